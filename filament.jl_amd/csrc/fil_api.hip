@@ -1,0 +1,628 @@
+// fil_api.hip -- the C ABI of libfilament_hip.so (declared in include/filament_hip.h).
+//
+// Device state of one RT problem:
+//   S   [Nrp][Nt/2]  positive-frequency spectrum of the field (the z-step state)
+//   X   [Nrp][Nt/2]  Hankel-space intermediate of the linear step
+//   M1, M2 [Nrp][Nrp] folded QDHT matrices; P [Nrp][Nt/2] propagator for the
+//   current dz.  Nrp = Nr rounded up to 64 (padding rows/columns are zero).
+// Rows are t/w-contiguous (the transpose of the reference's column-major
+// arrays), so the row phase reads whole rows and the Hankel phase reads the
+// spectrum as the B operand of a GEMM without a transpose.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fil_internal.h"
+#include "fil_kernels.h"
+
+using namespace fil;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t bytes) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    n = bytes;
+    if (!bytes) return hipSuccess;
+    return hipMalloc(&p, bytes);
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+#define HIPCHK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(FIL_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+constexpr double kPi = 3.14159265358979323846;
+
+double radius(const double* x, const double* y, int n) {
+  double ymax = -INFINITY;
+  for (int i = 0; i < n; i++) ymax = std::max(ymax, y[i]);
+  double lev = ymax * std::exp(-1.0), radl = 0, radr = 0;
+  for (int i = 0; i < n; i++) if (y[i] >= lev) { radl = x[i]; break; }
+  for (int i = n - 1; i >= 0; i--) if (y[i] >= lev) { radr = x[i]; break; }
+  return (std::fabs(radl) + std::fabs(radr)) / 2;
+}
+
+}  // namespace
+
+struct fil_handle {
+  fil_desc d{};
+  int Nr = 0, Nrp = 0, Nt = 0, Nh = 0;
+  bool fp64 = false, bf16x3 = false, scan64 = false;
+  hipStream_t st = nullptr;
+  bool tables = false, field = false;
+  bool pending_filter = false, plasma_valid = false;
+  double P_dz = NAN;
+  double dt = 0;
+  std::vector<double> r, rdr, t;
+  // device
+  DevBuf S, X, Et, M1, M2, M1h, M1l, M2h, M2l, KZ64, KG64, P, tw, Tg, Rg, coef;
+  DevBuf tab_ly[FIL_MAXCOMP], tab_sl[FIL_MAXCOMP];
+  DevBuf rho_end, rho_max, rho_full, kd_full;
+  DevBuf aFr, aIt, aSw, aImax, aFt, aS, ardr;
+  PlasmaParams<float> pp32{};
+  PlasmaParams<double> pp64{};
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;   // 4 per recorded step
+  size_t ev_used = 0;
+  double ms[3] = {0, 0, 0};
+  int64_t count = 0;
+
+  size_t rsz() const { return fp64 ? sizeof(double) : sizeof(float); }
+  size_t csz() const { return 2 * rsz(); }
+};
+
+namespace {
+
+template <typename R>
+void put_real(std::vector<unsigned char>& buf, size_t idx, double v) { reinterpret_cast<R*>(buf.data())[idx] = (R)v; }
+
+// host array of the working type
+struct HostArr {
+  std::vector<unsigned char> b;
+  bool fp64;
+  HostArr(size_t n, bool f64, bool complex) : b(n * (f64 ? 8 : 4) * (complex ? 2 : 1), 0), fp64(f64) {}
+  void set(size_t i, double v) { if (fp64) reinterpret_cast<double*>(b.data())[i] = v; else reinterpret_cast<float*>(b.data())[i] = (float)v; }
+  void setc(size_t i, double re, double im) { set(2 * i, re); set(2 * i + 1, im); }
+};
+
+hipError_t upload(DevBuf& d, const void* src, size_t bytes) {
+  hipError_t e = d.alloc(bytes);
+  if (e != hipSuccess) return e;
+  return hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice);
+}
+
+// round to bf16 (nearest even), returning the bits and the rounded value
+uint16_t to_bf16(float f, float* back) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+  uint16_t h = (uint16_t)(r >> 16);
+  uint32_t v = (uint32_t)h << 16;
+  std::memcpy(back, &v, 4);
+  return h;
+}
+
+int record(fil_handle* h, int slot) {
+  if (!h->timing) return 0;
+  size_t need = h->ev_used + 1;
+  if (need > h->ev.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    h->ev.push_back(e);
+  }
+  (void)slot;
+  HIPCHK(hipEventRecord(h->ev[h->ev_used], h->st));
+  h->ev_used++;
+  return 0;
+}
+
+int flush_timing(fil_handle* h) {
+  if (!h->ev_used) return 0;
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (size_t s = 0; s + 4 <= h->ev_used; s += 4) {
+    float a, b, c;
+    HIPCHK(hipEventElapsedTime(&a, h->ev[s], h->ev[s + 1]));
+    HIPCHK(hipEventElapsedTime(&b, h->ev[s + 1], h->ev[s + 2]));
+    HIPCHK(hipEventElapsedTime(&c, h->ev[s + 2], h->ev[s + 3]));
+    h->ms[0] += a; h->ms[1] += b; h->ms[2] += c;
+    h->count++;
+  }
+  h->ev_used = 0;
+  return 0;
+}
+
+template <typename R>
+int zstep_impl(fil_handle* h, double dz) {
+  if (!(h->P_dz == dz)) {
+    HIPCHK(launch_propagator<R>(h->KZ64.as<double>(), h->KG64.as<double>(), dz, h->P.as<cx<R>>(), h->Nrp, h->Nh, h->st));
+    h->P_dz = dz;
+  }
+  RowArgs<R> a{};
+  a.S = h->S.as<cx<R>>();
+  a.tw = h->tw.as<cx<R>>();
+  a.Tg = h->Tg.as<R>();
+  a.Rg = h->Rg.as<R>();
+  a.coef = h->coef.as<cx<R>>();
+  a.dz = (R)dz;
+  a.dt = (R)h->dt;
+  a.apply_filter = h->pending_filter ? 1 : 0;
+  a.nonlinearity = h->d.nonlinearity;
+  a.plasma = h->d.plasma;
+  a.responses = h->d.responses;
+  a.losses_ereal = h->d.losses_ereal;
+  if constexpr (sizeof(R) == 8) a.pp = h->pp64; else a.pp = h->pp32;
+  a.rho_end = h->rho_end.as<double>();
+  a.rho_max = h->rho_max.as<double>();
+  a.rho_full = h->d.keep_plasma ? h->rho_full.as<R>() : nullptr;
+  a.kd_full = h->d.keep_plasma ? h->kd_full.as<R>() : nullptr;
+  if (int rc = record(h, 0)) return rc;
+  HIPCHK(launch_row<R>(a, h->Nr, h->Nt, h->scan64, h->st));
+  if (int rc = record(h, 1)) return rc;
+  if constexpr (sizeof(R) == 4) {
+    if (h->bf16x3) {
+      HIPCHK(launch_dht_gemm_bf16x3(h->M1h.as<uint16_t>(), h->M1l.as<uint16_t>(), h->S.as<cx<float>>(),
+                                    h->X.as<cx<float>>(), h->P.as<cx<float>>(), h->Nrp, h->Nh, h->st));
+      if (int rc = record(h, 2)) return rc;
+      HIPCHK(launch_dht_gemm_bf16x3(h->M2h.as<uint16_t>(), h->M2l.as<uint16_t>(), h->X.as<cx<float>>(),
+                                    h->S.as<cx<float>>(), nullptr, h->Nrp, h->Nh, h->st));
+      if (int rc = record(h, 3)) return rc;
+      h->pending_filter = true;
+      h->plasma_valid = h->d.plasma != 0;
+      return 0;
+    }
+  }
+  HIPCHK(launch_dht_gemm<R>(h->M1.as<R>(), h->S.as<cx<R>>(), h->X.as<cx<R>>(), h->P.as<cx<R>>(), h->Nrp, h->Nh, h->st));
+  if (int rc = record(h, 2)) return rc;
+  HIPCHK(launch_dht_gemm<R>(h->M2.as<R>(), h->X.as<cx<R>>(), h->S.as<cx<R>>(), nullptr, h->Nrp, h->Nh, h->st));
+  if (int rc = record(h, 3)) return rc;
+  h->pending_filter = true;
+  h->plasma_valid = h->d.plasma != 0;
+  return 0;
+}
+
+// Row kernel without the nonlinear step: applies a pending field filter and
+// solves the plasma equation for the current field (Models.jl:66).
+template <typename R>
+int plasma_only(fil_handle* h) {
+  RowArgs<R> a{};
+  a.S = h->S.as<cx<R>>();
+  a.tw = h->tw.as<cx<R>>();
+  a.Tg = h->Tg.as<R>();
+  a.Rg = h->Rg.as<R>();
+  a.coef = h->coef.as<cx<R>>();
+  a.dz = 0;
+  a.dt = (R)h->dt;
+  a.apply_filter = h->pending_filter ? 1 : 0;
+  a.nonlinearity = 0;
+  a.plasma = 1;
+  a.responses = 0;
+  if constexpr (sizeof(R) == 8) a.pp = h->pp64; else a.pp = h->pp32;
+  a.rho_end = h->rho_end.as<double>();
+  a.rho_max = h->rho_max.as<double>();
+  a.rho_full = h->d.keep_plasma ? h->rho_full.as<R>() : nullptr;
+  a.kd_full = h->d.keep_plasma ? h->kd_full.as<R>() : nullptr;
+  HIPCHK(launch_row<R>(a, h->Nr, h->Nt, h->scan64, h->st));
+  h->pending_filter = false;
+  h->plasma_valid = true;
+  return 0;
+}
+
+template <typename R>
+int analyze_impl(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
+  const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh;
+  if (h->d.plasma && !h->plasma_valid)
+    if (int rc = plasma_only<R>(h)) return rc;
+  HIPCHK(launch_analyze_rows<R>(h->S.as<cx<R>>(), h->tw.as<cx<R>>(), h->Rg.as<R>(), h->Tg.as<R>(),
+                                h->pending_filter ? 1 : 0, (R)h->dt, h->aFr.as<double>(), h->aIt.as<float>(),
+                                h->aSw.as<float>(), h->aImax.as<double>(), Nr, Nt, h->st));
+  h->pending_filter = false;
+  HIPCHK(launch_colsum(h->aIt.as<float>(), Nr, Nt, h->ardr.as<double>(), h->aFt.as<double>(), h->st));
+  HIPCHK(launch_colsum(h->aSw.as<float>(), Nr, Nh + 1, h->ardr.as<double>(), h->aS.as<double>(), h->st));
+  std::vector<double> fr(Nr), ft(Nt), im(Nr), re(Nr, 0.0), rm(Nr, 0.0), sp(Nh + 1);
+  HIPCHK(hipMemcpyAsync(fr.data(), h->aFr.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(ft.data(), h->aFt.p, Nt * 8, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(im.data(), h->aImax.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(sp.data(), h->aS.p, (Nh + 1) * 8, hipMemcpyDeviceToHost, h->st));
+  if (h->d.plasma) {
+    HIPCHK(hipMemcpyAsync(re.data(), h->rho_end.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(rm.data(), h->rho_max.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  // scalar diagnostics (FieldAnalyzers.jl:201-213)
+  fil_diag o{};
+  o.z = z;
+  double Fmax = 0, Imax = 0, rhomax = 0, De = 0, W = 0;
+  for (int i = 0; i < Nr; i++) {
+    Fmax = std::max(Fmax, fr[i]);
+    Imax = std::max(Imax, im[i]);
+    rhomax = std::max(rhomax, rm[i]);
+    De += re[i] * h->rdr[i];
+    W += fr[i] * h->rdr[i];
+  }
+  o.Fmax = Fmax; o.Imax = Imax; o.rhomax = rhomax;
+  o.De = 2 * kPi * De;
+  o.rfil = 2 * radius(h->r.data(), fr.data(), Nr);
+  o.rpl = 2 * radius(h->r.data(), re.data(), Nr);
+  o.tau = radius(h->t.data(), ft.data(), Nt);
+  o.W = 2 * kPi * W;
+  if (d) *d = o;
+  if (Fr) std::memcpy(Fr, fr.data(), Nr * 8);
+  if (Ft) std::memcpy(Ft, ft.data(), Nt * 8);
+  if (rho_end) std::memcpy(rho_end, re.data(), Nr * 8);
+  if (S) std::memcpy(S, sp.data(), (Nh + 1) * 8);
+  return 0;
+}
+
+bool valid_handle(fil_handle* h) { return h != nullptr; }
+
+}  // namespace
+
+extern "C" {
+
+const char* fil_last_error(void) { return g_err.c_str(); }
+
+int fil_create(const fil_desc* d, fil_handle** out) {
+  if (!d || !out) return fail(FIL_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->abi_version != FIL_ABI_VERSION) return fail(FIL_EINVAL, "ABI version mismatch");
+  if (d->geometry != FIL_GEOM_RT) return fail(FIL_EINVAL, "only the RT geometry is implemented");
+  if (d->precision != FIL_FP32 && d->precision != FIL_FP64) return fail(FIL_EINVAL, "precision must be FIL_FP32 or FIL_FP64");
+  const int Nt = d->Nt;
+  if (Nt < 256 || Nt > 8192 || (Nt & (Nt - 1))) return fail(FIL_EINVAL, "Nt must be a power of two in [256, 8192]");
+  if (d->precision == FIL_FP64 && Nt > 4096) return fail(FIL_EINVAL, "FP64 supports Nt <= 4096");
+  if (d->Nr < 2 || d->Nr > 16384) return fail(FIL_EINVAL, "Nr must be in [2, 16384]");
+  if (d->plasma && (d->ncomp < 1 || d->ncomp > FIL_MAXCOMP)) return fail(FIL_EINVAL, "ncomp must be in [1, 4]");
+  if ((d->form & FIL_FORM_DHT_BF16X3) && d->precision != FIL_FP32)
+    return fail(FIL_EINVAL, "the split-BF16 Hankel form needs FIL_FP32");
+  if (d->nonlinearity && (d->responses & (FIL_RESP_RAMAN_THG | FIL_RESP_RAMAN_NOTHG)))
+    return fail(FIL_EINVAL, "the Raman response is not implemented");
+  if ((d->responses & FIL_RESP_CUBIC_THG) && (d->responses & FIL_RESP_CUBIC_NOTHG))
+    return fail(FIL_EINVAL, "choose one cubic response");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (d->device < 0 || d->device >= ndev) return fail(FIL_EINVAL, "no such HIP device");
+  HIPCHK(hipSetDevice(d->device));
+  fil_handle* h = new fil_handle();
+  h->d = *d;
+  h->Nr = d->Nr;
+  h->Nrp = (d->Nr + 63) / 64 * 64;
+  h->Nt = Nt;
+  h->Nh = Nt / 2;
+  h->fp64 = d->precision == FIL_FP64;
+  h->bf16x3 = (d->form & FIL_FORM_DHT_BF16X3) != 0;
+  h->scan64 = (d->form & FIL_FORM_SCAN_FP64) != 0 || h->fp64;
+  hipError_t e = hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete h; return fail(FIL_EDEVICE, hipGetErrorString(e)); }
+  const size_t cz = h->csz(), rz = h->rsz();
+  const size_t NS = (size_t)h->Nrp * h->Nh;
+  bool ok = h->S.alloc(NS * cz) == hipSuccess && h->X.alloc(NS * cz) == hipSuccess &&
+            h->P.alloc(NS * cz) == hipSuccess && h->Et.alloc((size_t)h->Nrp * Nt * cz) == hipSuccess &&
+            h->rho_end.alloc(h->Nr * 8) == hipSuccess && h->rho_max.alloc(h->Nr * 8) == hipSuccess &&
+            h->aFr.alloc(h->Nr * 8) == hipSuccess && h->aImax.alloc(h->Nr * 8) == hipSuccess &&
+            h->aIt.alloc((size_t)h->Nr * Nt * 4) == hipSuccess &&
+            h->aSw.alloc((size_t)h->Nr * (h->Nh + 1) * 4) == hipSuccess &&
+            h->aFt.alloc(Nt * 8) == hipSuccess && h->aS.alloc((h->Nh + 1) * 8) == hipSuccess;
+  if (ok && d->keep_plasma)
+    ok = h->rho_full.alloc((size_t)h->Nr * Nt * rz) == hipSuccess && h->kd_full.alloc((size_t)h->Nr * Nt * rz) == hipSuccess;
+  if (!ok) { delete h; return fail(FIL_ENOMEM, "device allocation failed"); }
+  (void)hipMemset(h->S.p, 0, NS * cz);
+  (void)hipMemset(h->X.p, 0, NS * cz);
+  (void)hipMemset(h->rho_end.p, 0, h->Nr * 8);
+  (void)hipMemset(h->rho_max.p, 0, h->Nr * 8);
+  if (d->keep_plasma) {
+    (void)hipMemset(h->rho_full.p, 0, (size_t)h->Nr * Nt * rz);
+    (void)hipMemset(h->kd_full.p, 0, (size_t)h->Nr * Nt * rz);
+  }
+  *out = h;
+  return FIL_OK;
+}
+
+int fil_set_tables(fil_handle* h, const fil_tables* t) {
+  if (!valid_handle(h) || !t) return fail(FIL_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(h->d.device));
+  const int Nr = h->Nr, Nrp = h->Nrp, Nt = h->Nt, Nh = h->Nh;
+  const bool f64 = h->fp64;
+  if (!t->r || !t->dr || !t->KZ || !t->Kguard || !t->Wguard || !t->Rguard || !t->Tguard || !t->dht_T ||
+      !t->dht_RJ || !t->dht_JV)
+    return fail(FIL_EINVAL, "missing linear-propagator table");
+  if (!(t->dt > 0)) return fail(FIL_EINVAL, "dt must be positive");
+  h->dt = t->dt;
+  h->r.assign(t->r, t->r + Nr);
+  h->rdr.resize(Nr);
+  for (int i = 0; i < Nr; i++) h->rdr[i] = t->r[i] * t->dr[i];
+  if (!t->t) return fail(FIL_EINVAL, "missing t grid");
+  h->t.assign(t->t, t->t + Nt);
+  HIPCHK(upload(h->ardr, h->rdr.data(), Nr * 8));
+
+  // folded QDHT matrices M1 = T diag(RJ), M2 = diag(1/RJ) T, zero padded
+  std::vector<double> M1((size_t)Nrp * Nrp, 0.0), M2((size_t)Nrp * Nrp, 0.0);
+  for (int m = 0; m < Nr; m++)
+    for (int n = 0; n < Nr; n++) {
+      double T = t->dht_T[m + (size_t)Nr * n];
+      M1[(size_t)m * Nrp + n] = T * t->dht_RJ[n];
+      M2[(size_t)m * Nrp + n] = T / t->dht_RJ[m];
+    }
+  {
+    HostArr a(M1.size(), f64, false), b(M2.size(), f64, false);
+    for (size_t q = 0; q < M1.size(); q++) { a.set(q, M1[q]); b.set(q, M2[q]); }
+    HIPCHK(upload(h->M1, a.b.data(), a.b.size()));
+    HIPCHK(upload(h->M2, b.b.data(), b.b.size()));
+  }
+  if (h->bf16x3) {
+    std::vector<uint16_t> h1(M1.size()), l1(M1.size()), h2(M2.size()), l2(M2.size());
+    for (size_t q = 0; q < M1.size(); q++) {
+      float back, back2;
+      float v = (float)M1[q];
+      h1[q] = to_bf16(v, &back);
+      l1[q] = to_bf16((float)(M1[q] - (double)back), &back2);
+      v = (float)M2[q];
+      h2[q] = to_bf16(v, &back);
+      l2[q] = to_bf16((float)(M2[q] - (double)back), &back2);
+    }
+    HIPCHK(upload(h->M1h, h1.data(), h1.size() * 2));
+    HIPCHK(upload(h->M1l, l1.data(), l1.size() * 2));
+    HIPCHK(upload(h->M2h, h2.data(), h2.size() * 2));
+    HIPCHK(upload(h->M2l, l2.data(), l2.size() * 2));
+  }
+  // propagator inputs in FP64: KZ and Kguard*Wguard on [Nrp][Nh]
+  {
+    std::vector<double> kz((size_t)Nrp * Nh * 2, 0.0), kg((size_t)Nrp * Nh, 0.0);
+    for (int m = 0; m < Nr; m++)
+      for (int w = 0; w < Nh; w++) {
+        size_t src = m + (size_t)Nr * w;
+        kz[2 * ((size_t)m * Nh + w)] = t->KZ[2 * src];
+        kz[2 * ((size_t)m * Nh + w) + 1] = t->KZ[2 * src + 1];
+        kg[(size_t)m * Nh + w] = t->Kguard[src] * t->Wguard[w];
+      }
+    HIPCHK(upload(h->KZ64, kz.data(), kz.size() * 8));
+    HIPCHK(upload(h->KG64, kg.data(), kg.size() * 8));
+    h->P_dz = NAN;
+  }
+  // twiddles, guards
+  {
+    HostArr tw(Nt, f64, true), tg(Nt, f64, false), rg(Nr, f64, false);
+    for (int j = 0; j < Nt; j++) {
+      double a = -2.0 * kPi * j / Nt;
+      tw.setc(j, std::cos(a), std::sin(a));
+      tg.set(j, t->Tguard[j]);
+    }
+    for (int i = 0; i < Nr; i++) rg.set(i, t->Rguard[i]);
+    HIPCHK(upload(h->tw, tw.b.data(), tw.b.size()));
+    HIPCHK(upload(h->Tg, tg.b.data(), tg.b.size()));
+    HIPCHK(upload(h->Rg, rg.b.data(), rg.b.size()));
+  }
+  // response coefficients i QZ Rnl d(w) / (2 Nt), d(0) = 1, d(w>0) = 2
+  {
+    HostArr cf((size_t)3 * Nh, f64, true);
+    const double* R[3] = {nullptr, nullptr, nullptr};
+    if (h->d.nonlinearity) {
+      if (h->d.responses & (FIL_RESP_CUBIC_THG | FIL_RESP_CUBIC_NOTHG)) R[0] = t->Rnl_cubic;
+      if (h->d.responses & FIL_RESP_FREE) R[1] = t->Rnl_free;
+      if (h->d.responses & FIL_RESP_LOSSES) R[2] = t->Rnl_losses;
+      if (!t->QZ) return fail(FIL_EINVAL, "missing QZ table");
+      for (int q = 0; q < 3; q++)
+        if ((q == 0 && (h->d.responses & (FIL_RESP_CUBIC_THG | FIL_RESP_CUBIC_NOTHG)) && !R[0]) ||
+            (q == 1 && (h->d.responses & FIL_RESP_FREE) && !R[1]) ||
+            (q == 2 && (h->d.responses & FIL_RESP_LOSSES) && !R[2]))
+          return fail(FIL_EINVAL, "missing Rnl table");
+    }
+    for (int q = 0; q < 3; q++)
+      for (int w = 0; w < Nh; w++) {
+        std::complex<double> c(0, 0);
+        if (R[q]) {
+          std::complex<double> qz(t->QZ[2 * w], t->QZ[2 * w + 1]);
+          std::complex<double> rn(R[q][2 * w], R[q][2 * w + 1]);
+          c = std::complex<double>(0, 1) * qz * rn * ((w == 0) ? 1.0 : 2.0) / (2.0 * Nt);
+        }
+        cf.setc((size_t)q * Nh + w, c.real(), c.imag());
+      }
+    HIPCHK(upload(h->coef, cf.b.data(), cf.b.size()));
+  }
+  // plasma parameters
+  if (h->d.plasma) {
+    PlasmaParams<double> p{};
+    p.ncomp = h->d.ncomp;
+    p.ereal = h->d.plasma_ereal;
+    p.kdep = h->d.plasma_kdep;
+    p.rho0 = t->rho0;
+    p.avalanche = 0;
+    const double L2 = std::log2(10.0);
+    PlasmaParams<float> p32{};
+    for (int q = 0; q < h->d.ncomp; q++) {
+      int n = t->ntab[q];
+      if (n < 2 || !t->tab_log10x[q] || !t->tab_log10y[q]) return fail(FIL_EINVAL, "rate table needs >= 2 nodes");
+      const double* lx = t->tab_log10x[q];
+      const double* ly = t->tab_log10y[q];
+      double d0 = lx[1] - lx[0];
+      if (!(d0 > 0)) return fail(FIL_EINVAL, "rate table x must increase");
+      for (int m = 1; m < n; m++)
+        if (std::fabs((lx[m] - lx[m - 1]) - d0) > 1e-9 * std::fabs(d0) + 1e-12)
+          return fail(FIL_EINVAL, "rate table must be uniformly spaced in log10(x)");
+      std::vector<double> ly2(n), sl(n - 1);
+      for (int m = 0; m < n; m++) ly2[m] = ly[m] * L2;
+      for (int m = 0; m < n - 1; m++) sl[m] = (ly[m + 1] - ly[m]) / (lx[m + 1] - lx[m]);
+      p.frhont[q] = t->frhont[q];
+      p.Rava[q] = t->Rava[q];
+      if (t->Rava[q] != 0) p.avalanche = 1;
+      p.K[q] = t->K[q];
+      p.ntab[q] = n;
+      p.lx0[q] = lx[0] * L2;
+      p.dlx[q] = d0 * L2;
+      p.inv_dlx[q] = 1.0 / (d0 * L2);
+      p.xmin[q] = std::pow(10.0, lx[0]);
+      if (f64) {
+        HIPCHK(upload(h->tab_ly[q], ly2.data(), n * 8));
+        HIPCHK(upload(h->tab_sl[q], sl.data(), (n - 1) * 8));
+        p.ly[q] = h->tab_ly[q].as<double>();
+        p.slope[q] = h->tab_sl[q].as<double>();
+      } else {
+        std::vector<float> a(ly2.begin(), ly2.end()), b(sl.begin(), sl.end());
+        HIPCHK(upload(h->tab_ly[q], a.data(), n * 4));
+        HIPCHK(upload(h->tab_sl[q], b.data(), (n - 1) * 4));
+        p32.ly[q] = h->tab_ly[q].as<float>();
+        p32.slope[q] = h->tab_sl[q].as<float>();
+      }
+    }
+    h->pp64 = p;
+    p32.ncomp = p.ncomp; p32.ereal = p.ereal; p32.kdep = p.kdep; p32.avalanche = p.avalanche;
+    p32.rho0 = (float)p.rho0;
+    for (int q = 0; q < FIL_MAXCOMP; q++) {
+      p32.frhont[q] = (float)p.frhont[q]; p32.Rava[q] = (float)p.Rava[q]; p32.K[q] = (float)p.K[q];
+      p32.ntab[q] = p.ntab[q]; p32.lx0[q] = (float)p.lx0[q]; p32.dlx[q] = (float)p.dlx[q];
+      p32.inv_dlx[q] = (float)p.inv_dlx[q]; p32.xmin[q] = (float)p.xmin[q];
+    }
+    h->pp32 = p32;
+  }
+  h->tables = true;
+  return FIL_OK;
+}
+
+int fil_upload_field(fil_handle* h, const double* E, int64_t ld) {
+  if (!valid_handle(h) || !E) return fail(FIL_EINVAL, "null argument");
+  if (!h->tables) return fail(FIL_ESTATE, "fil_set_tables first");
+  if (ld < h->Nr) return fail(FIL_EINVAL, "ld < Nr");
+  HIPCHK(hipSetDevice(h->d.device));
+  const int Nr = h->Nr, Nt = h->Nt;
+  HostArr a((size_t)h->Nrp * Nt, h->fp64, true);
+  for (int j = 0; j < Nt; j++)
+    for (int i = 0; i < Nr; i++) {
+      size_t s = (size_t)i + (size_t)ld * j;
+      a.setc((size_t)i * Nt + j, E[2 * s], E[2 * s + 1]);
+    }
+  HIPCHK(hipMemcpyAsync(h->Et.p, a.b.data(), a.b.size(), hipMemcpyHostToDevice, h->st));
+  if (h->fp64)
+    HIPCHK(launch_to_spectrum<double>(h->Et.as<cx<double>>(), h->S.as<cx<double>>(), h->tw.as<cx<double>>(), Nr, Nt, h->st));
+  else
+    HIPCHK(launch_to_spectrum<float>(h->Et.as<cx<float>>(), h->S.as<cx<float>>(), h->tw.as<cx<float>>(), Nr, Nt, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->field = true;
+  h->pending_filter = false;
+  h->plasma_valid = false;
+  return FIL_OK;
+}
+
+int fil_download_field(fil_handle* h, double* E, int64_t ld) {
+  if (!valid_handle(h) || !E) return fail(FIL_EINVAL, "null argument");
+  if (!h->field) return fail(FIL_ESTATE, "no field uploaded");
+  if (ld < h->Nr) return fail(FIL_EINVAL, "ld < Nr");
+  HIPCHK(hipSetDevice(h->d.device));
+  const int Nr = h->Nr, Nt = h->Nt;
+  if (h->fp64)
+    HIPCHK(launch_to_time<double>(h->S.as<cx<double>>(), h->Et.as<cx<double>>(), h->tw.as<cx<double>>(),
+                                  h->Rg.as<double>(), h->Tg.as<double>(), h->pending_filter, Nr, Nt, h->st));
+  else
+    HIPCHK(launch_to_time<float>(h->S.as<cx<float>>(), h->Et.as<cx<float>>(), h->tw.as<cx<float>>(),
+                                 h->Rg.as<float>(), h->Tg.as<float>(), h->pending_filter, Nr, Nt, h->st));
+  std::vector<unsigned char> buf((size_t)Nr * Nt * h->csz());
+  HIPCHK(hipMemcpyAsync(buf.data(), h->Et.p, buf.size(), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int i = 0; i < Nr; i++)
+    for (int j = 0; j < Nt; j++) {
+      size_t s = (size_t)i * Nt + j, o = (size_t)i + (size_t)ld * j;
+      if (h->fp64) {
+        E[2 * o] = reinterpret_cast<double*>(buf.data())[2 * s];
+        E[2 * o + 1] = reinterpret_cast<double*>(buf.data())[2 * s + 1];
+      } else {
+        E[2 * o] = reinterpret_cast<float*>(buf.data())[2 * s];
+        E[2 * o + 1] = reinterpret_cast<float*>(buf.data())[2 * s + 1];
+      }
+    }
+  return FIL_OK;
+}
+
+int fil_zstep(fil_handle* h, double z, double dz) {
+  (void)z;
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_zstep");
+  if (!std::isfinite(dz)) return fail(FIL_EINVAL, "dz must be finite");
+  HIPCHK(hipSetDevice(h->d.device));
+  return h->fp64 ? zstep_impl<double>(h, dz) : zstep_impl<float>(h, dz);
+}
+
+int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_analyze");
+  HIPCHK(hipSetDevice(h->d.device));
+  return h->fp64 ? analyze_impl<double>(h, z, d, Fr, Ft, rho_end, S) : analyze_impl<float>(h, z, d, Fr, Ft, rho_end, S);
+}
+
+int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (!h->d.plasma || !h->d.keep_plasma) return fail(FIL_ESTATE, "create the handle with plasma and keep_plasma");
+  if (ld < h->Nr) return fail(FIL_EINVAL, "ld < Nr");
+  HIPCHK(hipSetDevice(h->d.device));
+  if (!h->plasma_valid) {
+    int rc = h->fp64 ? plasma_only<double>(h) : plasma_only<float>(h);
+    if (rc) return rc;
+  }
+  const int Nr = h->Nr, Nt = h->Nt;
+  std::vector<unsigned char> a((size_t)Nr * Nt * h->rsz()), b((size_t)Nr * Nt * h->rsz());
+  HIPCHK(hipMemcpyAsync(a.data(), h->rho_full.p, a.size(), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(b.data(), h->kd_full.p, b.size(), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int i = 0; i < Nr; i++)
+    for (int j = 0; j < Nt; j++) {
+      size_t s = (size_t)i * Nt + j, o = (size_t)i + (size_t)ld * j;
+      double rv = h->fp64 ? reinterpret_cast<double*>(a.data())[s] : reinterpret_cast<float*>(a.data())[s];
+      double kv = h->fp64 ? reinterpret_cast<double*>(b.data())[s] : reinterpret_cast<float*>(b.data())[s];
+      if (rho) rho[o] = rv;
+      if (kdrho) kdrho[o] = kv;
+    }
+  return FIL_OK;
+}
+
+int fil_synchronize(fil_handle* h) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(h->st));
+  return FIL_OK;
+}
+
+int fil_destroy(fil_handle* h) {
+  if (!h) return FIL_OK;
+  (void)hipSetDevice(h->d.device);
+  (void)hipStreamSynchronize(h->st);
+  for (auto e : h->ev) (void)hipEventDestroy(e);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+  return FIL_OK;
+}
+
+int fil_timing_enable(fil_handle* h, int enable) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (int rc = flush_timing(h)) return rc;
+  h->timing = enable != 0;
+  h->ms[0] = h->ms[1] = h->ms[2] = 0;
+  h->count = 0;
+  return FIL_OK;
+}
+
+int fil_timing_read(fil_handle* h, double* ms_row, double* ms_dht1, double* ms_dht2, int64_t* count) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (int rc = flush_timing(h)) return rc;
+  if (ms_row) *ms_row = h->ms[0];
+  if (ms_dht1) *ms_dht1 = h->ms[1];
+  if (ms_dht2) *ms_dht2 = h->ms[2];
+  if (count) *count = h->count;
+  return FIL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// fil_internal.h -- shared types of libfilament_hip.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/filament_hip.h"
+
+namespace fil {
+
+// Complex value with the memory layout of float2 / double2.
+template <typename R>
+struct __align__(2 * sizeof(R)) cx {
+  R x, y;
+};
+
+template <typename R> __host__ __device__ __forceinline__ cx<R> mk(R a, R b) { cx<R> c; c.x = a; c.y = b; return c; }
+template <typename R> __host__ __device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename R> __host__ __device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename R> __host__ __device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename R> __host__ __device__ __forceinline__ cx<R> operator*(R s, cx<R> a) { return mk(s * a.x, s * a.y); }
+template <typename R> __host__ __device__ __forceinline__ cx<R> conj(cx<R> a) { return mk(a.x, -a.y); }
+// multiply by +i or -i
+template <typename R> __host__ __device__ __forceinline__ cx<R> mul_i(cx<R> a) { return mk(-a.y, a.x); }
+template <typename R> __host__ __device__ __forceinline__ cx<R> mul_mi(cx<R> a) { return mk(a.y, -a.x); }
+template <typename R> __host__ __device__ __forceinline__ R norm2(cx<R> a) { return a.x * a.x + a.y * a.y; }
+
+// Plasma-equation parameters passed by value to the row kernel
+// (photoionization_avalanche.jl:126-183), in the working precision.
+template <typename R>
+struct PlasmaParams {
+  int ncomp, ereal, kdep, avalanche;
+  R rho0;
+  R frhont[FIL_MAXCOMP], Rava[FIL_MAXCOMP], K[FIL_MAXCOMP];
+  // log2-domain rate tables, uniform in log2(x):
+  //   W = exp2(ly[i] + slope[i] (log2 I - lx0 - i dlx)), clamped to the end nodes
+  int ntab[FIL_MAXCOMP];
+  R lx0[FIL_MAXCOMP], inv_dlx[FIL_MAXCOMP], dlx[FIL_MAXCOMP];
+  R xmin[FIL_MAXCOMP];              // linear x of the first node
+  const R* ly[FIL_MAXCOMP];         // log2 y nodes (device)
+  const R* slope[FIL_MAXCOMP];      // per-interval log-log slope (device)
+};
+
+struct Handle;
+
+}  // namespace fil
+
+struct fil_handle;

@@ -1,0 +1,609 @@
+// fil_row.hip -- the row phase of one z-step: everything that is local to one
+// r row of the field, fused into one workgroup per row on gfx950.
+//
+// Per row (src/modules/Models.jl:75-121 restated for one r):
+//   (pending field filter of the previous step: t-transform back, multiply by
+//    Rguard(r)*Tguard(t), forward again -- Models.jl:109-119, Guards.jl:265-280)
+//   plasma density along t                      plasma_equations.jl:27-72
+//   RK3 in z of the nonlinear spectral RHS       nonlinear_propagators.jl:38-74
+//     (responses, field filter, analytic spectrum, update_dE!, i*QZ)
+// The row's spectrum stays in LDS for the whole step; HBM sees one read and
+// one write of the Nt/2 positive-frequency bins.
+#include "fil_internal.h"
+#include "fil_fft.h"
+#include "fil_kernels.h"
+
+namespace fil {
+
+// ---------------------------------------------------------------------------
+// Rate of one plasma component: TabulatedFunctions.TFunction evaluated as
+// log-log linear interpolation, clamped to the end nodes
+// (photoionization_avalanche.jl:158,184,196).  `power` gives the local
+// log-log slope (tfpower, :321).
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R tf_eval(const PlasmaParams<R>& pp, int q, R I, R* power) {
+  const int n = pp.ntab[q];
+  int i;
+  R lc;
+  if (!(I > (R)pp.xmin[q])) { i = 0; lc = (R)pp.lx0[q]; }
+  else {
+    R l = (sizeof(R) == 8) ? (R)log2((double)I) : (R)__log2f((float)I);
+    R u = (l - (R)pp.lx0[q]) * (R)pp.inv_dlx[q];
+    if (u >= (R)(n - 1)) { i = n - 2; lc = (R)pp.lx0[q] + (R)(n - 1) * (R)pp.dlx[q]; }
+    else { i = (int)u; if (i > n - 2) i = n - 2; lc = l; }
+  }
+  const R s = (R)pp.slope[q][i];
+  if (power) *power = s;
+  const R y = (R)pp.ly[q][i] + s * (lc - ((R)pp.lx0[q] + (R)i * (R)pp.dlx[q]));
+  return (sizeof(R) == 8) ? (R)exp2((double)y) : (R)exp2f((float)y);
+}
+
+template <typename R>
+__device__ __forceinline__ R fiarg(cx<R> e, int ereal) { return ereal ? e.x * e.x : norm2(e); }
+
+// Affine map rho -> a*rho + b, composed along t.  later o earlier.
+template <typename A>
+struct Aff { A a, b; };
+template <typename A>
+__device__ __forceinline__ Aff<A> compose(Aff<A> later, Aff<A> earlier) {
+  Aff<A> r; r.a = later.a * earlier.a; r.b = later.a * earlier.b + later.b; return r;
+}
+
+// Exclusive block scan of affine maps in thread order.  red: LDS scratch of
+// 2*(NTH/64) A values.
+template <typename A, int NTH>
+__device__ Aff<A> block_exclusive_scan(Aff<A> v, A* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  Aff<A> inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    A pa = __shfl_up(inc.a, off, 64), pb = __shfl_up(inc.b, off, 64);
+    if (lane >= off) { Aff<A> p; p.a = pa; p.b = pb; inc = compose(inc, p); }
+  }
+  // exclusive within the wave
+  A ea = __shfl_up(inc.a, 1, 64), eb = __shfl_up(inc.b, 1, 64);
+  Aff<A> ex; ex.a = (lane == 0) ? (A)1 : ea; ex.b = (lane == 0) ? (A)0 : eb;
+  constexpr int NW = NTH / 64;
+  if (lane == 63) { red[2 * wv] = inc.a; red[2 * wv + 1] = inc.b; }
+  __syncthreads();
+  Aff<A> wp; wp.a = (A)1; wp.b = (A)0;
+  for (int w = 0; w < wv && w < NW; w++) { Aff<A> t; t.a = red[2 * w]; t.b = red[2 * w + 1]; wp = compose(t, wp); }
+  __syncthreads();
+  return compose(ex, wp);
+}
+
+template <typename R>
+__device__ __forceinline__ R block_max(R v, R* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  R m = red[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = max(m, red[w]);
+  __syncthreads();
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// Plasma density of one row from the time-domain field Z (LDS).
+// Writes rho[t], kd[t] (LDS).  Without avalanche every component obeys the
+// linear ODE d rho/dt = W(t) (f - rho); one RK3 step of it (the reference's
+// ODEIntegrators.RK3, stages at t_j, t_j + dt/2, t_j + dt with the field
+// linearly interpolated) is exactly the affine map
+//   rho_{j+1} = rho_j + h_j (f - rho_j),
+//   h_j = dt [W1/6 + 2/3 W2 (1 - dt W1/2) + 1/6 W3 (1 + dt W1 - 2 dt W2 + dt^2 W1 W2)],
+// so the sequential t loop becomes a block-wide prefix scan of affine maps,
+// carried in A (FP64 for the fp64-scan form).  With avalanche the RHS is
+// quadratic and one lane walks the row sequentially.
+// ---------------------------------------------------------------------------
+template <typename R, typename A, int N, int NTH>
+__device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>& pp, R dt, A* red) {
+  const int tid = threadIdx.x;
+  const int nc = pp.ncomp;
+  if (pp.avalanche) {
+    if (tid == 0) {
+      // sequential reference loop (plasma_equations.jl:60-69)
+      R u[FIL_MAXCOMP], ut[FIL_MAXCOMP];
+      for (int q = 0; q < nc; q++) { u[q] = (R)pp.rho0; ut[q] = (R)pp.rho0; }
+      R s0 = 0; for (int q = 0; q < nc; q++) s0 += u[q];
+      rho[0] = s0;
+      for (int j = 0; j < N - 1; j++) {
+        cx<R> E0 = Z[j], E1 = Z[j + 1];
+        cx<R> Em = (R)0.5 * (E0 + E1);
+        R Ist[3] = {fiarg(E0, pp.ereal), fiarg(Em, pp.ereal), fiarg(E1, pp.ereal)};
+        R E2st[3] = {E0.x * E0.x, Em.x * Em.x, E1.x * E1.x};
+        R W[3][FIL_MAXCOMP];
+        for (int st = 0; st < 3; st++)
+          for (int q = 0; q < nc; q++) W[st][q] = tf_eval<R>(pp, q, Ist[st], nullptr);
+        // avalanche integrator
+        R k[3][FIL_MAXCOMP], tmp[FIL_MAXCOMP];
+        for (int st = 0; st < 3; st++) {
+          for (int q = 0; q < nc; q++) {
+            if (st == 0) tmp[q] = u[q];
+            else if (st == 1) tmp[q] = u[q] + dt * ((R)0.5 * k[0][q]);
+            else tmp[q] = u[q] + dt * (-k[0][q] + (R)2 * k[1][q]);
+          }
+          R tot = 0; for (int q = 0; q < nc; q++) tot += tmp[q];
+          for (int q = 0; q < nc; q++) {
+            R f = (R)pp.frhont[q];
+            k[st][q] = W[st][q] * (f - tmp[q]) + (R)pp.Rava[q] * E2st[st] * (f - tmp[q]) / f * tot;
+          }
+        }
+        R s = 0;
+        for (int q = 0; q < nc; q++) {
+          u[q] = u[q] + dt * (k[0][q] / (R)6 + (R)2 * k[1][q] / (R)3 + k[2][q] / (R)6);
+          s += u[q];
+        }
+        rho[j + 1] = s;
+        // avalanche-free integrator for kdrho
+        for (int st = 0; st < 3; st++) {
+          for (int q = 0; q < nc; q++) {
+            if (st == 0) tmp[q] = ut[q];
+            else if (st == 1) tmp[q] = ut[q] + dt * ((R)0.5 * k[0][q]);
+            else tmp[q] = ut[q] + dt * (-k[0][q] + (R)2 * k[1][q]);
+          }
+          for (int q = 0; q < nc; q++) k[st][q] = W[st][q] * ((R)pp.frhont[q] - tmp[q]);
+        }
+        R kdv = 0;
+        for (int q = 0; q < nc; q++) {
+          ut[q] = ut[q] + dt * (k[0][q] / (R)6 + (R)2 * k[1][q] / (R)3 + k[2][q] / (R)6);
+          R pw;
+          R w1 = tf_eval<R>(pp, q, Ist[0], &pw);
+          R K = pp.kdep ? pw : (R)pp.K[q];
+          kdv += K * (w1 * ((R)pp.frhont[q] - ut[q]));
+        }
+        kd[j] = kdv;
+      }
+      kd[N - 1] = 0;
+    }
+    __syncthreads();
+    return;
+  }
+
+  constexpr int C = N / NTH;   // contiguous chunk of t per thread
+  const int j0 = tid * C;
+  const A dtA = (A)dt;
+  // pass 1: this thread's composed affine map for each component
+  Aff<A> m[FIL_MAXCOMP];
+#pragma unroll
+  for (int q = 0; q < FIL_MAXCOMP; q++) { m[q].a = (A)1; m[q].b = (A)0; }
+  R Wprev[FIL_MAXCOMP];
+  bool have_prev = false;
+  for (int c = 0; c < C; c++) {
+    const int j = j0 + c;
+    if (j >= N - 1) break;
+    cx<R> E0 = Z[j], E1 = Z[j + 1];
+    cx<R> Em = (R)0.5 * (E0 + E1);
+    R I0 = fiarg(E0, pp.ereal), Im = fiarg(Em, pp.ereal), I1 = fiarg(E1, pp.ereal);
+#pragma unroll
+    for (int q = 0; q < FIL_MAXCOMP; q++) {
+      if (q >= nc) break;
+      R W1 = have_prev ? Wprev[q] : tf_eval<R>(pp, q, I0, nullptr);
+      R W2 = tf_eval<R>(pp, q, Im, nullptr);
+      R W3 = tf_eval<R>(pp, q, I1, nullptr);
+      Wprev[q] = W3;
+      A w1 = W1, w2 = W2, w3 = W3;
+      A h = dtA * (w1 / (A)6 + (A)2 / (A)3 * w2 * ((A)1 - dtA * w1 / (A)2) +
+                   w3 / (A)6 * ((A)1 + dtA * w1 - (A)2 * dtA * w2 + dtA * dtA * w1 * w2));
+      Aff<A> st; st.a = (A)1 - h; st.b = h * (A)pp.frhont[q];
+      m[q] = compose(st, m[q]);
+    }
+    have_prev = true;
+  }
+  // block scan per component -> density at the start of this thread's chunk
+  A r[FIL_MAXCOMP];
+#pragma unroll
+  for (int q = 0; q < FIL_MAXCOMP; q++) {
+    if (q >= nc) break;
+    Aff<A> ex = block_exclusive_scan<A, NTH>(m[q], red);
+    r[q] = ex.a * (A)pp.rho0 + ex.b;
+  }
+  if (tid == 0) {
+    R s0 = 0; for (int q = 0; q < nc; q++) s0 += (R)pp.rho0;
+    rho[0] = s0;
+  }
+  // pass 2: walk the chunk again
+  have_prev = false;
+  for (int c = 0; c < C; c++) {
+    const int j = j0 + c;
+    if (j >= N - 1) break;
+    cx<R> E0 = Z[j], E1 = Z[j + 1];
+    cx<R> Em = (R)0.5 * (E0 + E1);
+    R I0 = fiarg(E0, pp.ereal), Im = fiarg(Em, pp.ereal), I1 = fiarg(E1, pp.ereal);
+    A tot = 0;
+    R kdv = 0;
+#pragma unroll
+    for (int q = 0; q < FIL_MAXCOMP; q++) {
+      if (q >= nc) break;
+      R pw = 0;
+      R W1 = tf_eval<R>(pp, q, I0, pp.kdep ? &pw : nullptr);
+      R W2 = tf_eval<R>(pp, q, Im, nullptr);
+      R W3 = tf_eval<R>(pp, q, I1, nullptr);
+      A w1 = W1, w2 = W2, w3 = W3;
+      A h = dtA * (w1 / (A)6 + (A)2 / (A)3 * w2 * ((A)1 - dtA * w1 / (A)2) +
+                   w3 / (A)6 * ((A)1 + dtA * w1 - (A)2 * dtA * w2 + dtA * dtA * w1 * w2));
+      r[q] = ((A)1 - h) * r[q] + h * (A)pp.frhont[q];
+      tot += r[q];
+      R K = pp.kdep ? pw : (R)pp.K[q];
+      kdv += K * (W1 * (R)((A)pp.frhont[q] - r[q]));
+    }
+    rho[j + 1] = (R)tot;
+    kd[j] = kdv;
+  }
+  if (tid == NTH - 1) kd[N - 1] = 0;
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Responses of one RK stage.  Z holds the stage field in time (LDS); on exit
+// k[m] (spectral bins w = tid + NTH*m, w < N/2) holds func!'s dE.
+//   F_cubic = 3/4 |E|^2 Re E  or Re(E)^3          cubic.jl:36,44
+//   F_free  = rho Re E                             current_free.jl:35
+//   F_loss  = kdrho Re E / |E|^2 (0 below 1e-30)   current_losses.jl:131-133, :71-83
+// each multiplied by Rguard(r) Tguard(t) (nonlinear_propagators.jl:50), taken
+// to the analytic spectrum (Models.jl:180-183) and weighted by i QZ Rnl
+// (nonlinear_propagators.jl:51,57).  Two real signals share one complex FFT:
+// with Z = F_a + i F_b, 2 F_a^(w) = Z^(w) + conj Z^(N-w),
+// 2 F_b^(w) = -i (Z^(w) - conj Z^(N-w)).  coef = i QZ Rnl d(w) / (2N).
+// ---------------------------------------------------------------------------
+template <typename R, int N, int NTH>
+__device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, const cx<R>* __restrict__ tw,
+                                          const cx<R>* __restrict__ coef, R g_r, const R* __restrict__ Tg,
+                                          int resp, int losses_ereal, cx<R>* k) {
+  constexpr int TP = N / NTH;
+  constexpr int NH = N / 2;
+  constexpr int SP = NH / NTH;
+  const int tid = threadIdx.x;
+  const bool cub = resp & (FIL_RESP_CUBIC_THG | FIL_RESP_CUBIC_NOTHG);
+  const bool thg = resp & FIL_RESP_CUBIC_THG;
+  const bool fre = resp & FIL_RESP_FREE;
+  const bool los = resp & FIL_RESP_LOSSES;
+  R Fl[TP];
+#pragma unroll
+  for (int u = 0; u < TP; u++) {
+    const int t = tid + u * NTH;
+    cx<R> e = Z[t];
+    const R re = e.x, I = norm2(e);
+    const R g = g_r * Tg[t];
+    R fc = 0, ff = 0, fl = 0;
+    if (cub) fc = thg ? re * re * re : (R)0.75 * I * re;
+    if (fre) ff = rho[t] * re;
+    if (los) {
+      R fe = losses_ereal ? re * re : I;
+      R inv = (fe >= (R)1e-30) ? (R)1 / fe : (R)0;
+      fl = kd[t] * inv * re;
+    }
+    Fl[u] = g * fl;
+    Z[t] = mk(g * fc, g * ff);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < SP; m++) k[m] = mk((R)0, (R)0);
+  if (cub || fre) {
+    fft_lds<R, N, NTH, +1>(Z, tw);
+#pragma unroll
+    for (int m = 0; m < SP; m++) {
+      const int w = tid + m * NTH;
+      cx<R> zw = Z[w], zm = conj(Z[(N - w) & (N - 1)]);
+      cx<R> a2 = zw + zm, b2 = mul_mi(zw - zm);
+      k[m] = coef[w] * a2 + coef[NH + w] * b2;
+    }
+    __syncthreads();
+  }
+  if (los) {
+#pragma unroll
+    for (int u = 0; u < TP; u++) Z[tid + u * NTH] = mk(Fl[u], (R)0);
+    __syncthreads();
+    fft_lds<R, N, NTH, +1>(Z, tw);
+#pragma unroll
+    for (int m = 0; m < SP; m++) {
+      const int w = tid + m * NTH;
+      cx<R> zw = Z[w], zm = conj(Z[(N - w) & (N - 1)]);
+      k[m] = k[m] + coef[2 * NH + w] * (zw + zm);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R, int N, int NTH>
+__device__ __forceinline__ void store_padded(cx<R>* Z, const cx<R>* v) {
+  constexpr int SP = (N / 2) / NTH;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < SP; m++) { Z[tid + m * NTH] = v[m]; Z[N / 2 + tid + m * NTH] = mk((R)0, (R)0); }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// The row kernel.  One workgroup per r row; LDS: Z (N complex), rho, kd (N R).
+// ---------------------------------------------------------------------------
+template <typename R, typename A, int N, int NTH>
+__global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
+  R* rho = reinterpret_cast<R*>(Z + N);
+  R* kd = rho + N;
+  A* red = reinterpret_cast<A*>(kd + N);
+  constexpr int NH = N / 2;
+  constexpr int SP = NH / NTH;
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x;
+  cx<R>* Srow = a.S + (size_t)i * NH;
+  const R g_r = a.Rg[i];
+
+  cx<R> S0[SP];
+#pragma unroll
+  for (int m = 0; m < SP; m++) S0[m] = Srow[tid + m * NTH];
+
+  if (a.apply_filter) {
+    // field filter deferred from the end of the previous zstep
+    store_padded<R, N, NTH>(Z, S0);
+    fft_lds<R, N, NTH, -1>(Z, a.tw);
+    for (int t = tid; t < N; t += NTH) Z[t] = (g_r * a.Tg[t]) * Z[t];
+    __syncthreads();
+    fft_lds<R, N, NTH, +1>(Z, a.tw);
+    const R inv = (R)1 / (R)N;
+#pragma unroll
+    for (int m = 0; m < SP; m++) S0[m] = inv * Z[tid + m * NTH];
+    __syncthreads();
+  }
+  // time-domain field of the step's starting spectrum
+  store_padded<R, N, NTH>(Z, S0);
+  fft_lds<R, N, NTH, -1>(Z, a.tw);
+
+  if (a.plasma) {
+    plasma_row<R, A, N, NTH>(Z, rho, kd, a.pp, a.dt, red);
+    R mx = 0;
+    for (int t = tid; t < N; t += NTH) mx = max(mx, rho[t]);
+    mx = block_max<R>(mx, reinterpret_cast<R*>(red));
+    if (tid == 0) { a.rho_end[i] = (double)rho[N - 1]; a.rho_max[i] = (double)mx; }
+    if (a.rho_full) {
+      for (int t = tid; t < N; t += NTH) {
+        a.rho_full[(size_t)i * N + t] = rho[t];
+        a.kd_full[(size_t)i * N + t] = kd[t];
+      }
+    }
+  } else {
+    for (int t = tid; t < N; t += NTH) { rho[t] = 0; kd[t] = 0; }
+    __syncthreads();
+  }
+
+  if (!a.nonlinearity) {
+    if (a.apply_filter) {
+#pragma unroll
+      for (int m = 0; m < SP; m++) Srow[tid + m * NTH] = S0[m];
+    }
+    return;
+  }
+
+  // RK3 (Kutta) in z: k1 = f(S0); k2 = f(S0 + dz/2 k1); k3 = f(S0 - dz k1 + 2 dz k2)
+  const R dz = a.dz;
+  cx<R> k1[SP], k2[SP], k[SP], U[SP];
+  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, g_r, a.Tg, a.responses, a.losses_ereal, k1);
+#pragma unroll
+  for (int m = 0; m < SP; m++) U[m] = S0[m] + ((R)0.5 * dz) * k1[m];
+  store_padded<R, N, NTH>(Z, U);
+  fft_lds<R, N, NTH, -1>(Z, a.tw);
+  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, g_r, a.Tg, a.responses, a.losses_ereal, k2);
+#pragma unroll
+  for (int m = 0; m < SP; m++) U[m] = S0[m] + dz * (k2[m] + k2[m] - k1[m]);
+  store_padded<R, N, NTH>(Z, U);
+  fft_lds<R, N, NTH, -1>(Z, a.tw);
+  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, g_r, a.Tg, a.responses, a.losses_ereal, k);
+  const R c16 = (R)1 / (R)6, c23 = (R)2 / (R)3;
+#pragma unroll
+  for (int m = 0; m < SP; m++) {
+    cx<R> acc = c16 * k1[m] + c23 * k2[m] + c16 * k[m];
+    Srow[tid + m * NTH] = S0[m] + dz * acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Upload: time-domain rows [Nr][N] -> positive-frequency spectrum S [Nr][N/2]
+// (forward_transform_time!, Models.jl:162-165).
+// ---------------------------------------------------------------------------
+template <typename R, int N, int NTH>
+__global__ __launch_bounds__(NTH) void to_spectrum_kernel(const cx<R>* __restrict__ Et, cx<R>* S, const cx<R>* tw) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
+  const int i = blockIdx.x, tid = threadIdx.x;
+  for (int t = tid; t < N; t += NTH) Z[t] = Et[(size_t)i * N + t];
+  __syncthreads();
+  fft_lds<R, N, NTH, +1>(Z, tw);
+  const R inv = (R)1 / (R)N;
+  for (int w = tid; w < N / 2; w += NTH) S[(size_t)i * (N / 2) + w] = inv * Z[w];
+}
+
+// Download: spectrum -> time rows, with the pending field filter.
+template <typename R, int N, int NTH>
+__global__ __launch_bounds__(NTH) void to_time_kernel(const cx<R>* __restrict__ S, cx<R>* Et, const cx<R>* tw,
+                                                      const R* Rg, const R* Tg, int apply_filter) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
+  const int i = blockIdx.x, tid = threadIdx.x;
+  for (int w = tid; w < N / 2; w += NTH) { Z[w] = S[(size_t)i * (N / 2) + w]; Z[N / 2 + w] = mk((R)0, (R)0); }
+  __syncthreads();
+  fft_lds<R, N, NTH, -1>(Z, tw);
+  const R g = apply_filter ? Rg[i] : (R)1;
+  for (int t = tid; t < N; t += NTH) {
+    cx<R> v = Z[t];
+    if (apply_filter) v = (g * Tg[t]) * v;
+    Et[(size_t)i * N + t] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FieldAnalyzers.analyze! RT, per-row part (FieldAnalyzers.jl:166-215):
+// Fr(r) = dt sum_t |E|^2; It[r][t] = |E|^2 (for Ft); the projected spectrum
+// replaces the state (aspec2rspec!/rspec2aspec! round trip, :188-197);
+// Sw[r][w] = |2 Nt dt rspec(w)|^2 for w <= Nt/2; Imax of the projected field.
+// ---------------------------------------------------------------------------
+template <typename R, int N, int NTH>
+__global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>* tw, const R* Rg, const R* Tg,
+                                                           int apply_filter, R dt, double* Fr, float* It,
+                                                           float* Sw, double* Imax_row) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
+  double* red = reinterpret_cast<double*>(Z + N);
+  const int i = blockIdx.x, tid = threadIdx.x;
+  constexpr int NH = N / 2;
+  for (int w = tid; w < NH; w += NTH) { Z[w] = S[(size_t)i * NH + w]; Z[NH + w] = mk((R)0, (R)0); }
+  __syncthreads();
+  fft_lds<R, N, NTH, -1>(Z, tw);
+  const R g = apply_filter ? Rg[i] : (R)1;
+  double fr = 0;
+  for (int t = tid; t < N; t += NTH) {
+    cx<R> v = Z[t];
+    if (apply_filter) { v = (g * Tg[t]) * v; Z[t] = v; }
+    double I = (double)norm2(v);
+    fr += I;
+    It[(size_t)i * N + t] = (float)I;
+  }
+  // block sum of fr
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) fr += __shfl_xor(fr, off, 64);
+  if (lane == 0) red[wv] = fr;
+  __syncthreads();
+  if (tid == 0) { double s = 0; for (int w = 0; w < NTH / 64; w++) s += red[w]; Fr[i] = s * (double)dt; }
+  __syncthreads();
+  fft_lds<R, N, NTH, +1>(Z, tw);
+  const R inv = (R)1 / (R)N;
+  const double c = 2.0 * (double)N * (double)dt;
+  for (int w = tid; w <= NH; w += NTH) {
+    cx<R> v = inv * Z[w];
+    double re = (double)v.x, im = (double)v.y;
+    if (w >= 1 && w < NH) { re *= 0.5; im *= 0.5; }   // aspec2rspec
+    Sw[(size_t)i * (NH + 1) + w] = (float)(c * c * (re * re + im * im));
+  }
+  __syncthreads();
+  // projection: keep bins 0..N/2-1 as the new state
+  cx<R> keep[NH / NTH];
+#pragma unroll
+  for (int m = 0; m < NH / NTH; m++) keep[m] = inv * Z[tid + m * NTH];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < NH / NTH; m++) {
+    S[(size_t)i * NH + tid + m * NTH] = keep[m];
+    Z[tid + m * NTH] = keep[m];
+    Z[NH + tid + m * NTH] = mk((R)0, (R)0);
+  }
+  __syncthreads();
+  fft_lds<R, N, NTH, -1>(Z, tw);
+  double mx = 0;
+  for (int t = tid; t < N; t += NTH) mx = fmax(mx, (double)norm2(Z[t]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) red[wv] = mx;
+  __syncthreads();
+  if (tid == 0) { double m = 0; for (int w = 0; w < NTH / 64; w++) m = fmax(m, red[w]); Imax_row[i] = m; }
+}
+
+// Column sums over r: out[j] = 2 pi sum_i in[i][j] * rdr[i], in FP64, fixed order.
+__global__ void colsum_kernel(const float* __restrict__ in, int Nr, int ncol, const double* __restrict__ rdr,
+                              double* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  double s = 0;
+  for (int i = 0; i < Nr; i++) s += (double)in[(size_t)i * ncol + j] * rdr[i];
+  out[j] = 6.283185307179586 * s;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+template <int N> constexpr int threads_for() { return N >= 8192 ? 512 : (N >= 4096 ? 256 : (N >= 2048 ? 128 : 64)); }
+
+template <typename R, typename A>
+static size_t row_smem(int N, int nth) { return (size_t)N * (sizeof(cx<R>) + 2 * sizeof(R)) + 2 * (nth / 64) * sizeof(A) + 64; }
+
+template <typename R, typename A, int N>
+static hipError_t launch_row_n(const RowArgs<R>& a, int Nr, hipStream_t st) {
+  constexpr int NTH = threads_for<N>();
+  size_t sm = row_smem<R, A>(N, NTH);
+  auto k = row_kernel<R, A, N, NTH>;
+  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, a);
+  return hipGetLastError();
+}
+
+#define FIL_DISPATCH_N(N_, ...)                                  \
+  switch (N_) {                                                  \
+    case 256: { constexpr int NN = 256; __VA_ARGS__; } break;    \
+    case 512: { constexpr int NN = 512; __VA_ARGS__; } break;    \
+    case 1024: { constexpr int NN = 1024; __VA_ARGS__; } break;  \
+    case 2048: { constexpr int NN = 2048; __VA_ARGS__; } break;  \
+    case 4096: { constexpr int NN = 4096; __VA_ARGS__; } break;  \
+    case 8192: { constexpr int NN = 8192; __VA_ARGS__; } break;  \
+    default: return hipErrorInvalidValue;                        \
+  }
+
+template <typename R>
+hipError_t launch_row(const RowArgs<R>& a, int Nr, int N, bool scan64, hipStream_t st) {
+  if (scan64 || sizeof(R) == 8) {
+    FIL_DISPATCH_N(N, return (launch_row_n<R, double, NN>(a, Nr, st)));
+  } else {
+    FIL_DISPATCH_N(N, return (launch_row_n<R, float, NN>(a, Nr, st)));
+  }
+  return hipSuccess;
+}
+
+template <typename R>
+hipError_t launch_to_spectrum(const cx<R>* Et, cx<R>* S, const cx<R>* tw, int Nr, int N, hipStream_t st) {
+  FIL_DISPATCH_N(N, {
+    constexpr int NTH = threads_for<NN>();
+    size_t sm = (size_t)NN * sizeof(cx<R>);
+    auto k = to_spectrum_kernel<R, NN, NTH>;
+    { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
+    hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, Et, S, tw);
+    return hipGetLastError();
+  });
+  return hipSuccess;
+}
+
+template <typename R>
+hipError_t launch_to_time(const cx<R>* S, cx<R>* Et, const cx<R>* tw, const R* Rg, const R* Tg, int apply,
+                          int Nr, int N, hipStream_t st) {
+  FIL_DISPATCH_N(N, {
+    constexpr int NTH = threads_for<NN>();
+    size_t sm = (size_t)NN * sizeof(cx<R>);
+    auto k = to_time_kernel<R, NN, NTH>;
+    { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
+    hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, S, Et, tw, Rg, Tg, apply);
+    return hipGetLastError();
+  });
+  return hipSuccess;
+}
+
+template <typename R>
+hipError_t launch_analyze_rows(cx<R>* S, const cx<R>* tw, const R* Rg, const R* Tg, int apply, R dt, double* Fr,
+                               float* It, float* Sw, double* Imax_row, int Nr, int N, hipStream_t st) {
+  FIL_DISPATCH_N(N, {
+    constexpr int NTH = threads_for<NN>();
+    size_t sm = (size_t)NN * sizeof(cx<R>) + 16 * sizeof(double);
+    auto k = analyze_rows_kernel<R, NN, NTH>;
+    { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
+    hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, S, tw, Rg, Tg, apply, dt, Fr, It, Sw, Imax_row);
+    return hipGetLastError();
+  });
+  return hipSuccess;
+}
+
+hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* rdr, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(colsum_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, in, Nr, ncol, rdr, out);
+  return hipGetLastError();
+}
+
+template hipError_t launch_row<float>(const RowArgs<float>&, int, int, bool, hipStream_t);
+template hipError_t launch_row<double>(const RowArgs<double>&, int, int, bool, hipStream_t);
+template hipError_t launch_to_spectrum<float>(const cx<float>*, cx<float>*, const cx<float>*, int, int, hipStream_t);
+template hipError_t launch_to_spectrum<double>(const cx<double>*, cx<double>*, const cx<double>*, int, int, hipStream_t);
+template hipError_t launch_to_time<float>(const cx<float>*, cx<float>*, const cx<float>*, const float*, const float*, int, int, int, hipStream_t);
+template hipError_t launch_to_time<double>(const cx<double>*, cx<double>*, const cx<double>*, const double*, const double*, int, int, int, hipStream_t);
+template hipError_t launch_analyze_rows<float>(cx<float>*, const cx<float>*, const float*, const float*, int, float, double*, float*, float*, double*, int, int, hipStream_t);
+template hipError_t launch_analyze_rows<double>(cx<double>*, const cx<double>*, const double*, const double*, int, double, double*, float*, float*, double*, int, int, hipStream_t);
+
+}  // namespace fil

@@ -1,0 +1,98 @@
+"""ctypes binding of libfilament_hip.so (include/filament_hip.h).
+
+This is the reference-side FFI a host would write: the Julia host would hold
+the same structs behind ``ccall``.  Loading fails loudly when the library has
+not been built -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libfilament_hip.so")
+
+ABI_VERSION = 1
+MAXCOMP = 4
+GEOM_RT = 0
+FP32, FP64 = 4, 8
+FORM_DHT_BF16X3, FORM_SCAN_FP64 = 1, 2
+RESP_CUBIC_THG, RESP_CUBIC_NOTHG, RESP_FREE, RESP_LOSSES = 1, 2, 4, 8
+RESP_RAMAN_THG, RESP_RAMAN_NOTHG = 16, 32
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class FilDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "abi_version", "geometry", "Nr", "Nt", "precision", "form", "nonlinearity", "plasma",
+        "responses", "losses_ereal", "plasma_ereal", "plasma_kdep", "ncomp", "keep_plasma", "device")]
+
+
+class FilTables(ctypes.Structure):
+    _fields_ = [
+        ("r", _dp), ("dr", _dp), ("t", _dp), ("dt", ctypes.c_double),
+        ("KZ", _dp), ("Kguard", _dp), ("Wguard", _dp), ("Rguard", _dp), ("Tguard", _dp),
+        ("dht_T", _dp), ("dht_RJ", _dp), ("dht_JV", _dp),
+        ("QZ", _dp), ("Rnl_cubic", _dp), ("Rnl_free", _dp), ("Rnl_losses", _dp),
+        ("Rnl_raman", _dp), ("Hraman", _dp),
+        ("rho0", ctypes.c_double),
+        ("frhont", ctypes.c_double * MAXCOMP), ("Rava", ctypes.c_double * MAXCOMP),
+        ("K", ctypes.c_double * MAXCOMP), ("ntab", ctypes.c_int32 * MAXCOMP),
+        ("tab_log10x", _dp * MAXCOMP), ("tab_log10y", _dp * MAXCOMP),
+    ]
+
+
+class FilDiag(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("z", "Fmax", "Imax", "rhomax", "De", "rfil", "rpl", "tau", "W")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_upload_field", "fil_download_field",
+           "fil_zstep", "fil_analyze", "fil_download_plasma", "fil_synchronize", "fil_destroy",
+           "fil_timing_enable", "fil_timing_read")
+
+_lib = None
+
+
+class FilamentError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libfilament_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise FilamentError(f"{LIBPATH} is missing: build it first (make -C filament.jl_amd/csrc)")
+    L = ctypes.CDLL(LIBPATH)
+    L.fil_last_error.restype = ctypes.c_char_p
+    L.fil_create.argtypes = [ctypes.POINTER(FilDesc), ctypes.POINTER(ctypes.c_void_p)]
+    L.fil_set_tables.argtypes = [ctypes.c_void_p, ctypes.POINTER(FilTables)]
+    L.fil_upload_field.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
+    L.fil_download_field.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
+    L.fil_zstep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+    L.fil_analyze.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(FilDiag), _dp, _dp, _dp, _dp]
+    L.fil_download_plasma.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int64]
+    L.fil_synchronize.argtypes = [ctypes.c_void_p]
+    L.fil_destroy.argtypes = [ctypes.c_void_p]
+    L.fil_timing_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.fil_timing_read.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]
+    for name in EXPORTS:
+        if name != "fil_last_error":
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise FilamentError(f"libfilament_hip error {rc}: {load().fil_last_error().decode()}")
+
+
+def ptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None

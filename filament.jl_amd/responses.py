@@ -1,0 +1,35 @@
+"""The built-in nonlinear responses and the plasma equation, named as in the
+reference's medium files (``"init" => init_cubic`` etc.,
+test/diffraction/RT/air.jl:36-108).  User Julia closures cannot cross the
+C ABI, so a medium file selects one of these kinds; the library evaluates the
+matching ``calculate`` on the GPU."""
+
+
+class _Kind:
+    __slots__ = ("name",)
+
+    def __init__(self, name):
+        self.name = name
+
+    def __repr__(self):
+        return f"<response {self.name}>"
+
+
+init_cubic = _Kind("cubic")                     # medium_responses/cubic.jl
+init_current_free = _Kind("current_free")       # medium_responses/current_free.jl
+init_current_losses = _Kind("current_losses")   # medium_responses/current_losses.jl
+init_raman = _Kind("raman")                     # medium_responses/raman.jl
+init_photoionization_avalanche = _Kind("photoionization_avalanche")   # plasma_equations/
+
+
+def response_kind(init):
+    if isinstance(init, _Kind):
+        return init.name
+    if isinstance(init, str):
+        return init
+    raise TypeError(f"unsupported response init {init!r}")
+
+
+class RK3:
+    """Placeholder for ODEIntegrators.RK3() in input files (the only
+    tableau the library integrates with)."""

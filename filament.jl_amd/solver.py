@@ -1,0 +1,205 @@
+"""RT model on the host + hot path on libfilament_hip.so.
+
+``RTModel(p)`` plays the role of Filament.run's setup for geometry "RT"
+(src/Filament.jl:69-76): Unit, Grid, Field, Medium, Guard, Model and AStep
+are built on the host in FP64 and the tables are handed to the library once.
+``zstep`` is Models.zstep (src/modules/Models.jl:75-121) on the GPU and
+``analyze`` is FieldAnalyzers.analyze! RT (FieldAnalyzers.jl:166-215).
+
+``p`` carries the reference's input-file names (src/Input.jl:293-317 and the
+medium / initial-condition files), e.g. ``ru, zu, tu, Iu, rhou, rmax, Nr,
+tmin, tmax, Nt, lam0, permittivity, permeability, n2, NONLINEARITY, PLASMA,
+KPARAXIAL, QPARAXIAL, rguard, tguard, kguard, wguard, responses,
+plasma_equation, initial_condition``.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from . import lib as L
+from . import physics as ph
+from .responses import response_kind
+
+
+class RTModel:
+    def __init__(self, p, precision=L.FP32, form=0, device=0, keep_plasma=False, basedir="."):
+        self.p = p
+        self.unit = ph.UnitRT(p["ru"], p["zu"], p["tu"], p["Iu"], p["rhou"])
+        self.grid = ph.GridRT(p["rmax"], p["Nr"], p["tmin"], p["tmax"], p["Nt"])
+        self.medium = ph.Medium(p["permittivity"], p.get("permeability", ph.unit_permeability), p["n2"])
+        u, g, m = self.unit, self.grid, self.medium
+        self.w0 = 2 * math.pi * ph.C0 / p["lam0"]
+        self.n0 = float(np.real(m.refractive_index(self.w0)))
+        self.Eu = ph.field_unit(u, self.n0)
+        self.guard = ph.GuardRT(u, g, m, p["rguard"], p["tguard"], p["kguard"], p["wguard"])
+        self.KZ = ph.kz_table(u, g, m, self.w0, bool(p["KPARAXIAL"]))
+        self.precision, self.form = precision, form
+        self.nonlinearity = bool(p["NONLINEARITY"])
+        self.plasma = bool(p["PLASMA"])
+        if self.nonlinearity and not p.get("QPARAXIAL", True):
+            raise NotImplementedError("QPARAXIAL=false is not implemented")
+
+        # responses (nonlinear_propagators.jl:203-210)
+        resp_flags = 0
+        self.Rnl = {}
+        losses_ereal = 0
+        nuc_dz, mr_dz = p.get("nuc", 0.0), p.get("mr", 1.0)
+        if self.nonlinearity:
+            for item in p["responses"]:
+                kind = response_kind(item["init"])
+                if kind == "cubic":
+                    resp_flags |= L.RESP_CUBIC_THG if item["THG"] else L.RESP_CUBIC_NOTHG
+                    self.Rnl["cubic"] = ph.rnl_cubic(u, m, self.w0, self.Eu, g.Nt)
+                elif kind == "current_free":
+                    resp_flags |= L.RESP_FREE
+                    self.Rnl["free"] = ph.rnl_current_free(u, g, self.Eu, item["nuc"], item["mr"])
+                elif kind == "current_losses":
+                    resp_flags |= L.RESP_LOSSES
+                    losses_ereal = int(bool(item["EREAL"]))
+                    self.Rnl["losses"] = ph.rnl_current_losses(u, g, self.Eu, self.w0)
+                elif kind == "raman":
+                    raise NotImplementedError("the Raman response is not implemented on the GPU path")
+                else:
+                    raise ValueError(f"unknown response {kind}")
+            self.QZ = ph.qz_vector(u, g, m, self.w0, self.Eu)
+        else:
+            self.QZ = np.zeros(g.Nt, dtype=complex)
+        self.responses = resp_flags
+        # adaptive step constants (AdaptiveSteps.jl:37-43)
+        self.phik = ph.phi_kerr(u, m, self.w0, self.Eu) if p["n2"] != 0 else 0.0
+        self.phip = ph.phi_plasma(u, m, self.w0, self.Eu, mr_dz, nuc_dz)
+
+        # plasma (photoionization_avalanche.jl:1-116)
+        self.pt = None
+        if self.plasma:
+            self.pt = ph.plasma_tables(u, self.w0, self.Eu, p["plasma_equation"], basedir)
+
+        # initial field (Fields.jl:336-360): user IC on the host in FP64, analytic signal
+        E0 = np.asarray(p["initial_condition"](g.r, g.t, u.r, u.t, u.I), dtype=float)
+        self.E0 = ph.rsig2asig(E0)
+
+        self.T, self.RJ, self.JV = ph.dht_plan(g)
+
+        d = L.FilDesc()
+        d.abi_version = L.ABI_VERSION
+        d.geometry = L.GEOM_RT
+        d.Nr, d.Nt = g.Nr, g.Nt
+        d.precision = precision
+        d.form = form
+        d.nonlinearity = int(self.nonlinearity)
+        d.plasma = int(self.plasma)
+        d.responses = resp_flags
+        d.losses_ereal = losses_ereal
+        if self.plasma:
+            d.plasma_ereal = int(self.pt["EREAL"])
+            d.plasma_kdep = int(self.pt["KDEP"])
+            d.ncomp = len(self.pt["components"])
+        d.keep_plasma = int(keep_plasma)
+        d.device = device
+        self._lib = L.load()
+        h = ctypes.c_void_p()
+        L.check(self._lib.fil_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self._set_tables()
+        self.upload(self.E0)
+
+    # -- boundary ------------------------------------------------------------
+    def _set_tables(self):
+        g = self.grid
+        keep = []
+
+        def arr(a, cplx=False):
+            a = np.ascontiguousarray(a, dtype=np.complex128 if cplx else np.float64)
+            keep.append(a)
+            return L.ptr(a.view(np.float64) if cplx else a)
+
+        t = L.FilTables()
+        t.r, t.dr, t.t, t.dt = arr(g.r), arr(g.dr), arr(g.t), g.dt
+        t.KZ = arr(self.KZ.T, True)              # column-major [Nr x Nt]
+        t.Kguard = arr(self.guard.Kguard.T)
+        t.Wguard = arr(self.guard.Wguard)
+        t.Rguard = arr(self.guard.Rguard)
+        t.Tguard = arr(self.guard.Tguard)
+        t.dht_T, t.dht_RJ, t.dht_JV = arr(self.T.T), arr(self.RJ), arr(self.JV)
+        t.QZ = arr(self.QZ, True)
+        zero = np.zeros(g.Nt, dtype=complex)
+        t.Rnl_cubic = arr(self.Rnl.get("cubic", zero), True)
+        t.Rnl_free = arr(self.Rnl.get("free", zero), True)
+        t.Rnl_losses = arr(self.Rnl.get("losses", zero), True)
+        if self.pt is not None:
+            t.rho0 = self.pt["rho0"]
+            for q, c in enumerate(self.pt["components"]):
+                t.frhont[q], t.Rava[q], t.K[q] = c["frhont"], c["Rava"], c["K"]
+                t.ntab[q] = len(c["log10x"])
+                t.tab_log10x[q] = arr(c["log10x"])
+                t.tab_log10y[q] = arr(c["log10y"])
+        L.check(self._lib.fil_set_tables(self.h, ctypes.byref(t)))
+
+    def upload(self, E):
+        E = np.asfortranarray(np.asarray(E, dtype=np.complex128))
+        L.check(self._lib.fil_upload_field(self.h, L.ptr(E.ravel(order="F").view(np.float64)), self.grid.Nr))
+
+    def field(self):
+        """field.E after the last step (time domain, guarded), [Nr, Nt]."""
+        g = self.grid
+        out = np.zeros(g.Nr * g.Nt, dtype=np.complex128)
+        L.check(self._lib.fil_download_field(self.h, L.ptr(out.view(np.float64)), g.Nr))
+        return out.reshape(g.Nt, g.Nr).T.copy()
+
+    def plasma_arrays(self):
+        g = self.grid
+        rho = np.zeros(g.Nr * g.Nt)
+        kd = np.zeros(g.Nr * g.Nt)
+        L.check(self._lib.fil_download_plasma(self.h, L.ptr(rho), L.ptr(kd), g.Nr))
+        return rho.reshape(g.Nt, g.Nr).T.copy(), kd.reshape(g.Nt, g.Nr).T.copy()
+
+    def zstep(self, z, dz):
+        L.check(self._lib.fil_zstep(self.h, float(z), float(dz)))
+
+    def synchronize(self):
+        L.check(self._lib.fil_synchronize(self.h))
+
+    def analyze(self, z=0.0, arrays=False):
+        g = self.grid
+        d = L.FilDiag()
+        if arrays:
+            Fr, Ft, re, S = np.zeros(g.Nr), np.zeros(g.Nt), np.zeros(g.Nr), np.zeros(g.Nt // 2 + 1)
+            L.check(self._lib.fil_analyze(self.h, float(z), ctypes.byref(d), L.ptr(Fr), L.ptr(Ft),
+                                          L.ptr(re), L.ptr(S)))
+            out = d.as_dict()
+            out.update(Fr=Fr, Ft=Ft, rho_end=re, S=S)
+            return out
+        L.check(self._lib.fil_analyze(self.h, float(z), ctypes.byref(d), None, None, None, None))
+        return d.as_dict()
+
+    def dz_adaptive(self, dz0, phimax, Imax, rhomax):
+        """AStepKerrPlasma (AdaptiveSteps.jl:96-107)."""
+        if not self.nonlinearity:
+            return dz0
+        with np.errstate(divide="ignore"):
+            dzk = phimax / (self.phik * Imax) if self.phik * Imax != 0 else math.inf
+            dzp = phimax / (self.phip * rhomax) if self.phip * rhomax != 0 else math.inf
+        return min(dz0, dzk, dzp)
+
+    def timing(self, enable=True):
+        L.check(self._lib.fil_timing_enable(self.h, int(enable)))
+
+    def timing_read(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_int64()
+        L.check(self._lib.fil_timing_read(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                          ctypes.byref(n)))
+        return dict(row_ms=a.value, dht1_ms=b.value, dht2_ms=c.value, count=n.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.fil_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

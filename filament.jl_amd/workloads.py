@@ -1,0 +1,87 @@
+"""The RT workloads the throughput figures are quoted on, built from the
+reference's input conventions (examples/RT/input.jl, test/diffraction/RT/air.jl,
+examples/RT/gauss.jl) with synthetic, seeded initial fields.
+
+RT 256x2048  : air, 1 atm, 800 nm; Nr=256 on [0,10 mm] (Hankel nodes),
+               Nt=2048 on [-0.5,0.5] ps; Gaussian a0=1 mm, tau0=35 fs, 3 Pcr;
+               1e-3 uniform multiplicative amplitude noise; Kerr n2=1e-23 (no THG),
+               free current, MPI losses; O2 multiphoton ionisation K=8,
+               sigma_8 = 2.9e-99 s^-1 (cm^2/W)^8, neutrals 5.4e24 m^-3; dz=0.5 mm.
+RT 1024x4096 : same physics at Nr=1024, Nt=4096, 5 Pcr.
+"""
+import math
+
+import numpy as np
+
+from . import physics as ph
+from .responses import RK3, init_cubic, init_current_free, init_current_losses, init_photoionization_avalanche
+
+
+def o2_rate_table(sigma8_cm=2.9e-99, K=8, xmin=1e9, xmax=1e20, n=12):
+    """Power-law multiphoton rate W = sigma_K I^K as a two-column table
+    (x in W/m^2, y in 1/s), log-uniform in x like the reference's .tf files."""
+    x = np.logspace(math.log10(xmin), math.log10(xmax), n)
+    y = sigma8_cm * (x * 1e-4) ** K
+    return x, y
+
+
+def gaussian_ic(a0, tau0, I0, lam0, noise=0.0, seed=0):
+    """initial_condition(r, t, ru, tu, Iu) of examples/RT/gauss.jl form, with
+    multiplicative amplitude noise (1 + noise (2u - 1)), u ~ U[0,1) seeded."""
+    def initial_condition(r, t, ru, tu, Iu):
+        w0 = 2 * math.pi * ph.C0 / lam0
+        rr = np.asarray(r)[:, None] * ru
+        tt = np.asarray(t)[None, :] * tu
+        E = math.sqrt(I0 / Iu) * np.exp(-0.5 * rr ** 2 / a0 ** 2) * np.exp(-0.5 * tt ** 2 / tau0 ** 2) * np.cos(w0 * tt)
+        if noise:
+            u = np.random.default_rng(seed).random(E.shape)
+            E = E * (1 + noise * (2 * u - 1))
+        return E
+    return initial_condition
+
+
+def rt_air(Nr, Nt, power_pcr, rmax=10.0, tmin=-500.0, tmax=500.0, a0=1e-3, tau0=35e-15, noise=1e-3,
+           seed=0, n2=1e-23, nonlinearity=True, plasma=True, dz=5e-4):
+    """Parameter dict in the reference's input-file names (src/Input.jl:293-317)."""
+    lam0 = 800e-9
+    medium = ph.Medium(ph.peck_reeder_air, ph.unit_permeability, n2)
+    w0 = 2 * math.pi * ph.C0 / lam0
+    Pcr = medium.critical_power(w0)
+    I0 = power_pcr * Pcr / (math.pi * a0 ** 2)
+    nuc, mr = 5e12, 1.0
+    p = dict(
+        prefix="results/", geometry="RT",
+        ru=1e-3, zu=1.0, tu=1e-15, Iu=1e16, rhou=2.5e25,
+        rmax=rmax, Nr=Nr, tmin=tmin, tmax=tmax, Nt=Nt, lam0=lam0,
+        permittivity=ph.peck_reeder_air, permeability=ph.unit_permeability, n2=n2, nuc=nuc, mr=mr,
+        NONLINEARITY=nonlinearity, PLASMA=plasma, KPARAXIAL=False, QPARAXIAL=True,
+        rguard=1.0, tguard=20.0, kguard=45.0, wguard=1e16,
+        dz_initial=dz, dzphimax=math.inf, Istop=1e3, ALG=RK3(), z=0.0,
+        responses=[
+            {"init": init_cubic, "THG": False, "n2": n2},
+            {"init": init_current_free, "nuc": nuc, "mr": mr},
+            {"init": init_current_losses, "EREAL": False},
+        ],
+        plasma_equation={
+            "init": init_photoionization_avalanche, "ALG": RK3(), "EREAL": False, "KDEP": False,
+            "rho0": 0.0, "rho_nt": 5.4e24, "nuc": 0.0, "mr": mr,
+            "components": [{"fraction": 1.0, "ionization_energy": 12.063, "ionization_rate": o2_rate_table()}],
+        },
+        initial_condition=gaussian_ic(a0, tau0, I0, lam0, noise, seed),
+        I0=I0, Pcr=Pcr, power_pcr=power_pcr,
+    )
+    return p
+
+
+def rt_256x2048():
+    """The CPU-runnable plain-method case: 3 Pcr, 1000 steps of 0.5 mm."""
+    p = rt_air(256, 2048, 3.0)
+    p["nsteps"] = 1000
+    return p
+
+
+def rt_1024x4096():
+    """The precision-form case the z-steps/s metric is quoted on: 5 Pcr, 2000 steps."""
+    p = rt_air(1024, 4096, 5.0)
+    p["nsteps"] = 2000
+    return p

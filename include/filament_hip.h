@@ -1,0 +1,170 @@
+/*
+ * filament_hip.h -- C ABI of libfilament_hip.so, the MI355X (gfx950)
+ * implementation of Filament.jl's per-z-step propagation path for the
+ * axisymmetric RT geometry.
+ *
+ * The library replaces the device side of one reference call,
+ *   Models.zstep(z, dz, grid, field, guard, model)   (src/modules/Models.jl:75-121)
+ * together with the pieces it reaches: the plasma solve (plasma_equations.jl:27-72),
+ * the t-transforms (Models.jl:152-171), the nonlinear RK step
+ * (nonlinear_propagators.jl:38-74,246-283), the spectral linear step with the
+ * Hankel transform (linear_propagators/spectral.jl:16-26, Models.jl:135-148),
+ * the guard filters (Guards.jl:199-318) and the RT field analyzer
+ * (FieldAnalyzers.jl:166-215).  The host keeps Input.prepare, Units, Grids,
+ * Media and the table construction; it hands the tables over once.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a nonzero FIL_E* code on
+ *    failure; fil_last_error() then gives a thread-local message.
+ *  - 2-D host arrays use the reference's (Julia) column-major layout: element
+ *    (i, j) of an Nr x Nt array is at [i + ld*j], r (or k) index fastest.
+ *  - Complex host arrays are interleaved (re, im) doubles.
+ *  - The library owns all device memory; callers never see device pointers.
+ *  - A handle is not thread-safe; calls on one handle are issued in order on
+ *    the handle's own HIP stream.  fil_zstep is asynchronous; fil_analyze,
+ *    fil_download_* and fil_synchronize block.
+ */
+#ifndef FILAMENT_HIP_H
+#define FILAMENT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIL_ABI_VERSION 1
+#define FIL_MAXCOMP 4
+
+typedef struct fil_handle fil_handle;
+
+enum {
+  FIL_OK = 0,
+  FIL_EINVAL = 1,    /* bad argument or unsupported configuration */
+  FIL_EDEVICE = 2,   /* HIP runtime failure */
+  FIL_ESTATE = 3,    /* call out of order (e.g. zstep before tables) */
+  FIL_ENOMEM = 4
+};
+
+enum { FIL_GEOM_RT = 0 };
+
+enum { FIL_FP32 = 4, FIL_FP64 = 8 };
+
+/* Forms (bit flags; 0 = the plain port) */
+enum {
+  FIL_FORM_DHT_BF16X3 = 1,  /* Hankel transform as split-BF16 3-term MFMA products, FP32 accumulate */
+  FIL_FORM_SCAN_FP64 = 2    /* plasma-density t-integration carried in FP64 */
+};
+
+/* Nonlinear responses (src/modules/medium_responses), bit flags */
+enum {
+  FIL_RESP_CUBIC_THG = 1,     /* calc_cubic          cubic.jl:33-38  */
+  FIL_RESP_CUBIC_NOTHG = 2,   /* calc_cubic_nothg    cubic.jl:41-46  */
+  FIL_RESP_FREE = 4,          /* calc_current_free   current_free.jl:31-37 */
+  FIL_RESP_LOSSES = 8,        /* calc_current_losses current_losses.jl:36-83 */
+  FIL_RESP_RAMAN_THG = 16,    /* calc_raman          raman.jl:46-54  */
+  FIL_RESP_RAMAN_NOTHG = 32   /* calc_raman_nothg    raman.jl:57-65  */
+};
+
+/* Problem description (replaces the type parameters of Field/Model built by
+ * Filament.run, src/Filament.jl:69-75). */
+typedef struct fil_desc {
+  int32_t abi_version;     /* FIL_ABI_VERSION */
+  int32_t geometry;        /* FIL_GEOM_RT */
+  int32_t Nr, Nt;          /* Nt: power of two, 256..8192 (FP64: ..4096) */
+  int32_t precision;       /* FIL_FP32 (the reference's FloatGPU) or FIL_FP64 */
+  int32_t form;            /* FIL_FORM_* flags */
+  int32_t nonlinearity;    /* NONLINEARITY */
+  int32_t plasma;          /* PLASMA */
+  int32_t responses;       /* FIL_RESP_* flags (ignored unless nonlinearity) */
+  int32_t losses_ereal;    /* current_losses "EREAL" */
+  int32_t plasma_ereal;    /* plasma_equation "EREAL" */
+  int32_t plasma_kdep;     /* plasma_equation "KDEP" */
+  int32_t ncomp;           /* plasma components, 1..FIL_MAXCOMP */
+  int32_t keep_plasma;     /* store full rho/kdrho each step (for fil_download_plasma) */
+  int32_t device;          /* HIP device ordinal */
+} fil_desc;
+
+/* Host-precomputed tables, FP64 (the library rounds them to the working
+ * precision).  The library copies everything; the caller keeps ownership. */
+typedef struct fil_tables {
+  /* Grids (Grids.jl:97-121,204-223) */
+  const double* r;         /* [Nr] r nodes (grid units) */
+  const double* dr;        /* [Nr] */
+  const double* t;         /* [Nt] t nodes (grid units) */
+  double dt;               /* t step (grid units) */
+  /* Linear propagator (spectral.jl:78-101) and spectral guard (Guards.jl:302-313) */
+  const double* KZ;        /* [Nr x Nt] complex, column-major */
+  const double* Kguard;    /* [Nr x Nt] */
+  const double* Wguard;    /* [Nt] */
+  /* Field guard (Guards.jl:299-300) */
+  const double* Rguard;    /* [Nr] */
+  const double* Tguard;    /* [Nt] */
+  /* Hankel plan (HankelTransforms.plan_dht, Fields.jl:348-351):
+   * forward f -> JV .* (T * (RJ .* f)), inverse F -> (T * (F ./ JV)) ./ RJ */
+  const double* dht_T;     /* [Nr x Nr] */
+  const double* dht_RJ;    /* [Nr] */
+  const double* dht_JV;    /* [Nr] */
+  /* Nonlinear propagator (nonlinear_propagators.jl:323-343) and responses */
+  const double* QZ;        /* [Nt] complex: QPARAXIAL prefactor (independent of r) */
+  const double* Rnl_cubic; /* [Nt] complex (cubic.jl:11-12, broadcast) */
+  const double* Rnl_free;  /* [Nt] complex (current_free.jl:67-74) */
+  const double* Rnl_losses;/* [Nt] complex (current_losses.jl:102-108) */
+  const double* Rnl_raman; /* [Nt] complex (raman.jl:186-187, broadcast) */
+  const double* Hraman;    /* [Nt] complex (raman.jl:189-202) */
+  /* Plasma equation (photoionization_avalanche.jl:126-183), per component:
+   * dimensionless frhont, Rava, K; rate table as log10 pairs of the scaled
+   * table (x/Iu, y*tu), which must be uniformly spaced in log10(x). */
+  double rho0;             /* initial density per component [rhou] */
+  double frhont[FIL_MAXCOMP];
+  double Rava[FIL_MAXCOMP];
+  double K[FIL_MAXCOMP];
+  int32_t ntab[FIL_MAXCOMP];
+  const double* tab_log10x[FIL_MAXCOMP];
+  const double* tab_log10y[FIL_MAXCOMP];
+} fil_tables;
+
+/* Scalar diagnostics of FieldAnalyzerRT (FieldAnalyzers.jl:125-138) */
+typedef struct fil_diag {
+  double z, Fmax, Imax, rhomax, De, rfil, rpl, tau, W;
+} fil_diag;
+
+const char* fil_last_error(void);
+
+/* Replaces Fields.Field / Models.Model construction (Filament.jl:71-74). */
+int fil_create(const fil_desc* d, fil_handle** h);
+int fil_set_tables(fil_handle* h, const fil_tables* t);
+
+/* Field in the time domain, column-major E[i + ld*j] (Field.E, Fields.jl:342-355
+ * after rsig2asig!).  Upload replaces the state; download returns what the
+ * reference holds in field.E after the last zstep (guarded, time domain). */
+int fil_upload_field(fil_handle* h, const double* E, int64_t ld);
+int fil_download_field(fil_handle* h, double* E, int64_t ld);
+
+/* Models.zstep (Models.jl:75-121).  Asynchronous. */
+int fil_zstep(fil_handle* h, double z, double dz);
+
+/* FieldAnalyzers.analyze! RT (FieldAnalyzers.jl:166-215), including its side
+ * effect of projecting the field onto its analytic part.  Fr [Nr], Ft [Nt],
+ * rho_end [Nr] and S [Nt/2+1] may be NULL.  Blocks. */
+int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft,
+                double* rho_end, double* S);
+
+/* field.rho and field.kdrho (Fields.jl:357-358) as the last plasma solve left
+ * them, column-major [i + ld*j].  Blocks. */
+int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld);
+
+int fil_synchronize(fil_handle* h);
+int fil_destroy(fil_handle* h);
+
+/* Per-phase device timing with HIP events on the handle's stream (the
+ * reference's TimerOutputs sections "plasma/nonlinearity" and "linear",
+ * Models.jl:84-119).  names: "row" (plasma + t-FFTs + RK), "dht1", "dht2".
+ * fil_timing_read returns accumulated milliseconds and launch counts. */
+int fil_timing_enable(fil_handle* h, int enable);
+int fil_timing_read(fil_handle* h, double* ms_row, double* ms_dht1, double* ms_dht2, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

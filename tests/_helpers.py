@@ -1,0 +1,62 @@
+"""Test helpers: the same problem as an oracle configuration."""
+import numpy as np
+
+
+def load_pkg():
+    """Import the filament.jl_amd package under the name filament_amd."""
+    import importlib.util
+    import os
+    import sys
+    if "filament_amd" in sys.modules:
+        return sys.modules["filament_amd"]
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "filament.jl_amd")
+    spec = importlib.util.spec_from_file_location("filament_amd", os.path.join(root, "__init__.py"),
+                                                  submodule_search_locations=[root])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["filament_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def load_oracle():
+    import os
+    import sys
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle")
+    if d not in sys.path:
+        sys.path.insert(0, d)
+    import oracle
+    return oracle
+
+
+def oracle_config(p):
+    """The same problem as an oracle configuration dict (tests only)."""
+    resp = {response_name(r["init"]): r for r in p["responses"]} if p["NONLINEARITY"] else {}
+    pe = p["plasma_equation"]
+    comps = pe["components"]
+    tabs = [tuple(np.asarray(v, dtype=float) for v in c["ionization_rate"]) for c in comps]
+    medium = 0 if p["permittivity"] is load_pkg().physics.peck_reeder_air else None
+    if medium is None:
+        raise ValueError("the oracle restates only Peck-Reeder air")
+    cub = resp.get("cubic")
+    return dict(
+        geometry=0, Nr=p["Nr"], Nt=p["Nt"], rmax=p["rmax"], tmin=p["tmin"], tmax=p["tmax"],
+        ru=p["ru"], zu=p["zu"], tu=p["tu"], Iu=p["Iu"], rhou=p["rhou"], lam0=p["lam0"],
+        medium=medium, n_const=1.0, n2=p["n2"],
+        nonlinearity=int(p["NONLINEARITY"]), plasma=int(p["PLASMA"]),
+        kparaxial=int(p["KPARAXIAL"]), qparaxial=int(p["QPARAXIAL"]),
+        rguard=p["rguard"], tguard=p["tguard"], kguard=p["kguard"], wguard=p["wguard"],
+        cubic=(0 if cub is None else (1 if cub["THG"] else 2)), raman=0,
+        current_free=int("current_free" in resp),
+        nuc=resp["current_free"]["nuc"] if "current_free" in resp else p["nuc"],
+        mr=resp["current_free"]["mr"] if "current_free" in resp else p["mr"],
+        current_losses=int("current_losses" in resp),
+        losses_ereal=int(resp["current_losses"]["EREAL"]) if "current_losses" in resp else 0,
+        pe_ereal=int(pe["EREAL"]), pe_kdep=int(pe["KDEP"]), pe_rho0=pe["rho0"], pe_rho_nt=pe["rho_nt"],
+        pe_nuc=pe["nuc"], pe_mr=pe["mr"],
+        comp_frac=[c["fraction"] for c in comps], comp_Ui=[c["ionization_energy"] for c in comps],
+        tables=tabs,
+    )
+
+
+def response_name(init):
+    return load_pkg().responses.response_kind(init)

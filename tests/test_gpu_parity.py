@@ -1,0 +1,100 @@
+"""Parity of the HIP path (through the C ABI) against the FP64 oracle on
+small RT problems.  Tolerances: relative L2 field error <= 1e-4 for the FP32
+plain port and <= 1e-9 for the FP64 build over the stated number of steps."""
+import math
+
+import numpy as np
+import pytest
+
+import _helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def small_rt(pkg, Nr=64, Nt=512, power=3.0, nonlinearity=True, plasma=True, tau0=20e-15, noise=1e-3):
+    p = pkg.workloads.rt_air(Nr, Nt, power, tmin=-125.0, tmax=125.0, tau0=tau0, noise=noise,
+                             nonlinearity=nonlinearity, plasma=plasma)
+    return p
+
+
+def oracle_for(orc, p, precision=8):
+    pk = H.load_pkg()
+    g = pk.physics.GridRT(p["rmax"], p["Nr"], p["tmin"], p["tmax"], p["Nt"])
+    E0 = p["initial_condition"](g.r, g.t, p["ru"], p["tu"], p["Iu"])
+    return orc.Oracle(H.oracle_config(p), E0, precision)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def run_gpu(pkg, p, nsteps, dz, precision, analyze_each=True, form=0):
+    m = pkg.RTModel(p, precision=precision, form=form, keep_plasma=True)
+    m.analyze(0.0)
+    z = 0.0
+    for _ in range(nsteps):
+        z += dz
+        m.zstep(z, dz)
+        if analyze_each:
+            m.analyze(z)
+    return m
+
+
+@pytest.mark.parametrize("precision,tol", [(4, 2e-6), (8, 1e-12)])
+def test_upload_download_roundtrip(pkg, precision, tol):
+    p = small_rt(pkg)
+    m = pkg.RTModel(p, precision=precision)
+    E = m.field()
+    assert rel_l2(E, m.E0) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [(4, 1e-5), (8, 1e-11)])
+def test_linear_steps_match_oracle(pkg, orc, precision, tol):
+    p = small_rt(pkg, nonlinearity=False, plasma=False)
+    o = oracle_for(orc, p)
+    dz = 0.05
+    o.run(20, dz)
+    m = run_gpu(pkg, p, 20, dz, precision)
+    err = rel_l2(m.field(), o.field())
+    print(f"linear precision={precision} rel_l2={err:.3e}")
+    assert err < tol
+
+
+@pytest.mark.parametrize("precision,tol", [(4, 2e-5), (8, 1e-11)])
+def test_plasma_matches_oracle(pkg, orc, precision, tol):
+    p = small_rt(pkg, power=6.0, nonlinearity=False, plasma=True)
+    o = oracle_for(orc, p)
+    rho_o, kd_o = o.plasma()
+    m = pkg.RTModel(p, precision=precision, keep_plasma=True)
+    rho, kd = m.plasma_arrays()
+    e1, e2 = rel_l2(rho, rho_o), rel_l2(kd, kd_o)
+    print(f"plasma precision={precision} rho {e1:.3e} kdrho {e2:.3e} max rho {rho_o.max():.3e}")
+    assert rho_o.max() > 0
+    assert e1 < tol and e2 < tol
+
+
+@pytest.mark.parametrize("precision,tol", [(4, 1e-4), (8, 1e-9)])
+def test_full_steps_match_oracle(pkg, orc, precision, tol):
+    p = small_rt(pkg, power=6.0)
+    o = oracle_for(orc, p)
+    dz = 5e-3
+    rows = o.run(30, dz)
+    m = run_gpu(pkg, p, 30, dz, precision)
+    err = rel_l2(m.field(), o.field())
+    d = m.analyze(30 * dz)
+    print(f"full precision={precision} rel_l2={err:.3e} Imax gpu {d['Imax']:.6e} oracle {rows[-1]['Imax']:.6e}")
+    assert err < tol
+    for k in ("Fmax", "Imax", "W"):
+        assert abs(d[k] - rows[-1][k]) <= 10 * tol * abs(rows[-1][k])
+
+
+def test_zstep_without_analyze_tracks_oracle(pkg, orc):
+    """The bench path (zstep only, no analyzer projection in between)."""
+    p = small_rt(pkg, power=6.0)
+    o = oracle_for(orc, p)
+    dz = 5e-3
+    o.run(30, dz)
+    m = run_gpu(pkg, p, 30, dz, 4, analyze_each=False)
+    err = rel_l2(m.field(), o.field())
+    print(f"zstep-only rel_l2={err:.3e}")
+    assert err < 1e-4
