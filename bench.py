@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Throughput of the RT z-step on one MI355X: z-steps/s at Nr=1024 x Nt=4096.
+
+One "step" is Models.zstep (src/modules/Models.jl:75-121) on the whole
+1024 x 4096 field: plasma scan, t-FFTs and RK3 nonlinear step (row kernel),
+then the two Hankel-transform GEMMs with the propagator multiply.  The field
+is resident in HBM when the timed region starts.
+
+Prints one JSON line.  `value` is the chosen form's z-steps/s (split-BF16
+Hankel transform + FP64 plasma scan) when it holds <= 1e-3 relative fluence
+error against the FP64 build over the timed trajectory, otherwise the plain
+FP32 port's.  Both forms, their errors, the per-kernel times, the roofline of
+the dominant kernel and a CPU baseline (the C++ restatement run on the host
+cores over a bounded sample) are reported in the same line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+
+
+def load_pkg():
+    import importlib.util
+    pdir = os.path.join(ROOT, "filament.jl_amd")
+    spec = importlib.util.spec_from_file_location("filament_amd", os.path.join(pdir, "__init__.py"),
+                                                  submodule_search_locations=[pdir])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["filament_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def run_form(pkg, p, precision, form, warmup, steps, dz, profile_steps):
+    m = pkg.RTModel(p, precision=precision, form=form)
+    z = 0.0
+    for _ in range(warmup):
+        z += dz
+        m.zstep(z, dz)
+    m.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        z += dz
+        m.zstep(z, dz)
+    m.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    # per-kernel device times with HIP events on the library's stream
+    m.timing(True)
+    for _ in range(profile_steps):
+        z += dz
+        m.zstep(z, dz)
+    kt = m.timing_read()
+    m.timing(False)
+    d = m.analyze(z, arrays=True)
+    return m, ms, kt, d, z
+
+
+def fluence_error(F, Fref):
+    return float(np.linalg.norm(F - Fref) / np.linalg.norm(Fref))
+
+
+def cpu_baseline(pkg, p, sample_steps, dz):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _helpers as H
+    orc = H.load_oracle()
+    g = pkg.physics.GridRT(p["rmax"], p["Nr"], p["tmin"], p["tmax"], p["Nt"])
+    E0 = p["initial_condition"](g.r, g.t, p["ru"], p["tu"], p["Iu"])
+    o = orc.Oracle(H.oracle_config(p), E0, 4)
+    o.zstep(dz, dz)            # warm-up step excluded (Filament.jl:166-170)
+    t0 = time.perf_counter()
+    for n in range(sample_steps):
+        o.zstep((n + 2) * dz, dz)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=sample_steps / dt, unit="z-steps/s", cores=threads, kind="port",
+                sample=f"FP32 C++ restatement, RT 1024x4096, {sample_steps} z-steps after 1 warm-up step, "
+                       f"{dt:.1f} s wall")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the RT z-step runs on a single MI355X; "
+              "multi-GPU sharding is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_pkg()
+    L = pkg.lib
+    p = pkg.workloads.rt_1024x4096()
+    dz = p["dz_initial"]
+    K, W = args.steps, args.warmup
+    prof = min(K, 50)
+    Nr, Nt = p["Nr"], p["Nt"]
+    Nh = Nt // 2
+
+    # FP64 build: the error reference for both forms
+    m64, ms64, kt64, d64, _ = run_form(pkg, p, L.FP64, 0, W, K, dz, 1)
+    m64.close()
+    # plain FP32 port
+    mp, msp, ktp, dp, _ = run_form(pkg, p, L.FP32, 0, W, K, dz, prof)
+    mp.close()
+    err_plain = fluence_error(dp["Fr"], d64["Fr"])
+    forms = {"plain_fp32": dict(z_steps_per_s=1e3 / msp, ms_per_step=msp, fluence_err_vs_fp64=err_plain,
+                                kernel_ms=dict(row=ktp["row_ms"] / ktp["count"], dht1=ktp["dht1_ms"] / ktp["count"],
+                                               dht2=ktp["dht2_ms"] / ktp["count"]))}
+    chosen = None
+    try:
+        mc, msc, ktc, dc, _ = run_form(pkg, p, L.FP32, L.FORM_DHT_BF16X3 | L.FORM_SCAN_FP64, W, K, dz, prof)
+        mc.close()
+        err_c = fluence_error(dc["Fr"], d64["Fr"])
+        forms["bf16x3_dht_fp64_scan"] = dict(
+            z_steps_per_s=1e3 / msc, ms_per_step=msc, fluence_err_vs_fp64=err_c,
+            kernel_ms=dict(row=ktc["row_ms"] / ktc["count"], dht1=ktc["dht1_ms"] / ktc["count"],
+                           dht2=ktc["dht2_ms"] / ktc["count"]))
+        if err_c <= 1e-3:
+            chosen = ("bf16x3_dht_fp64_scan", msc, ktc)
+    except L.FilamentError as e:
+        forms["bf16x3_dht_fp64_scan"] = dict(error=str(e))
+    if chosen is None:
+        chosen = ("plain_fp32", msp, ktp)
+    name, ms, kt = chosen
+
+    # roofline of the dominant kernel of the reported form
+    n = kt["count"]
+    row_ms, g1_ms, g2_ms = kt["row_ms"] / n, kt["dht1_ms"] / n, kt["dht2_ms"] / n
+    gemm_flop = 2.0 * Nr * (2 * Nh) * Nr          # real GEMM M=Nr, N=2*Nh, K=Nr
+    row_bytes = 2.0 * 8 * Nr * Nh                 # one read + one write of the c64 half-spectrum
+    if max(g1_ms, g2_ms) >= row_ms:
+        kms = 0.5 * (g1_ms + g2_ms)
+        peak = PEAK["bf16_mfma_tflops"] if name.startswith("bf16") else PEAK["fp32_mfma_tflops"]
+        ach = gemm_flop / (kms * 1e-3) / 1e12
+        roof = dict(kernel="dht_gemm", bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+                    traffic=None, per_launch=f"{gemm_flop:.4g} flop (2*Nr*Nt*Nr)", avg_launch_ms=kms)
+    else:
+        ach = row_bytes / (row_ms * 1e-3) / 1e9
+        roof = dict(kernel="row_kernel", bound="hbm", achieved=ach, peak=PEAK["hbm_gbs"], unit="GB/s",
+                    frac=ach / PEAK["hbm_gbs"], traffic=None, per_launch=f"{row_bytes:.4g} B (2*8*Nr*Nt/2)",
+                    avg_launch_ms=row_ms)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(pkg, p, args.cpu_steps, dz)
+
+    out = {
+        "metric": "z-steps/s RT 1024x4096: plain FP32 vs chosen form at <=1e-3 fluence err",
+        "value": 1e3 / ms,
+        "unit": "z-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if name == "plain_fp32" else "fp32+bf16x3-mfma+fp64-scan",
+        "data": "synthetic seeded Gaussian pulse (RT air, 5 Pcr), 1e-3 amplitude noise",
+        "config": {"workload": "RT Nr=1024 x Nt=4096, air 800 nm, Kerr+plasma, RK3, dz=0.5 mm", "form": name,
+                   "Nr": Nr, "Nt": Nt, "parallelism": "single GPU"},
+        "forms": forms,
+        "fp64_ms_per_step": ms64,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

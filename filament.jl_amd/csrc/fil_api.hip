@@ -362,8 +362,13 @@ int fil_set_tables(fil_handle* h, const fil_tables* t) {
       M2[(size_t)m * Nrp + n] = T / t->dht_RJ[m];
     }
   {
+    // stored transposed (MT[k][m]) for the FP32 / FP64 GEMM kernels
     HostArr a(M1.size(), f64, false), b(M2.size(), f64, false);
-    for (size_t q = 0; q < M1.size(); q++) { a.set(q, M1[q]); b.set(q, M2[q]); }
+    for (int m = 0; m < Nrp; m++)
+      for (int n = 0; n < Nrp; n++) {
+        a.set((size_t)n * Nrp + m, M1[(size_t)m * Nrp + n]);
+        b.set((size_t)n * Nrp + m, M2[(size_t)m * Nrp + n]);
+      }
     HIPCHK(upload(h->M1, a.b.data(), a.b.size()));
     HIPCHK(upload(h->M2, b.b.data(), b.b.size()));
   }

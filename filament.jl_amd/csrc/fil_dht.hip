@@ -22,77 +22,99 @@ namespace fil {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-template <typename R> struct Acc;
-template <> struct Acc<float> {
-  typedef floatx4 T;
-  static __device__ __forceinline__ T mfma(float a, float b, T c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-};
-
-// Tile: BM rows x BW complex columns per workgroup, BK = 16, 4 waves in 2x2,
-// each wave BM/2 x BW/2 as (BM/32) x (BW/32) 16x16 MFMA tiles for re and im.
-// MFMA 16x16x4 operand layout (lane l): a = A[l%16][l/16], b = B[l/16][l%16],
-// c[i] = C[4*(l/16)+i][l%16].
-template <typename R, int BM, int BW>
-__global__ __launch_bounds__(256) void dht_gemm_kernel(const R* __restrict__ M, const cx<R>* __restrict__ In,
-                                                       cx<R>* __restrict__ Out, const cx<R>* __restrict__ P,
-                                                       int Nr, int Nh) {
-  constexpr int BK = 16;
-  constexpr int MT = BM / 32, WT = BW / 32;   // MFMA tiles per wave in m and w
-  __shared__ R As[BK][BM + 4];
-  __shared__ R Bre[BK][BW + 4];
-  __shared__ R Bim[BK][BW + 4];
+// All FP32/FP64 kernels take the real matrix transposed, MT[k][m] = M[m][k],
+// so an A tile (BK rows of k, BM consecutive m) is a contiguous copy.
+//
+// MFMA 16x16x4 f32 operand layout (lane l): a = A[l%16][l/16],
+// b = B[l/16][l%16], c[i] = C[4*(l/16)+i][l%16].  Each wave owns a
+// WM x WW (complex) sub-tile as (WM/16) x (WW/16) MFMA tiles for re and im.
+// LDS row strides are padded to 16 (mod 64) floats so that the four k rows
+// read by one MFMA operand load fall in disjoint bank groups.
+template <int BM, int BW, int BK, int WM, int WW>
+__global__ __launch_bounds__(256) void dht_gemm_f32_kernel(const float* __restrict__ MT, const cx<float>* __restrict__ In,
+                                                           cx<float>* __restrict__ Out, const cx<float>* __restrict__ P,
+                                                           int Nr, int Nh) {
+  constexpr int MT_ = WM / 16, WT = WW / 16;
+  constexpr int WAVES_M = BM / WM;
+  static_assert((BM / WM) * (BW / WW) == 4, "4 waves per workgroup");
+  constexpr int SA = BM + 16, SB = BW + 16;
+  __shared__ __align__(16) float As[2][BK][SA];
+  __shared__ __align__(16) float Bre[2][BK][SB];
+  __shared__ __align__(16) float Bim[2][BK][SB];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * BM, w0 = blockIdx.x * BW;
-  const int wm = (wv >> 1) * (BM / 2), ww = (wv & 1) * (BW / 2);
-  typename Acc<R>::T acc[MT][WT][2];
+  const int wm = (wv % WAVES_M) * WM, ww = (wv / WAVES_M) * WW;
+  const int l16 = lane & 15, l4 = lane >> 4;
+  constexpr int NA = BK * BM / 4 / 256;   // float4 of A per thread
+  constexpr int NB = BK * BW / 2 / 256;   // float4 (two complex) of B per thread
+  static_assert(NA >= 1 && NB >= 1, "tile too small for 256 threads");
+  float4 ra[NA], rb[NB];
+  auto gload = [&](int k0) {
 #pragma unroll
-  for (int a = 0; a < MT; a++)
+    for (int u = 0; u < NA; u++) {
+      const int e = tid + 256 * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;
+      ra[u] = *reinterpret_cast<const float4*>(MT + (size_t)(k0 + kk) * Nr + m0 + m4);
+    }
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+      const int e = tid + 256 * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;
+      rb[u] = *reinterpret_cast<const float4*>(In + (size_t)(k0 + kk) * Nh + w0 + c2);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < NA; u++) {
+      const int e = tid + 256 * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;
+      *reinterpret_cast<float4*>(&As[buf][kk][m4]) = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < NB; u++) {
+      const int e = tid + 256 * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;
+      *reinterpret_cast<float2*>(&Bre[buf][kk][c2]) = make_float2(rb[u].x, rb[u].z);
+      *reinterpret_cast<float2*>(&Bim[buf][kk][c2]) = make_float2(rb[u].y, rb[u].w);
+    }
+  };
+  floatx4 acc[MT_][WT][2];
+#pragma unroll
+  for (int a = 0; a < MT_; a++)
 #pragma unroll
     for (int b = 0; b < WT; b++) { acc[a][b][0] = {0, 0, 0, 0}; acc[a][b][1] = {0, 0, 0, 0}; }
 
-  const int l16 = lane & 15, l4 = lane >> 4;
-  for (int k0 = 0; k0 < Nr; k0 += BK) {
-    // A tile: BM x BK, stored k-major
-    for (int e = tid; e < BM * BK; e += 256) {
-      const int row = e / BK, kk = e % BK;
-      As[kk][row] = M[(size_t)(m0 + row) * Nr + k0 + kk];
-    }
-    // B tile: BK x BW complex, split into re / im planes
-    for (int e = tid; e < BK * BW; e += 256) {
-      const int kk = e / BW, c = e % BW;
-      cx<R> v = In[(size_t)(k0 + kk) * Nh + w0 + c];
-      Bre[kk][c] = v.x;
-      Bim[kk][c] = v.y;
-    }
-    __syncthreads();
+  const int nk = Nr / BK;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int kb = 0; kb < nk; kb++) {
+    const int buf = kb & 1;
+    if (kb + 1 < nk) gload((kb + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int k = ks + l4;
-      R af[MT], br[WT], bi[WT];
+      float af[MT_], br[WT], bi[WT];
 #pragma unroll
-      for (int a = 0; a < MT; a++) af[a] = As[k][wm + a * 16 + l16];
+      for (int a = 0; a < MT_; a++) af[a] = As[buf][k][wm + a * 16 + l16];
 #pragma unroll
-      for (int b = 0; b < WT; b++) { br[b] = Bre[k][ww + b * 16 + l16]; bi[b] = Bim[k][ww + b * 16 + l16]; }
+      for (int b = 0; b < WT; b++) { br[b] = Bre[buf][k][ww + b * 16 + l16]; bi[b] = Bim[buf][k][ww + b * 16 + l16]; }
 #pragma unroll
-      for (int a = 0; a < MT; a++)
+      for (int a = 0; a < MT_; a++)
 #pragma unroll
         for (int b = 0; b < WT; b++) {
-          acc[a][b][0] = Acc<R>::mfma(af[a], br[b], acc[a][b][0]);
-          acc[a][b][1] = Acc<R>::mfma(af[a], bi[b], acc[a][b][1]);
+          acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], br[b], acc[a][b][0], 0, 0, 0);
+          acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bi[b], acc[a][b][1], 0, 0, 0);
         }
     }
+    if (kb + 1 < nk) lstore(buf ^ 1);
     __syncthreads();
   }
-  // epilogue: optional propagator multiply, interleaved store
 #pragma unroll
-  for (int a = 0; a < MT; a++)
+  for (int a = 0; a < MT_; a++)
 #pragma unroll
     for (int b = 0; b < WT; b++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int m = m0 + wm + a * 16 + 4 * l4 + i;
         const int w = w0 + ww + b * 16 + l16;
-        cx<R> v = mk((R)acc[a][b][0][i], (R)acc[a][b][1][i]);
+        cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
         const size_t o = (size_t)m * Nh + w;
         if (P) v = v * P[o];
         Out[o] = v;
@@ -102,7 +124,7 @@ __global__ __launch_bounds__(256) void dht_gemm_kernel(const R* __restrict__ M, 
 // FP64 variant on the vector ALUs (MI355X's FP64 vector and matrix peaks
 // are equal): 64 x 64 complex tile, each thread a 4 x 4 block.
 template <typename R>
-__global__ __launch_bounds__(256) void dht_gemm_valu_kernel(const R* __restrict__ M, const cx<R>* __restrict__ In,
+__global__ __launch_bounds__(256) void dht_gemm_valu_kernel(const R* __restrict__ MT, const cx<R>* __restrict__ In,
                                                             cx<R>* __restrict__ Out, const cx<R>* __restrict__ P,
                                                             int Nr, int Nh) {
   constexpr int BM = 64, BW = 64, BK = 16;
@@ -118,8 +140,8 @@ __global__ __launch_bounds__(256) void dht_gemm_valu_kernel(const R* __restrict_
     for (int b = 0; b < 4; b++) acc[a][b] = mk((R)0, (R)0);
   for (int k0 = 0; k0 < Nr; k0 += BK) {
     for (int e = tid; e < BM * BK; e += 256) {
-      const int row = e / BK, kk = e % BK;
-      As[kk][row] = M[(size_t)(m0 + row) * Nr + k0 + kk];
+      const int kk = e / BM, row = e % BM;
+      As[kk][row] = MT[(size_t)(k0 + kk) * Nr + m0 + row];
     }
     for (int e = tid; e < BK * BW; e += 256) {
       const int kk = e / BW, c = e % BW;
@@ -151,15 +173,23 @@ __global__ __launch_bounds__(256) void dht_gemm_valu_kernel(const R* __restrict_
 }
 
 template <typename R>
-hipError_t launch_dht_gemm(const R* M, const cx<R>* In, cx<R>* Out, const cx<R>* P, int Nr, int Nh, hipStream_t st) {
-  constexpr int BM = 64, BW = 64;
-  if (Nr % BM || Nh % BW) return hipErrorInvalidValue;
-  dim3 grid(Nh / BW, Nr / BM);
-  if (sizeof(R) == 8)
-    hipLaunchKernelGGL((dht_gemm_valu_kernel<R>), grid, dim3(256), 0, st, M, In, Out, P, Nr, Nh);
-  else
-    hipLaunchKernelGGL((dht_gemm_kernel<float, BM, BW>), grid, dim3(256), 0, st, (const float*)M,
-                       (const cx<float>*)In, (cx<float>*)Out, (const cx<float>*)P, Nr, Nh);
+hipError_t launch_dht_gemm(const R* MT, const cx<R>* In, cx<R>* Out, const cx<R>* P, int Nr, int Nh, hipStream_t st) {
+  if (Nr % 64 || Nh % 64) return hipErrorInvalidValue;
+  if (sizeof(R) == 8) {
+    hipLaunchKernelGGL((dht_gemm_valu_kernel<R>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st, MT, In, Out, P, Nr, Nh);
+    return hipGetLastError();
+  }
+  const float* A = (const float*)MT;
+  const cx<float>* B = (const cx<float>*)In;
+  cx<float>* C = (cx<float>*)Out;
+  const cx<float>* PP = (const cx<float>*)P;
+  if (Nr % 128 == 0 && (long)(Nr / 128) * (Nh / 64) >= 256) {
+    hipLaunchKernelGGL((dht_gemm_f32_kernel<128, 64, 16, 64, 32>), dim3(Nh / 64, Nr / 128), dim3(256), 0, st,
+                       A, B, C, PP, Nr, Nh);
+  } else {
+    hipLaunchKernelGGL((dht_gemm_f32_kernel<64, 64, 16, 32, 32>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st,
+                       A, B, C, PP, Nr, Nh);
+  }
   return hipGetLastError();
 }
 
@@ -190,9 +220,112 @@ template hipError_t launch_dht_gemm<double>(const double*, const cx<double>*, cx
 template hipError_t launch_propagator<float>(const double*, const double*, double, cx<float>*, int, int, hipStream_t);
 template hipError_t launch_propagator<double>(const double*, const double*, double, cx<double>*, int, int, hipStream_t);
 
-hipError_t launch_dht_gemm_bf16x3(const uint16_t*, const uint16_t*, const cx<float>*, cx<float>*, const cx<float>*,
-                                  int, int, hipStream_t) {
-  return hipErrorNotSupported;
+// ---------------------------------------------------------------------------
+// Split-BF16 3-term form: with x = x_hi + x_lo (x_hi = bf16(x),
+// x_lo = bf16(x - x_hi)), M*S ~ M_hi S_hi + M_hi S_lo + M_lo S_hi, each on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  M is split once on the
+// host; the field is split in registers after the LDS read.
+// Operand layout (lane l): a[i] = A[l%16][8*(l/16)+i], b[i] = B[8*(l/16)+i][l%16],
+// c[i] = C[4*(l/16)+i][l%16].
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split8(const float* x, bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    __bf16 h = (__bf16)x[i];
+    hi[i] = h;
+    lo[i] = (__bf16)(x[i] - (float)h);
+  }
+}
+
+template <int BM, int BW>
+__global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __restrict__ Mhi,
+                                                              const uint16_t* __restrict__ Mlo,
+                                                              const cx<float>* __restrict__ In,
+                                                              cx<float>* __restrict__ Out,
+                                                              const cx<float>* __restrict__ P, int Nr, int Nh) {
+  constexpr int BK = 32;
+  constexpr int MT = BM / 32, WT = BW / 32;
+  __shared__ __align__(16) uint16_t Ah[BM][BK + 8];
+  __shared__ __align__(16) uint16_t Al[BM][BK + 8];
+  __shared__ __align__(16) float Bre[BW][BK + 4];
+  __shared__ __align__(16) float Bim[BW][BK + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m0 = blockIdx.y * BM, w0 = blockIdx.x * BW;
+  const int wm = (wv >> 1) * (BM / 2), ww = (wv & 1) * (BW / 2);
+  const int l16 = lane & 15, l4 = lane >> 4;
+  floatx4 acc[MT][WT][2];
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int b = 0; b < WT; b++) { acc[a][b][0] = {0, 0, 0, 0}; acc[a][b][1] = {0, 0, 0, 0}; }
+
+  for (int k0 = 0; k0 < Nr; k0 += BK) {
+    // A planes: BM x BK bf16, 16-byte chunks
+    for (int e = tid; e < BM * (BK / 8); e += 256) {
+      const int row = e / (BK / 8), kq = (e % (BK / 8)) * 8;
+      const size_t g = (size_t)(m0 + row) * Nr + k0 + kq;
+      *reinterpret_cast<uint4*>(&Ah[row][kq]) = *reinterpret_cast<const uint4*>(Mhi + g);
+      *reinterpret_cast<uint4*>(&Al[row][kq]) = *reinterpret_cast<const uint4*>(Mlo + g);
+    }
+    // B tile: BK x BW complex, transposed into re / im planes [col][k]
+    for (int e = tid; e < BK * BW; e += 256) {
+      const int kk = e / BW, c = e % BW;
+      cx<float> v = In[(size_t)(k0 + kk) * Nh + w0 + c];
+      Bre[c][kk] = v.x;
+      Bim[c][kk] = v.y;
+    }
+    __syncthreads();
+    bf16x8 ah[MT], al[MT];
+#pragma unroll
+    for (int a = 0; a < MT; a++) {
+      ah[a] = *reinterpret_cast<const bf16x8*>(&Ah[wm + a * 16 + l16][8 * l4]);
+      al[a] = *reinterpret_cast<const bf16x8*>(&Al[wm + a * 16 + l16][8 * l4]);
+    }
+#pragma unroll
+    for (int b = 0; b < WT; b++) {
+      const int col = ww + b * 16 + l16;
+      float xr[8], xi[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) { xr[i] = Bre[col][8 * l4 + i]; xi[i] = Bim[col][8 * l4 + i]; }
+      bf16x8 rh, rl, ih, il;
+      split8(xr, rh, rl);
+      split8(xi, ih, il);
+#pragma unroll
+      for (int a = 0; a < MT; a++) {
+        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], rh, acc[a][b][0], 0, 0, 0);
+        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rl, acc[a][b][0], 0, 0, 0);
+        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rh, acc[a][b][0], 0, 0, 0);
+        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ih, acc[a][b][1], 0, 0, 0);
+        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], il, acc[a][b][1], 0, 0, 0);
+        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], ih, acc[a][b][1], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int b = 0; b < WT; b++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int m = m0 + wm + a * 16 + 4 * l4 + i;
+        const int w = w0 + ww + b * 16 + l16;
+        cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
+        const size_t o = (size_t)m * Nh + w;
+        if (P) v = v * P[o];
+        Out[o] = v;
+      }
+}
+
+hipError_t launch_dht_gemm_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In, cx<float>* Out,
+                                  const cx<float>* P, int Nr, int Nh, hipStream_t st) {
+  constexpr int BM = 64, BW = 64;
+  if (Nr % BM || Nh % BW) return hipErrorInvalidValue;
+  dim3 grid(Nh / BW, Nr / BM);
+  hipLaunchKernelGGL((dht_gemm_bf16x3_kernel<BM, BW>), grid, dim3(256), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh);
+  return hipGetLastError();
 }
 
 }  // namespace fil

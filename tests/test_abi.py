@@ -1,0 +1,56 @@
+"""The C-ABI library loads and exports every entry point declared in
+include/filament_hip.h; its structs agree with the ctypes mirror.  No
+compute calls (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..")
+HEADER = os.path.join(ROOT, "include", "filament_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fil_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for n in ("fil_create", "fil_set_tables", "fil_upload_field", "fil_download_field", "fil_zstep",
+              "fil_analyze", "fil_download_plasma", "fil_destroy", "fil_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    L = pkg.lib.load()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(pkg.lib.EXPORTS) <= set(declared_functions())
+
+
+def test_struct_layouts_match_header(pkg):
+    # fil_desc: 15 int32 fields; fil_diag: 9 doubles
+    assert ctypes.sizeof(pkg.lib.FilDesc) == 15 * 4
+    assert ctypes.sizeof(pkg.lib.FilDiag) == 9 * 8
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = text[text.index("typedef struct fil_desc"):text.index("} fil_desc;")]
+    fields = re.findall(r"int32_t\s+([a-zA-Z0-9_, ]+);", body)
+    names = [f.strip() for group in fields for f in group.split(",")]
+    assert names == [n for n, _ in pkg.lib.FilDesc._fields_]
+    body = text[text.index("typedef struct fil_tables"):text.index("} fil_tables;")]
+    names = re.findall(r"\b([a-zA-Z0-9_]+)(?:\[FIL_MAXCOMP\])?;", body)
+    assert names == [n for n, _ in pkg.lib.FilTables._fields_]
+
+
+def test_errors_are_reported_without_a_device(pkg):
+    L = pkg.lib.load()
+    h = ctypes.c_void_p()
+    rc = L.fil_create(None, ctypes.byref(h))
+    assert rc == 1 and b"null" in L.fil_last_error()
+    d = pkg.lib.FilDesc()
+    d.abi_version = 99
+    assert L.fil_create(ctypes.byref(d), ctypes.byref(h)) == 1
+    assert b"ABI" in L.fil_last_error()
