@@ -37,7 +37,12 @@ def load_pkg():
     return mod
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_form(pkg, p, precision, form, warmup, steps, dz, profile_steps):
+    log(f"form precision={precision} flags={form}: setup")
     m = pkg.RTModel(p, precision=precision, form=form)
     z = 0.0
     for _ in range(warmup):
@@ -51,6 +56,7 @@ def run_form(pkg, p, precision, form, warmup, steps, dz, profile_steps):
     m.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / steps
+    log(f"form precision={precision} flags={form}: {ms:.4f} ms/step over {steps} steps")
     # per-kernel device times with HIP events on the library's stream
     m.timing(True)
     for _ in range(profile_steps):
@@ -72,12 +78,14 @@ def cpu_baseline(pkg, p, sample_steps, dz):
     orc = H.load_oracle()
     g = pkg.physics.GridRT(p["rmax"], p["Nr"], p["tmin"], p["tmax"], p["Nt"])
     E0 = p["initial_condition"](g.r, g.t, p["ru"], p["tu"], p["Iu"])
+    log("cpu baseline: oracle setup")
     o = orc.Oracle(H.oracle_config(p), E0, 4)
     o.zstep(dz, dz)            # warm-up step excluded (Filament.jl:166-170)
     t0 = time.perf_counter()
     for n in range(sample_steps):
         o.zstep((n + 2) * dz, dz)
     dt = time.perf_counter() - t0
+    log(f"cpu baseline: {sample_steps} steps in {dt:.2f} s")
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=sample_steps / dt, unit="z-steps/s", cores=threads, kind="port",
                 sample=f"FP32 C++ restatement, RT 1024x4096, {sample_steps} z-steps after 1 warm-up step, "
@@ -107,7 +115,8 @@ def main():
     Nh = Nt // 2
 
     # FP64 build: the error reference for both forms
-    m64, ms64, kt64, d64, _ = run_form(pkg, p, L.FP64, 0, W, K, dz, 1)
+    # (every form runs the same W + K + prof steps, so the fluences compare at the same z)
+    m64, ms64, kt64, d64, _ = run_form(pkg, p, L.FP64, 0, W, K, dz, prof)
     m64.close()
     # plain FP32 port
     mp, msp, ktp, dp, _ = run_form(pkg, p, L.FP32, 0, W, K, dz, prof)

@@ -8,6 +8,7 @@
 // Rows are t/w-contiguous (the transpose of the reference's column-major
 // arrays), so the row phase reads whole rows and the Hankel phase reads the
 // spectrum as the B operand of a GEMM without a transpose.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -71,6 +72,7 @@ struct fil_handle {
   bool pending_filter = false, plasma_valid = false;
   double P_dz = NAN;
   double dt = 0;
+  double scale[3] = {1, 1, 1};
   std::vector<double> r, rdr, t;
   // device
   DevBuf S, X, Et, M1, M2, M1h, M1l, M2h, M2l, KZ64, KG64, P, tw, Tg, Rg, coef;
@@ -162,6 +164,7 @@ int zstep_impl(fil_handle* h, double dz) {
   a.Tg = h->Tg.as<R>();
   a.Rg = h->Rg.as<R>();
   a.coef = h->coef.as<cx<R>>();
+  for (int q = 0; q < 3; q++) a.sc[q] = (R)h->scale[q];
   a.dz = (R)dz;
   a.dt = (R)h->dt;
   a.apply_filter = h->pending_filter ? 1 : 0;
@@ -430,16 +433,30 @@ int fil_set_tables(fil_handle* h, const fil_tables* t) {
             (q == 2 && (h->d.responses & FIL_RESP_LOSSES) && !R[2]))
           return fail(FIL_EINVAL, "missing Rnl table");
     }
-    for (int q = 0; q < 3; q++)
-      for (int w = 0; w < Nh; w++) {
-        std::complex<double> c(0, 0);
-        if (R[q]) {
+    // Each response's real signal is scaled by s_q = median_w |i QZ Rnl| before
+    // it shares a complex FFT with another one; the table holds coef / s_q.
+    for (int q = 0; q < 3; q++) {
+      std::vector<std::complex<double>> c(Nh, std::complex<double>(0, 0));
+      std::vector<double> mag;
+      if (R[q]) {
+        for (int w = 0; w < Nh; w++) {
           std::complex<double> qz(t->QZ[2 * w], t->QZ[2 * w + 1]);
           std::complex<double> rn(R[q][2 * w], R[q][2 * w + 1]);
-          c = std::complex<double>(0, 1) * qz * rn * ((w == 0) ? 1.0 : 2.0) / (2.0 * Nt);
+          c[w] = std::complex<double>(0, 1) * qz * rn;
+          if (w > 0 && std::abs(c[w]) > 0) mag.push_back(std::abs(c[w]));
         }
-        cf.setc((size_t)q * Nh + w, c.real(), c.imag());
       }
+      double s = 1.0;
+      if (!mag.empty()) {
+        std::nth_element(mag.begin(), mag.begin() + mag.size() / 2, mag.end());
+        s = mag[mag.size() / 2];
+      }
+      h->scale[q] = s;
+      for (int w = 0; w < Nh; w++) {
+        std::complex<double> v = c[w] * ((w == 0) ? 1.0 : 2.0) / (2.0 * Nt * s);
+        cf.setc((size_t)q * Nh + w, v.real(), v.imag());
+      }
+    }
     HIPCHK(upload(h->coef, cf.b.data(), cf.b.size()));
   }
   // plasma parameters

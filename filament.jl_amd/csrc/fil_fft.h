@@ -2,77 +2,138 @@
 //
 // Replaces the cuFFT plan along t of the reference (plan_fft!(E, [2]),
 // src/modules/Fields.jl:353; used at Models.jl:162-171, 180-183).
-// Stockham decimation-in-frequency, radix-4 passes plus one radix-2 pass when
-// log2(N) is odd.  Every pass reads its operands into registers, barriers,
-// then writes the outputs back into the same LDS array, so one N-point
-// buffer suffices.  No normalisation is applied.
+//
+// Stockham decimation-in-frequency with radix-16 passes (a last radix-2/4/8
+// pass when log2 N is not a multiple of 4): at N = 4096 three passes, each one
+// LDS read and one LDS write of the row.  Every pass reads its operands into
+// registers, barriers, then writes the outputs back into the same array, so
+// one buffer suffices.  The row is stored with one pad element per 16
+// (lds_index(i) = i + i/16): the strided outputs of the first pass and the
+// contiguous chunks of the plasma scan then fall in distinct LDS banks.
+// No normalisation is applied.
 #pragma once
 
 #include "fil_internal.h"
 
 namespace fil {
 
+__host__ __device__ constexpr int lds_index(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_size(int n) { return n + (n >> 4); }
+
+// cos, sin of 2 pi k / 16, k = 0..7
+__device__ __forceinline__ double c16(int k) {
+  constexpr double c[8] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                           0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613};
+  return c[k];
+}
+__device__ __forceinline__ double s16(int k) {
+  constexpr double s[8] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                           1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173};
+  return s[k];
+}
+
+// In-register DFT of size RAD (2, 4, 8, 16): a[k] <- sum_j a[j] exp(SIGN 2 pi i jk / RAD).
+// Radix-2 decimation in time, recursively.
+template <typename T, int RAD, int SIGN>
+struct Dft {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    constexpr int H = RAD / 2;
+    cx<T> e[H], o[H];
+#pragma unroll
+    for (int j = 0; j < H; j++) { e[j] = a[2 * j]; o[j] = a[2 * j + 1]; }
+    Dft<T, H, SIGN>::run(e);
+    Dft<T, H, SIGN>::run(o);
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+      cx<T> w;
+      if (k == 0) w = mk((T)1, (T)0);
+      else w = mk((T)c16(k * (16 / RAD)), (T)(SIGN * s16(k * (16 / RAD))));
+      cx<T> t = (k == 0) ? o[k] : w * o[k];
+      a[k] = e[k] + t;
+      a[k + H] = e[k] - t;
+    }
+  }
+};
+template <typename T, int SIGN>
+struct Dft<T, 2, SIGN> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    cx<T> t = a[0];
+    a[0] = t + a[1];
+    a[1] = t - a[1];
+  }
+};
+template <typename T, int SIGN>
+struct Dft<T, 4, SIGN> {
+  static __device__ __forceinline__ void run(cx<T>* a) {
+    cx<T> apc = a[0] + a[2], amc = a[0] - a[2], bpd = a[1] + a[3], bmd = a[1] - a[3];
+    cx<T> jb = (SIGN > 0) ? mul_i(bmd) : mul_mi(bmd);
+    a[0] = apc + bpd;
+    a[1] = amc + jb;
+    a[2] = apc - bpd;
+    a[3] = amc - jb;
+  }
+};
+
+// One Stockham pass of radix RAD at stride S over N points.
+// Inputs  x[q + S (p + k N/(S RAD))] = x[b + k N/RAD],  b = p S + q;
+// outputs y[q + S (RAD p + k)] = DFT_RAD(...)[k] * w^(p S k),  w = exp(SIGN 2 pi i / N).
+template <typename T, int N, int NTH, int SIGN, int RAD, int S>
+__device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict__ tw) {
+  constexpr int NB = N / RAD;
+  constexpr int BPT = (NB + NTH - 1) / NTH;
+  const int tid = threadIdx.x;
+  cx<T> a[BPT][RAD];
+#pragma unroll
+  for (int u = 0; u < BPT; u++) {
+    const int b = tid + u * NTH;
+    if (NB % NTH == 0 || b < NB) {
+#pragma unroll
+      for (int k = 0; k < RAD; k++) a[u][k] = z[lds_index(b + k * NB)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < BPT; u++) {
+    const int b = tid + u * NTH;
+    if (NB % NTH == 0 || b < NB) {
+      Dft<T, RAD, SIGN>::run(a[u]);
+      const int q = b % S, p = b / S;
+      if (S < N / RAD) {       // the last pass has p == 0: no twiddles
+        const int e = p * S;
+#pragma unroll
+        for (int k = 1; k < RAD; k++) {
+          cx<T> w = tw[e * k];
+          if (SIGN > 0) w = conj(w);
+          a[u][k] = a[u][k] * w;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < RAD; k++) z[lds_index(q + S * (RAD * p + k))] = a[u][k];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int NTH, int SIGN, int S>
+struct FftPasses {
+  static __device__ __forceinline__ void run(cx<T>* z, const cx<T>* tw) {
+    constexpr int REM = N / S;
+    if constexpr (REM >= 16) {
+      stockham_pass<T, N, NTH, SIGN, 16, S>(z, tw);
+      FftPasses<T, N, NTH, SIGN, S * 16>::run(z, tw);
+    } else if constexpr (REM > 1) {
+      stockham_pass<T, N, NTH, SIGN, REM, S>(z, tw);
+    }
+  }
+};
+
 // SIGN = -1: y_k = sum_j x_j exp(-2 pi i jk/N)   (the reference's P * E)
 // SIGN = +1: y_k = sum_j x_j exp(+2 pi i jk/N)   (N times the reference's P \ E)
-// tw: exp(-2 pi i j / N), j < N, in global memory (cached).
-template <typename R, int N, int NTH, int SIGN>
-__device__ __forceinline__ void fft_lds(cx<R>* z, const cx<R>* __restrict__ tw) {
-  static_assert((N & (N - 1)) == 0 && N >= 4, "N must be a power of two >= 4");
-  constexpr int Q = N / 4;
-  constexpr int BPT = (Q + NTH - 1) / NTH;
-  const int tid = threadIdx.x;
-  int s = 1;
-#pragma unroll
-  for (int n = N; n >= 4; n >>= 2) {
-    cx<R> a[BPT][4];
-#pragma unroll
-    for (int u = 0; u < BPT; u++) {
-      const int b = tid + u * NTH;
-      if (Q % NTH == 0 || b < Q) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) a[u][k] = z[b + k * Q];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < BPT; u++) {
-      const int b = tid + u * NTH;
-      if (Q % NTH == 0 || b < Q) {
-        const int q = b & (s - 1);
-        const int p = b / s;
-        cx<R> apc = a[u][0] + a[u][2], amc = a[u][0] - a[u][2];
-        cx<R> bpd = a[u][1] + a[u][3], bmd = a[u][1] - a[u][3];
-        cx<R> jb = (SIGN > 0) ? mul_i(bmd) : mul_mi(bmd);
-        cx<R> y0 = apc + bpd, y1 = amc + jb, y2 = apc - bpd, y3 = amc - jb;
-        const int e = p * s;
-        if (e != 0) {
-          cx<R> w1 = tw[e], w2 = tw[2 * e], w3 = tw[3 * e];
-          if (SIGN > 0) { w1 = conj(w1); w2 = conj(w2); w3 = conj(w3); }
-          y1 = y1 * w1; y2 = y2 * w2; y3 = y3 * w3;
-        }
-        const int o = q + 4 * s * p;
-        z[o] = y0;
-        z[o + s] = y1;
-        z[o + 2 * s] = y2;
-        z[o + 3 * s] = y3;
-      }
-    }
-    __syncthreads();
-    s <<= 2;
-  }
-  if (s == N / 2) {   // one radix-2 pass left (log2 N odd); in place per thread
-    constexpr int H = N / 2;
-#pragma unroll
-    for (int u = 0; u < (H + NTH - 1) / NTH; u++) {
-      const int b = tid + u * NTH;
-      if (H % NTH == 0 || b < H) {
-        cx<R> a0 = z[b], a1 = z[b + H];
-        z[b] = a0 + a1;
-        z[b + H] = a0 - a1;
-      }
-    }
-    __syncthreads();
-  }
+// z is stored with lds_index(); tw: exp(-2 pi i j / N), j < N, global memory.
+template <typename T, int N, int NTH, int SIGN>
+__device__ __forceinline__ void fft_lds(cx<T>* z, const cx<T>* __restrict__ tw) {
+  static_assert((N & (N - 1)) == 0 && N >= 16, "N must be a power of two >= 16");
+  FftPasses<T, N, NTH, SIGN, 1>::run(z, tw);
 }
 
 }  // namespace fil

@@ -11,7 +11,8 @@ struct RowArgs {
   const cx<R>* tw;          // [Nt] exp(-2 pi i j/Nt)
   const R* Tg;              // [Nt] Tguard
   const R* Rg;              // [Nr] Rguard
-  const cx<R>* coef;        // [3][Nt/2] i QZ Rnl d(w)/(2 Nt): cubic, free, losses
+  const cx<R>* coef;        // [3][Nt/2] i QZ Rnl d(w)/(2 Nt sc): cubic, free, losses
+  R sc[3];                  // per-response scale applied before the packed FFTs
   R dz, dt;
   int apply_filter;         // field filter of the previous step still pending
   int nonlinearity, plasma, responses, losses_ereal;

@@ -110,7 +110,7 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
       R s0 = 0; for (int q = 0; q < nc; q++) s0 += u[q];
       rho[0] = s0;
       for (int j = 0; j < N - 1; j++) {
-        cx<R> E0 = Z[j], E1 = Z[j + 1];
+        cx<R> E0 = Z[lds_index(j)], E1 = Z[lds_index(j + 1)];
         cx<R> Em = (R)0.5 * (E0 + E1);
         R Ist[3] = {fiarg(E0, pp.ereal), fiarg(Em, pp.ereal), fiarg(E1, pp.ereal)};
         R E2st[3] = {E0.x * E0.x, Em.x * Em.x, E1.x * E1.x};
@@ -174,7 +174,7 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
   for (int c = 0; c < C; c++) {
     const int j = j0 + c;
     if (j >= N - 1) break;
-    cx<R> E0 = Z[j], E1 = Z[j + 1];
+    cx<R> E0 = Z[lds_index(j)], E1 = Z[lds_index(j + 1)];
     cx<R> Em = (R)0.5 * (E0 + E1);
     R I0 = fiarg(E0, pp.ereal), Im = fiarg(Em, pp.ereal), I1 = fiarg(E1, pp.ereal);
 #pragma unroll
@@ -209,7 +209,7 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
   for (int c = 0; c < C; c++) {
     const int j = j0 + c;
     if (j >= N - 1) break;
-    cx<R> E0 = Z[j], E1 = Z[j + 1];
+    cx<R> E0 = Z[lds_index(j)], E1 = Z[lds_index(j + 1)];
     cx<R> Em = (R)0.5 * (E0 + E1);
     R I0 = fiarg(E0, pp.ereal), Im = fiarg(Em, pp.ereal), I1 = fiarg(E1, pp.ereal);
     A tot = 0;
@@ -246,12 +246,17 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
 // to the analytic spectrum (Models.jl:180-183) and weighted by i QZ Rnl
 // (nonlinear_propagators.jl:51,57).  Two real signals share one complex FFT:
 // with Z = F_a + i F_b, 2 F_a^(w) = Z^(w) + conj Z^(N-w),
-// 2 F_b^(w) = -i (Z^(w) - conj Z^(N-w)).  coef = i QZ Rnl d(w) / (2N).
+// 2 F_b^(w) = -i (Z^(w) - conj Z^(N-w)).  coef = i QZ Rnl d(w) / (2N s_q).
+// Each signal enters the FFT pre-multiplied by its scale s_q (the typical
+// size of |i QZ Rnl| for that response): the raw responses differ by up to
+// ~20 orders of magnitude (3/4 |E|^2 Re E ~ 1 against rho Re E ~ 1e-16), and
+// without the scaling the FFT rounding of the large one would swamp the
+// small one, whose coefficient is correspondingly huge.
 // ---------------------------------------------------------------------------
 template <typename R, int N, int NTH>
 __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, const cx<R>* __restrict__ tw,
-                                          const cx<R>* __restrict__ coef, R g_r, const R* __restrict__ Tg,
-                                          int resp, int losses_ereal, cx<R>* k) {
+                                          const cx<R>* __restrict__ coef, R sc0, R sc1, R sc2, R g_r,
+                                          const R* __restrict__ Tg, int resp, int losses_ereal, cx<R>* k) {
   constexpr int TP = N / NTH;
   constexpr int NH = N / 2;
   constexpr int SP = NH / NTH;
@@ -264,7 +269,7 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
 #pragma unroll
   for (int u = 0; u < TP; u++) {
     const int t = tid + u * NTH;
-    cx<R> e = Z[t];
+    cx<R> e = Z[lds_index(t)];
     const R re = e.x, I = norm2(e);
     const R g = g_r * Tg[t];
     R fc = 0, ff = 0, fl = 0;
@@ -275,8 +280,8 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
       R inv = (fe >= (R)1e-30) ? (R)1 / fe : (R)0;
       fl = kd[t] * inv * re;
     }
-    Fl[u] = g * fl;
-    Z[t] = mk(g * fc, g * ff);
+    Fl[u] = (g * sc2) * fl;
+    Z[lds_index(t)] = mk((g * sc0) * fc, (g * sc1) * ff);
   }
   __syncthreads();
 #pragma unroll
@@ -286,7 +291,7 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
 #pragma unroll
     for (int m = 0; m < SP; m++) {
       const int w = tid + m * NTH;
-      cx<R> zw = Z[w], zm = conj(Z[(N - w) & (N - 1)]);
+      cx<R> zw = Z[lds_index(w)], zm = conj(Z[lds_index((N - w) & (N - 1))]);
       cx<R> a2 = zw + zm, b2 = mul_mi(zw - zm);
       k[m] = coef[w] * a2 + coef[NH + w] * b2;
     }
@@ -294,13 +299,13 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
   }
   if (los) {
 #pragma unroll
-    for (int u = 0; u < TP; u++) Z[tid + u * NTH] = mk(Fl[u], (R)0);
+    for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = mk(Fl[u], (R)0);
     __syncthreads();
     fft_lds<R, N, NTH, +1>(Z, tw);
 #pragma unroll
     for (int m = 0; m < SP; m++) {
       const int w = tid + m * NTH;
-      cx<R> zw = Z[w], zm = conj(Z[(N - w) & (N - 1)]);
+      cx<R> zw = Z[lds_index(w)], zm = conj(Z[lds_index((N - w) & (N - 1))]);
       k[m] = k[m] + coef[2 * NH + w] * (zw + zm);
     }
     __syncthreads();
@@ -312,7 +317,7 @@ __device__ __forceinline__ void store_padded(cx<R>* Z, const cx<R>* v) {
   constexpr int SP = (N / 2) / NTH;
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int m = 0; m < SP; m++) { Z[tid + m * NTH] = v[m]; Z[N / 2 + tid + m * NTH] = mk((R)0, (R)0); }
+  for (int m = 0; m < SP; m++) { Z[lds_index(tid + m * NTH)] = v[m]; Z[lds_index(N / 2 + tid + m * NTH)] = mk((R)0, (R)0); }
   __syncthreads();
 }
 
@@ -323,7 +328,7 @@ template <typename R, typename A, int N, int NTH>
 __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
-  R* rho = reinterpret_cast<R*>(Z + N);
+  R* rho = reinterpret_cast<R*>(Z + lds_size(N));
   R* kd = rho + N;
   A* red = reinterpret_cast<A*>(kd + N);
   constexpr int NH = N / 2;
@@ -341,12 +346,12 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
     // field filter deferred from the end of the previous zstep
     store_padded<R, N, NTH>(Z, S0);
     fft_lds<R, N, NTH, -1>(Z, a.tw);
-    for (int t = tid; t < N; t += NTH) Z[t] = (g_r * a.Tg[t]) * Z[t];
+    for (int t = tid; t < N; t += NTH) Z[lds_index(t)] = (g_r * a.Tg[t]) * Z[lds_index(t)];
     __syncthreads();
     fft_lds<R, N, NTH, +1>(Z, a.tw);
     const R inv = (R)1 / (R)N;
 #pragma unroll
-    for (int m = 0; m < SP; m++) S0[m] = inv * Z[tid + m * NTH];
+    for (int m = 0; m < SP; m++) S0[m] = inv * Z[lds_index(tid + m * NTH)];
     __syncthreads();
   }
   // time-domain field of the step's starting spectrum
@@ -381,17 +386,17 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   // RK3 (Kutta) in z: k1 = f(S0); k2 = f(S0 + dz/2 k1); k3 = f(S0 - dz k1 + 2 dz k2)
   const R dz = a.dz;
   cx<R> k1[SP], k2[SP], k[SP], U[SP];
-  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, g_r, a.Tg, a.responses, a.losses_ereal, k1);
+  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, a.sc[0], a.sc[1], a.sc[2], g_r, a.Tg, a.responses, a.losses_ereal, k1);
 #pragma unroll
   for (int m = 0; m < SP; m++) U[m] = S0[m] + ((R)0.5 * dz) * k1[m];
   store_padded<R, N, NTH>(Z, U);
   fft_lds<R, N, NTH, -1>(Z, a.tw);
-  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, g_r, a.Tg, a.responses, a.losses_ereal, k2);
+  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, a.sc[0], a.sc[1], a.sc[2], g_r, a.Tg, a.responses, a.losses_ereal, k2);
 #pragma unroll
   for (int m = 0; m < SP; m++) U[m] = S0[m] + dz * (k2[m] + k2[m] - k1[m]);
   store_padded<R, N, NTH>(Z, U);
   fft_lds<R, N, NTH, -1>(Z, a.tw);
-  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, g_r, a.Tg, a.responses, a.losses_ereal, k);
+  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, a.sc[0], a.sc[1], a.sc[2], g_r, a.Tg, a.responses, a.losses_ereal, k);
   const R c16 = (R)1 / (R)6, c23 = (R)2 / (R)3;
 #pragma unroll
   for (int m = 0; m < SP; m++) {
@@ -409,11 +414,11 @@ __global__ __launch_bounds__(NTH) void to_spectrum_kernel(const cx<R>* __restric
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   const int i = blockIdx.x, tid = threadIdx.x;
-  for (int t = tid; t < N; t += NTH) Z[t] = Et[(size_t)i * N + t];
+  for (int t = tid; t < N; t += NTH) Z[lds_index(t)] = Et[(size_t)i * N + t];
   __syncthreads();
   fft_lds<R, N, NTH, +1>(Z, tw);
   const R inv = (R)1 / (R)N;
-  for (int w = tid; w < N / 2; w += NTH) S[(size_t)i * (N / 2) + w] = inv * Z[w];
+  for (int w = tid; w < N / 2; w += NTH) S[(size_t)i * (N / 2) + w] = inv * Z[lds_index(w)];
 }
 
 // Download: spectrum -> time rows, with the pending field filter.
@@ -423,12 +428,12 @@ __global__ __launch_bounds__(NTH) void to_time_kernel(const cx<R>* __restrict__ 
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   const int i = blockIdx.x, tid = threadIdx.x;
-  for (int w = tid; w < N / 2; w += NTH) { Z[w] = S[(size_t)i * (N / 2) + w]; Z[N / 2 + w] = mk((R)0, (R)0); }
+  for (int w = tid; w < N / 2; w += NTH) { Z[lds_index(w)] = S[(size_t)i * (N / 2) + w]; Z[lds_index(N / 2 + w)] = mk((R)0, (R)0); }
   __syncthreads();
   fft_lds<R, N, NTH, -1>(Z, tw);
   const R g = apply_filter ? Rg[i] : (R)1;
   for (int t = tid; t < N; t += NTH) {
-    cx<R> v = Z[t];
+    cx<R> v = Z[lds_index(t)];
     if (apply_filter) v = (g * Tg[t]) * v;
     Et[(size_t)i * N + t] = v;
   }
@@ -446,17 +451,17 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
                                                            float* Sw, double* Imax_row) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
-  double* red = reinterpret_cast<double*>(Z + N);
+  double* red = reinterpret_cast<double*>(Z + lds_size(N));
   const int i = blockIdx.x, tid = threadIdx.x;
   constexpr int NH = N / 2;
-  for (int w = tid; w < NH; w += NTH) { Z[w] = S[(size_t)i * NH + w]; Z[NH + w] = mk((R)0, (R)0); }
+  for (int w = tid; w < NH; w += NTH) { Z[lds_index(w)] = S[(size_t)i * NH + w]; Z[lds_index(NH + w)] = mk((R)0, (R)0); }
   __syncthreads();
   fft_lds<R, N, NTH, -1>(Z, tw);
   const R g = apply_filter ? Rg[i] : (R)1;
   double fr = 0;
   for (int t = tid; t < N; t += NTH) {
-    cx<R> v = Z[t];
-    if (apply_filter) { v = (g * Tg[t]) * v; Z[t] = v; }
+    cx<R> v = Z[lds_index(t)];
+    if (apply_filter) { v = (g * Tg[t]) * v; Z[lds_index(t)] = v; }
     double I = (double)norm2(v);
     fr += I;
     It[(size_t)i * N + t] = (float)I;
@@ -473,7 +478,7 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
   const R inv = (R)1 / (R)N;
   const double c = 2.0 * (double)N * (double)dt;
   for (int w = tid; w <= NH; w += NTH) {
-    cx<R> v = inv * Z[w];
+    cx<R> v = inv * Z[lds_index(w)];
     double re = (double)v.x, im = (double)v.y;
     if (w >= 1 && w < NH) { re *= 0.5; im *= 0.5; }   // aspec2rspec
     Sw[(size_t)i * (NH + 1) + w] = (float)(c * c * (re * re + im * im));
@@ -482,18 +487,18 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
   // projection: keep bins 0..N/2-1 as the new state
   cx<R> keep[NH / NTH];
 #pragma unroll
-  for (int m = 0; m < NH / NTH; m++) keep[m] = inv * Z[tid + m * NTH];
+  for (int m = 0; m < NH / NTH; m++) keep[m] = inv * Z[lds_index(tid + m * NTH)];
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < NH / NTH; m++) {
     S[(size_t)i * NH + tid + m * NTH] = keep[m];
-    Z[tid + m * NTH] = keep[m];
-    Z[NH + tid + m * NTH] = mk((R)0, (R)0);
+    Z[lds_index(tid + m * NTH)] = keep[m];
+    Z[lds_index(NH + tid + m * NTH)] = mk((R)0, (R)0);
   }
   __syncthreads();
   fft_lds<R, N, NTH, -1>(Z, tw);
   double mx = 0;
-  for (int t = tid; t < N; t += NTH) mx = fmax(mx, (double)norm2(Z[t]));
+  for (int t = tid; t < N; t += NTH) mx = fmax(mx, (double)norm2(Z[lds_index(t)]));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
   if (lane == 0) red[wv] = mx;
@@ -517,7 +522,9 @@ __global__ void colsum_kernel(const float* __restrict__ in, int Nr, int ncol, co
 template <int N> constexpr int threads_for() { return N >= 8192 ? 512 : (N >= 4096 ? 256 : (N >= 2048 ? 128 : 64)); }
 
 template <typename R, typename A>
-static size_t row_smem(int N, int nth) { return (size_t)N * (sizeof(cx<R>) + 2 * sizeof(R)) + 2 * (nth / 64) * sizeof(A) + 64; }
+static size_t row_smem(int N, int nth) {
+  return (size_t)lds_size(N) * sizeof(cx<R>) + (size_t)N * 2 * sizeof(R) + 2 * (nth / 64) * sizeof(A) + 64;
+}
 
 template <typename R, typename A, int N>
 static hipError_t launch_row_n(const RowArgs<R>& a, int Nr, hipStream_t st) {
@@ -555,7 +562,7 @@ template <typename R>
 hipError_t launch_to_spectrum(const cx<R>* Et, cx<R>* S, const cx<R>* tw, int Nr, int N, hipStream_t st) {
   FIL_DISPATCH_N(N, {
     constexpr int NTH = threads_for<NN>();
-    size_t sm = (size_t)NN * sizeof(cx<R>);
+    size_t sm = (size_t)lds_size(NN) * sizeof(cx<R>);
     auto k = to_spectrum_kernel<R, NN, NTH>;
     { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
     hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, Et, S, tw);
@@ -569,7 +576,7 @@ hipError_t launch_to_time(const cx<R>* S, cx<R>* Et, const cx<R>* tw, const R* R
                           int Nr, int N, hipStream_t st) {
   FIL_DISPATCH_N(N, {
     constexpr int NTH = threads_for<NN>();
-    size_t sm = (size_t)NN * sizeof(cx<R>);
+    size_t sm = (size_t)lds_size(NN) * sizeof(cx<R>);
     auto k = to_time_kernel<R, NN, NTH>;
     { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
     hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, S, Et, tw, Rg, Tg, apply);
@@ -583,7 +590,7 @@ hipError_t launch_analyze_rows(cx<R>* S, const cx<R>* tw, const R* Rg, const R* 
                                float* It, float* Sw, double* Imax_row, int Nr, int N, hipStream_t st) {
   FIL_DISPATCH_N(N, {
     constexpr int NTH = threads_for<NN>();
-    size_t sm = (size_t)NN * sizeof(cx<R>) + 16 * sizeof(double);
+    size_t sm = (size_t)lds_size(NN) * sizeof(cx<R>) + 16 * sizeof(double);
     auto k = analyze_rows_kernel<R, NN, NTH>;
     { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
     hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, S, tw, Rg, Tg, apply, dt, Fr, It, Sw, Imax_row);

@@ -99,22 +99,25 @@ def run(p, precision=L.FP32, form=0, device=0, verbose=True):
     if prefix_dir:
         os.makedirs(prefix_dir, exist_ok=True)
     model = RTModel(p, precision=precision, form=form, device=device, basedir=p.get("basedir", "."))
-    z = p["z"]
-    d = model.analyze(z)
+    # the reference keeps z, zmax and dz of the RT geometry in FloatGPU = Float32
+    # (Input.jl:294-297, AdaptiveSteps.jl:106), which fixes the number of steps
+    f32 = np.float32
+    z = f32(p["z"])
+    d = model.analyze(float(z))
     fplot = os.path.join(prefix_dir, prefix_name + "plot.dat")
     write_plotdat_header(fplot, p)
     write_plotdat_row(fplot, d)
     rows = [d]
-    zmax = p["zmax"]
+    zmax = f32(p["zmax"])
     t0 = time.time()
     try:
         while z < zmax:
             if verbose:
                 print(f"z={_fmt(z)}[zu] I={_fmt(d['Imax'])}[Iu] rho={_fmt(d['rhomax'])}[rhou]")
-            dz = model.dz_adaptive(p["dz_initial"], p["dzphimax"], d["Imax"], d["rhomax"])
-            z = z + dz
-            model.zstep(z, dz)
-            d = model.analyze(z)
+            dz = f32(model.dz_adaptive(p["dz_initial"], p["dzphimax"], d["Imax"], d["rhomax"]))
+            z = f32(z + dz)
+            model.zstep(float(z), float(dz))
+            d = model.analyze(float(z))
             write_plotdat_row(fplot, d)
             rows.append(d)
             if d["Imax"] > p["Istop"]:

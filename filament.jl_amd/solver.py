@@ -24,7 +24,10 @@ from .responses import response_kind
 
 
 class RTModel:
-    def __init__(self, p, precision=L.FP32, form=0, device=0, keep_plasma=False, basedir="."):
+    def __init__(self, p, precision=L.FP32, form=0, device=0, keep_plasma=False, basedir=".", KZ=None):
+        """``KZ`` overrides the host-computed linear-propagator table (an
+        [Nr, Nt] complex array), e.g. to drive the device with another
+        implementation's table."""
         self.p = p
         self.unit = ph.UnitRT(p["ru"], p["zu"], p["tu"], p["Iu"], p["rhou"])
         self.grid = ph.GridRT(p["rmax"], p["Nr"], p["tmin"], p["tmax"], p["Nt"])
@@ -34,7 +37,7 @@ class RTModel:
         self.n0 = float(np.real(m.refractive_index(self.w0)))
         self.Eu = ph.field_unit(u, self.n0)
         self.guard = ph.GuardRT(u, g, m, p["rguard"], p["tguard"], p["kguard"], p["wguard"])
-        self.KZ = ph.kz_table(u, g, m, self.w0, bool(p["KPARAXIAL"]))
+        self.KZ = ph.kz_table(u, g, m, self.w0, bool(p["KPARAXIAL"])) if KZ is None else np.asarray(KZ)
         self.precision, self.form = precision, form
         self.nonlinearity = bool(p["NONLINEARITY"])
         self.plasma = bool(p["PLASMA"])
