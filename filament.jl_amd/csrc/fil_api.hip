@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "fil_fft.h"
 #include "fil_internal.h"
 #include "fil_kernels.h"
 
@@ -407,12 +408,24 @@ int fil_set_tables(fil_handle* h, const fil_tables* t) {
   }
   // twiddles, guards
   {
-    HostArr tw(Nt, f64, true), tg(Nt, f64, false), rg(Nr, f64, false);
-    for (int j = 0; j < Nt; j++) {
-      double a = -2.0 * kPi * j / Nt;
-      tw.setc(j, std::cos(a), std::sin(a));
-      tg.set(j, t->Tguard[j]);
+    // per-pass twiddle table in the layout fft_lds reads (fil_fft.h)
+    HostArr tw(std::max(1, fft_tw_size(Nt)), f64, true), tg(Nt, f64, false), rg(Nr, f64, false);
+    {
+      size_t o = 0;
+      for (int s = 1; Nt / s > 1; s *= fft_radix(Nt, s)) {
+        const int rad = fft_radix(Nt, s);
+        if (s < Nt / rad) {
+          const int P = Nt / (s * rad);
+          for (int k = 1; k < rad; k++)
+            for (int p = 0; p < P; p++) {
+              double a = -2.0 * kPi * (double)(((long)p * s * k) % Nt) / Nt;
+              tw.setc(o++, std::cos(a), std::sin(a));
+            }
+        }
+      }
+      if ((int)o != fft_tw_size(Nt)) return fail(FIL_EINVAL, "internal: twiddle table size");
     }
+    for (int j = 0; j < Nt; j++) tg.set(j, t->Tguard[j]);
     for (int i = 0; i < Nr; i++) rg.set(i, t->Rguard[i]);
     HIPCHK(upload(h->tw, tw.b.data(), tw.b.size()));
     HIPCHK(upload(h->Tg, tg.b.data(), tg.b.size()));

@@ -49,31 +49,27 @@ __global__ __launch_bounds__(256) void dht_gemm_f32_kernel(const float* __restri
   constexpr int NB = BK * BW / 2 / 256;   // float4 (two complex) of B per thread
   static_assert(NA >= 1 && NB >= 1, "tile too small for 256 threads");
   float4 ra[NA], rb[NB];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int u = 0; u < NA; u++) {
-      const int e = tid + 256 * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;
-      ra[u] = *reinterpret_cast<const float4*>(MT + (size_t)(k0 + kk) * Nr + m0 + m4);
-    }
-#pragma unroll
-    for (int u = 0; u < NB; u++) {
-      const int e = tid + 256 * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;
-      rb[u] = *reinterpret_cast<const float4*>(In + (size_t)(k0 + kk) * Nh + w0 + c2);
-    }
-  };
-  auto lstore = [&](int buf) {
-#pragma unroll
-    for (int u = 0; u < NA; u++) {
-      const int e = tid + 256 * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;
-      *reinterpret_cast<float4*>(&As[buf][kk][m4]) = ra[u];
-    }
-#pragma unroll
-    for (int u = 0; u < NB; u++) {
-      const int e = tid + 256 * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;
-      *reinterpret_cast<float2*>(&Bre[buf][kk][c2]) = make_float2(rb[u].x, rb[u].z);
-      *reinterpret_cast<float2*>(&Bim[buf][kk][c2]) = make_float2(rb[u].y, rb[u].w);
-    }
-  };
+  // global -> registers (prefetch of the next k tile) and registers -> LDS,
+  // written as macros so the prefetch arrays stay in registers
+#define FIL_F32_GLOAD(k0_)                                                                      \
+  _Pragma("unroll") for (int u = 0; u < NA; u++) {                                               \
+    const int e = tid + 256 * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;                     \
+    ra[u] = *reinterpret_cast<const float4*>(MT + (size_t)((k0_) + kk) * Nr + m0 + m4);           \
+  }                                                                                              \
+  _Pragma("unroll") for (int u = 0; u < NB; u++) {                                               \
+    const int e = tid + 256 * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;                     \
+    rb[u] = *reinterpret_cast<const float4*>(In + (size_t)((k0_) + kk) * Nh + w0 + c2);           \
+  }
+#define FIL_F32_LSTORE(buf_)                                                                    \
+  _Pragma("unroll") for (int u = 0; u < NA; u++) {                                               \
+    const int e = tid + 256 * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;                     \
+    *reinterpret_cast<float4*>(&As[buf_][kk][m4]) = ra[u];                                       \
+  }                                                                                              \
+  _Pragma("unroll") for (int u = 0; u < NB; u++) {                                               \
+    const int e = tid + 256 * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;                     \
+    *reinterpret_cast<float2*>(&Bre[buf_][kk][c2]) = make_float2(rb[u].x, rb[u].z);              \
+    *reinterpret_cast<float2*>(&Bim[buf_][kk][c2]) = make_float2(rb[u].y, rb[u].w);              \
+  }
   floatx4 acc[MT_][WT][2];
 #pragma unroll
   for (int a = 0; a < MT_; a++)
@@ -81,12 +77,14 @@ __global__ __launch_bounds__(256) void dht_gemm_f32_kernel(const float* __restri
     for (int b = 0; b < WT; b++) { acc[a][b][0] = {0, 0, 0, 0}; acc[a][b][1] = {0, 0, 0, 0}; }
 
   const int nk = Nr / BK;
-  gload(0);
-  lstore(0);
+  FIL_F32_GLOAD(0);
+  FIL_F32_LSTORE(0);
   __syncthreads();
   for (int kb = 0; kb < nk; kb++) {
     const int buf = kb & 1;
-    if (kb + 1 < nk) gload((kb + 1) * BK);
+    // unconditional prefetch (the last iteration re-reads tile 0 into the idle buffer),
+    // which keeps the prefetch registers out of scratch
+    FIL_F32_GLOAD(((kb + 1) % nk) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int k = ks + l4;
@@ -103,9 +101,11 @@ __global__ __launch_bounds__(256) void dht_gemm_f32_kernel(const float* __restri
           acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bi[b], acc[a][b][1], 0, 0, 0);
         }
     }
-    if (kb + 1 < nk) lstore(buf ^ 1);
+    FIL_F32_LSTORE(buf ^ 1);
     __syncthreads();
   }
+#undef FIL_F32_GLOAD
+#undef FIL_F32_LSTORE
 #pragma unroll
   for (int a = 0; a < MT_; a++)
 #pragma unroll
@@ -184,7 +184,7 @@ hipError_t launch_dht_gemm(const R* MT, const cx<R>* In, cx<R>* Out, const cx<R>
   cx<float>* C = (cx<float>*)Out;
   const cx<float>* PP = (const cx<float>*)P;
   if (Nr % 128 == 0 && (long)(Nr / 128) * (Nh / 64) >= 256) {
-    hipLaunchKernelGGL((dht_gemm_f32_kernel<128, 64, 16, 64, 32>), dim3(Nh / 64, Nr / 128), dim3(256), 0, st,
+    hipLaunchKernelGGL((dht_gemm_f32_kernel<128, 64, 32, 64, 32>), dim3(Nh / 64, Nr / 128), dim3(256), 0, st,
                        A, B, C, PP, Nr, Nh);
   } else {
     hipLaunchKernelGGL((dht_gemm_f32_kernel<64, 64, 16, 32, 32>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st,
@@ -319,12 +319,130 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __
       }
 }
 
+// Second split-BF16 kernel: 128 (m) x 64 (complex w) x 32 (k) tiles, the
+// field split into four bf16 planes (re_hi, re_lo, im_hi, im_lo) once when a
+// tile is staged, stored [w][k] so each lane's MFMA operand (8 consecutive k)
+// is one 16-byte LDS read; register-prefetch double buffering.
+template <int BM, int BW, int BK>
+__global__ __launch_bounds__(256) void dht_gemm_bf16x3_v2_kernel(const uint16_t* __restrict__ Mhi,
+                                                                 const uint16_t* __restrict__ Mlo,
+                                                                 const cx<float>* __restrict__ In,
+                                                                 cx<float>* __restrict__ Out,
+                                                                 const cx<float>* __restrict__ P, int Nr, int Nh) {
+  constexpr int SK = BK + 8;              // padded row: (BK + 8) bf16
+  constexpr int WM = BM / 2, WW = BW / 2;
+  constexpr int MT = WM / 16, WT = WW / 16;
+  __shared__ __align__(16) uint16_t As[2][2][BM][SK];      // [buf][hi/lo][m][k]
+  __shared__ __align__(16) uint16_t Bs[2][4][BW][SK];      // [buf][re_hi,re_lo,im_hi,im_lo][w][k]
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m0 = blockIdx.y * BM, w0 = blockIdx.x * BW;
+  const int wm = (wv >> 1) * WM, ww = (wv & 1) * WW;
+  const int l16 = lane & 15, l4 = lane >> 4;
+  constexpr int NA = BM * BK / 8 / 256;     // 16-byte chunks of each A plane per thread
+  constexpr int NB = BW * BK / 8 / 256;     // (column, 8 consecutive k) B chunks per thread
+  static_assert(NA >= 1 && NB >= 1 && BK % 32 == 0, "tile shape");
+  uint4 rah[NA], ral[NA];
+  cx<float> rb[NB][8];
+#define FIL_BF_GLOAD(k0_)                                                                       \
+  _Pragma("unroll") for (int u = 0; u < NA; u++) {                                               \
+    const int e = tid + 256 * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;                    \
+    const size_t g = (size_t)(m0 + row) * Nr + (k0_) + kq;                                        \
+    rah[u] = *reinterpret_cast<const uint4*>(Mhi + g);                                           \
+    ral[u] = *reinterpret_cast<const uint4*>(Mlo + g);                                           \
+  }                                                                                              \
+  _Pragma("unroll") for (int u = 0; u < NB; u++) {                                               \
+    const int e = tid + 256 * u, bw = e % BW, bkg = e / BW;                                      \
+    _Pragma("unroll") for (int i = 0; i < 8; i++)                                                \
+      rb[u][i] = In[(size_t)((k0_) + 8 * bkg + i) * Nh + w0 + bw];                               \
+  }
+#define FIL_BF_LSTORE(buf_)                                                                     \
+  _Pragma("unroll") for (int u = 0; u < NA; u++) {                                               \
+    const int e = tid + 256 * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;                    \
+    *reinterpret_cast<uint4*>(&As[buf_][0][row][kq]) = rah[u];                                   \
+    *reinterpret_cast<uint4*>(&As[buf_][1][row][kq]) = ral[u];                                   \
+  }                                                                                              \
+  _Pragma("unroll") for (int u = 0; u < NB; u++) {                                               \
+    const int e = tid + 256 * u, bw = e % BW, bkg = e / BW;                                      \
+    bf16x8 rh, rl, ih, il;                                                                       \
+    _Pragma("unroll") for (int i = 0; i < 8; i++) {                                              \
+      __bf16 hh = (__bf16)rb[u][i].x;                                                            \
+      rh[i] = hh; rl[i] = (__bf16)(rb[u][i].x - (float)hh);                                      \
+      hh = (__bf16)rb[u][i].y;                                                                   \
+      ih[i] = hh; il[i] = (__bf16)(rb[u][i].y - (float)hh);                                      \
+    }                                                                                            \
+    *reinterpret_cast<bf16x8*>(&Bs[buf_][0][bw][8 * bkg]) = rh;                                  \
+    *reinterpret_cast<bf16x8*>(&Bs[buf_][1][bw][8 * bkg]) = rl;                                  \
+    *reinterpret_cast<bf16x8*>(&Bs[buf_][2][bw][8 * bkg]) = ih;                                  \
+    *reinterpret_cast<bf16x8*>(&Bs[buf_][3][bw][8 * bkg]) = il;                                  \
+  }
+  floatx4 acc[MT][WT][2];
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int b = 0; b < WT; b++) { acc[a][b][0] = {0, 0, 0, 0}; acc[a][b][1] = {0, 0, 0, 0}; }
+  const int nk = Nr / BK;
+  FIL_BF_GLOAD(0);
+  FIL_BF_LSTORE(0);
+  __syncthreads();
+  for (int kb = 0; kb < nk; kb++) {
+    const int buf = kb & 1;
+    FIL_BF_GLOAD(((kb + 1) % nk) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 32) {
+    bf16x8 ah[MT], al[MT];
+#pragma unroll
+    for (int a = 0; a < MT; a++) {
+      ah[a] = *reinterpret_cast<const bf16x8*>(&As[buf][0][wm + a * 16 + l16][ks + 8 * l4]);
+      al[a] = *reinterpret_cast<const bf16x8*>(&As[buf][1][wm + a * 16 + l16][ks + 8 * l4]);
+    }
+#pragma unroll
+    for (int b = 0; b < WT; b++) {
+      const int col = ww + b * 16 + l16;
+      const bf16x8 rh = *reinterpret_cast<const bf16x8*>(&Bs[buf][0][col][ks + 8 * l4]);
+      const bf16x8 rl = *reinterpret_cast<const bf16x8*>(&Bs[buf][1][col][ks + 8 * l4]);
+      const bf16x8 ih = *reinterpret_cast<const bf16x8*>(&Bs[buf][2][col][ks + 8 * l4]);
+      const bf16x8 il = *reinterpret_cast<const bf16x8*>(&Bs[buf][3][col][ks + 8 * l4]);
+#pragma unroll
+      for (int a = 0; a < MT; a++) {
+        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], rh, acc[a][b][0], 0, 0, 0);
+        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rl, acc[a][b][0], 0, 0, 0);
+        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rh, acc[a][b][0], 0, 0, 0);
+        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ih, acc[a][b][1], 0, 0, 0);
+        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], il, acc[a][b][1], 0, 0, 0);
+        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], ih, acc[a][b][1], 0, 0, 0);
+      }
+    }
+    }
+    FIL_BF_LSTORE(buf ^ 1);
+    __syncthreads();
+  }
+#undef FIL_BF_GLOAD
+#undef FIL_BF_LSTORE
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int b = 0; b < WT; b++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int m = m0 + wm + a * 16 + 4 * l4 + i;
+        const int w = w0 + ww + b * 16 + l16;
+        cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
+        const size_t o = (size_t)m * Nh + w;
+        if (P) v = v * P[o];
+        Out[o] = v;
+      }
+}
+
 hipError_t launch_dht_gemm_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In, cx<float>* Out,
                                   const cx<float>* P, int Nr, int Nh, hipStream_t st) {
-  constexpr int BM = 64, BW = 64;
-  if (Nr % BM || Nh % BW) return hipErrorInvalidValue;
-  dim3 grid(Nh / BW, Nr / BM);
-  hipLaunchKernelGGL((dht_gemm_bf16x3_kernel<BM, BW>), grid, dim3(256), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh);
+  if (Nr % 64 || Nh % 64) return hipErrorInvalidValue;
+  if (Nr % 128 == 0 && (long)(Nr / 128) * (Nh / 64) >= 256) {
+    hipLaunchKernelGGL((dht_gemm_bf16x3_v2_kernel<128, 64, 64>), dim3(Nh / 64, Nr / 128), dim3(256), 0, st,
+                       Mhi, Mlo, In, Out, P, Nr, Nh);
+  } else {
+    hipLaunchKernelGGL((dht_gemm_bf16x3_kernel<64, 64>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st,
+                       Mhi, Mlo, In, Out, P, Nr, Nh);
+  }
   return hipGetLastError();
 }
 

@@ -74,6 +74,21 @@ struct Dft<T, 4, SIGN> {
   }
 };
 
+// Twiddle table layout (built on the host by fil::fft_twiddles): for every
+// pass that has twiddles (stride S < N/RAD) a block of (RAD-1) x P entries,
+// P = N/(S RAD), entry [(k-1) P + p] = exp(-2 pi i p S k / N).  Lanes of one
+// pass read consecutive p, so every twiddle load is coalesced.
+__host__ __device__ constexpr int fft_radix(int N, int S) { return (N / S >= 16) ? 16 : N / S; }
+__host__ __device__ constexpr int fft_tw_offset(int N, int S) {
+  int off = 0;
+  for (int s = 1; s < S; s *= fft_radix(N, s)) {
+    const int rad = fft_radix(N, s);
+    if (s < N / rad) off += (rad - 1) * (N / (s * rad));
+  }
+  return off;
+}
+__host__ __device__ constexpr int fft_tw_size(int N) { return fft_tw_offset(N, N); }
+
 // One Stockham pass of radix RAD at stride S over N points.
 // Inputs  x[q + S (p + k N/(S RAD))] = x[b + k N/RAD],  b = p S + q;
 // outputs y[q + S (RAD p + k)] = DFT_RAD(...)[k] * w^(p S k),  w = exp(SIGN 2 pi i / N).
@@ -99,10 +114,11 @@ __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict_
       Dft<T, RAD, SIGN>::run(a[u]);
       const int q = b % S, p = b / S;
       if (S < N / RAD) {       // the last pass has p == 0: no twiddles
-        const int e = p * S;
+        constexpr int P = N / (S * RAD);
+        const cx<T>* twp = tw + fft_tw_offset(N, S);
 #pragma unroll
         for (int k = 1; k < RAD; k++) {
-          cx<T> w = tw[e * k];
+          cx<T> w = twp[(k - 1) * P + p];
           if (SIGN > 0) w = conj(w);
           a[u][k] = a[u][k] * w;
         }
@@ -129,7 +145,7 @@ struct FftPasses {
 
 // SIGN = -1: y_k = sum_j x_j exp(-2 pi i jk/N)   (the reference's P * E)
 // SIGN = +1: y_k = sum_j x_j exp(+2 pi i jk/N)   (N times the reference's P \ E)
-// z is stored with lds_index(); tw: exp(-2 pi i j / N), j < N, global memory.
+// z is stored with lds_index(); tw: the per-pass twiddle table (global memory).
 template <typename T, int N, int NTH, int SIGN>
 __device__ __forceinline__ void fft_lds(cx<T>* z, const cx<T>* __restrict__ tw) {
   static_assert((N & (N - 1)) == 0 && N >= 16, "N must be a power of two >= 16");

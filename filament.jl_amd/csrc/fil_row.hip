@@ -42,6 +42,22 @@ __device__ __forceinline__ R tf_eval(const PlasmaParams<R>& pp, int q, R I, R* p
 template <typename R>
 __device__ __forceinline__ R fiarg(cx<R> e, int ereal) { return ereal ? e.x * e.x : norm2(e); }
 
+// Gain of one Kutta RK3 step of d rho/dt = W(t)(f - rho) with stage rates
+// W1, W2, W3: rho -> rho + h (f - rho),
+// h = dt [W1/6 + 2/3 W2 (1 - dt W1/2) + 1/6 W3 (1 + dt W1 - 2 dt W2 + dt^2 W1 W2)].
+template <typename A>
+__device__ __forceinline__ A rk3_gain(A dt, A w1, A w2, A w3) {
+  constexpr A c6 = (A)1 / (A)6, c23 = (A)2 / (A)3;
+  const A dw1 = dt * w1, dw2 = dt * w2;
+  return dt * (c6 * w1 + c23 * w2 * ((A)1 - (A)0.5 * dw1) + c6 * w3 * ((A)1 + dw1 - (A)2 * dw2 + dw1 * dw2));
+}
+
+template <typename R>
+__device__ __forceinline__ R recip(R x) {
+  if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);
+  else return (R)1 / x;
+}
+
 // Affine map rho -> a*rho + b, composed along t.  later o earlier.
 template <typename A>
 struct Aff { A a, b; };
@@ -185,8 +201,7 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
       R W3 = tf_eval<R>(pp, q, I1, nullptr);
       Wprev[q] = W3;
       A w1 = W1, w2 = W2, w3 = W3;
-      A h = dtA * (w1 / (A)6 + (A)2 / (A)3 * w2 * ((A)1 - dtA * w1 / (A)2) +
-                   w3 / (A)6 * ((A)1 + dtA * w1 - (A)2 * dtA * w2 + dtA * dtA * w1 * w2));
+      const A h = rk3_gain<A>(dtA, w1, w2, w3);
       Aff<A> st; st.a = (A)1 - h; st.b = h * (A)pp.frhont[q];
       m[q] = compose(st, m[q]);
     }
@@ -222,8 +237,7 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
       R W2 = tf_eval<R>(pp, q, Im, nullptr);
       R W3 = tf_eval<R>(pp, q, I1, nullptr);
       A w1 = W1, w2 = W2, w3 = W3;
-      A h = dtA * (w1 / (A)6 + (A)2 / (A)3 * w2 * ((A)1 - dtA * w1 / (A)2) +
-                   w3 / (A)6 * ((A)1 + dtA * w1 - (A)2 * dtA * w2 + dtA * dtA * w1 * w2));
+      const A h = rk3_gain<A>(dtA, w1, w2, w3);
       r[q] = ((A)1 - h) * r[q] + h * (A)pp.frhont[q];
       tot += r[q];
       R K = pp.kdep ? pw : (R)pp.K[q];
@@ -277,7 +291,7 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
     if (fre) ff = rho[t] * re;
     if (los) {
       R fe = losses_ereal ? re * re : I;
-      R inv = (fe >= (R)1e-30) ? (R)1 / fe : (R)0;
+      R inv = (fe >= (R)1e-30) ? recip<R>(fe) : (R)0;
       fl = kd[t] * inv * re;
     }
     Fl[u] = (g * sc2) * fl;
