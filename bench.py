@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp64,plain,chosen",
+                    help="comma list of forms to run (fp64, plain, chosen); profiling runs pick one")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the RT z-step runs on a single MI355X; "
@@ -114,6 +116,17 @@ def main():
     Nr, Nt = p["Nr"], p["Nt"]
     Nh = Nt // 2
 
+    forms_run = set(args.forms.split(","))
+    if forms_run != {"fp64", "plain", "chosen"}:
+        # profiling mode: run the requested forms only, no JSON contract line
+        for f in ("fp64", "plain", "chosen"):
+            if f in forms_run:
+                prec = L.FP64 if f == "fp64" else L.FP32
+                flags = (L.FORM_DHT_BF16X3 | L.FORM_SCAN_FP64) if f == "chosen" else 0
+                m, ms, kt, d, _ = run_form(pkg, p, prec, flags, W, K, dz, prof)
+                m.close()
+                log(f"{f}: {ms:.4f} ms/step")
+        return
     # FP64 build: the error reference for both forms
     # (every form runs the same W + K + prof steps, so the fluences compare at the same z)
     m64, ms64, kt64, d64, _ = run_form(pkg, p, L.FP64, 0, W, K, dz, prof)

@@ -80,6 +80,11 @@ struct fil_handle {
   DevBuf tab_ly[FIL_MAXCOMP], tab_sl[FIL_MAXCOMP];
   DevBuf rho_end, rho_max, rho_full, kd_full;
   DevBuf aFr, aIt, aSw, aImax, aFt, aS, ardr;
+  // saved states (fil_state_save / fil_state_restore)
+  struct Slot {
+    DevBuf S, rho_end, rho_max;
+    bool used = false, pending_filter = false, plasma_valid = false;
+  } slot[FIL_NSLOTS];
   PlasmaParams<float> pp32{};
   PlasmaParams<double> pp64{};
   // timing
@@ -622,6 +627,57 @@ int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld) {
       if (rho) rho[o] = rv;
       if (kdrho) kdrho[o] = kv;
     }
+  return FIL_OK;
+}
+
+int fil_state_save(fil_handle* h, int slot) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (slot < 0 || slot >= FIL_NSLOTS) return fail(FIL_EINVAL, "slot out of range");
+  if (!h->field) return fail(FIL_ESTATE, "no field uploaded");
+  HIPCHK(hipSetDevice(h->d.device));
+  auto& s = h->slot[slot];
+  const size_t nb = (size_t)h->Nrp * h->Nh * h->csz();
+  if (!s.S.p) {
+    if (s.S.alloc(nb) != hipSuccess || s.rho_end.alloc(h->Nr * 8) != hipSuccess || s.rho_max.alloc(h->Nr * 8) != hipSuccess)
+      return fail(FIL_ENOMEM, "device allocation failed");
+  }
+  HIPCHK(hipMemcpyAsync(s.S.p, h->S.p, nb, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(s.rho_end.p, h->rho_end.p, h->Nr * 8, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(s.rho_max.p, h->rho_max.p, h->Nr * 8, hipMemcpyDeviceToDevice, h->st));
+  s.used = true;
+  s.pending_filter = h->pending_filter;
+  s.plasma_valid = h->plasma_valid;
+  return FIL_OK;
+}
+
+int fil_state_restore(fil_handle* h, int slot) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (slot < 0 || slot >= FIL_NSLOTS) return fail(FIL_EINVAL, "slot out of range");
+  auto& s = h->slot[slot];
+  if (!s.used) return fail(FIL_ESTATE, "slot is empty");
+  HIPCHK(hipSetDevice(h->d.device));
+  const size_t nb = (size_t)h->Nrp * h->Nh * h->csz();
+  HIPCHK(hipMemcpyAsync(h->S.p, s.S.p, nb, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->rho_end.p, s.rho_end.p, h->Nr * 8, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->rho_max.p, s.rho_max.p, h->Nr * 8, hipMemcpyDeviceToDevice, h->st));
+  h->pending_filter = s.pending_filter;
+  h->plasma_valid = s.plasma_valid;
+  return FIL_OK;
+}
+
+int fil_fluence(fil_handle* h, int slot, double* Fr) {
+  if (!valid_handle(h) || !Fr) return fail(FIL_EINVAL, "null argument");
+  if (!h->field) return fail(FIL_ESTATE, "no field uploaded");
+  if (slot >= FIL_NSLOTS || (slot >= 0 && !h->slot[slot].used)) return fail(FIL_EINVAL, "bad slot");
+  HIPCHK(hipSetDevice(h->d.device));
+  const void* S = slot < 0 ? h->S.p : h->slot[slot].S.p;
+  const double scale = (double)h->Nt * h->dt;   // dt sum_t |E|^2 = dt Nt sum_w |S_w|^2
+  if (h->fp64)
+    HIPCHK(launch_rowsum_abs2<double>(static_cast<const cx<double>*>(S), h->Nr, h->Nh, scale, h->aFr.as<double>(), h->st));
+  else
+    HIPCHK(launch_rowsum_abs2<float>(static_cast<const cx<float>*>(S), h->Nr, h->Nh, scale, h->aFr.as<double>(), h->st));
+  HIPCHK(hipMemcpyAsync(Fr, h->aFr.p, h->Nr * 8, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
   return FIL_OK;
 }
 

@@ -3,9 +3,9 @@
 // Replaces the cuFFT plan along t of the reference (plan_fft!(E, [2]),
 // src/modules/Fields.jl:353; used at Models.jl:162-171, 180-183).
 //
-// Stockham decimation-in-frequency with radix-16 passes (a last radix-2/4/8
-// pass when log2 N is not a multiple of 4): at N = 4096 three passes, each one
-// LDS read and one LDS write of the row.  Every pass reads its operands into
+// Stockham decimation-in-frequency with radix-FIL_FFT_MAXRAD passes (a last
+// smaller-radix pass when needed): at N = 4096 and radix 8 four passes, each
+// one LDS read and one LDS write of the row.  Every pass reads its operands into
 // registers, barriers, then writes the outputs back into the same array, so
 // one buffer suffices.  The row is stored with one pad element per 16
 // (lds_index(i) = i + i/16): the strided outputs of the first pass and the
@@ -16,6 +16,10 @@
 #include "fil_internal.h"
 
 namespace fil {
+
+// Largest radix of the Stockham passes.  With N/8 threads per row every
+// thread owns one radix-8 butterfly per pass.
+#define FIL_FFT_MAXRAD 8
 
 __host__ __device__ constexpr int lds_index(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int lds_size(int n) { return n + (n >> 4); }
@@ -78,7 +82,7 @@ struct Dft<T, 4, SIGN> {
 // pass that has twiddles (stride S < N/RAD) a block of (RAD-1) x P entries,
 // P = N/(S RAD), entry [(k-1) P + p] = exp(-2 pi i p S k / N).  Lanes of one
 // pass read consecutive p, so every twiddle load is coalesced.
-__host__ __device__ constexpr int fft_radix(int N, int S) { return (N / S >= 16) ? 16 : N / S; }
+__host__ __device__ constexpr int fft_radix(int N, int S) { return (N / S >= FIL_FFT_MAXRAD) ? FIL_FFT_MAXRAD : N / S; }
 __host__ __device__ constexpr int fft_tw_offset(int N, int S) {
   int off = 0;
   for (int s = 1; s < S; s *= fft_radix(N, s)) {
@@ -96,8 +100,27 @@ template <typename T, int N, int NTH, int SIGN, int RAD, int S>
 __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict__ tw) {
   constexpr int NB = N / RAD;
   constexpr int BPT = (NB + NTH - 1) / NTH;
-  const int tid = threadIdx.x;
+  constexpr bool TW = S < N / RAD;     // the last pass has p == 0: no twiddles
+  constexpr int P = N / (S * RAD);
+  // The LDS addresses of a pass depend only on the thread index; without this
+  // opaque copy the compiler hoists them out of every FFT of the kernel and
+  // keeps them all live, which costs ~50 VGPRs and halves the occupancy.
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
   cx<T> a[BPT][RAD];
+  cx<T> w[BPT][TW ? RAD - 1 : 1];
+  // twiddle loads first, so their latency overlaps the LDS reads and the barrier
+  if constexpr (TW) {
+    const cx<T>* twp = tw + fft_tw_offset(N, S);
+#pragma unroll
+    for (int u = 0; u < BPT; u++) {
+      const int b = tid + u * NTH;
+      if (NB % NTH == 0 || b < NB) {
+#pragma unroll
+        for (int k = 1; k < RAD; k++) w[u][k - 1] = twp[(k - 1) * P + b / S];
+      }
+    }
+  }
 #pragma unroll
   for (int u = 0; u < BPT; u++) {
     const int b = tid + u * NTH;
@@ -113,15 +136,9 @@ __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict_
     if (NB % NTH == 0 || b < NB) {
       Dft<T, RAD, SIGN>::run(a[u]);
       const int q = b % S, p = b / S;
-      if (S < N / RAD) {       // the last pass has p == 0: no twiddles
-        constexpr int P = N / (S * RAD);
-        const cx<T>* twp = tw + fft_tw_offset(N, S);
+      if constexpr (TW) {
 #pragma unroll
-        for (int k = 1; k < RAD; k++) {
-          cx<T> w = twp[(k - 1) * P + p];
-          if (SIGN > 0) w = conj(w);
-          a[u][k] = a[u][k] * w;
-        }
+        for (int k = 1; k < RAD; k++) a[u][k] = a[u][k] * ((SIGN > 0) ? conj(w[u][k - 1]) : w[u][k - 1]);
       }
 #pragma unroll
       for (int k = 0; k < RAD; k++) z[lds_index(q + S * (RAD * p + k))] = a[u][k];
@@ -134,9 +151,9 @@ template <typename T, int N, int NTH, int SIGN, int S>
 struct FftPasses {
   static __device__ __forceinline__ void run(cx<T>* z, const cx<T>* tw) {
     constexpr int REM = N / S;
-    if constexpr (REM >= 16) {
-      stockham_pass<T, N, NTH, SIGN, 16, S>(z, tw);
-      FftPasses<T, N, NTH, SIGN, S * 16>::run(z, tw);
+    if constexpr (REM >= FIL_FFT_MAXRAD) {
+      stockham_pass<T, N, NTH, SIGN, FIL_FFT_MAXRAD, S>(z, tw);
+      FftPasses<T, N, NTH, SIGN, S * FIL_FFT_MAXRAD>::run(z, tw);
     } else if constexpr (REM > 1) {
       stockham_pass<T, N, NTH, SIGN, REM, S>(z, tw);
     }

@@ -34,6 +34,8 @@ template <typename R>
 hipError_t launch_analyze_rows(cx<R>* S, const cx<R>* tw, const R* Rg, const R* Tg, int apply, R dt, double* Fr,
                                float* It, float* Sw, double* Imax_row, int Nr, int N, hipStream_t st);
 hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* rdr, double* out, hipStream_t st);
+template <typename R>
+hipError_t launch_rowsum_abs2(const cx<R>* S, int Nr, int Nh, double scale, double* out, hipStream_t st);
 
 // Hankel-transform phase (fil_dht.hip)
 // Propagator P[k][w] = exp(i KZ dz) Kguard Wguard computed in FP64.

@@ -520,6 +520,29 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
   if (tid == 0) { double m = 0; for (int w = 0; w < NTH / 64; w++) m = fmax(m, red[w]); Imax_row[i] = m; }
 }
 
+// Parseval fluence of each row from its half-spectrum: out[i] = scale * sum_w |S[i][w]|^2.
+template <typename R>
+__global__ __launch_bounds__(256) void rowsum_abs2_kernel(const cx<R>* __restrict__ S, int Nh, double scale,
+                                                          double* out) {
+  __shared__ double red[4];
+  const int i = blockIdx.x, tid = threadIdx.x;
+  double s = 0;
+  for (int w = tid; w < Nh; w += 256) s += (double)norm2(S[(size_t)i * Nh + w]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = s;
+  __syncthreads();
+  if (tid == 0) out[i] = scale * (red[0] + red[1] + red[2] + red[3]);
+}
+
+template <typename R>
+hipError_t launch_rowsum_abs2(const cx<R>* S, int Nr, int Nh, double scale, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(rowsum_abs2_kernel<R>, dim3(Nr), dim3(256), 0, st, S, Nh, scale, out);
+  return hipGetLastError();
+}
+template hipError_t launch_rowsum_abs2<float>(const cx<float>*, int, int, double, double*, hipStream_t);
+template hipError_t launch_rowsum_abs2<double>(const cx<double>*, int, int, double, double*, hipStream_t);
+
 // Column sums over r: out[j] = 2 pi sum_i in[i][j] * rdr[i], in FP64, fixed order.
 __global__ void colsum_kernel(const float* __restrict__ in, int Nr, int ncol, const double* __restrict__ rdr,
                               double* out) {
@@ -533,7 +556,10 @@ __global__ void colsum_kernel(const float* __restrict__ in, int Nr, int ncol, co
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-template <int N> constexpr int threads_for() { return N >= 8192 ? 512 : (N >= 4096 ? 256 : (N >= 2048 ? 128 : 64)); }
+// one radix-FIL_FFT_MAXRAD butterfly per thread per FFT pass
+template <int N> constexpr int threads_for() {
+  return N / FIL_FFT_MAXRAD < 64 ? 64 : (N / FIL_FFT_MAXRAD > 1024 ? 1024 : N / FIL_FFT_MAXRAD);
+}
 
 template <typename R, typename A>
 static size_t row_smem(int N, int nth) {

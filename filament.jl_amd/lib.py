@@ -53,7 +53,7 @@ class FilDiag(ctypes.Structure):
 
 EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_upload_field", "fil_download_field",
            "fil_zstep", "fil_analyze", "fil_download_plasma", "fil_synchronize", "fil_destroy",
-           "fil_timing_enable", "fil_timing_read")
+           "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_fluence")
 
 _lib = None
 
@@ -82,6 +82,9 @@ def load():
     L.fil_destroy.argtypes = [ctypes.c_void_p]
     L.fil_timing_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.fil_timing_read.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]
+    L.fil_state_save.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.fil_state_restore.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.fil_fluence.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
     for name in EXPORTS:
         if name != "fil_last_error":
             getattr(L, name).restype = ctypes.c_int

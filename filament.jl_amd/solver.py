@@ -186,6 +186,18 @@ class RTModel:
             dzp = phimax / (self.phip * rhomax) if self.phip * rhomax != 0 else math.inf
         return min(dz0, dzk, dzp)
 
+    def state_save(self, slot):
+        L.check(self._lib.fil_state_save(self.h, int(slot)))
+
+    def state_restore(self, slot):
+        L.check(self._lib.fil_state_restore(self.h, int(slot)))
+
+    def fluence(self, slot=-1):
+        """Fr(r) of the current state (slot < 0) or of a saved slot, by Parseval."""
+        Fr = np.zeros(self.grid.Nr)
+        L.check(self._lib.fil_fluence(self.h, int(slot), L.ptr(Fr)))
+        return Fr
+
     def timing(self, enable=True):
         L.check(self._lib.fil_timing_enable(self.h, int(enable)))
 

@@ -155,6 +155,20 @@ int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft,
 int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld);
 
 int fil_synchronize(fil_handle* h);
+
+/* Device-side copies of the z-step state (slots 0..FIL_NSLOTS-1), used to
+ * run a coarse step and the fine steps it replaces from the same state.  No
+ * counterpart in the reference (its driver never branches the state); the
+ * host-side equivalent would be deepcopy(field). */
+#define FIL_NSLOTS 4
+int fil_state_save(fil_handle* h, int slot);
+int fil_state_restore(fil_handle* h, int slot);
+
+/* Fluence profile Fr(r) = dt * sum_t |E(r,t)|^2 (FieldAnalyzers.jl:171) of the
+ * current state (slot < 0) or of a saved slot, computed on the device from
+ * the spectrum by Parseval's identity, before the pending field filter.
+ * Fr has Nr entries.  Blocks. */
+int fil_fluence(fil_handle* h, int slot, double* Fr);
 int fil_destroy(fil_handle* h);
 
 /* Per-phase device timing with HIP events on the handle's stream (the
