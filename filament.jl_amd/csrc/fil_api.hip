@@ -34,6 +34,17 @@ int fail(int code, const std::string& msg) {
 struct DevBuf {
   void* p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p; n = o.n;
+      o.p = nullptr; o.n = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { if (p) (void)hipFree(p); }
   hipError_t alloc(size_t bytes) {
     if (p) { (void)hipFree(p); p = nullptr; }
@@ -85,6 +96,19 @@ struct fil_handle {
     DevBuf S, rho_end, rho_max;
     bool used = false, pending_filter = false, plasma_valid = false;
   } slot[FIL_NSLOTS];
+  // learned corrector (fil_corrector_*)
+  struct Corr {
+    bool on = false;
+    CorrGeom g{};
+    int cap = 0, count = 0, next = 0, bmax = 0;
+    double y_scale = 0;
+    int64_t adam_t = 0;
+    DevBuf w, wbf, m, v, grad;        // flat parameters, BF16 copy, Adam moments, gradient
+    DevBuf X, Y;                      // stored pairs [cap][4][HW], [cap][2][HW]
+    DevBuf Frow, feat, carrier;       // feature scratch, features [4][HW], exp(i w0 t_j) [Nt]
+    DevBuf act[3], out, gA, gB, gout; // training activations and gradients [bmax][..][HW]
+    DevBuf idx, part;                 // batch indices, reduction partials
+  } corr;
   PlasmaParams<float> pp32{};
   PlasmaParams<double> pp64{};
   // timing
@@ -282,6 +306,88 @@ int analyze_impl(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, d
 }
 
 bool valid_handle(fil_handle* h) { return h != nullptr; }
+
+// ---------------------------------------------------------------------------
+// Learned corrector helpers
+// ---------------------------------------------------------------------------
+static_assert(corr_nparam() == FIL_CORR_NPARAM, "FIL_CORR_NPARAM must match the network");
+
+uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+double normal01(uint64_t& s) {
+  double u1 = uniform01(s), u2 = uniform01(s);
+  if (u1 < 1e-300) u1 = 1e-300;
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kPi * u2);
+}
+
+int corr_bias_off(int l) { return corr_woff(l) + corr_cout(l) * corr_cin(l) * 9; }
+
+// network forward over B samples of `in` (gathered by idx when given) into corr.out
+int corr_forward(fil_handle* h, const float* in, const int* idx, int B) {
+  auto& c = h->corr;
+  const float* w = c.w.as<float>();
+  const uint16_t* wb = c.wbf.as<uint16_t>();
+  const float* src = in;
+  const int* sidx = idx;
+  for (int l = 0; l < CORR_LAYERS; l++) {
+    float* dst = (l == CORR_LAYERS - 1) ? c.out.as<float>() : c.act[l].as<float>();
+    HIPCHK(launch_conv_fwd(l, src, sidx, wb + corr_woff(l), w + corr_bias_off(l), dst, B, c.g.H, c.g.W,
+                           l < CORR_LAYERS - 1 ? 1 : 0, h->st));
+    src = dst;
+    sidx = nullptr;
+  }
+  return 0;
+}
+
+// gradient of the loss (corr.gout already filled) into corr.grad
+int corr_backward(fil_handle* h, const float* in, const int* idx, int B) {
+  auto& c = h->corr;
+  const uint16_t* wb = c.wbf.as<uint16_t>();
+  float* gr = c.grad.as<float>();
+  const int H = c.g.H, W = c.g.W;
+  float* g = c.gout.as<float>();
+  float* bufs[2] = {c.gA.as<float>(), c.gB.as<float>()};
+  for (int l = CORR_LAYERS - 1; l >= 0; l--) {
+    const float* a_in = (l == 0) ? in : c.act[l - 1].as<float>();
+    HIPCHK(launch_conv_bwd_weight(l, g, a_in, l == 0 ? idx : nullptr, gr + corr_woff(l), gr + corr_bias_off(l), B, H, W,
+                                  h->st));
+    if (l > 0) {
+      float* d = bufs[l & 1];
+      HIPCHK(launch_conv_bwd_data(l, g, wb + corr_woff(l), c.act[l - 1].as<float>(), d, B, H, W, h->st));
+      g = d;
+    }
+  }
+  return 0;
+}
+
+// loss over B samples whose targets are corr.Y[idx[b]]; fills corr.gout; returns the partial count
+int corr_loss(fil_handle* h, const int* idx, int B, int* nblk) {
+  auto& c = h->corr;
+  HIPCHK(launch_corr_loss(c.out.as<float>(), c.Y.as<float>(), idx, (float)(1.0 / c.y_scale), c.gout.as<float>(),
+                          c.part.as<double>(), B, c.g.H * c.g.W, nblk, h->st));
+  return 0;
+}
+
+int corr_read_loss(fil_handle* h, int nblk, int B, double* loss) {
+  std::vector<double> part(nblk);
+  HIPCHK(hipMemcpyAsync(part.data(), h->corr.part.p, nblk * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  double s = 0;
+  for (double v : part) s += v;
+  if (loss) *loss = s / ((double)B * 2 * h->corr.g.H * h->corr.g.W);
+  return 0;
+}
+
+int corr_ready(fil_handle* h) {
+  if (!h) return fail(FIL_EINVAL, "null handle");
+  if (!h->corr.on) return fail(FIL_ESTATE, "fil_corrector_init first");
+  return 0;
+}
 
 }  // namespace
 
@@ -713,6 +819,227 @@ int fil_timing_read(fil_handle* h, double* ms_row, double* ms_dht1, double* ms_d
   if (ms_dht1) *ms_dht1 = h->ms[1];
   if (ms_dht2) *ms_dht2 = h->ms[2];
   if (count) *count = h->count;
+  return FIL_OK;
+}
+
+int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
+  if (!valid_handle(h) || !cd) return fail(FIL_EINVAL, "null argument");
+  if (h->fp64) return fail(FIL_EINVAL, "the corrector needs an FP32 handle");
+  if (!h->tables) return fail(FIL_ESTATE, "fil_set_tables first");
+  if (cd->pool_r < 1 || cd->pool_t < 1) return fail(FIL_EINVAL, "pool sizes must be >= 1");
+  if (cd->nr < 1 || cd->nr > h->Nr || cd->nr % cd->pool_r) return fail(FIL_EINVAL, "nr must be in [1, Nr] and a multiple of pool_r");
+  if (cd->t0 < 0 || cd->nt < 1 || cd->t0 + cd->nt > h->Nt || cd->nt % cd->pool_t)
+    return fail(FIL_EINVAL, "t window must lie in [0, Nt) with nt a multiple of pool_t");
+  if (cd->capacity < 0 || cd->batch_max < 1) return fail(FIL_EINVAL, "capacity >= 0 and batch_max >= 1 required");
+  if (!(cd->a_scale > 0) || !(cd->rho_scale > 0) || !(cd->dt > 0) || !std::isfinite(cd->w0) || !std::isfinite(cd->tmin))
+    return fail(FIL_EINVAL, "a_scale, rho_scale and dt must be positive, w0 and tmin finite");
+  const int H = cd->nr / cd->pool_r, W = cd->nt / cd->pool_t;
+  if ((int64_t)H * W > (1 << 22)) return fail(FIL_EINVAL, "pooled grid too large");
+  HIPCHK(hipSetDevice(h->d.device));
+  HIPCHK(hipStreamSynchronize(h->st));
+  auto& c = h->corr;
+  c = fil_handle::Corr();
+  c.g.nr = cd->nr; c.g.t0 = cd->t0; c.g.nt = cd->nt; c.g.pr = cd->pool_r; c.g.pt = cd->pool_t;
+  c.g.H = H; c.g.W = W;
+  c.g.dt = (float)cd->dt;
+  c.g.inv_as = (float)(1.0 / cd->a_scale); c.g.inv_rs = (float)(1.0 / cd->rho_scale);
+  c.cap = cd->capacity;
+  c.bmax = cd->batch_max;
+  const size_t HW = (size_t)H * W, NP = FIL_CORR_NPARAM, B = c.bmax;
+  bool ok = c.w.alloc(NP * 4) == hipSuccess && c.wbf.alloc(NP * 2) == hipSuccess && c.m.alloc(NP * 4) == hipSuccess &&
+            c.v.alloc(NP * 4) == hipSuccess && c.grad.alloc(NP * 4) == hipSuccess &&
+            c.X.alloc(std::max<size_t>(1, (size_t)c.cap) * 4 * HW * 4) == hipSuccess &&
+            c.Y.alloc(std::max<size_t>(1, (size_t)c.cap) * 2 * HW * 4) == hipSuccess &&
+            c.Frow.alloc((size_t)cd->nr * 4 * W * 4) == hipSuccess && c.feat.alloc(4 * HW * 4) == hipSuccess &&
+            c.out.alloc(B * 2 * HW * 4) == hipSuccess && c.gout.alloc(B * 2 * HW * 4) == hipSuccess &&
+            c.gA.alloc(B * CORR_CH * HW * 4) == hipSuccess && c.gB.alloc(B * CORR_CH * HW * 4) == hipSuccess &&
+            c.idx.alloc(B * 4) == hipSuccess && c.part.alloc((B * 2 * HW / 256 + 1024) * 8) == hipSuccess;
+  for (int l = 0; l < 3 && ok; l++) ok = c.act[l].alloc(B * CORR_CH * HW * 4) == hipSuccess;
+  if (ok) ok = c.carrier.alloc((size_t)h->Nt * 8) == hipSuccess;
+  if (!ok) { c = fil_handle::Corr(); return fail(FIL_ENOMEM, "device allocation failed"); }
+  {
+    std::vector<float> car(2 * (size_t)h->Nt);
+    for (int j = 0; j < h->Nt; j++) {
+      const double ph = cd->w0 * (cd->tmin + j * cd->dt);
+      car[2 * j] = (float)std::cos(ph);
+      car[2 * j + 1] = (float)std::sin(ph);
+    }
+    HIPCHK(hipMemcpyAsync(c.carrier.p, car.data(), car.size() * 4, hipMemcpyHostToDevice, h->st));
+    c.g.carrier = c.carrier.as<cx<float>>();
+  }
+  // He-normal weights for the ReLU layers, zero last layer and biases
+  std::vector<float> w(NP, 0.0f);
+  uint64_t s = seed;
+  for (int l = 0; l < CORR_LAYERS - 1; l++) {
+    const double sd = std::sqrt(2.0 / (corr_cin(l) * 9.0));
+    for (int q = 0; q < corr_cout(l) * corr_cin(l) * 9; q++) w[corr_woff(l) + q] = (float)(sd * normal01(s));
+  }
+  HIPCHK(hipMemcpyAsync(c.w.p, w.data(), NP * 4, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemsetAsync(c.m.p, 0, NP * 4, h->st));
+  HIPCHK(hipMemsetAsync(c.v.p, 0, NP * 4, h->st));
+  HIPCHK(launch_round_bf16(c.w.as<float>(), c.wbf.as<uint16_t>(), (int)NP, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  c.on = true;
+  return FIL_OK;
+}
+
+int fil_corrector_params(fil_handle* h, float* w, int set) {
+  if (int rc = corr_ready(h)) return rc;
+  if (!w) return fail(FIL_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(h->d.device));
+  auto& c = h->corr;
+  const size_t NP = FIL_CORR_NPARAM;
+  if (set) {
+    for (size_t q = 0; q < NP; q++)
+      if (!std::isfinite(w[q])) return fail(FIL_EINVAL, "parameters must be finite");
+    HIPCHK(hipMemcpyAsync(c.w.p, w, NP * 4, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemsetAsync(c.m.p, 0, NP * 4, h->st));
+    HIPCHK(hipMemsetAsync(c.v.p, 0, NP * 4, h->st));
+    HIPCHK(launch_round_bf16(c.w.as<float>(), c.wbf.as<uint16_t>(), (int)NP, h->st));
+    c.adam_t = 0;
+  } else {
+    HIPCHK(hipMemcpyAsync(w, c.w.p, NP * 4, hipMemcpyDeviceToHost, h->st));
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  return FIL_OK;
+}
+
+int fil_corrector_scale(fil_handle* h, double* y_scale, int set) {
+  if (int rc = corr_ready(h)) return rc;
+  if (!y_scale) return fail(FIL_EINVAL, "null argument");
+  if (set) {
+    if (!(*y_scale >= 0) || !std::isfinite(*y_scale)) return fail(FIL_EINVAL, "y_scale must be finite and >= 0");
+    h->corr.y_scale = *y_scale;
+  } else {
+    *y_scale = h->corr.y_scale;
+  }
+  return FIL_OK;
+}
+
+int fil_corrector_features(fil_handle* h, float* X) {
+  if (int rc = corr_ready(h)) return rc;
+  if (!h->field) return fail(FIL_ESTATE, "no field uploaded");
+  HIPCHK(hipSetDevice(h->d.device));
+  auto& c = h->corr;
+  HIPCHK(launch_corr_features(h->S.as<cx<float>>(), h->tw.as<cx<float>>(), h->pp32, h->d.plasma, c.g, h->Nt,
+                              c.Frow.as<float>(), c.feat.as<float>(), h->st));
+  if (X) HIPCHK(hipMemcpyAsync(X, c.feat.p, (size_t)4 * c.g.H * c.g.W * 4, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return FIL_OK;
+}
+
+int fil_corrector_store_pair(fil_handle* h, const float* X, const float* T) {
+  if (int rc = corr_ready(h)) return rc;
+  if (!X || !T) return fail(FIL_EINVAL, "null argument");
+  auto& c = h->corr;
+  if (c.cap < 1) return fail(FIL_EINVAL, "the corrector was created with capacity 0");
+  HIPCHK(hipSetDevice(h->d.device));
+  const size_t HW = (size_t)c.g.H * c.g.W;
+  HIPCHK(hipMemcpyAsync(c.X.as<float>() + (size_t)c.next * 4 * HW, X, 4 * HW * 4, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(c.Y.as<float>() + (size_t)c.next * 2 * HW, T, 2 * HW * 4, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  c.next = (c.next + 1) % c.cap;
+  c.count = std::min(c.count + 1, c.cap);
+  return FIL_OK;
+}
+
+int fil_corrector_train(fil_handle* h, int iters, int batch, double lr, uint64_t seed, double* loss) {
+  if (int rc = corr_ready(h)) return rc;
+  auto& c = h->corr;
+  if (iters < 1 || batch < 1 || batch > c.bmax) return fail(FIL_EINVAL, "iters >= 1 and 1 <= batch <= batch_max required");
+  if (batch > c.count) return fail(FIL_ESTATE, "fewer stored pairs than the batch size");
+  if (!(lr > 0) || !std::isfinite(lr)) return fail(FIL_EINVAL, "lr must be positive");
+  HIPCHK(hipSetDevice(h->d.device));
+  const int HW = c.g.H * c.g.W;
+  if (c.y_scale == 0) {
+    int nb = 0;
+    HIPCHK(launch_sum_squares(c.Y.as<float>(), c.count * 2 * HW, c.part.as<double>(), &nb, h->st));
+    std::vector<double> part(nb);
+    HIPCHK(hipMemcpyAsync(part.data(), c.part.p, nb * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    double s = 0;
+    for (double v : part) s += v;
+    const double rms = std::sqrt(s / ((double)c.count * 2 * HW));
+    if (!(rms > 0) || !std::isfinite(rms)) return fail(FIL_ESTATE, "stored targets are all zero");
+    c.y_scale = rms;
+  }
+  // batch indices of every iteration, drawn without replacement within a batch
+  std::vector<int> ids((size_t)iters * batch);
+  std::vector<int> perm(c.count);
+  uint64_t s = seed;
+  for (int it = 0; it < iters; it++) {
+    for (int q = 0; q < c.count; q++) perm[q] = q;
+    for (int q = 0; q < batch; q++) {
+      const int r = q + (int)(splitmix64(s) % (uint64_t)(c.count - q));
+      std::swap(perm[q], perm[r]);
+      ids[(size_t)it * batch + q] = perm[q];
+    }
+  }
+  DevBuf dids;
+  HIPCHK(dids.alloc(ids.size() * 4));
+  HIPCHK(hipMemcpyAsync(dids.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, h->st));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int nblk = 0;
+  for (int it = 0; it < iters; it++) {
+    const int* idx = dids.as<int>() + (size_t)it * batch;
+    if (int rc = corr_forward(h, c.X.as<float>(), idx, batch)) return rc;
+    if (int rc = corr_loss(h, idx, batch, &nblk)) return rc;
+    if (int rc = corr_backward(h, c.X.as<float>(), idx, batch)) return rc;
+    c.adam_t++;
+    const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)c.adam_t)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)c.adam_t)));
+    HIPCHK(launch_adam(c.w.as<float>(), c.grad.as<float>(), c.m.as<float>(), c.v.as<float>(), c.wbf.as<uint16_t>(),
+                       FIL_CORR_NPARAM, (float)lr, b1, b2, eps, c1, c2, h->st));
+  }
+  if (int rc = corr_read_loss(h, nblk, batch, loss)) return rc;
+  return FIL_OK;
+}
+
+int fil_corrector_eval(fil_handle* h, int first, int count, double* loss, float* grad) {
+  if (int rc = corr_ready(h)) return rc;
+  auto& c = h->corr;
+  if (first < 0 || count < 1 || count > c.bmax || first + count > c.count)
+    return fail(FIL_EINVAL, "pairs [first, first+count) must be stored and count <= batch_max");
+  if (!(c.y_scale > 0)) return fail(FIL_ESTATE, "y_scale is not set (train or set it first)");
+  HIPCHK(hipSetDevice(h->d.device));
+  std::vector<int> ids(count);
+  for (int q = 0; q < count; q++) ids[q] = first + q;
+  HIPCHK(hipMemcpyAsync(c.idx.p, ids.data(), count * 4, hipMemcpyHostToDevice, h->st));
+  int nblk = 0;
+  if (int rc = corr_forward(h, c.X.as<float>(), c.idx.as<int>(), count)) return rc;
+  if (int rc = corr_loss(h, c.idx.as<int>(), count, &nblk)) return rc;
+  if (grad) {
+    if (int rc = corr_backward(h, c.X.as<float>(), c.idx.as<int>(), count)) return rc;
+    HIPCHK(hipMemcpyAsync(grad, c.grad.p, (size_t)FIL_CORR_NPARAM * 4, hipMemcpyDeviceToHost, h->st));
+  }
+  return corr_read_loss(h, nblk, count, loss);
+}
+
+int fil_corrector_forward(fil_handle* h, const float* X, float* Y) {
+  if (int rc = corr_ready(h)) return rc;
+  if (!X || !Y) return fail(FIL_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(h->d.device));
+  auto& c = h->corr;
+  const size_t HW = (size_t)c.g.H * c.g.W;
+  HIPCHK(hipMemcpyAsync(c.feat.p, X, 4 * HW * 4, hipMemcpyHostToDevice, h->st));
+  if (int rc = corr_forward(h, c.feat.as<float>(), nullptr, 1)) return rc;
+  HIPCHK(hipMemcpyAsync(Y, c.out.p, 2 * HW * 4, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return FIL_OK;
+}
+
+int fil_corrector_apply(fil_handle* h) {
+  if (int rc = corr_ready(h)) return rc;
+  if (!h->field) return fail(FIL_ESTATE, "no field uploaded");
+  auto& c = h->corr;
+  if (!(c.y_scale > 0)) return fail(FIL_ESTATE, "the corrector is untrained (y_scale is 0)");
+  HIPCHK(hipSetDevice(h->d.device));
+  HIPCHK(launch_corr_features(h->S.as<cx<float>>(), h->tw.as<cx<float>>(), h->pp32, h->d.plasma, c.g, h->Nt,
+                              c.Frow.as<float>(), c.feat.as<float>(), h->st));
+  if (int rc = corr_forward(h, c.feat.as<float>(), nullptr, 1)) return rc;
+  const float scale = (float)(c.y_scale / (double)c.g.inv_as);
+  HIPCHK(launch_corr_apply(h->S.as<cx<float>>(), h->tw.as<cx<float>>(), c.g, h->Nt, c.out.as<float>(), scale, h->st));
+  h->plasma_valid = false;
   return FIL_OK;
 }
 

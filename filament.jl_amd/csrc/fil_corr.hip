@@ -1,0 +1,267 @@
+// fil_corr.hip -- network of the learned coarse-step corrector (no counterpart
+// in the reference): forward and backward of 3x3 convolutions, MSE loss and
+// Adam, all on the device.
+//
+// Layer l maps [B][cin][H][W] -> [B][cout][H][W] with zero padding:
+//   out[co][y][x] = b[co] + sum_ci sum_{dy,dx in -1..1} w[co][ci][(dy+1)*3+(dx+1)] in[ci][y+dy][x+dx]
+// Weights are read as BF16 (the rounded copy of the FP32 master parameters)
+// and every product is accumulated in FP32.  One thread owns one pixel and all
+// its output channels; the layer's weights sit in LDS in the order the inner
+// loop reads them, so each LDS read feeds several FMAs.
+#include <algorithm>
+
+#include "fil_internal.h"
+#include "fil_kernels.h"
+
+namespace fil {
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+constexpr int CT = 256;   // threads per block of the convolution kernels
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(CT) void conv_fwd_kernel(const float* __restrict__ in, const int* __restrict__ idx,
+                                                      const uint16_t* __restrict__ wbf, const float* __restrict__ bias,
+                                                      float* __restrict__ out, int H, int W, int relu) {
+  __shared__ __align__(16) float wl[CIN * 9 * COUT];   // [ci][tap][co]
+  __shared__ float bl[COUT];
+  for (int q = threadIdx.x; q < CIN * 9 * COUT; q += CT) {
+    const int co = q % COUT, r = q / COUT, tap = r % 9, ci = r / 9;
+    wl[q] = bf2f(wbf[(co * CIN + ci) * 9 + tap]);
+  }
+  for (int q = threadIdx.x; q < COUT; q += CT) bl[q] = bias[q];
+  __syncthreads();
+  const int HW = H * W;
+  const int b = blockIdx.y;
+  const int p = blockIdx.x * CT + threadIdx.x;
+  if (p >= HW) return;
+  const int y = p / W, x = p % W;
+  const float* inb = in + (size_t)(idx ? idx[b] : b) * CIN * HW;
+  float acc[COUT];
+#pragma unroll
+  for (int co = 0; co < COUT; co++) acc[co] = bl[co];
+  for (int ci = 0; ci < CIN; ci++) {
+    const float* ip = inb + (size_t)ci * HW;
+    float v[9];
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+      const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+      v[tap] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? ip[yy * W + xx] : 0.0f;
+    }
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+      const float* wr = wl + (ci * 9 + tap) * COUT;
+#pragma unroll
+      for (int co = 0; co < COUT; co++) acc[co] += wr[co] * v[tap];
+    }
+  }
+  float* ob = out + (size_t)b * COUT * HW + p;
+#pragma unroll
+  for (int co = 0; co < COUT; co++) ob[(size_t)co * HW] = relu ? fmaxf(acc[co], 0.0f) : acc[co];
+}
+
+// din[ci][q] = sum_co sum_tap w[co][ci][tap] dout[co][q - d(tap)], times [act > 0]
+template <int CIN, int COUT>
+__global__ __launch_bounds__(CT) void conv_bwd_data_kernel(const float* __restrict__ dout, const uint16_t* __restrict__ wbf,
+                                                           const float* __restrict__ act, float* __restrict__ din,
+                                                           int H, int W) {
+  __shared__ __align__(16) float wl[COUT * 9 * CIN];   // [co][tap][ci]
+  for (int q = threadIdx.x; q < COUT * 9 * CIN; q += CT) {
+    const int ci = q % CIN, r = q / CIN, tap = r % 9, co = r / 9;
+    wl[q] = bf2f(wbf[(co * CIN + ci) * 9 + tap]);
+  }
+  __syncthreads();
+  const int HW = H * W;
+  const int b = blockIdx.y;
+  const int p = blockIdx.x * CT + threadIdx.x;
+  if (p >= HW) return;
+  const int y = p / W, x = p % W;
+  const float* gb = dout + (size_t)b * COUT * HW;
+  float acc[CIN];
+#pragma unroll
+  for (int ci = 0; ci < CIN; ci++) acc[ci] = 0.0f;
+  for (int co = 0; co < COUT; co++) {
+    const float* gp = gb + (size_t)co * HW;
+    // not unrolled: unrolling lets the compiler hoist all 9 x CIN weight reads
+#pragma unroll 1
+    for (int tap = 0; tap < 9; tap++) {
+      const int yy = y - (tap / 3 - 1), xx = x - (tap % 3 - 1);
+      const float gv = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? gp[yy * W + xx] : 0.0f;
+      const float* wr = wl + (co * 9 + tap) * CIN;
+#pragma unroll
+      for (int ci = 0; ci < CIN; ci++) acc[ci] += wr[ci] * gv;
+    }
+  }
+  float* db = din + (size_t)b * CIN * HW + p;
+  const float* ab = act ? act + (size_t)b * CIN * HW + p : nullptr;
+#pragma unroll
+  for (int ci = 0; ci < CIN; ci++) {
+    float g = acc[ci];
+    if (ab && !(ab[(size_t)ci * HW] > 0.0f)) g = 0.0f;
+    db[(size_t)ci * HW] = g;
+  }
+}
+
+// One block per (ci, co); ci == CIN gives the bias gradient.  Fixed summation
+// order (thread-strided pixels, then a fixed tree), so results are reproducible.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(CT) void conv_bwd_weight_kernel(const float* __restrict__ dout, const float* __restrict__ in,
+                                                             const int* __restrict__ idx, float* __restrict__ dW,
+                                                             float* __restrict__ dbias, int B, int H, int W) {
+  __shared__ float red[9][CT / 64];
+  const int ci = blockIdx.x, co = blockIdx.y, tid = threadIdx.x;
+  const int HW = H * W;
+  float s[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) s[k] = 0.0f;
+  for (int b = 0; b < B; b++) {
+    const float* gp = dout + ((size_t)b * COUT + co) * HW;
+    if (ci == CIN) {
+      for (int p = tid; p < HW; p += CT) s[0] += gp[p];
+    } else {
+      const float* ip = in + ((size_t)(idx ? idx[b] : b) * CIN + ci) * HW;
+      for (int p = tid; p < HW; p += CT) {
+        const int y = p / W, x = p % W;
+        const float g = gp[p];
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+          const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+          if (yy >= 0 && yy < H && xx >= 0 && xx < W) s[tap] += g * ip[yy * W + xx];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    float v = s[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((tid & 63) == 0) red[k][tid >> 6] = v;
+  }
+  __syncthreads();
+  if (tid < 9) {
+    float v = 0.0f;
+    for (int w = 0; w < CT / 64; w++) v += red[tid][w];
+    if (ci == CIN) { if (tid == 0) dbias[co] = v; }
+    else dW[(co * CIN + ci) * 9 + tid] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void corr_loss_kernel(const float* __restrict__ out, const float* __restrict__ Y,
+                                                        const int* __restrict__ idx, float inv_ys, float* dout,
+                                                        double* part, int B, int HW) {
+  __shared__ double red[4];
+  const size_t n = (size_t)B * 2 * HW;
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double s = 0;
+  if (q < n) {
+    const int b = (int)(q / (2 * (size_t)HW));
+    const size_t r = q % (2 * (size_t)HW);
+    const float y = Y[(size_t)idx[b] * 2 * HW + r] * inv_ys;
+    const float d = out[q] - y;
+    dout[q] = d * (2.0f / (float)n);
+    s = (double)d * (double)d;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void sum_squares_kernel(const float* __restrict__ x, int n, double* part) {
+  __shared__ double red[4];
+  double s = 0;
+  for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) s += (double)x[q] * (double)x[q];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void adam_kernel(float* w, const float* __restrict__ g, float* m, float* v, uint16_t* wbf, int n, float lr,
+                            float b1, float b2, float eps, float c1, float c2) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const float gq = g[q];
+  const float mq = b1 * m[q] + (1.0f - b1) * gq;
+  const float vq = b2 * v[q] + (1.0f - b2) * gq * gq;
+  m[q] = mq;
+  v[q] = vq;
+  const float wq = w[q] - lr * (mq * c1) / (sqrtf(vq * c2) + eps);
+  w[q] = wq;
+  wbf[q] = f2bf(wq);
+}
+
+__global__ void round_bf16_kernel(const float* __restrict__ w, uint16_t* wbf, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) wbf[q] = f2bf(w[q]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define FIL_CORR_LAYER(l_, ...)                                                  \
+  switch (l_) {                                                                  \
+    case 0: { constexpr int CI = CORR_IN, CO = CORR_CH; __VA_ARGS__; } break;    \
+    case 1: { constexpr int CI = CORR_CH, CO = CORR_CH; __VA_ARGS__; } break;    \
+    case 2: { constexpr int CI = CORR_CH, CO = CORR_CH; __VA_ARGS__; } break;    \
+    case 3: { constexpr int CI = CORR_CH, CO = CORR_OUT; __VA_ARGS__; } break;   \
+    default: return hipErrorInvalidValue;                                        \
+  }
+static_assert(CORR_LAYERS == 4, "layer dispatch covers four layers");
+
+hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const uint16_t* wbf, const float* bias, float* out,
+                           int B, int H, int W, int relu, hipStream_t st) {
+  const dim3 grid((H * W + CT - 1) / CT, B);
+  FIL_CORR_LAYER(l, hipLaunchKernelGGL((conv_fwd_kernel<CI, CO>), grid, dim3(CT), 0, st, in, idx, wbf, bias, out, H, W, relu));
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_bwd_data(int l, const float* dout, const uint16_t* wbf, const float* act, float* din, int B,
+                                int H, int W, hipStream_t st) {
+  const dim3 grid((H * W + CT - 1) / CT, B);
+  FIL_CORR_LAYER(l, hipLaunchKernelGGL((conv_bwd_data_kernel<CI, CO>), grid, dim3(CT), 0, st, dout, wbf, act, din, H, W));
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_bwd_weight(int l, const float* dout, const float* in, const int* idx, float* dW, float* db,
+                                  int B, int H, int W, hipStream_t st) {
+  FIL_CORR_LAYER(l, hipLaunchKernelGGL((conv_bwd_weight_kernel<CI, CO>), dim3(CI + 1, CO), dim3(CT), 0, st, dout, in,
+                                       idx, dW, db, B, H, W));
+  return hipGetLastError();
+}
+
+hipError_t launch_corr_loss(const float* out, const float* Y, const int* idx, float inv_ys, float* dout, double* part,
+                            int B, int HW, int* nblk, hipStream_t st) {
+  const size_t n = (size_t)B * 2 * HW;
+  *nblk = (int)((n + 255) / 256);
+  hipLaunchKernelGGL(corr_loss_kernel, dim3(*nblk), dim3(256), 0, st, out, Y, idx, inv_ys, dout, part, B, HW);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_squares(const float* x, int n, double* part, int* nblk, hipStream_t st) {
+  *nblk = std::min(1024, (n + 255) / 256);
+  if (*nblk < 1) *nblk = 1;
+  hipLaunchKernelGGL(sum_squares_kernel, dim3(*nblk), dim3(256), 0, st, x, n, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wbf, int n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, hipStream_t st) {
+  hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, st, w, g, m, v, wbf, n, lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_round_bf16(const float* w, uint16_t* wbf, int n, hipStream_t st) {
+  hipLaunchKernelGGL(round_bf16_kernel, dim3((n + 255) / 256), dim3(256), 0, st, w, wbf, n);
+  return hipGetLastError();
+}
+
+}  // namespace fil

@@ -37,6 +37,61 @@ hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* rdr, d
 template <typename R>
 hipError_t launch_rowsum_abs2(const cx<R>* S, int Nr, int Nh, double scale, double* out, hipStream_t st);
 
+// Learned coarse-step corrector (fil_row.hip: features / apply; fil_corr.hip: network).
+// Window rows [0, nr), t samples [t0, t0 + nt); pooled grid H = nr/pr by W = nt/pt.
+struct CorrGeom {
+  int nr, t0, nt, pr, pt, H, W;
+  const cx<float>* carrier;  // [Nt] exp(i w0 t_j) (rounded from FP64): envelope A = E carrier
+  float dt;        // t step (plasma scan)
+  float inv_as;    // 1 / field scale
+  float inv_rs;    // 1 / density scale
+};
+// Features of the current state, X[4][H][W]: Re A, Im A (/a_s), |E|^2 (/a_s^2),
+// rho (/rho_s), each averaged over its pr x pt block.  Frow: scratch [nr][4][W].
+hipError_t launch_corr_features(const cx<float>* S, const cx<float>* tw, const PlasmaParams<float>& pp, int plasma,
+                                const CorrGeom& g, int N, float* Frow, float* X, hipStream_t st);
+// S rows [0, nr): E += scale * D(r, t) exp(-i w0 t) with D[2][H][W] bilinearly
+// interpolated from the pooled-block centres, in the t window.
+hipError_t launch_corr_apply(cx<float>* S, const cx<float>* tw, const CorrGeom& g, int N, const float* D, float scale,
+                             hipStream_t st);
+
+// Network of the corrector: 4 layers of 3x3 convolutions (zero padding),
+// 4 -> C -> C -> C -> 2 channels, ReLU between layers, BF16 weights, FP32
+// accumulation.  Activations [B][ch][H][W] FP32.
+constexpr int CORR_CH = 32;
+constexpr int CORR_IN = 4;
+constexpr int CORR_OUT = 2;
+constexpr int CORR_LAYERS = 4;
+__host__ __device__ constexpr int corr_cin(int l) { return l == 0 ? CORR_IN : CORR_CH; }
+__host__ __device__ constexpr int corr_cout(int l) { return l == CORR_LAYERS - 1 ? CORR_OUT : CORR_CH; }
+// offset of layer l's weights [cout][cin][9] in the flat parameter vector; its bias [cout] follows
+__host__ __device__ constexpr int corr_woff(int l) {
+  int o = 0;
+  for (int k = 0; k < l; k++) o += corr_cout(k) * corr_cin(k) * 9 + corr_cout(k);
+  return o;
+}
+__host__ __device__ constexpr int corr_nparam() { return corr_woff(CORR_LAYERS); }
+
+// forward of layer l: out = act(conv(in) + b); in batch element b is taken from
+// in + (idx ? idx[b] : b) * cin*H*W.
+hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const uint16_t* wbf, const float* bias, float* out,
+                           int B, int H, int W, int relu, hipStream_t st);
+// din = conv^T(dout) (* [act > 0] when act != null)
+hipError_t launch_conv_bwd_data(int l, const float* dout, const uint16_t* wbf, const float* act, float* din, int B,
+                                int H, int W, hipStream_t st);
+// dW[cout][cin][9], db[cout] summed over the batch (deterministic order)
+hipError_t launch_conv_bwd_weight(int l, const float* dout, const float* in, const int* idx, float* dW, float* db,
+                                  int B, int H, int W, hipStream_t st);
+// dout = 2 (out - Y[idx[b]] * inv_ys) / (B*2*H*W); part[blk] = partial sums of squares
+hipError_t launch_corr_loss(const float* out, const float* Y, const int* idx, float inv_ys, float* dout, double* part,
+                            int B, int HW, int* nblk, hipStream_t st);
+// Adam step on the flat parameter vector; refreshes the BF16 copy of the weights.
+hipError_t launch_adam(float* w, const float* g, float* m, float* v, uint16_t* wbf, int n, float lr, float b1,
+                       float b2, float eps, float c1, float c2, hipStream_t st);
+hipError_t launch_round_bf16(const float* w, uint16_t* wbf, int n, hipStream_t st);
+// part[blk] = partial sums of x^2 over x[0, n)
+hipError_t launch_sum_squares(const float* x, int n, double* part, int* nblk, hipStream_t st);
+
 // Hankel-transform phase (fil_dht.hip)
 // Propagator P[k][w] = exp(i KZ dz) Kguard Wguard computed in FP64.
 template <typename R>

@@ -554,6 +554,101 @@ __global__ void colsum_kernel(const float* __restrict__ in, int Nr, int ncol, co
 }
 
 // ---------------------------------------------------------------------------
+// Learned corrector, field side.  Window rows i < g.nr.
+// Features: time-domain field of the row, its plasma density (FP32 scan),
+// then block averages over pt samples of A = E exp(i w0 t), |E|^2 and rho.
+// The carrier exp(i w0 t_j) is a table rounded from FP64: a phase computed in
+// FP32 from t_j would be off by ~1e-5 rad at |t| ~ 100 carrier periods.
+// ---------------------------------------------------------------------------
+template <int N, int NTH>
+__global__ __launch_bounds__(NTH) void corr_features_kernel(const cx<float>* __restrict__ S, const cx<float>* tw,
+                                                            PlasmaParams<float> pp, int plasma, CorrGeom g,
+                                                            float* Frow) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<float>* Z = reinterpret_cast<cx<float>*>(smem);
+  float* rho = reinterpret_cast<float*>(Z + lds_size(N));
+  float* kd = rho + N;
+  float* red = kd + N;
+  constexpr int NH = N / 2, SP = NH / NTH;
+  const int tid = threadIdx.x, i = blockIdx.x;
+  cx<float> v[SP];
+#pragma unroll
+  for (int m = 0; m < SP; m++) v[m] = S[(size_t)i * NH + tid + m * NTH];
+  store_padded<float, N, NTH>(Z, v);
+  fft_lds<float, N, NTH, -1>(Z, tw);
+  if (plasma) plasma_row<float, float, N, NTH>(Z, rho, kd, pp, g.dt, red);
+  float* out = Frow + (size_t)i * 4 * g.W;
+  const float inv = 1.0f / (float)g.pt;
+  for (int b = tid; b < g.W; b += NTH) {
+    float sr = 0, si = 0, sI = 0, sn = 0;
+    for (int s = 0; s < g.pt; s++) {
+      const int j = g.t0 + b * g.pt + s;
+      const cx<float> e = Z[lds_index(j)];
+      const cx<float> a = e * g.carrier[j];
+      sr += a.x;
+      si += a.y;
+      sI += norm2(e);
+      if (plasma) sn += rho[j];
+    }
+    out[b] = sr * inv * g.inv_as;
+    out[g.W + b] = si * inv * g.inv_as;
+    out[2 * g.W + b] = sI * inv * g.inv_as * g.inv_as;
+    out[3 * g.W + b] = sn * inv * g.inv_rs;
+  }
+}
+
+// X[c][h][b] = mean over the pr rows of block h of Frow[row][c][b]
+__global__ void corr_pool_rows_kernel(const float* __restrict__ Frow, CorrGeom g, float* X) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int HW = g.H * g.W;
+  if (q >= 4 * HW) return;
+  const int c = q / HW, rem = q % HW, h = rem / g.W, b = rem % g.W;
+  float s = 0;
+  for (int k = 0; k < g.pr; k++) s += Frow[((size_t)(h * g.pr + k) * 4 + c) * g.W + b];
+  X[q] = s / (float)g.pr;
+}
+
+// bilinear interpolation of one pooled plane at pooled coordinates (y, x), clamped
+__device__ __forceinline__ float bilinear(const float* P, int H, int W, float y, float x) {
+  y = fminf(fmaxf(y, 0.0f), (float)(H - 1));
+  x = fminf(fmaxf(x, 0.0f), (float)(W - 1));
+  const int y0 = (int)y, x0 = (int)x;
+  const int y1 = min(y0 + 1, H - 1), x1 = min(x0 + 1, W - 1);
+  const float fy = y - (float)y0, fx = x - (float)x0;
+  const float a = P[y0 * W + x0] + fx * (P[y0 * W + x1] - P[y0 * W + x0]);
+  const float b = P[y1 * W + x0] + fx * (P[y1 * W + x1] - P[y1 * W + x0]);
+  return a + fy * (b - a);
+}
+
+template <int N, int NTH>
+__global__ __launch_bounds__(NTH) void corr_apply_kernel(cx<float>* S, const cx<float>* tw, CorrGeom g,
+                                                         const float* __restrict__ D, float scale) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<float>* Z = reinterpret_cast<cx<float>*>(smem);
+  constexpr int NH = N / 2, SP = NH / NTH;
+  const int tid = threadIdx.x, i = blockIdx.x;
+  cx<float> v[SP];
+#pragma unroll
+  for (int m = 0; m < SP; m++) v[m] = S[(size_t)i * NH + tid + m * NTH];
+  store_padded<float, N, NTH>(Z, v);
+  fft_lds<float, N, NTH, -1>(Z, tw);
+  // pooled block h covers rows [h pr, (h+1) pr); its centre is at row (h + 1/2) pr - 1/2
+  const float y = ((float)i + 0.5f) / (float)g.pr - 0.5f;
+  const int HW = g.H * g.W;
+  for (int j = g.t0 + tid; j < g.t0 + g.nt; j += NTH) {
+    const float x = ((float)(j - g.t0) + 0.5f) / (float)g.pt - 0.5f;
+    const cx<float> d = mk(scale * bilinear(D, g.H, g.W, y, x), scale * bilinear(D + HW, g.H, g.W, y, x));
+    // (dre + i dim) exp(-i w0 t)
+    Z[lds_index(j)] = Z[lds_index(j)] + d * conj(g.carrier[j]);
+  }
+  __syncthreads();
+  fft_lds<float, N, NTH, +1>(Z, tw);
+  const float inv = 1.0f / (float)N;
+#pragma unroll
+  for (int m = 0; m < SP; m++) S[(size_t)i * NH + tid + m * NTH] = inv * Z[lds_index(tid + m * NTH)];
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 // one radix-FIL_FFT_MAXRAD butterfly per thread per FFT pass
@@ -642,6 +737,35 @@ hipError_t launch_analyze_rows(cx<R>* S, const cx<R>* tw, const R* Rg, const R* 
 hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* rdr, double* out, hipStream_t st) {
   hipLaunchKernelGGL(colsum_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, in, Nr, ncol, rdr, out);
   return hipGetLastError();
+}
+
+hipError_t launch_corr_features(const cx<float>* S, const cx<float>* tw, const PlasmaParams<float>& pp, int plasma,
+                                const CorrGeom& g, int N, float* Frow, float* X, hipStream_t st) {
+  FIL_DISPATCH_N(N, {
+    constexpr int NTH = threads_for<NN>();
+    size_t sm = row_smem<float, float>(NN, NTH);
+    auto k = corr_features_kernel<NN, NTH>;
+    { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
+    hipLaunchKernelGGL(k, dim3(g.nr), dim3(NTH), sm, st, S, tw, pp, plasma, g, Frow);
+    hipError_t e1 = hipGetLastError();
+    if (e1 != hipSuccess) return e1;
+  });
+  const int n = 4 * g.H * g.W;
+  hipLaunchKernelGGL(corr_pool_rows_kernel, dim3((n + 255) / 256), dim3(256), 0, st, Frow, g, X);
+  return hipGetLastError();
+}
+
+hipError_t launch_corr_apply(cx<float>* S, const cx<float>* tw, const CorrGeom& g, int N, const float* D, float scale,
+                             hipStream_t st) {
+  FIL_DISPATCH_N(N, {
+    constexpr int NTH = threads_for<NN>();
+    size_t sm = (size_t)lds_size(NN) * sizeof(cx<float>);
+    auto k = corr_apply_kernel<NN, NTH>;
+    { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
+    hipLaunchKernelGGL(k, dim3(g.nr), dim3(NTH), sm, st, S, tw, g, D, scale);
+    return hipGetLastError();
+  });
+  return hipSuccess;
 }
 
 template hipError_t launch_row<float>(const RowArgs<float>&, int, int, bool, hipStream_t);

@@ -51,9 +51,19 @@ class FilDiag(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class FilCorrDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("nr", "t0", "nt", "pool_r", "pool_t", "capacity", "batch_max")] + \
+               [(n, ctypes.c_double) for n in ("w0", "tmin", "dt", "a_scale", "rho_scale")]
+
+
+CORR_NPARAM = 20258
+
 EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_upload_field", "fil_download_field",
            "fil_zstep", "fil_analyze", "fil_download_plasma", "fil_synchronize", "fil_destroy",
-           "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_fluence")
+           "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_fluence",
+           "fil_corrector_init", "fil_corrector_params", "fil_corrector_scale", "fil_corrector_features",
+           "fil_corrector_store_pair", "fil_corrector_train", "fil_corrector_eval", "fil_corrector_forward",
+           "fil_corrector_apply")
 
 _lib = None
 
@@ -85,6 +95,17 @@ def load():
     L.fil_state_save.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.fil_state_restore.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.fil_fluence.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.fil_corrector_init.argtypes = [ctypes.c_void_p, ctypes.POINTER(FilCorrDesc), ctypes.c_uint64]
+    L.fil_corrector_params.argtypes = [ctypes.c_void_p, _fp, ctypes.c_int]
+    L.fil_corrector_scale.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+    L.fil_corrector_features.argtypes = [ctypes.c_void_p, _fp]
+    L.fil_corrector_store_pair.argtypes = [ctypes.c_void_p, _fp, _fp]
+    L.fil_corrector_train.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
+                                      _dp]
+    L.fil_corrector_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _fp]
+    L.fil_corrector_forward.argtypes = [ctypes.c_void_p, _fp, _fp]
+    L.fil_corrector_apply.argtypes = [ctypes.c_void_p]
     for name in EXPORTS:
         if name != "fil_last_error":
             getattr(L, name).restype = ctypes.c_int
@@ -99,3 +120,8 @@ def check(rc):
 
 def ptr(a):
     return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def fptr(a):
+    """float32 array -> float*"""
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if a is not None else None

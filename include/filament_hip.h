@@ -171,6 +171,61 @@ int fil_state_restore(fil_handle* h, int slot);
 int fil_fluence(fil_handle* h, int slot, double* Fr);
 int fil_destroy(fil_handle* h);
 
+/* ---------------------------------------------------------------------------
+ * Learned coarse-step corrector.  No counterpart in the reference: it serves
+ * a driver that advances by coarse steps (a multiple of the reference's dz)
+ * and corrects each coarse result towards what the fine steps would give.
+ *
+ * Features of a state (FP32 handles only): the field of rows [0, nr) in the
+ * time window [t0, t0 + nt), as the envelope A = E exp(i w0 t), averaged over
+ * pool_r x pool_t blocks into an H x W grid (H = nr/pool_r, W = nt/pool_t):
+ *   X[0] = Re A / a_scale, X[1] = Im A / a_scale, X[2] = |E|^2 / a_scale^2,
+ *   X[3] = rho / rho_scale  (plasma density of that field, zero without plasma)
+ * X is [4][H][W] floats, W fastest.  The network (4 layers of 3x3
+ * convolutions, 4 -> 32 -> 32 -> 32 -> 2 channels, ReLU between, BF16 weights
+ * with FP32 accumulation) maps X to Y [2][H][W]; the correction added to the
+ * field is a_scale * y_scale * (Y[0] + i Y[1]) exp(-i w0 t), bilinearly
+ * interpolated from the block centres.  Training targets are
+ * (X_fine[0:2] - X_coarse[0:2]) for the same step, in feature units.
+ * Parameters: a flat FP32 vector of FIL_CORR_NPARAM values, layer by layer
+ * w[cout][cin][3][3] then b[cout].
+ * ------------------------------------------------------------------------- */
+#define FIL_CORR_NPARAM 20258
+
+typedef struct fil_corr_desc {
+  int32_t nr;              /* window rows [0, nr) */
+  int32_t t0, nt;          /* window t samples [t0, t0 + nt) */
+  int32_t pool_r, pool_t;  /* block size; nr % pool_r == 0, nt % pool_t == 0 */
+  int32_t capacity;        /* training pairs held on the device (ring) */
+  int32_t batch_max;       /* largest training batch */
+  double w0;               /* carrier frequency, radians per t unit */
+  double tmin, dt;         /* t_j = tmin + j dt (t units) */
+  double a_scale;          /* field scale of the features */
+  double rho_scale;        /* density scale of the features */
+} fil_corr_desc;
+
+/* Creates (or re-creates) the handle's corrector with seeded He-initialised
+ * weights and a zero last layer (the initial correction is zero). */
+int fil_corrector_init(fil_handle* h, const fil_corr_desc* c, uint64_t seed);
+/* Get (set = 0) or set (set != 0) the FP32 parameters; setting resets Adam. */
+int fil_corrector_params(fil_handle* h, float* w, int set);
+/* Get or set the target scale y_scale (0 = not yet trained). */
+int fil_corrector_scale(fil_handle* h, double* y_scale, int set);
+/* Features of the current state; X (may be NULL) receives [4][H][W].  Blocks. */
+int fil_corrector_features(fil_handle* h, float* X);
+/* Stores one training pair (X [4][H][W], target T [2][H][W]). */
+int fil_corrector_store_pair(fil_handle* h, const float* X, const float* T);
+/* Adam on the stored pairs: iters steps of random batches of `batch` pairs.
+ * y_scale is set to the RMS of the stored targets if it is 0.  *loss: the
+ * mean-square error (in y_scale units) of the last batch.  Blocks. */
+int fil_corrector_train(fil_handle* h, int iters, int batch, double lr, uint64_t seed, double* loss);
+/* Loss and (grad != NULL) its gradient over stored pairs [first, first+count). Blocks. */
+int fil_corrector_eval(fil_handle* h, int first, int count, double* loss, float* grad);
+/* Network output Y [2][H][W] for features X [4][H][W] from the host.  Blocks. */
+int fil_corrector_forward(fil_handle* h, const float* X, float* Y);
+/* Adds the network's correction to the current state.  Asynchronous. */
+int fil_corrector_apply(fil_handle* h);
+
 /* Per-phase device timing with HIP events on the handle's stream (the
  * reference's TimerOutputs sections "plasma/nonlinearity" and "linear",
  * Models.jl:84-119).  names: "row" (plasma + t-FFTs + RK), "dht1", "dht2".

@@ -43,6 +43,14 @@ def test_struct_layouts_match_header(pkg):
     body = text[text.index("typedef struct fil_tables"):text.index("} fil_tables;")]
     names = re.findall(r"\b([a-zA-Z0-9_]+)(?:\[FIL_MAXCOMP\])?;", body)
     assert names == [n for n, _ in pkg.lib.FilTables._fields_]
+    body = text[text.index("typedef struct fil_corr_desc"):text.index("} fil_corr_desc;")]
+    names = [f.strip() for group in re.findall(r"(?:int32_t|double)\s+([a-zA-Z0-9_, ]+);", body)
+             for f in group.split(",")]
+    assert names == [n for n, _ in pkg.lib.FilCorrDesc._fields_]
+    assert ctypes.sizeof(pkg.lib.FilCorrDesc) == 7 * 4 + 4 + 5 * 8   # 4 bytes of padding before w0
+    nparam = int(re.search(r"#define FIL_CORR_NPARAM (\d+)", text).group(1))
+    # 4 -> 32 -> 32 -> 32 -> 2 channels of 3x3 weights plus biases
+    assert nparam == pkg.lib.CORR_NPARAM == (32 * 4 * 9 + 32) + 2 * (32 * 32 * 9 + 32) + (2 * 32 * 9 + 2)
 
 
 def test_errors_are_reported_without_a_device(pkg):
@@ -54,3 +62,7 @@ def test_errors_are_reported_without_a_device(pkg):
     d.abi_version = 99
     assert L.fil_create(ctypes.byref(d), ctypes.byref(h)) == 1
     assert b"ABI" in L.fil_last_error()
+    c = pkg.lib.FilCorrDesc()
+    assert L.fil_corrector_init(None, ctypes.byref(c), 0) == 1
+    assert L.fil_corrector_apply(None) == 1 and b"null" in L.fil_last_error()
+    assert L.fil_state_save(None, 0) == 1
