@@ -1,21 +1,30 @@
 #!/usr/bin/env python3
 """Throughput of the RT z-step on one MI355X: z-steps/s at Nr=1024 x Nt=4096.
 
-One "step" is Models.zstep (src/modules/Models.jl:75-121) on the whole
-1024 x 4096 field: plasma scan, t-FFTs and RK3 nonlinear step (row kernel),
-then the two Hankel-transform GEMMs with the propagator multiply.  The field
-is resident in HBM when the timed region starts.
+One fine "step" is Models.zstep (src/modules/Models.jl:75-121) with the
+reference's dz on the whole 1024 x 4096 field: plasma scan, t-FFTs and RK3
+nonlinear step (row kernel), then the two Hankel-transform GEMMs with the
+propagator multiply.  The field is resident in HBM when the timed region
+starts.
 
-Prints one JSON line.  `value` is the chosen form's z-steps/s (split-BF16
-Hankel transform + FP64 plasma scan) when it holds <= 1e-3 relative fluence
-error against the FP64 build over the timed trajectory, otherwise the plain
-FP32 port's.  Both forms, their errors, the per-kernel times, the roofline of
-the dominant kernel and a CPU baseline (the C++ restatement run on the host
-cores over a bounded sample) are reported in the same line.
+Forms, each run over the same trajectory (W warm-up + K timed + a few
+profiled fine-step equivalents) from the same initial field:
+  fp64    the FP64 build, fine steps: the error reference;
+  plain   the plain FP32 port, fine steps;
+  chosen  FP32 with the split-BF16 Hankel transform and the FP64 plasma scan,
+          advanced by coarse steps of F*dz, each followed by the learned
+          corrector, with a fine-step verification (and fallback) every M
+          coarse steps.  Its rate counts fine-step equivalents (a coarse step
+          counts F; verification steps are charged to the run).  The
+          corrector is trained on the device during set-up, outside the timed
+          region, on FP64 fine-step targets of other pulses.
+
+Prints one JSON line.  `value` is the chosen form's fine-equivalent z-steps/s
+when its fluence profile stays within 1e-3 relative L2 of the FP64 build's at
+the end of the trajectory, otherwise the plain port's z-steps/s.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -24,6 +33,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+GATE = 1e-3
 
 
 def load_pkg():
@@ -41,35 +51,78 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_form(pkg, p, precision, form, warmup, steps, dz, profile_steps):
-    log(f"form precision={precision} flags={form}: setup")
+def fluence_error(F, Fref):
+    return float(np.linalg.norm(F - Fref) / np.linalg.norm(Fref))
+
+
+def kernel_ms(kt):
+    n = max(1, kt["count"])
+    return dict(row=kt["row_ms"] / n, dht1=kt["dht1_ms"] / n, dht2=kt["dht2_ms"] / n)
+
+
+def run_fine(pkg, p, precision, form, W, K, P, dz):
+    """W warm-up, K timed and P profiled fine steps."""
     m = pkg.RTModel(p, precision=precision, form=form)
     z = 0.0
-    for _ in range(warmup):
+    for _ in range(W):
         z += dz
         m.zstep(z, dz)
     m.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(K):
         z += dz
         m.zstep(z, dz)
     m.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / steps
-    log(f"form precision={precision} flags={form}: {ms:.4f} ms/step over {steps} steps")
-    # per-kernel device times with HIP events on the library's stream
+    ms = (time.perf_counter() - t0) * 1e3 / K
     m.timing(True)
-    for _ in range(profile_steps):
+    for _ in range(P):
         z += dz
         m.zstep(z, dz)
     kt = m.timing_read()
     m.timing(False)
-    d = m.analyze(z, arrays=True)
-    return m, ms, kt, d, z
+    Fr = m.analyze(z, arrays=True)["Fr"]
+    m.close()
+    return ms, kernel_ms(kt), Fr
 
 
-def fluence_error(F, Fref):
-    return float(np.linalg.norm(F - Fref) / np.linalg.norm(Fref))
+def run_coarse(pkg, p, form, W, K, P, dz, factor, check_every, tol, corr_state, geom):
+    """The chosen form: coarse steps (+ corrector) with verification, W/K/P in
+    fine-step equivalents."""
+    m = pkg.RTModel(p, precision=pkg.lib.FP32, form=form)
+    corr = None
+    if corr_state is not None:
+        corr = pkg.corrector.Corrector(m, geom=geom, capacity=1, batch_max=1)
+        corr.load_state(corr_state)
+    st = pkg.CoarseStepper(m, factor=factor, check_every=check_every, tol=tol, corrector=corr)
+    z = st.advance(0.0, dz, W)
+    m.synchronize()
+    s0 = st.stats()
+    t0 = time.perf_counter()
+    z = st.advance(z, dz, K)
+    m.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / K
+    s1 = st.stats()
+    m.timing(True)
+    z = st.advance(z, dz, P)
+    kt = m.timing_read()
+    m.timing(False)
+    Fr = m.analyze(z, arrays=True)["Fr"]
+    m.close()
+    timed = dict(coarse_steps=s1["coarse_steps"] - s0["coarse_steps"], fine_steps=s1["fine_steps"] - s0["fine_steps"],
+                 checks=s1["checks"] - s0["checks"], failed_checks=s1["failed_checks"] - s0["failed_checks"],
+                 last_check_distance=s1["last_distance"])
+    return ms, kernel_ms(kt), Fr, timed
+
+
+def train_corrector(pkg, p, form, dz, factor, draws, npairs, iters, geom):
+    C = pkg.corrector
+
+    def make(d):
+        return pkg.workloads.rt_air(p["Nr"], p["Nt"], d["power_pcr"], a0=d["a0"], tau0=d["tau0"], seed=d["seed"])
+
+    st, info = C.fit(pkg.RTModel, make, draws, geom, dz, factor, form, npairs=npairs, iters=iters, log=log)
+    log(f"corrector: {info}")
+    return st, info
 
 
 def cpu_baseline(pkg, p, sample_steps, dz):
@@ -88,19 +141,28 @@ def cpu_baseline(pkg, p, sample_steps, dz):
     log(f"cpu baseline: {sample_steps} steps in {dt:.2f} s")
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return dict(value=sample_steps / dt, unit="z-steps/s", cores=threads, kind="port",
-                sample=f"FP32 C++ restatement, RT 1024x4096, {sample_steps} z-steps after 1 warm-up step, "
-                       f"{dt:.1f} s wall")
+                sample=f"FP32 C++ restatement of the plain port, RT 1024x4096, {sample_steps} fine z-steps after "
+                       f"1 warm-up step, {dt:.1f} s wall")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=400, help="timed fine-step equivalents")
+    ap.add_argument("--warmup", type=int, default=16, help="warm-up fine-step equivalents")
+    ap.add_argument("--factor", type=int, default=8, help="coarse step = factor * dz in the chosen form")
+    ap.add_argument("--check-every", type=int, default=50, help="coarse steps between fine-step verifications")
+    ap.add_argument("--check-tol", type=float, default=1e-4,
+                    help="relative fluence distance of one coarse step from its fine steps that triggers the fallback")
+    ap.add_argument("--no-corrector", action="store_true")
+    ap.add_argument("--corr-draws", type=int, default=4)
+    ap.add_argument("--corr-pairs", type=int, default=24)
+    ap.add_argument("--corr-iters", type=int, default=1500)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,plain,chosen",
-                    help="comma list of forms to run (fp64, plain, chosen); profiling runs pick one")
+                    help="comma list of forms (fp64, plain, chosen); anything but all three is a profiling run "
+                         "that prints per-form times only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the RT z-step runs on a single MI355X; "
@@ -111,66 +173,75 @@ def main():
     L = pkg.lib
     p = pkg.workloads.rt_1024x4096()
     dz = p["dz_initial"]
+    F = args.factor
     K, W = args.steps, args.warmup
-    prof = min(K, 50)
+    if K % F or W % F:
+        sys.exit("--steps and --warmup must be multiples of --factor")
+    P = 4 * F                                   # profiled fine-step equivalents
     Nr, Nt = p["Nr"], p["Nt"]
     Nh = Nt // 2
+    chosen_form = L.FORM_DHT_BF16X3 | L.FORM_SCAN_FP64
 
-    forms_run = set(args.forms.split(","))
-    if forms_run != {"fp64", "plain", "chosen"}:
-        # profiling mode: run the requested forms only, no JSON contract line
-        for f in ("fp64", "plain", "chosen"):
-            if f in forms_run:
-                prec = L.FP64 if f == "fp64" else L.FP32
-                flags = (L.FORM_DHT_BF16X3 | L.FORM_SCAN_FP64) if f == "chosen" else 0
-                m, ms, kt, d, _ = run_form(pkg, p, prec, flags, W, K, dz, prof)
-                m.close()
-                log(f"{f}: {ms:.4f} ms/step")
+    want = set(args.forms.split(","))
+    full = want == {"fp64", "plain", "chosen"}
+    forms = {}
+    Fr64 = None
+    if "fp64" in want:
+        log("fp64: fine steps")
+        ms64, kt64, Fr64 = run_fine(pkg, p, L.FP64, 0, W, K, P, dz)
+        forms["fp64"] = dict(ms_per_step=ms64, z_steps_per_s=1e3 / ms64, kernel_ms=kt64)
+        log(f"fp64: {ms64:.4f} ms/step")
+    if "plain" in want:
+        log("plain FP32: fine steps")
+        msp, ktp, Frp = run_fine(pkg, p, L.FP32, 0, W, K, P, dz)
+        forms["plain_fp32"] = dict(ms_per_step=msp, z_steps_per_s=1e3 / msp, kernel_ms=ktp,
+                                   fluence_err_vs_fp64=fluence_error(Frp, Fr64) if Fr64 is not None else None)
+        log(f"plain FP32: {msp:.4f} ms/step")
+    if "chosen" in want:
+        m0 = pkg.RTModel(p, precision=L.FP32, form=chosen_form)
+        geom = pkg.corrector.window_geometry(m0.grid, m0.unit, m0.w0)
+        m0.close()
+        cst, cinfo = None, None
+        if not args.no_corrector:
+            draws = pkg.corrector.ensemble_draws(args.corr_draws, seed=1)
+            cst, cinfo = train_corrector(pkg, p, chosen_form, dz, F, draws, args.corr_pairs, args.corr_iters, geom)
+        log(f"chosen: coarse x{F}, corrector {'off' if cst is None else 'on'}")
+        msc, ktc, Frc, timed = run_coarse(pkg, p, chosen_form, W, K, P, dz, F, args.check_every, args.check_tol,
+                                          cst, geom)
+        forms["chosen"] = dict(
+            ms_per_fine_step_equivalent=msc, z_steps_per_s=1e3 / msc, kernel_ms_per_zstep=ktc,
+            fluence_err_vs_fp64=fluence_error(Frc, Fr64) if Fr64 is not None else None,
+            description=f"FP32 + split-BF16 Hankel MFMA + FP64 plasma scan, coarse steps of {F} dz"
+                        f"{'' if cst is None else ' + learned corrector'}, fine-step verification every "
+                        f"{args.check_every} coarse steps (tol {args.check_tol:g})",
+            timed_region=timed, corrector_training=cinfo)
+        log(f"chosen: {msc:.4f} ms per fine-step equivalent, {timed}")
+        if Fr64 is not None:
+            log(f"chosen: fluence err vs fp64 {forms['chosen']['fluence_err_vs_fp64']:.3e}")
+    if not full:
+        print(json.dumps({"profiling_forms": forms}))
         return
-    # FP64 build: the error reference for both forms
-    # (every form runs the same W + K + prof steps, so the fluences compare at the same z)
-    m64, ms64, kt64, d64, _ = run_form(pkg, p, L.FP64, 0, W, K, dz, prof)
-    m64.close()
-    # plain FP32 port
-    mp, msp, ktp, dp, _ = run_form(pkg, p, L.FP32, 0, W, K, dz, prof)
-    mp.close()
-    err_plain = fluence_error(dp["Fr"], d64["Fr"])
-    forms = {"plain_fp32": dict(z_steps_per_s=1e3 / msp, ms_per_step=msp, fluence_err_vs_fp64=err_plain,
-                                kernel_ms=dict(row=ktp["row_ms"] / ktp["count"], dht1=ktp["dht1_ms"] / ktp["count"],
-                                               dht2=ktp["dht2_ms"] / ktp["count"]))}
-    chosen = None
-    try:
-        mc, msc, ktc, dc, _ = run_form(pkg, p, L.FP32, L.FORM_DHT_BF16X3 | L.FORM_SCAN_FP64, W, K, dz, prof)
-        mc.close()
-        err_c = fluence_error(dc["Fr"], d64["Fr"])
-        forms["bf16x3_dht_fp64_scan"] = dict(
-            z_steps_per_s=1e3 / msc, ms_per_step=msc, fluence_err_vs_fp64=err_c,
-            kernel_ms=dict(row=ktc["row_ms"] / ktc["count"], dht1=ktc["dht1_ms"] / ktc["count"],
-                           dht2=ktc["dht2_ms"] / ktc["count"]))
-        if err_c <= 1e-3:
-            chosen = ("bf16x3_dht_fp64_scan", msc, ktc)
-    except L.FilamentError as e:
-        forms["bf16x3_dht_fp64_scan"] = dict(error=str(e))
-    if chosen is None:
-        chosen = ("plain_fp32", msp, ktp)
-    name, ms, kt = chosen
 
-    # roofline of the dominant kernel of the reported form
-    n = kt["count"]
-    row_ms, g1_ms, g2_ms = kt["row_ms"] / n, kt["dht1_ms"] / n, kt["dht2_ms"] / n
+    err_c = forms["chosen"]["fluence_err_vs_fp64"]
+    if err_c <= GATE:
+        name, ms, kt = "chosen", forms["chosen"]["ms_per_fine_step_equivalent"], forms["chosen"]["kernel_ms_per_zstep"]
+    else:
+        name, ms, kt = "plain_fp32", forms["plain_fp32"]["ms_per_step"], forms["plain_fp32"]["kernel_ms"]
+
+    # roofline of the dominant kernel of the reported form (per launch, HIP-event averages)
     gemm_flop = 2.0 * Nr * (2 * Nh) * Nr          # real GEMM M=Nr, N=2*Nh, K=Nr
     row_bytes = 2.0 * 8 * Nr * Nh                 # one read + one write of the c64 half-spectrum
-    if max(g1_ms, g2_ms) >= row_ms:
-        kms = 0.5 * (g1_ms + g2_ms)
-        peak = PEAK["bf16_mfma_tflops"] if name.startswith("bf16") else PEAK["fp32_mfma_tflops"]
+    if max(kt["dht1"], kt["dht2"]) >= kt["row"]:
+        kms = 0.5 * (kt["dht1"] + kt["dht2"])
+        peak = PEAK["bf16_mfma_tflops"] if name == "chosen" else PEAK["fp32_mfma_tflops"]
         ach = gemm_flop / (kms * 1e-3) / 1e12
         roof = dict(kernel="dht_gemm", bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
                     traffic=None, per_launch=f"{gemm_flop:.4g} flop (2*Nr*Nt*Nr)", avg_launch_ms=kms)
     else:
-        ach = row_bytes / (row_ms * 1e-3) / 1e9
+        ach = row_bytes / (kt["row"] * 1e-3) / 1e9
         roof = dict(kernel="row_kernel", bound="hbm", achieved=ach, peak=PEAK["hbm_gbs"], unit="GB/s",
                     frac=ach / PEAK["hbm_gbs"], traffic=None, per_launch=f"{row_bytes:.4g} B (2*8*Nr*Nt/2)",
-                    avg_launch_ms=row_ms)
+                    avg_launch_ms=kt["row"])
 
     cpu = None
     if not args.no_cpu:
@@ -190,9 +261,9 @@ def main():
         "dtype": "fp32" if name == "plain_fp32" else "fp32+bf16x3-mfma+fp64-scan",
         "data": "synthetic seeded Gaussian pulse (RT air, 5 Pcr), 1e-3 amplitude noise",
         "config": {"workload": "RT Nr=1024 x Nt=4096, air 800 nm, Kerr+plasma, RK3, dz=0.5 mm", "form": name,
-                   "Nr": Nr, "Nt": Nt, "parallelism": "single GPU"},
+                   "Nr": Nr, "Nt": Nt, "steps_unit": "fine-step equivalents of dz", "parallelism": "single GPU"},
         "forms": forms,
-        "fp64_ms_per_step": ms64,
+        "speedup_chosen_vs_plain": forms["plain_fp32"]["ms_per_step"] / forms["chosen"]["ms_per_fine_step_equivalent"],
         "roofline": roof,
         "cpu_baseline": cpu,
     }

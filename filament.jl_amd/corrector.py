@@ -156,18 +156,16 @@ def capture_vs_fp64(m32, c32, m64, probe, z, dz, factor):
     model takes the coarse step, the FP64 model the ``factor`` fine steps from
     the same field, and ``probe`` (a Corrector on a second FP32 model with the
     same window) reads the features of the FP64 result.  The FP32 model is left
-    at the FP64 result.  The field filter is 1 inside the window, so features
-    of the filtered (downloaded) field equal those of the state."""
-    E0 = m32.field()
+    at the FP64 result.  States move between the models on the device
+    (fil_state_copy), pending field filter included."""
+    m64.copy_state_from(m32)
     m32.zstep(z + factor * dz, factor * dz)
     Xc = c32.features()
-    m64.upload(E0)
     for k in range(factor):
         m64.zstep(z + (k + 1) * dz, dz)
-    E64 = m64.field()
-    probe.m.upload(E64)
+    probe.m.copy_state_from(m64)
     Xf = probe.features()
-    m32.upload(E64)
+    m32.copy_state_from(m64)
     return Xc, (Xf[:2] - Xc[:2]).astype(np.float32)
 
 

@@ -124,6 +124,13 @@ struct fil_handle {
 
 namespace {
 
+// precision-converting copy of a half-spectrum state
+template <typename D, typename S>
+__global__ void convert_state_kernel(const cx<S>* __restrict__ src, cx<D>* __restrict__ dst, size_t n) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = mk((D)src[q].x, (D)src[q].y);
+}
+
 template <typename R>
 void put_real(std::vector<unsigned char>& buf, size_t idx, double v) { reinterpret_cast<R*>(buf.data())[idx] = (R)v; }
 
@@ -753,6 +760,38 @@ int fil_state_save(fil_handle* h, int slot) {
   s.used = true;
   s.pending_filter = h->pending_filter;
   s.plasma_valid = h->plasma_valid;
+  return FIL_OK;
+}
+
+int fil_state_copy(fil_handle* dst, const fil_handle* src) {
+  if (!valid_handle(dst) || !src) return fail(FIL_EINVAL, "null handle");
+  if (dst == src) return FIL_OK;
+  if (dst->d.device != src->d.device) return fail(FIL_EINVAL, "handles on different devices");
+  if (dst->Nr != src->Nr || dst->Nt != src->Nt) return fail(FIL_EINVAL, "handles of different grids");
+  if (!src->field || !dst->tables) return fail(FIL_ESTATE, "source needs a field and destination its tables");
+  HIPCHK(hipSetDevice(dst->d.device));
+  // the source's pending work completes before the destination's stream reads it
+  HIPCHK(hipStreamSynchronize(src->st));
+  const size_t n = (size_t)dst->Nrp * dst->Nh;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if (dst->fp64 == src->fp64) {
+    HIPCHK(hipMemcpyAsync(dst->S.p, src->S.p, n * dst->csz(), hipMemcpyDeviceToDevice, dst->st));
+  } else if (dst->fp64) {
+    hipLaunchKernelGGL((convert_state_kernel<double, float>), grid, dim3(256), 0, dst->st, src->S.as<cx<float>>(),
+                       dst->S.as<cx<double>>(), n);
+    HIPCHK(hipGetLastError());
+  } else {
+    hipLaunchKernelGGL((convert_state_kernel<float, double>), grid, dim3(256), 0, dst->st, src->S.as<cx<double>>(),
+                       dst->S.as<cx<float>>(), n);
+    HIPCHK(hipGetLastError());
+  }
+  if (src->d.plasma && dst->d.plasma) {
+    HIPCHK(hipMemcpyAsync(dst->rho_end.p, src->rho_end.p, dst->Nr * 8, hipMemcpyDeviceToDevice, dst->st));
+    HIPCHK(hipMemcpyAsync(dst->rho_max.p, src->rho_max.p, dst->Nr * 8, hipMemcpyDeviceToDevice, dst->st));
+  }
+  dst->field = true;
+  dst->pending_filter = src->pending_filter;
+  dst->plasma_valid = src->plasma_valid && src->d.plasma && dst->d.plasma;
   return FIL_OK;
 }
 

@@ -22,19 +22,22 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 
-constexpr int CT = 256;   // threads per block of the convolution kernels
+constexpr int CT = 128;   // threads per block of the convolution kernels
+constexpr int CG = 8;     // channels per thread in the forward / data-gradient kernels
 
-template <int CIN, int COUT>
+// Output channels are split into groups of G per thread (blockIdx.z), so a
+// single sample still spreads over the whole chip.
+template <int CIN, int COUT, int G>
 __global__ __launch_bounds__(CT) void conv_fwd_kernel(const float* __restrict__ in, const int* __restrict__ idx,
                                                       const uint16_t* __restrict__ wbf, const float* __restrict__ bias,
                                                       float* __restrict__ out, int H, int W, int relu) {
-  __shared__ __align__(16) float wl[CIN * 9 * COUT];   // [ci][tap][co]
-  __shared__ float bl[COUT];
-  for (int q = threadIdx.x; q < CIN * 9 * COUT; q += CT) {
-    const int co = q % COUT, r = q / COUT, tap = r % 9, ci = r / 9;
-    wl[q] = bf2f(wbf[(co * CIN + ci) * 9 + tap]);
+  static_assert(COUT % G == 0, "group size must divide the output channels");
+  __shared__ __align__(16) float wl[CIN * 9 * G];   // [ci][tap][co in group]
+  const int co0 = blockIdx.z * G;
+  for (int q = threadIdx.x; q < CIN * 9 * G; q += CT) {
+    const int co = q % G, r = q / G, tap = r % 9, ci = r / 9;
+    wl[q] = bf2f(wbf[((co0 + co) * CIN + ci) * 9 + tap]);
   }
-  for (int q = threadIdx.x; q < COUT; q += CT) bl[q] = bias[q];
   __syncthreads();
   const int HW = H * W;
   const int b = blockIdx.y;
@@ -42,9 +45,10 @@ __global__ __launch_bounds__(CT) void conv_fwd_kernel(const float* __restrict__ 
   if (p >= HW) return;
   const int y = p / W, x = p % W;
   const float* inb = in + (size_t)(idx ? idx[b] : b) * CIN * HW;
-  float acc[COUT];
+  float acc[G];
 #pragma unroll
-  for (int co = 0; co < COUT; co++) acc[co] = bl[co];
+  for (int co = 0; co < G; co++) acc[co] = bias[co0 + co];
+#pragma unroll 1
   for (int ci = 0; ci < CIN; ci++) {
     const float* ip = inb + (size_t)ci * HW;
     float v[9];
@@ -55,25 +59,28 @@ __global__ __launch_bounds__(CT) void conv_fwd_kernel(const float* __restrict__ 
     }
 #pragma unroll
     for (int tap = 0; tap < 9; tap++) {
-      const float* wr = wl + (ci * 9 + tap) * COUT;
+      const float* wr = wl + (ci * 9 + tap) * G;
 #pragma unroll
-      for (int co = 0; co < COUT; co++) acc[co] += wr[co] * v[tap];
+      for (int co = 0; co < G; co++) acc[co] += wr[co] * v[tap];
     }
   }
-  float* ob = out + (size_t)b * COUT * HW + p;
+  float* ob = out + ((size_t)b * COUT + co0) * HW + p;
 #pragma unroll
-  for (int co = 0; co < COUT; co++) ob[(size_t)co * HW] = relu ? fmaxf(acc[co], 0.0f) : acc[co];
+  for (int co = 0; co < G; co++) ob[(size_t)co * HW] = relu ? fmaxf(acc[co], 0.0f) : acc[co];
 }
 
-// din[ci][q] = sum_co sum_tap w[co][ci][tap] dout[co][q - d(tap)], times [act > 0]
-template <int CIN, int COUT>
+// din[ci][q] = sum_co sum_tap w[co][ci][tap] dout[co][q - d(tap)], times [act > 0];
+// input channels in groups of G per thread (blockIdx.z)
+template <int CIN, int COUT, int G>
 __global__ __launch_bounds__(CT) void conv_bwd_data_kernel(const float* __restrict__ dout, const uint16_t* __restrict__ wbf,
                                                            const float* __restrict__ act, float* __restrict__ din,
                                                            int H, int W) {
-  __shared__ __align__(16) float wl[COUT * 9 * CIN];   // [co][tap][ci]
-  for (int q = threadIdx.x; q < COUT * 9 * CIN; q += CT) {
-    const int ci = q % CIN, r = q / CIN, tap = r % 9, co = r / 9;
-    wl[q] = bf2f(wbf[(co * CIN + ci) * 9 + tap]);
+  static_assert(CIN % G == 0, "group size must divide the input channels");
+  __shared__ __align__(16) float wl[COUT * 9 * G];   // [co][tap][ci in group]
+  const int ci0 = blockIdx.z * G;
+  for (int q = threadIdx.x; q < COUT * 9 * G; q += CT) {
+    const int ci = q % G, r = q / G, tap = r % 9, co = r / 9;
+    wl[q] = bf2f(wbf[(co * CIN + ci0 + ci) * 9 + tap]);
   }
   __syncthreads();
   const int HW = H * W;
@@ -82,25 +89,25 @@ __global__ __launch_bounds__(CT) void conv_bwd_data_kernel(const float* __restri
   if (p >= HW) return;
   const int y = p / W, x = p % W;
   const float* gb = dout + (size_t)b * COUT * HW;
-  float acc[CIN];
+  float acc[G];
 #pragma unroll
-  for (int ci = 0; ci < CIN; ci++) acc[ci] = 0.0f;
+  for (int ci = 0; ci < G; ci++) acc[ci] = 0.0f;
+#pragma unroll 1
   for (int co = 0; co < COUT; co++) {
     const float* gp = gb + (size_t)co * HW;
-    // not unrolled: unrolling lets the compiler hoist all 9 x CIN weight reads
-#pragma unroll 1
+#pragma unroll
     for (int tap = 0; tap < 9; tap++) {
       const int yy = y - (tap / 3 - 1), xx = x - (tap % 3 - 1);
       const float gv = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? gp[yy * W + xx] : 0.0f;
-      const float* wr = wl + (co * 9 + tap) * CIN;
+      const float* wr = wl + (co * 9 + tap) * G;
 #pragma unroll
-      for (int ci = 0; ci < CIN; ci++) acc[ci] += wr[ci] * gv;
+      for (int ci = 0; ci < G; ci++) acc[ci] += wr[ci] * gv;
     }
   }
-  float* db = din + (size_t)b * CIN * HW + p;
-  const float* ab = act ? act + (size_t)b * CIN * HW + p : nullptr;
+  float* db = din + ((size_t)b * CIN + ci0) * HW + p;
+  const float* ab = act ? act + ((size_t)b * CIN + ci0) * HW + p : nullptr;
 #pragma unroll
-  for (int ci = 0; ci < CIN; ci++) {
+  for (int ci = 0; ci < G; ci++) {
     float g = acc[ci];
     if (ab && !(ab[(size_t)ci * HW] > 0.0f)) g = 0.0f;
     db[(size_t)ci * HW] = g;
@@ -219,15 +226,21 @@ static_assert(CORR_LAYERS == 4, "layer dispatch covers four layers");
 
 hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const uint16_t* wbf, const float* bias, float* out,
                            int B, int H, int W, int relu, hipStream_t st) {
-  const dim3 grid((H * W + CT - 1) / CT, B);
-  FIL_CORR_LAYER(l, hipLaunchKernelGGL((conv_fwd_kernel<CI, CO>), grid, dim3(CT), 0, st, in, idx, wbf, bias, out, H, W, relu));
+  FIL_CORR_LAYER(l, {
+    constexpr int G = CO < CG ? CO : CG;
+    const dim3 grid((H * W + CT - 1) / CT, B, CO / G);
+    hipLaunchKernelGGL((conv_fwd_kernel<CI, CO, G>), grid, dim3(CT), 0, st, in, idx, wbf, bias, out, H, W, relu);
+  });
   return hipGetLastError();
 }
 
 hipError_t launch_conv_bwd_data(int l, const float* dout, const uint16_t* wbf, const float* act, float* din, int B,
                                 int H, int W, hipStream_t st) {
-  const dim3 grid((H * W + CT - 1) / CT, B);
-  FIL_CORR_LAYER(l, hipLaunchKernelGGL((conv_bwd_data_kernel<CI, CO>), grid, dim3(CT), 0, st, dout, wbf, act, din, H, W));
+  FIL_CORR_LAYER(l, {
+    constexpr int G = CI < CG ? CI : CG;
+    const dim3 grid((H * W + CT - 1) / CT, B, CI / G);
+    hipLaunchKernelGGL((conv_bwd_data_kernel<CI, CO, G>), grid, dim3(CT), 0, st, dout, wbf, act, din, H, W);
+  });
   return hipGetLastError();
 }
 

@@ -60,7 +60,8 @@ CORR_NPARAM = 20258
 
 EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_upload_field", "fil_download_field",
            "fil_zstep", "fil_analyze", "fil_download_plasma", "fil_synchronize", "fil_destroy",
-           "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_fluence",
+           "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_state_copy",
+           "fil_fluence",
            "fil_corrector_init", "fil_corrector_params", "fil_corrector_scale", "fil_corrector_features",
            "fil_corrector_store_pair", "fil_corrector_train", "fil_corrector_eval", "fil_corrector_forward",
            "fil_corrector_apply")
@@ -95,6 +96,7 @@ def load():
     L.fil_state_save.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.fil_state_restore.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.fil_fluence.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    L.fil_state_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     _fp = ctypes.POINTER(ctypes.c_float)
     L.fil_corrector_init.argtypes = [ctypes.c_void_p, ctypes.POINTER(FilCorrDesc), ctypes.c_uint64]
     L.fil_corrector_params.argtypes = [ctypes.c_void_p, _fp, ctypes.c_int]

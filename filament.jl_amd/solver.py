@@ -192,6 +192,10 @@ class RTModel:
     def state_restore(self, slot):
         L.check(self._lib.fil_state_restore(self.h, int(slot)))
 
+    def copy_state_from(self, other):
+        """Take over the z-step state of another model on the same grid (device copy)."""
+        L.check(self._lib.fil_state_copy(self.h, other.h))
+
     def fluence(self, slot=-1):
         """Fr(r) of the current state (slot < 0) or of a saved slot, by Parseval."""
         Fr = np.zeros(self.grid.Nr)

@@ -163,6 +163,10 @@ int fil_synchronize(fil_handle* h);
 #define FIL_NSLOTS 4
 int fil_state_save(fil_handle* h, int slot);
 int fil_state_restore(fil_handle* h, int slot);
+/* Copies the z-step state of src into dst (same device and grid), converting
+ * between FP32 and FP64 handles; dst's previous state is replaced.  Waits for
+ * src's stream, then runs on dst's stream. */
+int fil_state_copy(fil_handle* dst, const fil_handle* src);
 
 /* Fluence profile Fr(r) = dt * sum_t |E(r,t)|^2 (FieldAnalyzers.jl:171) of the
  * current state (slot < 0) or of a saved slot, computed on the device from
