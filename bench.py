@@ -7,8 +7,8 @@ nonlinear step (row kernel), then the two Hankel-transform GEMMs with the
 propagator multiply.  The field is resident in HBM when the timed region
 starts.
 
-Forms, each run over the same trajectory (W warm-up + K timed + a few
-profiled fine-step equivalents) from the same initial field:
+Forms, each run over the same trajectory (W warm-up + K timed fine-step
+equivalents) from the same initial field:
   fp64    the FP64 build, fine steps: the error reference;
   plain   the plain FP32 port, fine steps;
   chosen  FP32 with the split-BF16 Hankel transform and the FP64 plasma scan,
@@ -33,6 +33,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (fetch x 2.00, write x 1.00,
+# calibrated on a 16 MiB device copy), RT 1024x4096: profiles/rt1024x4096_hbm_traffic_pmc.txt
+TRAFFIC_BYTES = {"row_kernel": 38.2e6, "dht_gemm_bf16x3": 78.8e6}
 GATE = 1e-3
 
 
@@ -60,24 +63,22 @@ def kernel_ms(kt):
     return dict(row=kt["row_ms"] / n, dht1=kt["dht1_ms"] / n, dht2=kt["dht2_ms"] / n)
 
 
-def run_fine(pkg, p, precision, form, W, K, P, dz):
-    """W warm-up, K timed and P profiled fine steps."""
+def run_fine(pkg, p, precision, form, W, K, dz):
+    """W warm-up and K timed fine steps; per-kernel HIP-event times over the
+    timed steps (events on the library's stream)."""
     m = pkg.RTModel(p, precision=precision, form=form)
     z = 0.0
     for _ in range(W):
         z += dz
         m.zstep(z, dz)
     m.synchronize()
+    m.timing(True)
     t0 = time.perf_counter()
     for _ in range(K):
         z += dz
         m.zstep(z, dz)
     m.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / K
-    m.timing(True)
-    for _ in range(P):
-        z += dz
-        m.zstep(z, dz)
     kt = m.timing_read()
     m.timing(False)
     Fr = m.analyze(z, arrays=True)["Fr"]
@@ -85,8 +86,8 @@ def run_fine(pkg, p, precision, form, W, K, P, dz):
     return ms, kernel_ms(kt), Fr
 
 
-def run_coarse(pkg, p, form, W, K, P, dz, factor, check_every, tol, corr_state, geom):
-    """The chosen form: coarse steps (+ corrector) with verification, W/K/P in
+def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geom):
+    """The chosen form: coarse steps (+ corrector) with verification, W/K in
     fine-step equivalents."""
     m = pkg.RTModel(p, precision=pkg.lib.FP32, form=form)
     corr = None
@@ -97,15 +98,14 @@ def run_coarse(pkg, p, form, W, K, P, dz, factor, check_every, tol, corr_state, 
     z = st.advance(0.0, dz, W)
     m.synchronize()
     s0 = st.stats()
+    m.timing(True)
     t0 = time.perf_counter()
     z = st.advance(z, dz, K)
     m.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / K
-    s1 = st.stats()
-    m.timing(True)
-    z = st.advance(z, dz, P)
     kt = m.timing_read()
     m.timing(False)
+    s1 = st.stats()
     Fr = m.analyze(z, arrays=True)["Fr"]
     m.close()
     timed = dict(coarse_steps=s1["coarse_steps"] - s0["coarse_steps"], fine_steps=s1["fine_steps"] - s0["fine_steps"],
@@ -177,7 +177,6 @@ def main():
     K, W = args.steps, args.warmup
     if K % F or W % F:
         sys.exit("--steps and --warmup must be multiples of --factor")
-    P = 4 * F                                   # profiled fine-step equivalents
     Nr, Nt = p["Nr"], p["Nt"]
     Nh = Nt // 2
     chosen_form = L.FORM_DHT_BF16X3 | L.FORM_SCAN_FP64
@@ -188,12 +187,12 @@ def main():
     Fr64 = None
     if "fp64" in want:
         log("fp64: fine steps")
-        ms64, kt64, Fr64 = run_fine(pkg, p, L.FP64, 0, W, K, P, dz)
+        ms64, kt64, Fr64 = run_fine(pkg, p, L.FP64, 0, W, K, dz)
         forms["fp64"] = dict(ms_per_step=ms64, z_steps_per_s=1e3 / ms64, kernel_ms=kt64)
         log(f"fp64: {ms64:.4f} ms/step")
     if "plain" in want:
         log("plain FP32: fine steps")
-        msp, ktp, Frp = run_fine(pkg, p, L.FP32, 0, W, K, P, dz)
+        msp, ktp, Frp = run_fine(pkg, p, L.FP32, 0, W, K, dz)
         forms["plain_fp32"] = dict(ms_per_step=msp, z_steps_per_s=1e3 / msp, kernel_ms=ktp,
                                    fluence_err_vs_fp64=fluence_error(Frp, Fr64) if Fr64 is not None else None)
         log(f"plain FP32: {msp:.4f} ms/step")
@@ -206,7 +205,7 @@ def main():
             draws = pkg.corrector.ensemble_draws(args.corr_draws, seed=1)
             cst, cinfo = train_corrector(pkg, p, chosen_form, dz, F, draws, args.corr_pairs, args.corr_iters, geom)
         log(f"chosen: coarse x{F}, corrector {'off' if cst is None else 'on'}")
-        msc, ktc, Frc, timed = run_coarse(pkg, p, chosen_form, W, K, P, dz, F, args.check_every, args.check_tol,
+        msc, ktc, Frc, timed = run_coarse(pkg, p, chosen_form, W, K, dz, F, args.check_every, args.check_tol,
                                           cst, geom)
         forms["chosen"] = dict(
             ms_per_fine_step_equivalent=msc, z_steps_per_s=1e3 / msc, kernel_ms_per_zstep=ktc,
@@ -236,11 +235,13 @@ def main():
         peak = PEAK["bf16_mfma_tflops"] if name == "chosen" else PEAK["fp32_mfma_tflops"]
         ach = gemm_flop / (kms * 1e-3) / 1e12
         roof = dict(kernel="dht_gemm", bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                    traffic=None, per_launch=f"{gemm_flop:.4g} flop (2*Nr*Nt*Nr)", avg_launch_ms=kms)
+                    traffic=TRAFFIC_BYTES["dht_gemm_bf16x3"] if name == "chosen" else None,
+                    per_launch=f"{gemm_flop:.4g} flop (2*Nr*Nt*Nr)", avg_launch_ms=kms)
     else:
         ach = row_bytes / (kt["row"] * 1e-3) / 1e9
         roof = dict(kernel="row_kernel", bound="hbm", achieved=ach, peak=PEAK["hbm_gbs"], unit="GB/s",
-                    frac=ach / PEAK["hbm_gbs"], traffic=None, per_launch=f"{row_bytes:.4g} B (2*8*Nr*Nt/2)",
+                    frac=ach / PEAK["hbm_gbs"], traffic=TRAFFIC_BYTES["row_kernel"],
+                    per_launch=f"{row_bytes:.4g} B (2*8*Nr*Nt/2)",
                     avg_launch_ms=kt["row"])
 
     cpu = None
