@@ -534,10 +534,11 @@ int fil_set_tables(fil_handle* h, const fil_tables* t) {
         const int rad = fft_radix(Nt, s);
         if (s < Nt / rad) {
           const int P = Nt / (s * rad);
-          for (int k = 1; k < rad; k++)
-            for (int p = 0; p < P; p++) {
-              double a = -2.0 * kPi * (double)(((long)p * s * k) % Nt) / Nt;
-              tw.setc(o++, std::cos(a), std::sin(a));
+          for (int p = 0; p < P; p++)
+            for (int k = 0; k < FIL_TW_STRIDE; k++) {
+              const double a = -2.0 * kPi * (double)(((long)p * s * k) % Nt) / Nt;
+              if (k < rad) tw.setc(o, std::cos(a), std::sin(a));
+              o++;
             }
         }
       }

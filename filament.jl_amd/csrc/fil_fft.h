@@ -78,37 +78,49 @@ struct Dft<T, 4, SIGN> {
   }
 };
 
-// Twiddle table layout (built on the host by fil::fft_twiddles): for every
-// pass that has twiddles (stride S < N/RAD) a block of (RAD-1) x P entries,
-// P = N/(S RAD), entry [(k-1) P + p] = exp(-2 pi i p S k / N).  Lanes of one
-// pass read consecutive p, so every twiddle load is coalesced.
+// Twiddle table layout (built on the host, fil_set_tables): for every pass
+// that has twiddles (stride S < N/RAD) a block of P x FIL_TW_STRIDE entries,
+// P = N/(S RAD), entry [p * FIL_TW_STRIDE + k] = exp(-2 pi i p S k / N),
+// k = 0..RAD-1 (k = 0 is 1 and unused).  A thread's twiddles are contiguous, so
+// it reads them with a few wide loads at constant offsets from one address.
+#define FIL_TW_STRIDE FIL_FFT_MAXRAD
 __host__ __device__ constexpr int fft_radix(int N, int S) { return (N / S >= FIL_FFT_MAXRAD) ? FIL_FFT_MAXRAD : N / S; }
 __host__ __device__ constexpr int fft_tw_offset(int N, int S) {
   int off = 0;
   for (int s = 1; s < S; s *= fft_radix(N, s)) {
     const int rad = fft_radix(N, s);
-    if (s < N / rad) off += (rad - 1) * (N / (s * rad));
+    if (s < N / rad) off += (N / (s * rad)) * FIL_TW_STRIDE;
   }
   return off;
 }
 __host__ __device__ constexpr int fft_tw_size(int N) { return fft_tw_offset(N, N); }
 
+// LDS offset of element i0 + c for a compile-time c that is a multiple of a
+// power of two C with i0 mod C + c's carry staying below 16 (every access of
+// the passes below): lds_index(i0 + c) = lds_index(i0) + c + (c >> 4).
+__host__ __device__ constexpr int lds_step(int c) { return c + (c >> 4); }
+
 // One Stockham pass of radix RAD at stride S over N points.
 // Inputs  x[q + S (p + k N/(S RAD))] = x[b + k N/RAD],  b = p S + q;
 // outputs y[q + S (RAD p + k)] = DFT_RAD(...)[k] * w^(p S k),  w = exp(SIGN 2 pi i / N).
+// LDS addresses are one padded base per thread plus compile-time offsets:
+// for reads lds_index(b + k NB) = lds_index(b) + lds_step(k NB) (NB a multiple
+// of 16); for writes lds_index(w0 + S k) = lds_index(w0) + lds_step(S k) with
+// w0 = q + S RAD p, because S, RAD and 16 are powers of two and q < S keeps
+// the low four bits of w0 + S k from carrying.
 template <typename T, int N, int NTH, int SIGN, int RAD, int S>
 __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict__ tw) {
   constexpr int NB = N / RAD;
   constexpr int BPT = (NB + NTH - 1) / NTH;
   constexpr bool TW = S < N / RAD;     // the last pass has p == 0: no twiddles
-  constexpr int P = N / (S * RAD);
+  static_assert(NB % 16 == 0, "read offsets assume N/RAD to be a multiple of 16");
   // The LDS addresses of a pass depend only on the thread index; without this
   // opaque copy the compiler hoists them out of every FFT of the kernel and
   // keeps them all live, which costs ~50 VGPRs and halves the occupancy.
   int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
   cx<T> a[BPT][RAD];
-  cx<T> w[BPT][TW ? RAD - 1 : 1];
+  cx<T> w[BPT][TW ? RAD : 1];
   // twiddle loads first, so their latency overlaps the LDS reads and the barrier
   if constexpr (TW) {
     const cx<T>* twp = tw + fft_tw_offset(N, S);
@@ -116,8 +128,9 @@ __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict_
     for (int u = 0; u < BPT; u++) {
       const int b = tid + u * NTH;
       if (NB % NTH == 0 || b < NB) {
+        const cx<T>* t = twp + (b / S) * FIL_TW_STRIDE;
 #pragma unroll
-        for (int k = 1; k < RAD; k++) w[u][k - 1] = twp[(k - 1) * P + b / S];
+        for (int k = 1; k < RAD; k++) w[u][k] = t[k];
       }
     }
   }
@@ -125,8 +138,9 @@ __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict_
   for (int u = 0; u < BPT; u++) {
     const int b = tid + u * NTH;
     if (NB % NTH == 0 || b < NB) {
+      const cx<T>* zr = z + lds_index(b);
 #pragma unroll
-      for (int k = 0; k < RAD; k++) a[u][k] = z[lds_index(b + k * NB)];
+      for (int k = 0; k < RAD; k++) a[u][k] = zr[lds_step(k * NB)];
     }
   }
   __syncthreads();
@@ -138,10 +152,11 @@ __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict_
       const int q = b % S, p = b / S;
       if constexpr (TW) {
 #pragma unroll
-        for (int k = 1; k < RAD; k++) a[u][k] = a[u][k] * ((SIGN > 0) ? conj(w[u][k - 1]) : w[u][k - 1]);
+        for (int k = 1; k < RAD; k++) a[u][k] = a[u][k] * ((SIGN > 0) ? conj(w[u][k]) : w[u][k]);
       }
+      cx<T>* zw = z + lds_index(q + S * RAD * p);
 #pragma unroll
-      for (int k = 0; k < RAD; k++) z[lds_index(q + S * (RAD * p + k))] = a[u][k];
+      for (int k = 0; k < RAD; k++) zw[lds_step(S * k)] = a[u][k];
     }
   }
   __syncthreads();
