@@ -229,6 +229,7 @@ def fit(RTModel, make_params, draws, geom, dz, factor, form, npairs=32, spacing=
         if log:
             log(f"corrector data: draw {d}: {len(ps)} pairs, target rms {rms(np.array([T for _, T in ps])):.3e}")
     t1 = time.perf_counter()
+    batch = min(batch, len(pairs))
     tm = RTModel(make_params(draws[0]), precision=_L.FP32, form=form)
     try:
         tc = Corrector(tm, geom=geom, capacity=len(pairs), batch_max=batch, seed=seed)

@@ -90,7 +90,15 @@ struct fil_handle {
   DevBuf S, X, Et, M1, M2, M1h, M1l, M2h, M2l, KZ64, KG64, P, tw, Tg, Rg, coef;
   DevBuf tab_ly[FIL_MAXCOMP], tab_sl[FIL_MAXCOMP];
   DevBuf rho_end, rho_max, rho_full, kd_full;
-  DevBuf aFr, aIt, aSw, aImax, aFt, aS, ardr;
+  DevBuf aFr, aIt, aSw, aImax, aFt, aS, ardr, acol;
+  // XYT geometry (rows are the Nx*Ny points, x fastest)
+  bool xyt = false;
+  int Nx = 0, Ny = 0;
+  double dx = 0, dy = 0;
+  std::vector<double> x, y;
+  DevBuf twx, twy;                                      // FFT twiddles along x and y
+  DevBuf xkt2x, xkt2y, xKX, xKY, xbeta, xwvf, xWg;      // FP64 propagator inputs
+  XytProp xprop{};
   // saved states (fil_state_save / fil_state_restore)
   struct Slot {
     DevBuf S, rho_end, rho_max;
@@ -107,7 +115,7 @@ struct fil_handle {
     DevBuf X, Y;                      // stored pairs [cap][4][HW], [cap][2][HW]
     DevBuf Frow, feat, carrier;       // feature scratch, features [4][HW], exp(i w0 t_j) [Nt]
     DevBuf act[3], out, gA, gB, gout; // training activations and gradients [bmax][..][HW]
-    DevBuf idx, part;                 // batch indices, reduction partials
+    DevBuf idx, part, wpart;          // batch indices, loss partials, weight-gradient partials
   } corr;
   PlasmaParams<float> pp32{};
   PlasmaParams<double> pp64{};
@@ -192,7 +200,10 @@ int flush_timing(fil_handle* h) {
 template <typename R>
 int zstep_impl(fil_handle* h, double dz) {
   if (!(h->P_dz == dz)) {
-    HIPCHK(launch_propagator<R>(h->KZ64.as<double>(), h->KG64.as<double>(), dz, h->P.as<cx<R>>(), h->Nrp, h->Nh, h->st));
+    if (h->xyt)
+      HIPCHK(launch_xyt_propagator<R>(h->xprop, dz, h->P.as<cx<R>>(), h->st));
+    else
+      HIPCHK(launch_propagator<R>(h->KZ64.as<double>(), h->KG64.as<double>(), dz, h->P.as<cx<R>>(), h->Nrp, h->Nh, h->st));
     h->P_dz = dz;
   }
   RowArgs<R> a{};
@@ -217,6 +228,16 @@ int zstep_impl(fil_handle* h, double dz) {
   if (int rc = record(h, 0)) return rc;
   HIPCHK(launch_row<R>(a, h->Nr, h->Nt, h->scan64, h->st));
   if (int rc = record(h, 1)) return rc;
+  if (h->xyt) {
+    // the three transform passes are timed together as the "dht1" phase
+    HIPCHK(launch_xyt_linear<R>(h->S.as<cx<R>>(), h->P.as<cx<R>>(), h->twx.as<cx<R>>(), h->twy.as<cx<R>>(), h->Nx,
+                                h->Ny, h->Nh, h->st));
+    if (int rc = record(h, 2)) return rc;
+    if (int rc = record(h, 3)) return rc;
+    h->pending_filter = true;
+    h->plasma_valid = h->d.plasma != 0;
+    return 0;
+  }
   if constexpr (sizeof(R) == 4) {
     if (h->bf16x3) {
       HIPCHK(launch_dht_gemm_bf16x3(h->M1h.as<uint16_t>(), h->M1l.as<uint16_t>(), h->S.as<cx<float>>(),
@@ -266,8 +287,15 @@ int plasma_only(fil_handle* h) {
   return 0;
 }
 
+// Device part of the analyzers: the per-row pass (pending filter, projection,
+// Fr / Imax / spectrum rows) and the weighted column sums (RT: 2 pi r dr,
+// XYT: dx dy).  Fills host vectors; blocks.
+struct AnalyzerRows {
+  std::vector<double> fr, ft, im, re, rm, sp;
+};
+
 template <typename R>
-int analyze_impl(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
+int analyze_device(fil_handle* h, AnalyzerRows& v) {
   const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh;
   if (h->d.plasma && !h->plasma_valid)
     if (int rc = plasma_only<R>(h)) return rc;
@@ -275,40 +303,82 @@ int analyze_impl(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, d
                                 h->pending_filter ? 1 : 0, (R)h->dt, h->aFr.as<double>(), h->aIt.as<float>(),
                                 h->aSw.as<float>(), h->aImax.as<double>(), Nr, Nt, h->st));
   h->pending_filter = false;
-  HIPCHK(launch_colsum(h->aIt.as<float>(), Nr, Nt, h->ardr.as<double>(), h->aFt.as<double>(), h->st));
-  HIPCHK(launch_colsum(h->aSw.as<float>(), Nr, Nh + 1, h->ardr.as<double>(), h->aS.as<double>(), h->st));
-  std::vector<double> fr(Nr), ft(Nt), im(Nr), re(Nr, 0.0), rm(Nr, 0.0), sp(Nh + 1);
-  HIPCHK(hipMemcpyAsync(fr.data(), h->aFr.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipMemcpyAsync(ft.data(), h->aFt.p, Nt * 8, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipMemcpyAsync(im.data(), h->aImax.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipMemcpyAsync(sp.data(), h->aS.p, (Nh + 1) * 8, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(launch_colsum(h->aIt.as<float>(), Nr, Nt, h->ardr.as<double>(), h->acol.as<double>(), h->aFt.as<double>(),
+                       h->st));
+  HIPCHK(launch_colsum(h->aSw.as<float>(), Nr, Nh + 1, h->ardr.as<double>(), h->acol.as<double>(),
+                       h->aS.as<double>(), h->st));
+  v.fr.assign(Nr, 0.0); v.ft.assign(Nt, 0.0); v.im.assign(Nr, 0.0);
+  v.re.assign(Nr, 0.0); v.rm.assign(Nr, 0.0); v.sp.assign(Nh + 1, 0.0);
+  HIPCHK(hipMemcpyAsync(v.fr.data(), h->aFr.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(v.ft.data(), h->aFt.p, Nt * 8, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(v.im.data(), h->aImax.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(v.sp.data(), h->aS.p, (Nh + 1) * 8, hipMemcpyDeviceToHost, h->st));
   if (h->d.plasma) {
-    HIPCHK(hipMemcpyAsync(re.data(), h->rho_end.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
-    HIPCHK(hipMemcpyAsync(rm.data(), h->rho_max.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(v.re.data(), h->rho_end.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(v.rm.data(), h->rho_max.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
   }
   HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+template <typename R>
+int analyze_impl(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
+  const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh;
+  AnalyzerRows v;
+  if (int rc = analyze_device<R>(h, v)) return rc;
   // scalar diagnostics (FieldAnalyzers.jl:201-213)
   fil_diag o{};
   o.z = z;
   double Fmax = 0, Imax = 0, rhomax = 0, De = 0, W = 0;
   for (int i = 0; i < Nr; i++) {
-    Fmax = std::max(Fmax, fr[i]);
-    Imax = std::max(Imax, im[i]);
-    rhomax = std::max(rhomax, rm[i]);
-    De += re[i] * h->rdr[i];
-    W += fr[i] * h->rdr[i];
+    Fmax = std::max(Fmax, v.fr[i]);
+    Imax = std::max(Imax, v.im[i]);
+    rhomax = std::max(rhomax, v.rm[i]);
+    De += v.re[i] * h->rdr[i];
+    W += v.fr[i] * h->rdr[i];
   }
   o.Fmax = Fmax; o.Imax = Imax; o.rhomax = rhomax;
   o.De = 2 * kPi * De;
-  o.rfil = 2 * radius(h->r.data(), fr.data(), Nr);
-  o.rpl = 2 * radius(h->r.data(), re.data(), Nr);
-  o.tau = radius(h->t.data(), ft.data(), Nt);
+  o.rfil = 2 * radius(h->r.data(), v.fr.data(), Nr);
+  o.rpl = 2 * radius(h->r.data(), v.re.data(), Nr);
+  o.tau = radius(h->t.data(), v.ft.data(), Nt);
   o.W = 2 * kPi * W;
   if (d) *d = o;
-  if (Fr) std::memcpy(Fr, fr.data(), Nr * 8);
-  if (Ft) std::memcpy(Ft, ft.data(), Nt * 8);
-  if (rho_end) std::memcpy(rho_end, re.data(), Nr * 8);
-  if (S) std::memcpy(S, sp.data(), (Nh + 1) * 8);
+  if (Fr) std::memcpy(Fr, v.fr.data(), Nr * 8);
+  if (Ft) std::memcpy(Ft, v.ft.data(), Nt * 8);
+  if (rho_end) std::memcpy(rho_end, v.re.data(), Nr * 8);
+  if (S) std::memcpy(S, v.sp.data(), (Nh + 1) * 8);
+  return 0;
+}
+
+// FieldAnalyzers.analyze! XYT (FieldAnalyzers.jl:287-353)
+template <typename R>
+int analyze_xyt_impl(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, double* Ft, double* rho_end, double* S) {
+  const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh, Nx = h->Nx, Ny = h->Ny;
+  AnalyzerRows v;
+  if (int rc = analyze_device<R>(h, v)) return rc;
+  fil_diag_xyt o{};
+  o.z = z;
+  const int imax = (int)(std::max_element(v.fr.begin(), v.fr.end()) - v.fr.begin());   // findmax, x fastest
+  o.Fmax = v.fr[imax];
+  o.Imax = *std::max_element(v.im.begin(), v.im.end());
+  o.rhomax = *std::max_element(v.rm.begin(), v.rm.end());
+  double De = 0, W = 0;
+  for (int i = 0; i < Nr; i++) De += v.re[i];
+  o.De = De * h->dx * h->dy;
+  for (int j = 0; j < Nt; j++) W += v.ft[j];
+  o.W = W * h->dt;
+  const int ixm = imax % Nx, iym = imax / Nx;
+  std::vector<double> px(Nx), py(Ny);
+  for (int ix = 0; ix < Nx; ix++) px[ix] = v.fr[(size_t)ix + (size_t)Nx * iym];
+  for (int iy = 0; iy < Ny; iy++) py[iy] = v.fr[(size_t)ixm + (size_t)Nx * iy];
+  o.ax = radius(h->x.data(), px.data(), Nx);
+  o.ay = radius(h->y.data(), py.data(), Ny);
+  if (d) *d = o;
+  if (Fxy) std::memcpy(Fxy, v.fr.data(), (size_t)Nr * 8);
+  if (Ft) std::memcpy(Ft, v.ft.data(), Nt * 8);
+  if (rho_end) std::memcpy(rho_end, v.re.data(), (size_t)Nr * 8);
+  if (S) std::memcpy(S, v.sp.data(), (Nh + 1) * 8);
   return 0;
 }
 
@@ -361,8 +431,8 @@ int corr_backward(fil_handle* h, const float* in, const int* idx, int B) {
   float* bufs[2] = {c.gA.as<float>(), c.gB.as<float>()};
   for (int l = CORR_LAYERS - 1; l >= 0; l--) {
     const float* a_in = (l == 0) ? in : c.act[l - 1].as<float>();
-    HIPCHK(launch_conv_bwd_weight(l, g, a_in, l == 0 ? idx : nullptr, gr + corr_woff(l), gr + corr_bias_off(l), B, H, W,
-                                  h->st));
+    HIPCHK(launch_conv_bwd_weight(l, g, a_in, l == 0 ? idx : nullptr, gr + corr_woff(l), gr + corr_bias_off(l),
+                                  c.wpart.as<float>(), B, H, W, h->st));
     if (l > 0) {
       float* d = bufs[l & 1];
       HIPCHK(launch_conv_bwd_data(l, g, wb + corr_woff(l), c.act[l - 1].as<float>(), d, B, H, W, h->st));
@@ -406,12 +476,17 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (!d || !out) return fail(FIL_EINVAL, "null argument");
   *out = nullptr;
   if (d->abi_version != FIL_ABI_VERSION) return fail(FIL_EINVAL, "ABI version mismatch");
-  if (d->geometry != FIL_GEOM_RT) return fail(FIL_EINVAL, "only the RT geometry is implemented");
+  if (d->geometry != FIL_GEOM_RT && d->geometry != FIL_GEOM_XYT) return fail(FIL_EINVAL, "geometry must be RT or XYT");
+  const bool xyt = d->geometry == FIL_GEOM_XYT;
+  auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
+  if (xyt && (!pow2(d->Nx) || !pow2(d->Ny) || d->Nx < 16 || d->Ny < 16 || d->Nx > 1024 || d->Ny > 1024))
+    return fail(FIL_EINVAL, "XYT: Nx and Ny must be powers of two in [16, 1024]");
+  if (xyt && d->guard3d_bug && d->Ny > d->Nt) return fail(FIL_EINVAL, "guard3d_bug needs Ny <= Nt (the reference indexes the t window with y)");
   if (d->precision != FIL_FP32 && d->precision != FIL_FP64) return fail(FIL_EINVAL, "precision must be FIL_FP32 or FIL_FP64");
   const int Nt = d->Nt;
   if (Nt < 256 || Nt > 8192 || (Nt & (Nt - 1))) return fail(FIL_EINVAL, "Nt must be a power of two in [256, 8192]");
   if (d->precision == FIL_FP64 && Nt > 4096) return fail(FIL_EINVAL, "FP64 supports Nt <= 4096");
-  if (d->Nr < 2 || d->Nr > 16384) return fail(FIL_EINVAL, "Nr must be in [2, 16384]");
+  if (!xyt && (d->Nr < 2 || d->Nr > 16384)) return fail(FIL_EINVAL, "Nr must be in [2, 16384]");
   if (d->plasma && (d->ncomp < 1 || d->ncomp > FIL_MAXCOMP)) return fail(FIL_EINVAL, "ncomp must be in [1, 4]");
   if ((d->form & FIL_FORM_DHT_BF16X3) && d->precision != FIL_FP32)
     return fail(FIL_EINVAL, "the split-BF16 Hankel form needs FIL_FP32");
@@ -425,8 +500,11 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   HIPCHK(hipSetDevice(d->device));
   fil_handle* h = new fil_handle();
   h->d = *d;
-  h->Nr = d->Nr;
-  h->Nrp = (d->Nr + 63) / 64 * 64;
+  h->xyt = xyt;
+  h->Nx = xyt ? d->Nx : 0;
+  h->Ny = xyt ? d->Ny : 0;
+  h->Nr = xyt ? d->Nx * d->Ny : d->Nr;
+  h->Nrp = (h->Nr + 63) / 64 * 64;
   h->Nt = Nt;
   h->Nh = Nt / 2;
   h->fp64 = d->precision == FIL_FP64;
@@ -436,18 +514,19 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (e != hipSuccess) { delete h; return fail(FIL_EDEVICE, hipGetErrorString(e)); }
   const size_t cz = h->csz(), rz = h->rsz();
   const size_t NS = (size_t)h->Nrp * h->Nh;
-  bool ok = h->S.alloc(NS * cz) == hipSuccess && h->X.alloc(NS * cz) == hipSuccess &&
+  bool ok = h->S.alloc(NS * cz) == hipSuccess && h->X.alloc(xyt ? 0 : NS * cz) == hipSuccess &&
             h->P.alloc(NS * cz) == hipSuccess && h->Et.alloc((size_t)h->Nrp * Nt * cz) == hipSuccess &&
             h->rho_end.alloc(h->Nr * 8) == hipSuccess && h->rho_max.alloc(h->Nr * 8) == hipSuccess &&
             h->aFr.alloc(h->Nr * 8) == hipSuccess && h->aImax.alloc(h->Nr * 8) == hipSuccess &&
             h->aIt.alloc((size_t)h->Nr * Nt * 4) == hipSuccess &&
             h->aSw.alloc((size_t)h->Nr * (h->Nh + 1) * 4) == hipSuccess &&
-            h->aFt.alloc(Nt * 8) == hipSuccess && h->aS.alloc((h->Nh + 1) * 8) == hipSuccess;
+            h->aFt.alloc(Nt * 8) == hipSuccess && h->aS.alloc((h->Nh + 1) * 8) == hipSuccess &&
+            h->acol.alloc((size_t)COLSUM_MAXCHUNK * Nt * 8) == hipSuccess;
   if (ok && d->keep_plasma)
     ok = h->rho_full.alloc((size_t)h->Nr * Nt * rz) == hipSuccess && h->kd_full.alloc((size_t)h->Nr * Nt * rz) == hipSuccess;
   if (!ok) { delete h; return fail(FIL_ENOMEM, "device allocation failed"); }
   (void)hipMemset(h->S.p, 0, NS * cz);
-  (void)hipMemset(h->X.p, 0, NS * cz);
+  if (!xyt) (void)hipMemset(h->X.p, 0, NS * cz);
   (void)hipMemset(h->rho_end.p, 0, h->Nr * 8);
   (void)hipMemset(h->rho_max.p, 0, h->Nr * 8);
   if (d->keep_plasma) {
@@ -458,23 +537,85 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   return FIL_OK;
 }
 
-int fil_set_tables(fil_handle* h, const fil_tables* t) {
-  if (!valid_handle(h) || !t) return fail(FIL_EINVAL, "null argument");
-  HIPCHK(hipSetDevice(h->d.device));
+}  // extern "C"
+
+namespace {
+
+// per-pass twiddle table of an N-point LDS FFT in the layout fft_lds reads (fil_fft.h)
+int build_twiddles(DevBuf& buf, int N, bool f64) {
+  HostArr tw(std::max(1, fft_tw_size(N)), f64, true);
+  size_t o = 0;
+  for (int s = 1; N / s > 1; s *= fft_radix(N, s)) {
+    const int rad = fft_radix(N, s);
+    if (s < N / rad) {
+      const int P = N / (s * rad);
+      for (int p = 0; p < P; p++)
+        for (int k = 0; k < FIL_TW_STRIDE; k++) {
+          const double a = -2.0 * kPi * (double)(((long)p * s * k) % N) / N;
+          if (k < rad) tw.setc(o, std::cos(a), std::sin(a));
+          o++;
+        }
+    }
+  }
+  if ((int)o != fft_tw_size(N)) return fail(FIL_EINVAL, "internal: twiddle table size");
+  HIPCHK(upload(buf, tw.b.data(), tw.b.size()));
+  return 0;
+}
+
+int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt) {
   const int Nr = h->Nr, Nrp = h->Nrp, Nt = h->Nt, Nh = h->Nh;
   const bool f64 = h->fp64;
-  if (!t->r || !t->dr || !t->KZ || !t->Kguard || !t->Wguard || !t->Rguard || !t->Tguard || !t->dht_T ||
-      !t->dht_RJ || !t->dht_JV)
-    return fail(FIL_EINVAL, "missing linear-propagator table");
+  const bool bug = h->xyt && h->d.guard3d_bug;
+  if (!t->Wguard || !t->Tguard || !t->t) return fail(FIL_EINVAL, "missing t grid or guard table");
   if (!(t->dt > 0)) return fail(FIL_EINVAL, "dt must be positive");
   h->dt = t->dt;
-  h->r.assign(t->r, t->r + Nr);
-  h->rdr.resize(Nr);
-  for (int i = 0; i < Nr; i++) h->rdr[i] = t->r[i] * t->dr[i];
-  if (!t->t) return fail(FIL_EINVAL, "missing t grid");
   h->t.assign(t->t, t->t + Nt);
-  HIPCHK(upload(h->ardr, h->rdr.data(), Nr * 8));
+  std::vector<double> wt(Nr);    // analyzer weights of the rows
+  if (!h->xyt) {
+    if (!t->r || !t->dr || !t->KZ || !t->Kguard || !t->Rguard || !t->dht_T || !t->dht_RJ || !t->dht_JV)
+      return fail(FIL_EINVAL, "missing linear-propagator table");
+    h->r.assign(t->r, t->r + Nr);
+    h->rdr.resize(Nr);
+    for (int i = 0; i < Nr; i++) { h->rdr[i] = t->r[i] * t->dr[i]; wt[i] = 2 * kPi * h->rdr[i]; }
+  } else {
+    if (!xt->x || !xt->y || !xt->kt2x || !xt->kt2y || !xt->Xguard || !xt->Yguard || !xt->KXguard || !xt->KYguard ||
+        !xt->beta || !xt->w_vf)
+      return fail(FIL_EINVAL, "missing XYT table");
+    if (!(xt->dx > 0) || !(xt->dy > 0) || !(xt->zu > 0)) return fail(FIL_EINVAL, "dx, dy and zu must be positive");
+    h->x.assign(xt->x, xt->x + h->Nx);
+    h->y.assign(xt->y, xt->y + h->Ny);
+    h->dx = xt->dx;
+    h->dy = xt->dy;
+    for (int i = 0; i < Nr; i++) wt[i] = xt->dx * xt->dy;
+  }
+  HIPCHK(upload(h->ardr, wt.data(), Nr * 8));
 
+  if (h->xyt) {
+    // propagator inputs (spectral.jl:131-184, Guards.jl:162-195) and the x / y FFT twiddles
+    std::vector<double> beta(2 * (size_t)Nh), wvf(Nh);
+    for (int w = 0; w < Nh; w++) { beta[2 * w] = xt->beta[2 * w]; beta[2 * w + 1] = xt->beta[2 * w + 1]; wvf[w] = xt->w_vf[w]; }
+    HIPCHK(upload(h->xkt2x, xt->kt2x, h->Nx * 8));
+    HIPCHK(upload(h->xkt2y, xt->kt2y, h->Ny * 8));
+    HIPCHK(upload(h->xKX, xt->KXguard, h->Nx * 8));
+    HIPCHK(upload(h->xKY, xt->KYguard, h->Ny * 8));
+    HIPCHK(upload(h->xbeta, beta.data(), beta.size() * 8));
+    HIPCHK(upload(h->xwvf, wvf.data(), wvf.size() * 8));
+    HIPCHK(upload(h->xWg, t->Wguard, (size_t)Nt * 8));
+    XytProp& xp = h->xprop;
+    xp.Nx = h->Nx; xp.Ny = h->Ny; xp.Nh = Nh;
+    xp.kparaxial = xt->kparaxial != 0;
+    xp.guard3d_bug = bug ? 1 : 0;
+    xp.kt2x = h->xkt2x.as<double>(); xp.kt2y = h->xkt2y.as<double>();
+    xp.KX = h->xKX.as<double>(); xp.KY = h->xKY.as<double>();
+    xp.beta = h->xbeta.as<double>(); xp.wvf = h->xwvf.as<double>(); xp.Wg = h->xWg.as<double>();
+    xp.zu = xt->zu;
+    xp.norm = 1.0 / ((double)h->Nx * h->Ny);
+    h->P_dz = NAN;
+    if (int rc = build_twiddles(h->twx, h->Nx, f64)) return rc;
+    if (int rc = build_twiddles(h->twy, h->Ny, f64)) return rc;
+  }
+
+  if (!h->xyt) {
   // folded QDHT matrices M1 = T diag(RJ), M2 = diag(1/RJ) T, zero padded
   std::vector<double> M1((size_t)Nrp * Nrp, 0.0), M2((size_t)Nrp * Nrp, 0.0);
   for (int m = 0; m < Nr; m++)
@@ -524,29 +665,22 @@ int fil_set_tables(fil_handle* h, const fil_tables* t) {
     HIPCHK(upload(h->KG64, kg.data(), kg.size() * 8));
     h->P_dz = NAN;
   }
-  // twiddles, guards
+  }  // RT linear-step tables
+  // twiddles, field guards: Rg per row, Tg per t sample (Guards.jl:265-318).  The
+  // XYT filter is X(x) Y(y) T(t); with guard3d_bug the reference's kernel
+  // multiplies by T at the y index instead (Guards.jl:314), which the tables
+  // express as Rg = X Y T(iy) and Tg = 1.
   {
-    // per-pass twiddle table in the layout fft_lds reads (fil_fft.h)
-    HostArr tw(std::max(1, fft_tw_size(Nt)), f64, true), tg(Nt, f64, false), rg(Nr, f64, false);
-    {
-      size_t o = 0;
-      for (int s = 1; Nt / s > 1; s *= fft_radix(Nt, s)) {
-        const int rad = fft_radix(Nt, s);
-        if (s < Nt / rad) {
-          const int P = Nt / (s * rad);
-          for (int p = 0; p < P; p++)
-            for (int k = 0; k < FIL_TW_STRIDE; k++) {
-              const double a = -2.0 * kPi * (double)(((long)p * s * k) % Nt) / Nt;
-              if (k < rad) tw.setc(o, std::cos(a), std::sin(a));
-              o++;
-            }
-        }
-      }
-      if ((int)o != fft_tw_size(Nt)) return fail(FIL_EINVAL, "internal: twiddle table size");
+    if (int rc = build_twiddles(h->tw, Nt, f64)) return rc;
+    HostArr tg(Nt, f64, false), rg(Nr, f64, false);
+    for (int j = 0; j < Nt; j++) tg.set(j, bug ? 1.0 : t->Tguard[j]);
+    if (!h->xyt) {
+      for (int i = 0; i < Nr; i++) rg.set(i, t->Rguard[i]);
+    } else {
+      for (int iy = 0; iy < h->Ny; iy++)
+        for (int ix = 0; ix < h->Nx; ix++)
+          rg.set((size_t)ix + (size_t)h->Nx * iy, xt->Xguard[ix] * xt->Yguard[iy] * (bug ? t->Tguard[iy] : 1.0));
     }
-    for (int j = 0; j < Nt; j++) tg.set(j, t->Tguard[j]);
-    for (int i = 0; i < Nr; i++) rg.set(i, t->Rguard[i]);
-    HIPCHK(upload(h->tw, tw.b.data(), tw.b.size()));
     HIPCHK(upload(h->Tg, tg.b.data(), tg.b.size()));
     HIPCHK(upload(h->Rg, rg.b.data(), rg.b.size()));
   }
@@ -650,6 +784,24 @@ int fil_set_tables(fil_handle* h, const fil_tables* t) {
   return FIL_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int fil_set_tables(fil_handle* h, const fil_tables* t) {
+  if (!valid_handle(h) || !t) return fail(FIL_EINVAL, "null argument");
+  if (h->xyt) return fail(FIL_EINVAL, "XYT handles take fil_set_tables_xyt");
+  HIPCHK(hipSetDevice(h->d.device));
+  return set_tables_impl(h, t, nullptr);
+}
+
+int fil_set_tables_xyt(fil_handle* h, const fil_tables* t, const fil_tables_xyt* x) {
+  if (!valid_handle(h) || !t || !x) return fail(FIL_EINVAL, "null argument");
+  if (!h->xyt) return fail(FIL_EINVAL, "fil_set_tables_xyt needs an XYT handle");
+  HIPCHK(hipSetDevice(h->d.device));
+  return set_tables_impl(h, t, x);
+}
+
 int fil_upload_field(fil_handle* h, const double* E, int64_t ld) {
   if (!valid_handle(h) || !E) return fail(FIL_EINVAL, "null argument");
   if (!h->tables) return fail(FIL_ESTATE, "fil_set_tables first");
@@ -714,9 +866,19 @@ int fil_zstep(fil_handle* h, double z, double dz) {
 
 int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
   if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (h->xyt) return fail(FIL_EINVAL, "XYT handles take fil_analyze_xyt");
   if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_analyze");
   HIPCHK(hipSetDevice(h->d.device));
   return h->fp64 ? analyze_impl<double>(h, z, d, Fr, Ft, rho_end, S) : analyze_impl<float>(h, z, d, Fr, Ft, rho_end, S);
+}
+
+int fil_analyze_xyt(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, double* Ft, double* rho_end, double* S) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (!h->xyt) return fail(FIL_EINVAL, "fil_analyze_xyt needs an XYT handle");
+  if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_analyze_xyt");
+  HIPCHK(hipSetDevice(h->d.device));
+  return h->fp64 ? analyze_xyt_impl<double>(h, z, d, Fxy, Ft, rho_end, S)
+                 : analyze_xyt_impl<float>(h, z, d, Fxy, Ft, rho_end, S);
 }
 
 int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld) {
@@ -865,6 +1027,7 @@ int fil_timing_read(fil_handle* h, double* ms_row, double* ms_dht1, double* ms_d
 int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
   if (!valid_handle(h) || !cd) return fail(FIL_EINVAL, "null argument");
   if (h->fp64) return fail(FIL_EINVAL, "the corrector needs an FP32 handle");
+  if (h->xyt) return fail(FIL_EINVAL, "the corrector is defined for the RT geometry");
   if (!h->tables) return fail(FIL_ESTATE, "fil_set_tables first");
   if (cd->pool_r < 1 || cd->pool_t < 1) return fail(FIL_EINVAL, "pool sizes must be >= 1");
   if (cd->nr < 1 || cd->nr > h->Nr || cd->nr % cd->pool_r) return fail(FIL_EINVAL, "nr must be in [1, Nr] and a multiple of pool_r");
@@ -895,7 +1058,8 @@ int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
             c.gA.alloc(B * CORR_CH * HW * 4) == hipSuccess && c.gB.alloc(B * CORR_CH * HW * 4) == hipSuccess &&
             c.idx.alloc(B * 4) == hipSuccess && c.part.alloc((B * 2 * HW / 256 + 1024) * 8) == hipSuccess;
   for (int l = 0; l < 3 && ok; l++) ok = c.act[l].alloc(B * CORR_CH * HW * 4) == hipSuccess;
-  if (ok) ok = c.carrier.alloc((size_t)h->Nt * 8) == hipSuccess;
+  if (ok) ok = c.carrier.alloc((size_t)h->Nt * 8) == hipSuccess &&
+              c.wpart.alloc((size_t)CORR_WCHUNK_MAX * CORR_CH * (CORR_CH + 1) * 9 * 4) == hipSuccess;
   if (!ok) { c = fil_handle::Corr(); return fail(FIL_ENOMEM, "device allocation failed"); }
   {
     std::vector<float> car(2 * (size_t)h->Nt);

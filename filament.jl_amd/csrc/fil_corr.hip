@@ -25,46 +25,60 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 constexpr int CT = 128;   // threads per block of the convolution kernels
 constexpr int CG = 8;     // channels per thread in the forward / data-gradient kernels
 
-// Output channels are split into groups of G per thread (blockIdx.z), so a
-// single sample still spreads over the whole chip.
+// Forward: a block owns a TY x TX tile of output pixels of one sample and G
+// output channels (blockIdx.z); the input tile with its one-pixel halo is
+// staged in LDS eight input channels at a time, so each input value is read
+// from memory once per block instead of nine times per thread.
+constexpr int FTY = 4, FTX = 32;           // output tile of the forward kernel (FTY*FTX == CT)
+static_assert(FTY * FTX == CT, "one thread per output pixel of the tile");
+
 template <int CIN, int COUT, int G>
 __global__ __launch_bounds__(CT) void conv_fwd_kernel(const float* __restrict__ in, const int* __restrict__ idx,
                                                       const uint16_t* __restrict__ wbf, const float* __restrict__ bias,
                                                       float* __restrict__ out, int H, int W, int relu) {
   static_assert(COUT % G == 0, "group size must divide the output channels");
+  constexpr int CC = CIN < 8 ? CIN : 8;     // input channels per staged chunk
+  static_assert(CIN % CC == 0, "chunk must divide the input channels");
+  constexpr int HY = FTY + 2, HX = FTX + 2;
   __shared__ __align__(16) float wl[CIN * 9 * G];   // [ci][tap][co in group]
+  __shared__ float tin[CC][HY][HX];
   const int co0 = blockIdx.z * G;
-  for (int q = threadIdx.x; q < CIN * 9 * G; q += CT) {
+  const int tid = threadIdx.x;
+  for (int q = tid; q < CIN * 9 * G; q += CT) {
     const int co = q % G, r = q / G, tap = r % 9, ci = r / 9;
     wl[q] = bf2f(wbf[((co0 + co) * CIN + ci) * 9 + tap]);
   }
-  __syncthreads();
   const int HW = H * W;
   const int b = blockIdx.y;
-  const int p = blockIdx.x * CT + threadIdx.x;
-  if (p >= HW) return;
-  const int y = p / W, x = p % W;
+  const int ntx = (W + FTX - 1) / FTX;
+  const int y0 = (blockIdx.x / ntx) * FTY, x0 = (blockIdx.x % ntx) * FTX;
+  const int ty = tid / FTX, tx = tid % FTX;
+  const int y = y0 + ty, x = x0 + tx;
   const float* inb = in + (size_t)(idx ? idx[b] : b) * CIN * HW;
   float acc[G];
 #pragma unroll
   for (int co = 0; co < G; co++) acc[co] = bias[co0 + co];
 #pragma unroll 1
-  for (int ci = 0; ci < CIN; ci++) {
-    const float* ip = inb + (size_t)ci * HW;
-    float v[9];
-#pragma unroll
-    for (int tap = 0; tap < 9; tap++) {
-      const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-      v[tap] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? ip[yy * W + xx] : 0.0f;
+  for (int c0 = 0; c0 < CIN; c0 += CC) {
+    __syncthreads();
+    for (int e = tid; e < CC * HY * HX; e += CT) {
+      const int ci = e / (HY * HX), r = e % (HY * HX), yy = y0 - 1 + r / HX, xx = x0 - 1 + r % HX;
+      tin[ci][r / HX][r % HX] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? inb[(size_t)(c0 + ci) * HW + yy * W + xx] : 0.0f;
     }
+    __syncthreads();
 #pragma unroll
-    for (int tap = 0; tap < 9; tap++) {
-      const float* wr = wl + (ci * 9 + tap) * G;
+    for (int ci = 0; ci < CC; ci++) {
 #pragma unroll
-      for (int co = 0; co < G; co++) acc[co] += wr[co] * v[tap];
+      for (int tap = 0; tap < 9; tap++) {
+        const float v = tin[ci][ty + tap / 3][tx + tap % 3];
+        const float* wr = wl + ((c0 + ci) * 9 + tap) * G;
+#pragma unroll
+        for (int co = 0; co < G; co++) acc[co] += wr[co] * v;
+      }
     }
   }
-  float* ob = out + ((size_t)b * COUT + co0) * HW + p;
+  if (y >= H || x >= W) return;
+  float* ob = out + ((size_t)b * COUT + co0) * HW + (size_t)y * W + x;
 #pragma unroll
   for (int co = 0; co < G; co++) ob[(size_t)co * HW] = relu ? fmaxf(acc[co], 0.0f) : acc[co];
 }
@@ -114,32 +128,33 @@ __global__ __launch_bounds__(CT) void conv_bwd_data_kernel(const float* __restri
   }
 }
 
-// One block per (ci, co); ci == CIN gives the bias gradient.  Fixed summation
-// order (thread-strided pixels, then a fixed tree), so results are reproducible.
+// Weight gradient in two deterministic passes.  Pass 1: one block per (ci, co,
+// chunk of the batch's pixels); ci == CIN gives the bias gradient; it writes
+// its nine partial sums.  Pass 2 adds the chunks in order.
 template <int CIN, int COUT>
 __global__ __launch_bounds__(CT) void conv_bwd_weight_kernel(const float* __restrict__ dout, const float* __restrict__ in,
-                                                             const int* __restrict__ idx, float* __restrict__ dW,
-                                                             float* __restrict__ dbias, int B, int H, int W) {
+                                                             const int* __restrict__ idx, float* __restrict__ part,
+                                                             int B, int H, int W, int chunk) {
   __shared__ float red[9][CT / 64];
-  const int ci = blockIdx.x, co = blockIdx.y, tid = threadIdx.x;
+  const int ci = blockIdx.x, co = blockIdx.y, ch = blockIdx.z, tid = threadIdx.x;
   const int HW = H * W;
+  const long total = (long)B * HW;
+  const long p0 = (long)ch * chunk, p1 = p0 + chunk < total ? p0 + chunk : total;
   float s[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) s[k] = 0.0f;
-  for (int b = 0; b < B; b++) {
-    const float* gp = dout + ((size_t)b * COUT + co) * HW;
+  for (long pg = p0 + tid; pg < p1; pg += CT) {
+    const int b = (int)(pg / HW), p = (int)(pg % HW);
+    const float g = dout[((size_t)b * COUT + co) * HW + p];
     if (ci == CIN) {
-      for (int p = tid; p < HW; p += CT) s[0] += gp[p];
+      s[0] += g;
     } else {
       const float* ip = in + ((size_t)(idx ? idx[b] : b) * CIN + ci) * HW;
-      for (int p = tid; p < HW; p += CT) {
-        const int y = p / W, x = p % W;
-        const float g = gp[p];
+      const int y = p / W, x = p % W;
 #pragma unroll
-        for (int tap = 0; tap < 9; tap++) {
-          const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
-          if (yy >= 0 && yy < H && xx >= 0 && xx < W) s[tap] += g * ip[yy * W + xx];
-        }
+      for (int tap = 0; tap < 9; tap++) {
+        const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) s[tap] += g * ip[yy * W + xx];
       }
     }
   }
@@ -154,9 +169,20 @@ __global__ __launch_bounds__(CT) void conv_bwd_weight_kernel(const float* __rest
   if (tid < 9) {
     float v = 0.0f;
     for (int w = 0; w < CT / 64; w++) v += red[tid][w];
-    if (ci == CIN) { if (tid == 0) dbias[co] = v; }
-    else dW[(co * CIN + ci) * 9 + tid] = v;
+    part[(((size_t)ch * COUT + co) * (CIN + 1) + ci) * 9 + tid] = v;
   }
+}
+
+template <int CIN, int COUT>
+__global__ void conv_bwd_weight_reduce_kernel(const float* __restrict__ part, int nchunk, float* __restrict__ dW,
+                                              float* __restrict__ dbias) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;     // over COUT x (CIN+1) x 9
+  if (q >= COUT * (CIN + 1) * 9) return;
+  float v = 0.0f;
+  for (int c = 0; c < nchunk; c++) v += part[(size_t)c * COUT * (CIN + 1) * 9 + q];
+  const int tap = q % 9, r = q / 9, ci = r % (CIN + 1), co = r / (CIN + 1);
+  if (ci == CIN) { if (tap == 0) dbias[co] = v; }
+  else dW[(co * CIN + ci) * 9 + tap] = v;
 }
 
 __global__ __launch_bounds__(256) void corr_loss_kernel(const float* __restrict__ out, const float* __restrict__ Y,
@@ -227,8 +253,9 @@ static_assert(CORR_LAYERS == 4, "layer dispatch covers four layers");
 hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const uint16_t* wbf, const float* bias, float* out,
                            int B, int H, int W, int relu, hipStream_t st) {
   FIL_CORR_LAYER(l, {
-    constexpr int G = CO < CG ? CO : CG;
-    const dim3 grid((H * W + CT - 1) / CT, B, CO / G);
+    constexpr int G = CO < 4 ? CO : 4;
+    const int tiles = ((H + FTY - 1) / FTY) * ((W + FTX - 1) / FTX);
+    const dim3 grid(tiles, B, CO / G);
     hipLaunchKernelGGL((conv_fwd_kernel<CI, CO, G>), grid, dim3(CT), 0, st, in, idx, wbf, bias, out, H, W, relu);
   });
   return hipGetLastError();
@@ -245,9 +272,18 @@ hipError_t launch_conv_bwd_data(int l, const float* dout, const uint16_t* wbf, c
 }
 
 hipError_t launch_conv_bwd_weight(int l, const float* dout, const float* in, const int* idx, float* dW, float* db,
-                                  int B, int H, int W, hipStream_t st) {
-  FIL_CORR_LAYER(l, hipLaunchKernelGGL((conv_bwd_weight_kernel<CI, CO>), dim3(CI + 1, CO), dim3(CT), 0, st, dout, in,
-                                       idx, dW, db, B, H, W));
+                                  float* part, int B, int H, int W, hipStream_t st) {
+  const long total = (long)B * H * W;
+  int nchunk = (int)((total + CORR_WCHUNK_PIX - 1) / CORR_WCHUNK_PIX);
+  if (nchunk > CORR_WCHUNK_MAX) nchunk = CORR_WCHUNK_MAX;
+  const int chunk = (int)((total + nchunk - 1) / nchunk);
+  FIL_CORR_LAYER(l, {
+    hipLaunchKernelGGL((conv_bwd_weight_kernel<CI, CO>), dim3(CI + 1, CO, nchunk), dim3(CT), 0, st, dout, in, idx,
+                       part, B, H, W, chunk);
+    const int n = CO * (CI + 1) * 9;
+    hipLaunchKernelGGL((conv_bwd_weight_reduce_kernel<CI, CO>), dim3((n + 255) / 256), dim3(256), 0, st, part, nchunk,
+                       dW, db);
+  });
   return hipGetLastError();
 }
 

@@ -104,20 +104,18 @@ __host__ __device__ constexpr int lds_step(int c) { return c + (c >> 4); }
 // Inputs  x[q + S (p + k N/(S RAD))] = x[b + k N/RAD],  b = p S + q;
 // outputs y[q + S (RAD p + k)] = DFT_RAD(...)[k] * w^(p S k),  w = exp(SIGN 2 pi i / N).
 // LDS addresses are one padded base per thread plus compile-time offsets:
-// for reads lds_index(b + k NB) = lds_index(b) + lds_step(k NB) (NB a multiple
-// of 16); for writes lds_index(w0 + S k) = lds_index(w0) + lds_step(S k) with
+// for reads lds_index(b + k NB) = lds_index(b) + lds_step(k NB) when NB is a
+// multiple of 16 (N >= 128); for writes lds_index(w0 + S k) = lds_index(w0) + lds_step(S k) with
 // w0 = q + S RAD p, because S, RAD and 16 are powers of two and q < S keeps
 // the low four bits of w0 + S k from carrying.
 template <typename T, int N, int NTH, int SIGN, int RAD, int S>
-__device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict__ tw) {
+__device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict__ tw, int tid) {
   constexpr int NB = N / RAD;
   constexpr int BPT = (NB + NTH - 1) / NTH;
   constexpr bool TW = S < N / RAD;     // the last pass has p == 0: no twiddles
-  static_assert(NB % 16 == 0, "read offsets assume N/RAD to be a multiple of 16");
   // The LDS addresses of a pass depend only on the thread index; without this
   // opaque copy the compiler hoists them out of every FFT of the kernel and
   // keeps them all live, which costs ~50 VGPRs and halves the occupancy.
-  int tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
   cx<T> a[BPT][RAD];
   cx<T> w[BPT][TW ? RAD : 1];
@@ -138,9 +136,14 @@ __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict_
   for (int u = 0; u < BPT; u++) {
     const int b = tid + u * NTH;
     if (NB % NTH == 0 || b < NB) {
-      const cx<T>* zr = z + lds_index(b);
+      if constexpr (NB % 16 == 0) {
+        const cx<T>* zr = z + lds_index(b);
 #pragma unroll
-      for (int k = 0; k < RAD; k++) a[u][k] = zr[lds_step(k * NB)];
+        for (int k = 0; k < RAD; k++) a[u][k] = zr[lds_step(k * NB)];
+      } else {   // short transforms: the read stride does not keep the padding aligned
+#pragma unroll
+        for (int k = 0; k < RAD; k++) a[u][k] = z[lds_index(b + k * NB)];
+      }
     }
   }
   __syncthreads();
@@ -164,13 +167,13 @@ __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict_
 
 template <typename T, int N, int NTH, int SIGN, int S>
 struct FftPasses {
-  static __device__ __forceinline__ void run(cx<T>* z, const cx<T>* tw) {
+  static __device__ __forceinline__ void run(cx<T>* z, const cx<T>* tw, int tid) {
     constexpr int REM = N / S;
     if constexpr (REM >= FIL_FFT_MAXRAD) {
-      stockham_pass<T, N, NTH, SIGN, FIL_FFT_MAXRAD, S>(z, tw);
-      FftPasses<T, N, NTH, SIGN, S * FIL_FFT_MAXRAD>::run(z, tw);
+      stockham_pass<T, N, NTH, SIGN, FIL_FFT_MAXRAD, S>(z, tw, tid);
+      FftPasses<T, N, NTH, SIGN, S * FIL_FFT_MAXRAD>::run(z, tw, tid);
     } else if constexpr (REM > 1) {
-      stockham_pass<T, N, NTH, SIGN, REM, S>(z, tw);
+      stockham_pass<T, N, NTH, SIGN, REM, S>(z, tw, tid);
     }
   }
 };
@@ -178,10 +181,17 @@ struct FftPasses {
 // SIGN = -1: y_k = sum_j x_j exp(-2 pi i jk/N)   (the reference's P * E)
 // SIGN = +1: y_k = sum_j x_j exp(+2 pi i jk/N)   (N times the reference's P \ E)
 // z is stored with lds_index(); tw: the per-pass twiddle table (global memory).
+// tid: the thread's index among the NTH threads working on this transform
+// (several transforms may share a workgroup; every pass ends in a
+// workgroup-wide barrier, so all threads of the workgroup must call it).
+template <typename T, int N, int NTH, int SIGN>
+__device__ __forceinline__ void fft_lds(cx<T>* z, const cx<T>* __restrict__ tw, int tid) {
+  static_assert((N & (N - 1)) == 0 && N >= 16, "N must be a power of two >= 16");
+  FftPasses<T, N, NTH, SIGN, 1>::run(z, tw, tid);
+}
 template <typename T, int N, int NTH, int SIGN>
 __device__ __forceinline__ void fft_lds(cx<T>* z, const cx<T>* __restrict__ tw) {
-  static_assert((N & (N - 1)) == 0 && N >= 16, "N must be a power of two >= 16");
-  FftPasses<T, N, NTH, SIGN, 1>::run(z, tw);
+  fft_lds<T, N, NTH, SIGN>(z, tw, (int)threadIdx.x);
 }
 
 }  // namespace fil

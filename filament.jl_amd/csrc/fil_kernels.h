@@ -33,7 +33,11 @@ hipError_t launch_to_time(const cx<R>* S, cx<R>* Et, const cx<R>* tw, const R* R
 template <typename R>
 hipError_t launch_analyze_rows(cx<R>* S, const cx<R>* tw, const R* Rg, const R* Tg, int apply, R dt, double* Fr,
                                float* It, float* Sw, double* Imax_row, int Nr, int N, hipStream_t st);
-hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* rdr, double* out, hipStream_t st);
+// out[j] = sum_i in[i][j] * wt[i] over Nr rows, FP64, in a fixed order (row chunks, then the
+// chunks in order); part: scratch of COLSUM_MAXCHUNK * ncol doubles
+constexpr int COLSUM_MAXCHUNK = 256;
+hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* wt, double* part, double* out,
+                         hipStream_t st);
 template <typename R>
 hipError_t launch_rowsum_abs2(const cx<R>* S, int Nr, int Nh, double scale, double* out, hipStream_t st);
 
@@ -79,9 +83,12 @@ hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const uint16_
 // din = conv^T(dout) (* [act > 0] when act != null)
 hipError_t launch_conv_bwd_data(int l, const float* dout, const uint16_t* wbf, const float* act, float* din, int B,
                                 int H, int W, hipStream_t st);
-// dW[cout][cin][9], db[cout] summed over the batch (deterministic order)
+// dW[cout][cin][9], db[cout] summed over the batch (deterministic order); part: scratch of
+// CORR_WCHUNK_MAX * CORR_CH * (CORR_CH + 1) * 9 floats for the per-chunk partial sums
+constexpr int CORR_WCHUNK_PIX = 4096;   // pixels per partial-sum chunk
+constexpr int CORR_WCHUNK_MAX = 64;
 hipError_t launch_conv_bwd_weight(int l, const float* dout, const float* in, const int* idx, float* dW, float* db,
-                                  int B, int H, int W, hipStream_t st);
+                                  float* part, int B, int H, int W, hipStream_t st);
 // dout = 2 (out - Y[idx[b]] * inv_ys) / (B*2*H*W); part[blk] = partial sums of squares
 hipError_t launch_corr_loss(const float* out, const float* Y, const int* idx, float inv_ys, float* dout, double* part,
                             int B, int HW, int* nblk, hipStream_t st);
@@ -91,6 +98,27 @@ hipError_t launch_adam(float* w, const float* g, float* m, float* v, uint16_t* w
 hipError_t launch_round_bf16(const float* w, uint16_t* wbf, int n, hipStream_t st);
 // part[blk] = partial sums of x^2 over x[0, n)
 hipError_t launch_sum_squares(const float* x, int n, double* part, int* nblk, hipStream_t st);
+
+// XYT linear step (fil_xyt.hip).  Propagator inputs in FP64, device pointers.
+struct XytProp {
+  int Nx, Ny, Nh;
+  int kparaxial, guard3d_bug;
+  const double* kt2x;   // [Nx] (kx kxu)^2 [1/m^2]
+  const double* kt2y;   // [Ny]
+  const double* KX;     // [Nx] KXguard
+  const double* KY;     // [Ny] KYguard
+  const double* beta;   // [Nh] complex beta(w) [1/m]
+  const double* wvf;    // [Nh] w / vf [1/m]
+  const double* Wg;     // [Nt] Wguard
+  double zu;            // [m]
+  double norm;          // 1 / (Nx Ny), the inverse transform's normalisation
+};
+template <typename R>
+hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStream_t st);
+// S <- IFFT_xy[P .* FFT_xy[S]] for every w column of S [(ix + Nx iy)][Nh]
+template <typename R>
+hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const cx<R>* twx, const cx<R>* twy, int Nx, int Ny, int Nh,
+                             hipStream_t st);
 
 // Hankel-transform phase (fil_dht.hip)
 // Propagator P[k][w] = exp(i KZ dz) Kguard Wguard computed in FP64.

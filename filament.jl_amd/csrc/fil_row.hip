@@ -543,14 +543,23 @@ hipError_t launch_rowsum_abs2(const cx<R>* S, int Nr, int Nh, double scale, doub
 template hipError_t launch_rowsum_abs2<float>(const cx<float>*, int, int, double, double*, hipStream_t);
 template hipError_t launch_rowsum_abs2<double>(const cx<double>*, int, int, double, double*, hipStream_t);
 
-// Column sums over r: out[j] = 2 pi sum_i in[i][j] * rdr[i], in FP64, fixed order.
-__global__ void colsum_kernel(const float* __restrict__ in, int Nr, int ncol, const double* __restrict__ rdr,
-                              double* out) {
+// Column sums: out[j] = sum_i in[i][j] * wt[i] in FP64, in a fixed order.
+__global__ void colsum_part_kernel(const float* __restrict__ in, int Nr, int ncol, const double* __restrict__ wt,
+                                   int rows, double* __restrict__ part) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  const int i0 = blockIdx.y * rows, i1 = min(Nr, i0 + rows);
+  double s = 0;
+  for (int i = i0; i < i1; i++) s += (double)in[(size_t)i * ncol + j] * wt[i];
+  part[(size_t)blockIdx.y * ncol + j] = s;
+}
+
+__global__ void colsum_final_kernel(const double* __restrict__ part, int nchunk, int ncol, double* __restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ncol) return;
   double s = 0;
-  for (int i = 0; i < Nr; i++) s += (double)in[(size_t)i * ncol + j] * rdr[i];
-  out[j] = 6.283185307179586 * s;
+  for (int c = 0; c < nchunk; c++) s += part[(size_t)c * ncol + j];
+  out[j] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -734,8 +743,14 @@ hipError_t launch_analyze_rows(cx<R>* S, const cx<R>* tw, const R* Rg, const R* 
   return hipSuccess;
 }
 
-hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* rdr, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(colsum_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, in, Nr, ncol, rdr, out);
+hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* wt, double* part, double* out,
+                         hipStream_t st) {
+  int nchunk = (Nr + 63) / 64;
+  if (nchunk > COLSUM_MAXCHUNK) nchunk = COLSUM_MAXCHUNK;
+  const int rows = (Nr + nchunk - 1) / nchunk;
+  hipLaunchKernelGGL(colsum_part_kernel, dim3((ncol + 255) / 256, nchunk), dim3(256), 0, st, in, Nr, ncol, wt, rows,
+                     part);
+  hipLaunchKernelGGL(colsum_final_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, part, nchunk, ncol, out);
   return hipGetLastError();
 }
 

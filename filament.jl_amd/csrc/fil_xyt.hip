@@ -1,0 +1,180 @@
+// fil_xyt.hip -- the linear step of the XYT geometry on gfx950.
+//
+// propagate!(E, LP::LinearPropagatorSpectral, dz) (linear_propagators/spectral.jl:16-26)
+// with the 2-D FFT space plan of Field XYT (Fields.jl:152, plan_fft!(E, [1, 2])):
+//   E <- IFFT_xy[ P .* FFT_xy[E] ],  P = exp(i KZ dz) KXguard KYguard Wguard,
+// forward_transform_space! = P * E (unnormalised, exp(-i k x)),
+// inverse_transform_space! = P \ E (normalised) (Models.jl:135-148).
+//
+// The state is the half spectrum S[(ix + Nx iy)][w], w < Nt/2 contiguous.  The
+// 2-D transform of every w column is done in three passes over HBM:
+//   x pass      FFT along x of every (iy, w) line       (stride Nt/2)
+//   y pass      FFT along y, multiply by P, inverse FFT along y (stride Nx Nt/2)
+//   x pass      inverse FFT along x
+// Each workgroup owns one line (iy or ix) and WB consecutive w columns: it
+// reads WB contiguous complex values per grid point (full cache lines), runs
+// WB transforms in LDS, one group of N/8 threads per transform, and writes
+// them back.  1/(Nx Ny) of the inverse transform is folded into P.
+#include "fil_internal.h"
+#include "fil_fft.h"
+#include "fil_kernels.h"
+
+namespace fil {
+
+namespace {
+
+// complex FP64 helpers for the propagator
+struct cd { double x, y; };
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd cdiv(cd a, cd b) {
+  const double d = b.x * b.x + b.y * b.y;
+  return {(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+// principal square root; a zero imaginary part counts as +0 (the reference
+// adds 0im before sqrt, spectral.jl:182)
+__device__ __forceinline__ cd csqrt(cd z) {
+  const double r = hypot(z.x, z.y);
+  const double re = sqrt(0.5 * (r + z.x));
+  const double im = sqrt(0.5 * fmax(r - z.x, 0.0));
+  return {re, (z.y < 0.0) ? -im : im};
+}
+
+}  // namespace
+
+// P[(ix + Nx iy) Nh + w] for w < Nh (Kfunc spectral.jl:160-184, KZ XYT :131-156,
+// spectral filter Guards.jl:250-261 with kernel! :301-318).
+template <typename R>
+__global__ void xyt_propagator_kernel(XytProp pp, double dz, cx<R>* __restrict__ P) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)pp.Nx * pp.Ny * pp.Nh;
+  if (q >= n) return;
+  const int w = (int)(q % pp.Nh);
+  const size_t row = q / pp.Nh;
+  const int ix = (int)(row % pp.Nx), iy = (int)(row / pp.Nx);
+  const double kt2 = pp.kt2x[ix] + pp.kt2y[iy];
+  const cd beta = {pp.beta[2 * w], pp.beta[2 * w + 1]};
+  cd K;
+  if (pp.kparaxial) {
+    if (beta.x == 0.0 && beta.y == 0.0) K = {0.0, 0.0};
+    else {
+      const cd d = cdiv({kt2, 0.0}, {2.0 * beta.x, 2.0 * beta.y});
+      K = {beta.x - d.x, beta.y - d.y};
+    }
+  } else {
+    const cd b2 = cmul(beta, beta);
+    K = csqrt({b2.x - kt2, b2.y});
+  }
+  const double kzr = (K.x - pp.wvf[w]) * pp.zu, kzi = K.y * pp.zu;
+  // exp(i KZ dz)
+  const double mag = exp(-kzi * dz);
+  double sn, cs;
+  sincos(kzr * dz, &sn, &cs);
+  const double g = pp.KX[ix] * pp.KY[iy] * (pp.guard3d_bug ? pp.Wg[iy] : pp.Wg[w]) * pp.norm;
+  P[q] = mk((R)(g * mag * cs), (R)(g * mag * sn));
+}
+
+template <typename R>
+hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStream_t st) {
+  const size_t n = (size_t)pp.Nx * pp.Ny * pp.Nh;
+  hipLaunchKernelGGL(xyt_propagator_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pp, dz, P);
+  return hipGetLastError();
+}
+
+// Transforms of length N along a strided dimension: WB columns per workgroup.
+// MODE 0: forward (exp(-i)), 1: inverse (exp(+i), unnormalised),
+// 2: forward, multiply by P (same layout as S), inverse.
+template <int N> constexpr int xyt_col_threads() { return N / FIL_FFT_MAXRAD >= 2 ? N / FIL_FFT_MAXRAD : 2; }
+template <typename R, int N> constexpr int xyt_cols() {
+  return (sizeof(R) == 4) ? (N <= 512 ? 16 : 8) : (N <= 512 ? 8 : 4);
+}
+template <int N> constexpr int xyt_col_stride() { return lds_size(N) + 1; }   // +1: columns start in different banks
+
+template <typename R, int N, int MODE>
+__global__ __launch_bounds__((xyt_cols<R, N>() * xyt_col_threads<N>())) void xyt_fft_kernel(
+    cx<R>* S, const cx<R>* __restrict__ tw, const cx<R>* __restrict__ P, long line_stride, long elem_stride, int Nh) {
+  constexpr int WB = xyt_cols<R, N>();
+  constexpr int NTHC = xyt_col_threads<N>();
+  constexpr int NT = WB * NTHC;
+  constexpr int CS = xyt_col_stride<N>();
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
+  const int tid = threadIdx.x, col = tid / NTHC, ctid = tid % NTHC;
+  const int nwb = Nh / WB;
+  const long line = blockIdx.x / nwb;
+  const int w0 = (blockIdx.x % nwb) * WB;
+  const size_t base = (size_t)(line * line_stride + w0);
+  for (int q = tid; q < N * WB; q += NT) {
+    const int e = q / WB, c = q % WB;
+    Z[c * CS + lds_index(e)] = S[base + (size_t)e * elem_stride + c];
+  }
+  __syncthreads();
+  cx<R>* Zc = Z + col * CS;
+  if constexpr (MODE == 1) {
+    fft_lds<R, N, NTHC, +1>(Zc, tw, ctid);
+  } else {
+    fft_lds<R, N, NTHC, -1>(Zc, tw, ctid);
+  }
+  if constexpr (MODE == 2) {
+    for (int q = tid; q < N * WB; q += NT) {
+      const int e = q / WB, c = q % WB;
+      Z[c * CS + lds_index(e)] = Z[c * CS + lds_index(e)] * P[base + (size_t)e * elem_stride + c];
+    }
+    __syncthreads();
+    fft_lds<R, N, NTHC, +1>(Zc, tw, ctid);
+  }
+  for (int q = tid; q < N * WB; q += NT) {
+    const int e = q / WB, c = q % WB;
+    S[base + (size_t)e * elem_stride + c] = Z[c * CS + lds_index(e)];
+  }
+}
+
+#define FIL_DISPATCH_XY(N_, ...)                                  \
+  switch (N_) {                                                   \
+    case 16: { constexpr int NN = 16; __VA_ARGS__; } break;       \
+    case 32: { constexpr int NN = 32; __VA_ARGS__; } break;       \
+    case 64: { constexpr int NN = 64; __VA_ARGS__; } break;       \
+    case 128: { constexpr int NN = 128; __VA_ARGS__; } break;     \
+    case 256: { constexpr int NN = 256; __VA_ARGS__; } break;     \
+    case 512: { constexpr int NN = 512; __VA_ARGS__; } break;     \
+    case 1024: { constexpr int NN = 1024; __VA_ARGS__; } break;   \
+    default: return hipErrorInvalidValue;                         \
+  }
+
+template <typename R, int N, int MODE>
+static hipError_t launch_xyt_fft_n(cx<R>* S, const cx<R>* tw, const cx<R>* P, long nlines, long line_stride,
+                                   long elem_stride, int Nh, hipStream_t st) {
+  constexpr int WB = xyt_cols<R, N>();
+  constexpr int NT = WB * xyt_col_threads<N>();
+  if (Nh % WB) return hipErrorInvalidValue;
+  const size_t sm = (size_t)WB * xyt_col_stride<N>() * sizeof(cx<R>);
+  auto k = xyt_fft_kernel<R, N, MODE>;
+  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3((unsigned)(nlines * (Nh / WB))), dim3(NT), sm, st, S, tw, P, line_stride, elem_stride, Nh);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const cx<R>* twx, const cx<R>* twy, int Nx, int Ny, int Nh,
+                             hipStream_t st) {
+  const long nh = Nh;
+  hipError_t e;
+  // forward along x: lines iy, elements ix
+  FIL_DISPATCH_XY(Nx, e = (launch_xyt_fft_n<R, NN, 0>(S, twx, nullptr, Ny, (long)Nx * nh, nh, Nh, st)));
+  if (e != hipSuccess) return e;
+  // along y: forward, propagator, inverse; lines ix, elements iy
+  FIL_DISPATCH_XY(Ny, e = (launch_xyt_fft_n<R, NN, 2>(S, twy, P, Nx, nh, (long)Nx * nh, Nh, st)));
+  if (e != hipSuccess) return e;
+  // inverse along x
+  FIL_DISPATCH_XY(Nx, e = (launch_xyt_fft_n<R, NN, 1>(S, twx, nullptr, Ny, (long)Nx * nh, nh, Nh, st)));
+  return e;
+}
+
+template hipError_t launch_xyt_propagator<float>(const XytProp&, double, cx<float>*, hipStream_t);
+template hipError_t launch_xyt_propagator<double>(const XytProp&, double, cx<double>*, hipStream_t);
+template hipError_t launch_xyt_linear<float>(cx<float>*, const cx<float>*, const cx<float>*, const cx<float>*, int, int,
+                                             int, hipStream_t);
+template hipError_t launch_xyt_linear<double>(cx<double>*, const cx<double>*, const cx<double>*, const cx<double>*, int,
+                                              int, int, hipStream_t);
+
+}  // namespace fil

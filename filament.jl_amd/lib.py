@@ -12,9 +12,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libfilament_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAXCOMP = 4
-GEOM_RT = 0
+GEOM_RT, GEOM_XYT = 0, 1
 FP32, FP64 = 4, 8
 FORM_DHT_BF16X3, FORM_SCAN_FP64 = 1, 2
 RESP_CUBIC_THG, RESP_CUBIC_NOTHG, RESP_FREE, RESP_LOSSES = 1, 2, 4, 8
@@ -26,7 +26,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 class FilDesc(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "abi_version", "geometry", "Nr", "Nt", "precision", "form", "nonlinearity", "plasma",
-        "responses", "losses_ereal", "plasma_ereal", "plasma_kdep", "ncomp", "keep_plasma", "device")]
+        "responses", "losses_ereal", "plasma_ereal", "plasma_kdep", "ncomp", "keep_plasma", "device",
+        "Nx", "Ny", "guard3d_bug")]
 
 
 class FilTables(ctypes.Structure):
@@ -41,6 +42,19 @@ class FilTables(ctypes.Structure):
         ("K", ctypes.c_double * MAXCOMP), ("ntab", ctypes.c_int32 * MAXCOMP),
         ("tab_log10x", _dp * MAXCOMP), ("tab_log10y", _dp * MAXCOMP),
     ]
+
+
+class FilTablesXYT(ctypes.Structure):
+    _fields_ = [("x", _dp), ("y", _dp), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("kt2x", _dp), ("kt2y", _dp), ("Xguard", _dp), ("Yguard", _dp), ("KXguard", _dp), ("KYguard", _dp),
+                ("beta", _dp), ("w_vf", _dp), ("zu", ctypes.c_double), ("kparaxial", ctypes.c_int32)]
+
+
+class FilDiagXYT(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("z", "Fmax", "Imax", "rhomax", "De", "ax", "ay", "W")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class FilDiag(ctypes.Structure):
@@ -58,8 +72,8 @@ class FilCorrDesc(ctypes.Structure):
 
 CORR_NPARAM = 20258
 
-EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_upload_field", "fil_download_field",
-           "fil_zstep", "fil_analyze", "fil_download_plasma", "fil_synchronize", "fil_destroy",
+EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_set_tables_xyt", "fil_upload_field",
+           "fil_download_field", "fil_zstep", "fil_analyze", "fil_analyze_xyt", "fil_download_plasma", "fil_synchronize", "fil_destroy",
            "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_state_copy",
            "fil_fluence",
            "fil_corrector_init", "fil_corrector_params", "fil_corrector_scale", "fil_corrector_features",
@@ -84,6 +98,8 @@ def load():
     L.fil_last_error.restype = ctypes.c_char_p
     L.fil_create.argtypes = [ctypes.POINTER(FilDesc), ctypes.POINTER(ctypes.c_void_p)]
     L.fil_set_tables.argtypes = [ctypes.c_void_p, ctypes.POINTER(FilTables)]
+    L.fil_set_tables_xyt.argtypes = [ctypes.c_void_p, ctypes.POINTER(FilTables), ctypes.POINTER(FilTablesXYT)]
+    L.fil_analyze_xyt.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(FilDiagXYT), _dp, _dp, _dp, _dp]
     L.fil_upload_field.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
     L.fil_download_field.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
     L.fil_zstep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]

@@ -1,4 +1,4 @@
-"""Host-side setup of the RT problem in FP64 (the part the reference keeps in
+"""Host-side setup of the RT and XYT problems in FP64 (the part the reference keeps in
 Julia on the host): units, grids, medium, guards and the operator tables that
 are handed to libfilament_hip.so once.
 
@@ -25,6 +25,14 @@ class UnitRT:
     def __init__(self, ru, zu, tu, Iu, rhou):
         self.r, self.k, self.z, self.t = ru, 1.0 / ru, zu, tu
         self.w, self.lam, self.I, self.rho = 1.0 / tu, tu, Iu, rhou
+
+
+class UnitXYT:
+    """Units.jl:110-130 (XYT): the same time, intensity and density units as
+    RT, with x and y in place of r."""
+    def __init__(self, xu, yu, zu, tu, Iu, rhou):
+        self.x, self.y, self.kx, self.ky = xu, yu, 1.0 / xu, 1.0 / yu
+        self.z, self.t, self.w, self.lam, self.I, self.rho = zu, tu, 1.0 / tu, tu, Iu, rhou
 
 
 def field_unit(unit, n0):
@@ -54,6 +62,23 @@ class GridRT:
         self.k = j[:Nr] / rmax
         self.dr = _step(self.r)
         self.tmin, self.tmax, self.Nt = tmin, tmax, Nt
+        self.t = np.linspace(tmin, tmax, Nt)
+        self.dt = (tmax - tmin) / (Nt - 1)
+        self.w = 2 * np.pi * np.fft.fftfreq(Nt, self.dt)
+
+
+class GridXYT:
+    """Grids.jl:158-201: uniform x, y (range(min, max, length=N)),
+    kx = 2 pi fftfreq(Nx, 1/dx); t as for RT."""
+    def __init__(self, xmin, xmax, Nx, ymin, ymax, Ny, tmin, tmax, Nt):
+        self.Nx, self.Ny, self.Nt = Nx, Ny, Nt
+        self.x = np.linspace(xmin, xmax, Nx)
+        self.dx = (xmax - xmin) / (Nx - 1)
+        self.kx = 2 * np.pi * np.fft.fftfreq(Nx, self.dx)
+        self.y = np.linspace(ymin, ymax, Ny)
+        self.dy = (ymax - ymin) / (Ny - 1)
+        self.ky = 2 * np.pi * np.fft.fftfreq(Ny, self.dy)
+        self.tmin, self.tmax = tmin, tmax
         self.t = np.linspace(tmin, tmax, Nt)
         self.dt = (tmax - tmin) / (Nt - 1)
         self.w = 2 * np.pi * np.fft.fftfreq(Nt, self.dt)
@@ -161,6 +186,43 @@ class GuardRT:
         Kg[:, kmax == 0] = 0.0
         self.Kguard = Kg                                             # [Nr, Nt]
         self.Wguard = np.exp(-(((grid.w * unit.w) ** 2 / wguard ** 2) ** 20))
+
+
+class GuardXYT:
+    """Guards.jl:162-195: field windows X(x), Y(y), T(t); the spectral KX, KY
+    guards are ones in the reference (its kmax lines are commented out,
+    :176-184); W(w) = exp(-((w wu)^2 / wguard^2)^20)."""
+    def __init__(self, unit, grid, xguard, yguard, tguard, wguard):
+        self.Xguard = guard_window_both(grid.x, xguard)
+        self.Yguard = guard_window_both(grid.y, yguard)
+        self.Tguard = guard_window_both(grid.t, tguard)
+        self.KXguard = np.ones(grid.Nx)
+        self.KYguard = np.ones(grid.Ny)
+        self.Wguard = np.exp(-(((grid.w * unit.w) ** 2 / wguard ** 2) ** 20))
+
+
+def kz_factors_xyt(unit, grid, medium, w0, vf=None):
+    """The separable factors of KZ XYT (spectral.jl:131-156): beta(w) at the grid
+    frequencies, w/vf, and kt^2 = (kx kxu)^2 + (ky kyu)^2 split over x and y.
+    ``vf`` overrides the frame velocity (Media.group_velocity, a 5-point
+    derivative that two FP64 implementations round differently)."""
+    vf = medium.group_velocity(w0) if vf is None else vf
+    w = grid.w * unit.w
+    beta = np.asarray(medium.beta(w), dtype=complex)
+    return dict(beta=beta, w_vf=w / vf, kt2x=(grid.kx * unit.kx) ** 2, kt2y=(grid.ky * unit.ky) ** 2)
+
+
+def kz_table_xyt(unit, grid, medium, w0, paraxial):
+    """Full KZ[ix, iy, iw] (spectral.jl:131-156, Kfunc :160-184), for checks."""
+    f = kz_factors_xyt(unit, grid, medium, w0)
+    kt2 = f["kt2x"][:, None, None] + f["kt2y"][None, :, None]
+    beta = f["beta"][None, None, :]
+    if paraxial:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            K = np.where(beta != 0, beta - kt2 / (2 * beta), 0.0)
+    else:
+        K = np.sqrt(beta ** 2 - kt2 + 0j)
+    return (K - f["w_vf"][None, None, :]) * unit.z
 
 
 # ---------------------------------------------------------------------------

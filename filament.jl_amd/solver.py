@@ -1,20 +1,21 @@
-"""RT model on the host + hot path on libfilament_hip.so.
+"""RT and XYT models on the host + hot path on libfilament_hip.so.
 
-``RTModel(p)`` plays the role of Filament.run's setup for geometry "RT"
-(src/Filament.jl:69-76): Unit, Grid, Field, Medium, Guard, Model and AStep
-are built on the host in FP64 and the tables are handed to the library once.
-``zstep`` is Models.zstep (src/modules/Models.jl:75-121) on the GPU and
-``analyze`` is FieldAnalyzers.analyze! RT (FieldAnalyzers.jl:166-215).
+``RTModel(p)`` / ``XYTModel(p)`` play the role of Filament.run's setup for
+geometry "RT" / "XYT" (src/Filament.jl:69-76): Unit, Grid, Field, Medium,
+Guard, Model and AStep are built on the host in FP64 and the tables are handed
+to the library once.  ``zstep`` is Models.zstep (src/modules/Models.jl:75-121)
+on the GPU and ``analyze`` is FieldAnalyzers.analyze! (RT
+FieldAnalyzers.jl:166-215, XYT :287-353).
 
-``p`` carries the reference's input-file names (src/Input.jl:293-317 and the
-medium / initial-condition files), e.g. ``ru, zu, tu, Iu, rhou, rmax, Nr,
-tmin, tmax, Nt, lam0, permittivity, permeability, n2, NONLINEARITY, PLASMA,
-KPARAXIAL, QPARAXIAL, rguard, tguard, kguard, wguard, responses,
-plasma_equation, initial_condition``.
+``p`` carries the reference's input-file names (src/Input.jl:106-178 and the
+medium / initial-condition files), e.g. for RT ``ru, zu, tu, Iu, rhou, rmax,
+Nr, tmin, tmax, Nt, lam0, permittivity, permeability, n2, NONLINEARITY,
+PLASMA, KPARAXIAL, QPARAXIAL, rguard, tguard, kguard, wguard, responses,
+plasma_equation, initial_condition``; for XYT ``xu, yu`` and ``xmin, xmax, Nx,
+ymin, ymax, Ny, xguard, yguard, kxguard, kyguard`` in place of the r entries.
 """
 import ctypes
 import math
-import os
 
 import numpy as np
 
@@ -23,31 +24,27 @@ from . import physics as ph
 from .responses import response_kind
 
 
-class RTModel:
-    def __init__(self, p, precision=L.FP32, form=0, device=0, keep_plasma=False, basedir=".", KZ=None):
-        """``KZ`` overrides the host-computed linear-propagator table (an
-        [Nr, Nt] complex array), e.g. to drive the device with another
-        implementation's table."""
-        self.p = p
-        self.unit = ph.UnitRT(p["ru"], p["zu"], p["tu"], p["Iu"], p["rhou"])
-        self.grid = ph.GridRT(p["rmax"], p["Nr"], p["tmin"], p["tmax"], p["Nt"])
-        self.medium = ph.Medium(p["permittivity"], p.get("permeability", ph.unit_permeability), p["n2"])
-        u, g, m = self.unit, self.grid, self.medium
+class _Model:
+    """Host setup and boundary calls shared by the geometries.  Rows of the
+    device field are the spatial points (r, or (x, y) with x fastest); the t
+    axis is last."""
+
+    nrows = 0
+
+    def _physics(self, p, basedir):
+        u, g = self.unit, self.grid
+        self.medium = m = ph.Medium(p["permittivity"], p.get("permeability", ph.unit_permeability), p["n2"])
         self.w0 = 2 * math.pi * ph.C0 / p["lam0"]
         self.n0 = float(np.real(m.refractive_index(self.w0)))
         self.Eu = ph.field_unit(u, self.n0)
-        self.guard = ph.GuardRT(u, g, m, p["rguard"], p["tguard"], p["kguard"], p["wguard"])
-        self.KZ = ph.kz_table(u, g, m, self.w0, bool(p["KPARAXIAL"])) if KZ is None else np.asarray(KZ)
-        self.precision, self.form = precision, form
         self.nonlinearity = bool(p["NONLINEARITY"])
         self.plasma = bool(p["PLASMA"])
         if self.nonlinearity and not p.get("QPARAXIAL", True):
             raise NotImplementedError("QPARAXIAL=false is not implemented")
-
         # responses (nonlinear_propagators.jl:203-210)
         resp_flags = 0
         self.Rnl = {}
-        losses_ereal = 0
+        self.losses_ereal = 0
         nuc_dz, mr_dz = p.get("nuc", 0.0), p.get("mr", 1.0)
         if self.nonlinearity:
             for item in p["responses"]:
@@ -60,12 +57,13 @@ class RTModel:
                     self.Rnl["free"] = ph.rnl_current_free(u, g, self.Eu, item["nuc"], item["mr"])
                 elif kind == "current_losses":
                     resp_flags |= L.RESP_LOSSES
-                    losses_ereal = int(bool(item["EREAL"]))
+                    self.losses_ereal = int(bool(item["EREAL"]))
                     self.Rnl["losses"] = ph.rnl_current_losses(u, g, self.Eu, self.w0)
                 elif kind == "raman":
                     raise NotImplementedError("the Raman response is not implemented on the GPU path")
                 else:
                     raise ValueError(f"unknown response {kind}")
+            # QPARAXIAL: QZ depends on w only, also in XYT (nonlinear_propagators.jl:184-242)
             self.QZ = ph.qz_vector(u, g, m, self.w0, self.Eu)
         else:
             self.QZ = np.zeros(g.Nt, dtype=complex)
@@ -73,45 +71,39 @@ class RTModel:
         # adaptive step constants (AdaptiveSteps.jl:37-43)
         self.phik = ph.phi_kerr(u, m, self.w0, self.Eu) if p["n2"] != 0 else 0.0
         self.phip = ph.phi_plasma(u, m, self.w0, self.Eu, mr_dz, nuc_dz)
-
         # plasma (photoionization_avalanche.jl:1-116)
         self.pt = None
         if self.plasma:
             self.pt = ph.plasma_tables(u, self.w0, self.Eu, p["plasma_equation"], basedir)
 
-        # initial field (Fields.jl:336-360): user IC on the host in FP64, analytic signal
-        E0 = np.asarray(p["initial_condition"](g.r, g.t, u.r, u.t, u.I), dtype=float)
-        self.E0 = ph.rsig2asig(E0)
-
-        self.T, self.RJ, self.JV = ph.dht_plan(g)
-
+    def _desc(self, geometry, precision, form, keep_plasma, device):
         d = L.FilDesc()
         d.abi_version = L.ABI_VERSION
-        d.geometry = L.GEOM_RT
-        d.Nr, d.Nt = g.Nr, g.Nt
+        d.geometry = geometry
+        d.Nt = self.grid.Nt
         d.precision = precision
         d.form = form
         d.nonlinearity = int(self.nonlinearity)
         d.plasma = int(self.plasma)
-        d.responses = resp_flags
-        d.losses_ereal = losses_ereal
+        d.responses = self.responses
+        d.losses_ereal = self.losses_ereal
         if self.plasma:
             d.plasma_ereal = int(self.pt["EREAL"])
             d.plasma_kdep = int(self.pt["KDEP"])
             d.ncomp = len(self.pt["components"])
         d.keep_plasma = int(keep_plasma)
         d.device = device
+        return d
+
+    def _create(self, d):
         self._lib = L.load()
         h = ctypes.c_void_p()
         L.check(self._lib.fil_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
-        self._set_tables()
-        self.upload(self.E0)
 
-    # -- boundary ------------------------------------------------------------
-    def _set_tables(self):
+    def _common_tables(self, keep):
+        """fil_tables fields the geometries share; ``keep`` holds the arrays alive."""
         g = self.grid
-        keep = []
 
         def arr(a, cplx=False):
             a = np.ascontiguousarray(a, dtype=np.complex128 if cplx else np.float64)
@@ -119,13 +111,9 @@ class RTModel:
             return L.ptr(a.view(np.float64) if cplx else a)
 
         t = L.FilTables()
-        t.r, t.dr, t.t, t.dt = arr(g.r), arr(g.dr), arr(g.t), g.dt
-        t.KZ = arr(self.KZ.T, True)              # column-major [Nr x Nt]
-        t.Kguard = arr(self.guard.Kguard.T)
+        t.t, t.dt = arr(g.t), g.dt
         t.Wguard = arr(self.guard.Wguard)
-        t.Rguard = arr(self.guard.Rguard)
         t.Tguard = arr(self.guard.Tguard)
-        t.dht_T, t.dht_RJ, t.dht_JV = arr(self.T.T), arr(self.RJ), arr(self.JV)
         t.QZ = arr(self.QZ, True)
         zero = np.zeros(g.Nt, dtype=complex)
         t.Rnl_cubic = arr(self.Rnl.get("cubic", zero), True)
@@ -138,44 +126,39 @@ class RTModel:
                 t.ntab[q] = len(c["log10x"])
                 t.tab_log10x[q] = arr(c["log10x"])
                 t.tab_log10y[q] = arr(c["log10y"])
-        L.check(self._lib.fil_set_tables(self.h, ctypes.byref(t)))
+        return t, arr
 
+    # -- layout: rows x t on the device, the reference's array shape on the host
+    def _to_rows(self, E):
+        return np.asarray(E).reshape(self.nrows, self.grid.Nt, order="F")
+
+    def _from_rows(self, A):
+        return A
+
+    # -- boundary ------------------------------------------------------------
     def upload(self, E):
-        E = np.asfortranarray(np.asarray(E, dtype=np.complex128))
-        L.check(self._lib.fil_upload_field(self.h, L.ptr(E.ravel(order="F").view(np.float64)), self.grid.Nr))
+        E = np.asfortranarray(self._to_rows(np.asarray(E, dtype=np.complex128)))
+        L.check(self._lib.fil_upload_field(self.h, L.ptr(E.ravel(order="F").view(np.float64)), self.nrows))
 
     def field(self):
-        """field.E after the last step (time domain, guarded), [Nr, Nt]."""
-        g = self.grid
-        out = np.zeros(g.Nr * g.Nt, dtype=np.complex128)
-        L.check(self._lib.fil_download_field(self.h, L.ptr(out.view(np.float64)), g.Nr))
-        return out.reshape(g.Nt, g.Nr).T.copy()
+        """field.E after the last step (time domain, guarded), the reference's array shape."""
+        n, Nt = self.nrows, self.grid.Nt
+        out = np.zeros(n * Nt, dtype=np.complex128)
+        L.check(self._lib.fil_download_field(self.h, L.ptr(out.view(np.float64)), n))
+        return self._from_rows(out.reshape(Nt, n).T.copy())
 
     def plasma_arrays(self):
-        g = self.grid
-        rho = np.zeros(g.Nr * g.Nt)
-        kd = np.zeros(g.Nr * g.Nt)
-        L.check(self._lib.fil_download_plasma(self.h, L.ptr(rho), L.ptr(kd), g.Nr))
-        return rho.reshape(g.Nt, g.Nr).T.copy(), kd.reshape(g.Nt, g.Nr).T.copy()
+        n, Nt = self.nrows, self.grid.Nt
+        rho = np.zeros(n * Nt)
+        kd = np.zeros(n * Nt)
+        L.check(self._lib.fil_download_plasma(self.h, L.ptr(rho), L.ptr(kd), n))
+        return self._from_rows(rho.reshape(Nt, n).T.copy()), self._from_rows(kd.reshape(Nt, n).T.copy())
 
     def zstep(self, z, dz):
         L.check(self._lib.fil_zstep(self.h, float(z), float(dz)))
 
     def synchronize(self):
         L.check(self._lib.fil_synchronize(self.h))
-
-    def analyze(self, z=0.0, arrays=False):
-        g = self.grid
-        d = L.FilDiag()
-        if arrays:
-            Fr, Ft, re, S = np.zeros(g.Nr), np.zeros(g.Nt), np.zeros(g.Nr), np.zeros(g.Nt // 2 + 1)
-            L.check(self._lib.fil_analyze(self.h, float(z), ctypes.byref(d), L.ptr(Fr), L.ptr(Ft),
-                                          L.ptr(re), L.ptr(S)))
-            out = d.as_dict()
-            out.update(Fr=Fr, Ft=Ft, rho_end=re, S=S)
-            return out
-        L.check(self._lib.fil_analyze(self.h, float(z), ctypes.byref(d), None, None, None, None))
-        return d.as_dict()
 
     def dz_adaptive(self, dz0, phimax, Imax, rhomax):
         """AStepKerrPlasma (AdaptiveSteps.jl:96-107)."""
@@ -197,8 +180,9 @@ class RTModel:
         L.check(self._lib.fil_state_copy(self.h, other.h))
 
     def fluence(self, slot=-1):
-        """Fr(r) of the current state (slot < 0) or of a saved slot, by Parseval."""
-        Fr = np.zeros(self.grid.Nr)
+        """dt sum_t |E|^2 per row of the current state (slot < 0) or of a saved
+        slot, by Parseval on the device, before the pending field filter."""
+        Fr = np.zeros(self.nrows)
         L.check(self._lib.fil_fluence(self.h, int(slot), L.ptr(Fr)))
         return Fr
 
@@ -222,3 +206,120 @@ class RTModel:
             self.close()
         except Exception:
             pass
+
+
+class RTModel(_Model):
+    def __init__(self, p, precision=L.FP32, form=0, device=0, keep_plasma=False, basedir=".", KZ=None):
+        """``KZ`` overrides the host-computed linear-propagator table (an
+        [Nr, Nt] complex array), e.g. to drive the device with another
+        implementation's table."""
+        self.p = p
+        self.unit = u = ph.UnitRT(p["ru"], p["zu"], p["tu"], p["Iu"], p["rhou"])
+        self.grid = g = ph.GridRT(p["rmax"], p["Nr"], p["tmin"], p["tmax"], p["Nt"])
+        self.nrows = g.Nr
+        self.precision, self.form = precision, form
+        self._physics(p, basedir)
+        m = self.medium
+        self.guard = ph.GuardRT(u, g, m, p["rguard"], p["tguard"], p["kguard"], p["wguard"])
+        self.KZ = ph.kz_table(u, g, m, self.w0, bool(p["KPARAXIAL"])) if KZ is None else np.asarray(KZ)
+
+        # initial field (Fields.jl:336-360): user IC on the host in FP64, analytic signal
+        E0 = np.asarray(p["initial_condition"](g.r, g.t, u.r, u.t, u.I), dtype=float)
+        self.E0 = ph.rsig2asig(E0)
+        self.T, self.RJ, self.JV = ph.dht_plan(g)
+
+        d = self._desc(L.GEOM_RT, precision, form, keep_plasma, device)
+        d.Nr = g.Nr
+        self._create(d)
+        self._set_tables()
+        self.upload(self.E0)
+
+    def _set_tables(self):
+        g = self.grid
+        keep = []
+        t, arr = self._common_tables(keep)
+        t.r, t.dr = arr(g.r), arr(g.dr)
+        t.KZ = arr(self.KZ.T, True)              # column-major [Nr x Nt]
+        t.Kguard = arr(self.guard.Kguard.T)
+        t.Rguard = arr(self.guard.Rguard)
+        t.dht_T, t.dht_RJ, t.dht_JV = arr(self.T.T), arr(self.RJ), arr(self.JV)
+        L.check(self._lib.fil_set_tables(self.h, ctypes.byref(t)))
+
+    def analyze(self, z=0.0, arrays=False):
+        g = self.grid
+        d = L.FilDiag()
+        if arrays:
+            Fr, Ft, re, S = np.zeros(g.Nr), np.zeros(g.Nt), np.zeros(g.Nr), np.zeros(g.Nt // 2 + 1)
+            L.check(self._lib.fil_analyze(self.h, float(z), ctypes.byref(d), L.ptr(Fr), L.ptr(Ft),
+                                          L.ptr(re), L.ptr(S)))
+            out = d.as_dict()
+            out.update(Fr=Fr, Ft=Ft, rho_end=re, S=S)
+            return out
+        L.check(self._lib.fil_analyze(self.h, float(z), ctypes.byref(d), None, None, None, None))
+        return d.as_dict()
+
+
+class XYTModel(_Model):
+    def __init__(self, p, precision=L.FP32, form=0, device=0, keep_plasma=False, basedir=".", guard3d_bug=False,
+                 vf=None):
+        """``guard3d_bug``: reproduce the reference's 3-D filter kernel, which
+        takes the t (and w) factor at the y index (Guards.jl:314).  ``vf``
+        overrides the moving-frame velocity of KZ (e.g. another
+        implementation's value)."""
+        if form & L.FORM_DHT_BF16X3:
+            raise ValueError("the split-BF16 form applies to the RT Hankel transform only")
+        self.p = p
+        self.unit = u = ph.UnitXYT(p["xu"], p["yu"], p["zu"], p["tu"], p["Iu"], p["rhou"])
+        self.grid = g = ph.GridXYT(p["xmin"], p["xmax"], p["Nx"], p["ymin"], p["ymax"], p["Ny"],
+                                   p["tmin"], p["tmax"], p["Nt"])
+        self.nrows = g.Nx * g.Ny
+        self.precision, self.form = precision, form
+        self.guard3d_bug = bool(guard3d_bug)
+        self._physics(p, basedir)
+        self.guard = ph.GuardXYT(u, g, p["xguard"], p["yguard"], p["tguard"], p["wguard"])
+        self.kz = ph.kz_factors_xyt(u, g, self.medium, self.w0, vf)
+        self.kparaxial = bool(p["KPARAXIAL"])
+
+        # initial field (Fields.jl:141-160)
+        E0 = np.asarray(p["initial_condition"](g.x, g.y, g.t, u.x, u.y, u.t, u.I), dtype=float)
+        self.E0 = ph.rsig2asig(E0)
+
+        d = self._desc(L.GEOM_XYT, precision, form, keep_plasma, device)
+        d.Nx, d.Ny = g.Nx, g.Ny
+        d.guard3d_bug = int(self.guard3d_bug)
+        self._create(d)
+        self._set_tables()
+        self.upload(self.E0)
+
+    def _from_rows(self, A):
+        g = self.grid
+        return A.reshape(g.Nx, g.Ny, g.Nt, order="F")
+
+    def _set_tables(self):
+        g = self.grid
+        keep = []
+        t, arr = self._common_tables(keep)
+        x = L.FilTablesXYT()
+        x.x, x.y, x.dx, x.dy = arr(g.x), arr(g.y), g.dx, g.dy
+        x.kt2x, x.kt2y = arr(self.kz["kt2x"]), arr(self.kz["kt2y"])
+        x.Xguard, x.Yguard = arr(self.guard.Xguard), arr(self.guard.Yguard)
+        x.KXguard, x.KYguard = arr(self.guard.KXguard), arr(self.guard.KYguard)
+        x.beta = arr(self.kz["beta"], True)
+        x.w_vf = arr(self.kz["w_vf"])
+        x.zu = self.unit.z
+        x.kparaxial = int(self.kparaxial)
+        L.check(self._lib.fil_set_tables_xyt(self.h, ctypes.byref(t), ctypes.byref(x)))
+
+    def analyze(self, z=0.0, arrays=False):
+        g = self.grid
+        d = L.FilDiagXYT()
+        if arrays:
+            Fxy, Ft, re, S = np.zeros(self.nrows), np.zeros(g.Nt), np.zeros(self.nrows), np.zeros(g.Nt // 2 + 1)
+            L.check(self._lib.fil_analyze_xyt(self.h, float(z), ctypes.byref(d), L.ptr(Fxy), L.ptr(Ft),
+                                              L.ptr(re), L.ptr(S)))
+            out = d.as_dict()
+            out.update(Fxy=Fxy.reshape(g.Nx, g.Ny, order="F"), Ft=Ft, rho_end=re.reshape(g.Nx, g.Ny, order="F"),
+                       S=S)
+            return out
+        L.check(self._lib.fil_analyze_xyt(self.h, float(z), ctypes.byref(d), None, None, None, None))
+        return d.as_dict()

@@ -8,6 +8,10 @@ RT 256x2048  : air, 1 atm, 800 nm; Nr=256 on [0,10 mm] (Hankel nodes),
                free current, MPI losses; O2 multiphoton ionisation K=8,
                sigma_8 = 2.9e-99 s^-1 (cm^2/W)^8, neutrals 5.4e24 m^-3; dz=0.5 mm.
 RT 1024x4096 : same physics at Nr=1024, Nt=4096, 5 Pcr.
+XYT 512^3    : air, 800 nm; x, y on [-10, 10] mm (512 points each), t on
+               [-250, 250] fs (512); elliptical Gaussian 3 x 2 mm, 50 fs, 30 Pcr,
+               5% amplitude noise (white noise smoothed by a Gaussian of 0.1 mm
+               correlation length, seeded); dz = 0.2 mm.
 """
 import math
 
@@ -70,6 +74,72 @@ def rt_air(Nr, Nt, power_pcr, rmax=10.0, tmin=-500.0, tmax=500.0, a0=1e-3, tau0=
         initial_condition=gaussian_ic(a0, tau0, I0, lam0, noise, seed),
         I0=I0, Pcr=Pcr, power_pcr=power_pcr,
     )
+    return p
+
+
+def gaussian_ic_xyt(ax, ay, tau0, I0, lam0, noise=0.0, corr=1e-4, seed=0):
+    """initial_condition(x, y, t, xu, yu, tu, Iu) of examples/XYT/gauss.jl form:
+    an elliptical Gaussian beam and pulse, times (1 + noise N(x, y)) with N
+    seeded white noise smoothed by a Gaussian kernel of correlation length
+    ``corr`` [m] and scaled to unit standard deviation."""
+    def initial_condition(x, y, t, xu, yu, tu, Iu):
+        w0 = 2 * math.pi * ph.C0 / lam0
+        xx = np.asarray(x)[:, None, None] * xu
+        yy = np.asarray(y)[None, :, None] * yu
+        tt = np.asarray(t)[None, None, :] * tu
+        E = (math.sqrt(I0 / Iu) * np.exp(-0.5 * xx ** 2 / ax ** 2 - 0.5 * yy ** 2 / ay ** 2)
+             * np.exp(-0.5 * tt ** 2 / tau0 ** 2) * np.cos(w0 * tt))
+        if noise:
+            rng = np.random.default_rng(seed)
+            nx, ny = len(x), len(y)
+            wn = rng.standard_normal((nx, ny))
+            kx = 2 * np.pi * np.fft.fftfreq(nx, (x[1] - x[0]) * xu)
+            ky = 2 * np.pi * np.fft.fftfreq(ny, (y[1] - y[0]) * yu)
+            G = np.exp(-0.5 * corr ** 2 * (kx[:, None] ** 2 + ky[None, :] ** 2))
+            N = np.real(np.fft.ifft2(np.fft.fft2(wn) * G))
+            N /= N.std()
+            E = E * (1 + noise * N)[:, :, None]
+        return E
+    return initial_condition
+
+
+def xyt_air(Nx, Ny, Nt, power_pcr, xmax=10.0, ymax=10.0, tmin=-250.0, tmax=250.0, ax=3e-3, ay=2e-3, tau0=50e-15,
+            noise=0.05, seed=0, n2=1e-23, nonlinearity=True, plasma=True, dz=2e-4):
+    """XYT parameter dict in the reference's input-file names (src/Input.jl:151-178,
+    examples/XYT/input.jl)."""
+    lam0 = 800e-9
+    medium = ph.Medium(ph.peck_reeder_air, ph.unit_permeability, n2)
+    w0 = 2 * math.pi * ph.C0 / lam0
+    Pcr = medium.critical_power(w0)
+    I0 = power_pcr * Pcr / (math.pi * ax * ay)
+    nuc, mr = 5e12, 1.0
+    return dict(
+        prefix="results/", geometry="XYT",
+        xu=1e-3, yu=1e-3, zu=1.0, tu=1e-15, Iu=1e16, rhou=2.5e25,
+        xmin=-xmax, xmax=xmax, Nx=Nx, ymin=-ymax, ymax=ymax, Ny=Ny, tmin=tmin, tmax=tmax, Nt=Nt, lam0=lam0,
+        permittivity=ph.peck_reeder_air, permeability=ph.unit_permeability, n2=n2, nuc=nuc, mr=mr,
+        NONLINEARITY=nonlinearity, PLASMA=plasma, KPARAXIAL=False, QPARAXIAL=True,
+        xguard=1.0, yguard=1.0, tguard=20.0, kxguard=45.0, kyguard=45.0, wguard=1e16,
+        dz_initial=dz, dzphimax=math.inf, Istop=1e3, ALG=RK3(), z=0.0,
+        responses=[
+            {"init": init_cubic, "THG": False, "n2": n2},
+            {"init": init_current_free, "nuc": nuc, "mr": mr},
+            {"init": init_current_losses, "EREAL": False},
+        ],
+        plasma_equation={
+            "init": init_photoionization_avalanche, "ALG": RK3(), "EREAL": False, "KDEP": False,
+            "rho0": 0.0, "rho_nt": 5.4e24, "nuc": 0.0, "mr": mr,
+            "components": [{"fraction": 1.0, "ionization_energy": 12.063, "ionization_rate": o2_rate_table()}],
+        },
+        initial_condition=gaussian_ic_xyt(ax, ay, tau0, I0, lam0, noise, 1e-4, seed),
+        I0=I0, Pcr=Pcr, power_pcr=power_pcr,
+    )
+
+
+def xyt_512():
+    """The XYT case: 512^3, 30 Pcr, 500 steps of 0.2 mm."""
+    p = xyt_air(512, 512, 512, 30.0)
+    p["nsteps"] = 500
     return p
 
 

@@ -1,16 +1,16 @@
 /*
  * filament_hip.h -- C ABI of libfilament_hip.so, the MI355X (gfx950)
  * implementation of Filament.jl's per-z-step propagation path for the
- * axisymmetric RT geometry.
+ * axisymmetric RT and the three-dimensional XYT geometries.
  *
  * The library replaces the device side of one reference call,
  *   Models.zstep(z, dz, grid, field, guard, model)   (src/modules/Models.jl:75-121)
  * together with the pieces it reaches: the plasma solve (plasma_equations.jl:27-72),
  * the t-transforms (Models.jl:152-171), the nonlinear RK step
  * (nonlinear_propagators.jl:38-74,246-283), the spectral linear step with the
- * Hankel transform (linear_propagators/spectral.jl:16-26, Models.jl:135-148),
- * the guard filters (Guards.jl:199-318) and the RT field analyzer
- * (FieldAnalyzers.jl:166-215).  The host keeps Input.prepare, Units, Grids,
+ * Hankel transform (RT) or the 2-D FFT (XYT) (linear_propagators/spectral.jl:16-26,
+ * Models.jl:135-148), the guard filters (Guards.jl:199-318) and the RT and XYT
+ * field analyzers (FieldAnalyzers.jl:166-215, 287-353).  The host keeps Input.prepare, Units, Grids,
  * Media and the table construction; it hands the tables over once.
  *
  * Conventions
@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define FIL_ABI_VERSION 1
+#define FIL_ABI_VERSION 2
 #define FIL_MAXCOMP 4
 
 typedef struct fil_handle fil_handle;
@@ -46,7 +46,7 @@ enum {
   FIL_ENOMEM = 4
 };
 
-enum { FIL_GEOM_RT = 0 };
+enum { FIL_GEOM_RT = 0, FIL_GEOM_XYT = 1 };
 
 enum { FIL_FP32 = 4, FIL_FP64 = 8 };
 
@@ -83,6 +83,13 @@ typedef struct fil_desc {
   int32_t ncomp;           /* plasma components, 1..FIL_MAXCOMP */
   int32_t keep_plasma;     /* store full rho/kdrho each step (for fil_download_plasma) */
   int32_t device;          /* HIP device ordinal */
+  /* XYT only (ignored for RT): grid sizes, powers of two in [16, 1024]; the
+   * rows of the field are the Nx*Ny points (x fastest), Nr is ignored. */
+  int32_t Nx, Ny;
+  /* XYT only: reproduce the reference's 3-D filter kernel, which indexes the
+   * t (and w) factor with the y index (Guards.jl:314); 0 = the intended
+   * filter X(x) Y(y) T(t) and KX(kx) KY(ky) W(w).  Needs Ny <= Nt. */
+  int32_t guard3d_bug;
 } fil_desc;
 
 /* Host-precomputed tables, FP64 (the library rounds them to the working
@@ -124,6 +131,29 @@ typedef struct fil_tables {
   const double* tab_log10y[FIL_MAXCOMP];
 } fil_tables;
 
+/* XYT tables (Grids.jl:158-201, Guards.jl:162-195, spectral.jl:131-184), FP64.
+ * fil_set_tables_xyt takes these together with the fil_tables fields that the
+ * two geometries share: t, dt, Wguard, Tguard, QZ, Rnl_*, the plasma tables
+ * (r, dr, KZ, Kguard, Rguard and dht_* are not used).  KZ is evaluated on the
+ * device from its separable factors:
+ *   KZ = (Kfunc(beta(w), kt) - w/vf) zu,  kt^2 = kt2x[ix] + kt2y[iy],
+ *   Kfunc = beta - kt^2/(2 beta) (kparaxial) or sqrt(beta^2 - kt^2 + 0im). */
+typedef struct fil_tables_xyt {
+  const double* x;         /* [Nx] grid units */
+  const double* y;         /* [Ny] */
+  double dx, dy;           /* grid units */
+  const double* kt2x;      /* [Nx] (kx * kxu)^2 [1/m^2] */
+  const double* kt2y;      /* [Ny] (ky * kyu)^2 */
+  const double* Xguard;    /* [Nx] field guard windows */
+  const double* Yguard;    /* [Ny] */
+  const double* KXguard;   /* [Nx] spectral guards (all ones in the reference, Guards.jl:183-184) */
+  const double* KYguard;   /* [Ny] */
+  const double* beta;      /* [Nt] complex beta(w) [1/m] at the grid frequencies */
+  const double* w_vf;      /* [Nt] w / vf [1/m] (moving frame, spectral.jl:141-152) */
+  double zu;               /* [m] */
+  int32_t kparaxial;       /* KPARAXIAL */
+} fil_tables_xyt;
+
 /* Scalar diagnostics of FieldAnalyzerRT (FieldAnalyzers.jl:125-138) */
 typedef struct fil_diag {
   double z, Fmax, Imax, rhomax, De, rfil, rpl, tau, W;
@@ -134,6 +164,8 @@ const char* fil_last_error(void);
 /* Replaces Fields.Field / Models.Model construction (Filament.jl:71-74). */
 int fil_create(const fil_desc* d, fil_handle** h);
 int fil_set_tables(fil_handle* h, const fil_tables* t);
+/* XYT handles (Field/Guard/LinearPropagator XYT, Fields.jl:141-160). */
+int fil_set_tables_xyt(fil_handle* h, const fil_tables* t, const fil_tables_xyt* x);
 
 /* Field in the time domain, column-major E[i + ld*j] (Field.E, Fields.jl:342-355
  * after rsig2asig!).  Upload replaces the state; download returns what the
@@ -149,6 +181,16 @@ int fil_zstep(fil_handle* h, double z, double dz);
  * rho_end [Nr] and S [Nt/2+1] may be NULL.  Blocks. */
 int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft,
                 double* rho_end, double* S);
+
+/* FieldAnalyzerXYT (FieldAnalyzers.jl:262-353): Fxy [Nx x Ny] (x fastest),
+ * Ft [Nt], rho_end [Nx x Ny], S [Nt/2+1] may be NULL.  ax, ay: radii of Fxy
+ * along x and y through its maximum.  Projects the field onto its analytic
+ * part like the RT analyzer.  Blocks. */
+typedef struct fil_diag_xyt {
+  double z, Fmax, Imax, rhomax, De, ax, ay, W;
+} fil_diag_xyt;
+int fil_analyze_xyt(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, double* Ft,
+                    double* rho_end, double* S);
 
 /* field.rho and field.kdrho (Fields.jl:357-358) as the last plasma solve left
  * them, column-major [i + ld*j].  Blocks. */

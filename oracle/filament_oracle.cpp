@@ -243,8 +243,11 @@ struct TabFunc {
 struct Base {
   virtual ~Base() {}
   orc_config cfg;
-  int Nr = 1, Nt = 0, Nh = 0;
-  bool rt = true;
+  int Nr = 1, Nt = 0, Nh = 0;       // Nr: number of rows (r nodes, or Nx * Ny points)
+  bool rt = true, xyt = false;
+  int Nx = 0, Ny = 0;
+  std::vector<double> x, y, kx, ky;
+  double dx = 0, dy = 0;
   Medium med{};
   // FP64 grids and scalars
   std::vector<double> r, dr, k, t, w, rdr;
@@ -266,7 +269,7 @@ struct State : Base {
   int Rkind[4] = {0, 0, 0, 0};
   std::vector<C> Hraman;
   std::vector<T> Tm, RJ, JV;   // DHT
-  std::unique_ptr<FFTPlan<T>> plan;
+  std::unique_ptr<FFTPlan<T>> plan, planx, plany;
   std::vector<T> Rguard_dummy;
 
   void build(const orc_config& c, const double* E0);
@@ -274,6 +277,7 @@ struct State : Base {
   void time_fwd(C* row) const { plan->run(row, +1); }    // P \ E
   void time_inv(C* row) const { plan->run(row, -1); }    // P * E
   void dht_columns(C* A, int inverse);
+  void fft2_columns(C* A, int inverse);
   void plasma_row(int i, const C* Erow, T* rho_row, T* kd_row) const;
   void plasma_solve();
   void rhs_row(int i, const C* Es, C* dE, C* Et, C* F) const;
@@ -287,10 +291,32 @@ template <typename T>
 void State<T>::build(const orc_config& c, const double* E0) {
   cfg = c;
   rt = (c.geometry == ORC_GEOM_RT);
+  xyt = (c.geometry == ORC_GEOM_XYT);
   Nt = c.Nt;
-  Nr = rt ? c.Nr : 1;
+  Nr = rt ? c.Nr : (xyt ? c.Nx * c.Ny : 1);
   if (Nt < 4 || (Nt & (Nt - 1))) throw std::runtime_error("Nt must be a power of two >= 4");
   if (rt && Nr < 2) throw std::runtime_error("Nr must be >= 2");
+  if (xyt) {
+    Nx = c.Nx; Ny = c.Ny;
+    if (Nx < 2 || Ny < 2 || (Nx & (Nx - 1)) || (Ny & (Ny - 1))) throw std::runtime_error("Nx, Ny must be powers of two");
+    if (c.guard3d_bug && Ny > Nt) throw std::runtime_error("guard3d_bug needs Ny <= Nt");
+    planx.reset(new FFTPlan<T>(Nx));
+    plany.reset(new FFTPlan<T>(Ny));
+    // Grids.jl:196-201: range(min, max, length=N), kx = 2 pi fftfreq(N, 1/dx)
+    x.resize(Nx); y.resize(Ny); kx.resize(Nx); ky.resize(Ny);
+    dx = (c.xmax - c.xmin) / (Nx - 1);
+    dy = (c.ymax - c.ymin) / (Ny - 1);
+    for (int i = 0; i < Nx; i++) {
+      x[i] = c.xmin + i * dx;
+      int f = (i < (Nx + 1) / 2) ? i : i - Nx;
+      kx[i] = 2 * PI * f / (Nx * dx);
+    }
+    for (int i = 0; i < Ny; i++) {
+      y[i] = c.ymin + i * dy;
+      int f = (i < (Ny + 1) / 2) ? i : i - Ny;
+      ky[i] = 2 * PI * f / (Ny * dy);
+    }
+  }
   Nh = Nt / 2;
   med.kind = c.medium; med.n_const = c.n_const; med.n2 = c.n2;
   plan.reset(new FFTPlan<T>(Nt));
@@ -337,7 +363,7 @@ void State<T>::build(const orc_config& c, const double* E0) {
   rho.assign((size_t)Nr * Nt, 0); kdrho.assign((size_t)Nr * Nt, 0);
   Ftmp.assign((size_t)Nr * Nt, C(0));
 
-  // Guards.jl:289-320 (RT), :272-283 (T)
+  // Guards.jl:289-320 (RT), :272-283 (T), :162-195 (XYT)
   auto Tgd = guard_window_both(t, c.tguard);
   Tg.assign(Tgd.begin(), Tgd.end());
   Rg.assign(Nr, 1);
@@ -345,6 +371,22 @@ void State<T>::build(const orc_config& c, const double* E0) {
   Wg.resize(Nt);
   for (int j = 0; j < Nt; j++) Wg[j] = (T)std::exp(-std::pow(std::pow(w[j] * wu, 2) / (c.wguard * c.wguard), 20));
   Kg.assign((size_t)Nr * Nt, rt ? 0 : 1);
+  if (xyt) {
+    // field filter X(x) Y(y) T(t) and spectral filter KX(kx) KY(ky) W(w) with
+    // KX = KY = 1 (Guards.jl:176-184); kernel! (:301-318) takes its third factor
+    // at i3 = the y index, which guard3d_bug reproduces: X Y T(iy) and W(iky).
+    auto Xg = guard_window_both(x, c.xguard), Yg = guard_window_both(y, c.yguard);
+    std::vector<T> Wfull = Wg;
+    for (int iy = 0; iy < Ny; iy++)
+      for (int ix = 0; ix < Nx; ix++) {
+        const int i = ix + Nx * iy;
+        Rg[i] = (T)(Xg[ix] * Yg[iy] * (c.guard3d_bug ? Tgd[iy] : 1.0));
+        for (int j = 0; j < Nt; j++) Kg[(size_t)i * Nt + j] = c.guard3d_bug ? Wfull[iy] : (T)1;
+      }
+    if (c.guard3d_bug) {
+      for (int j = 0; j < Nt; j++) { Tg[j] = 1; Wg[j] = 1; }
+    }
+  }
   if (rt) {
     for (int j = 0; j < Nt; j++) {
       double kk = med.k(w[j] * wu);
@@ -355,14 +397,18 @@ void State<T>::build(const orc_config& c, const double* E0) {
     }
   }
 
-  // Linear propagator KZ (spectral.jl:78-101 RT, :55-75 T; Kfunc :160-184)
-  bool kpar = rt ? (c.kparaxial != 0) : true;   // Input.jl:285-291 forces T paraxial
+  // Linear propagator KZ (spectral.jl:78-101 RT, :131-156 XYT, :55-75 T; Kfunc :160-184)
+  bool kpar = (rt || xyt) ? (c.kparaxial != 0) : true;   // Input.jl:285-291 forces T paraxial
   KZ.assign((size_t)Nr * Nt, C(0));
   for (int j = 0; j < Nt; j++) {
     double ww = w[j] * wu;
     cd beta = med.beta(ww);
     for (int i = 0; i < Nr; i++) {
       double kt = k[i] * ku;
+      if (xyt) {
+        const double kxs = kx[i % Nx] / c.xu, kys = ky[i / Nx] / c.yu;
+        kt = std::sqrt(kxs * kxs + kys * kys);
+      }
       cd K;
       if (kpar) K = (beta != cd(0)) ? beta - kt * kt / (2.0 * beta) : cd(0);
       else K = std::sqrt(beta * beta - kt * kt + cd(0));
@@ -371,7 +417,7 @@ void State<T>::build(const orc_config& c, const double* E0) {
     }
   }
   // Nonlinear prefactor QZ (nonlinear_propagators.jl:323-343; Qfunc :394-426)
-  bool qpar = rt ? (c.qparaxial != 0) : true;
+  bool qpar = (rt || xyt) ? (c.qparaxial != 0) : true;
   if (!qpar) throw std::runtime_error("QPARAXIAL=false is not restated");
   QZ.assign((size_t)Nr * Nt, C(0));
   for (int j = 0; j < Nt; j++) {
@@ -646,9 +692,35 @@ void State<T>::nl_step_row(int i, C* u, T dz, std::vector<C>& s) const {
 }
 
 // propagate!(E, LP::LinearPropagatorSpectral, dz) (spectral.jl:16-26)
+// The 2-D FFT over (x, y) of every t/w column of A [rows][Nt] (Field XYT's
+// space plan plan_fft!(E, [1, 2]), Fields.jl:152; forward P * E unnormalised
+// exp(-i k x), inverse P \ E normalised, Models.jl:135-148).
+template <typename T>
+void State<T>::fft2_columns(C* A, int inverse) {
+  const int sgn = inverse ? +1 : -1;   // FFTPlan::run: +1 = normalised exp(+), -1 = exp(-)
+#pragma omp parallel
+  {
+    std::vector<C> v(std::max(Nx, Ny));
+#pragma omp for schedule(static)
+    for (int j = 0; j < Nt; j++) {
+      for (int iy = 0; iy < Ny; iy++) {
+        for (int ix = 0; ix < Nx; ix++) v[ix] = A[(size_t)(ix + Nx * iy) * Nt + j];
+        planx->run(v.data(), sgn);
+        for (int ix = 0; ix < Nx; ix++) A[(size_t)(ix + Nx * iy) * Nt + j] = v[ix];
+      }
+      for (int ix = 0; ix < Nx; ix++) {
+        for (int iy = 0; iy < Ny; iy++) v[iy] = A[(size_t)(ix + Nx * iy) * Nt + j];
+        plany->run(v.data(), sgn);
+        for (int iy = 0; iy < Ny; iy++) A[(size_t)(ix + Nx * iy) * Nt + j] = v[iy];
+      }
+    }
+  }
+}
+
 template <typename T>
 void State<T>::linear_step(T dz) {
   if (rt) dht_columns(E.data(), 0);
+  if (xyt) fft2_columns(E.data(), 0);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < Nr; i++)
     for (int j = 0; j < Nt; j++) {
@@ -658,6 +730,7 @@ void State<T>::linear_step(T dz) {
       E[q] = E[q] * (Kg[q] * Wg[j]);      // apply_spectral_filter! (Guards.jl:283-298)
     }
   if (rt) dht_columns(E.data(), 1);
+  if (xyt) fft2_columns(E.data(), 1);
 }
 
 // Models.zstep (Models.jl:75-121)
@@ -704,7 +777,9 @@ void State<T>::analyze(double z, orc_diag* d, double* Fr, double* Ft, double* rh
     fr[i] = s * dt;
     re[i] = rho[(size_t)i * Nt + Nt - 1];
   }
-  for (int j = 0; j < Nt; j++) ft[j] *= twopi;
+  // row weights: 2 pi r dr (RT, rdr holds r dr), dx dy (XYT, rdr = 1)
+  const double wscale = xyt ? dx * dy : twopi;
+  for (int j = 0; j < Nt; j++) ft[j] *= wscale;
   std::vector<C> row(Nt);
   for (int i = 0; i < Nr; i++) {
     C* Er = &E[(size_t)i * Nt];
@@ -719,9 +794,27 @@ void State<T>::analyze(double z, orc_diag* d, double* Fr, double* Ft, double* rh
     // Imax is taken after the projection (FieldAnalyzers.jl:203)
     for (int j = 0; j < Nt; j++) Imax = std::max(Imax, (double)std::norm(std::complex<double>(Er[j])));
   }
-  for (int j = 0; j < Nw; j++) sp[j] *= twopi;
+  for (int j = 0; j < Nw; j++) sp[j] *= wscale;
   d->z = z;
-  if (rt) {
+  if (xyt) {
+    // FieldAnalyzers.jl:330-352
+    const int imax = (int)(std::max_element(fr.begin(), fr.end()) - fr.begin());
+    d->Fmax = fr[imax];
+    d->Imax = Imax;
+    d->rhomax = rhomax;
+    double De = 0, W = 0;
+    for (int i = 0; i < Nr; i++) De += re[i];
+    for (int j = 0; j < Nt; j++) W += ft[j];
+    d->De = De * dx * dy;
+    d->W = W * dt;
+    const int ixm = imax % Nx, iym = imax / Nx;
+    std::vector<double> px(Nx), py(Ny);
+    for (int ix = 0; ix < Nx; ix++) px[ix] = fr[ix + Nx * iym];
+    for (int iy = 0; iy < Ny; iy++) py[iy] = fr[ixm + Nx * iy];
+    d->rfil = radius(x.data(), px.data(), Nx);   // ax
+    d->rpl = radius(y.data(), py.data(), Ny);    // ay
+    d->tau = 0;
+  } else if (rt) {
     d->Fmax = *std::max_element(fr.begin(), fr.end());
     d->Imax = Imax;
     d->rhomax = rhomax;
@@ -792,6 +885,14 @@ int orc_get_grid(void* h, double* r, double* dr, double* k, double* t, double* w
     std::copy(s.r.begin(), s.r.end(), r); std::copy(s.dr.begin(), s.dr.end(), dr);
     std::copy(s.k.begin(), s.k.end(), k); std::copy(s.t.begin(), s.t.end(), t);
     std::copy(s.w.begin(), s.w.end(), w);
+  });
+}
+
+int orc_get_grid_xyt(void* h, double* xo, double* yo, double* kxo, double* kyo) {
+  return dispatch(h, [&](auto& s) {
+    if (!s.xyt) throw std::runtime_error("not an XYT oracle");
+    std::copy(s.x.begin(), s.x.end(), xo); std::copy(s.y.begin(), s.y.end(), yo);
+    std::copy(s.kx.begin(), s.kx.end(), kxo); std::copy(s.ky.begin(), s.ky.end(), kyo);
   });
 }
 

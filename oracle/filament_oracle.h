@@ -1,6 +1,6 @@
 /*
  * filament_oracle.h -- CPU restatement of Filament.jl's per-z-step propagation
- * path (RT and T geometries), in FP64 and FP32.
+ * path (RT, XYT and T geometries), in FP64 and FP32.
  *
  * THIS IS TEST INFRASTRUCTURE.  It is the checker that the HIP library is
  * compared against; only tests/, __graft_entry__.smoke() and bench.py's CPU
@@ -9,7 +9,9 @@
  * Each function in filament_oracle.cpp cites the reference file:line it
  * restates (paths relative to the Filament.jl source tree).
  *
- * Layout: fields are row-major [Nr][Nt] (t fastest), complex interleaved.
+ * Layout: fields are row-major [rows][Nt] (t fastest), complex interleaved;
+ * the rows are the r nodes (RT), the (x, y) points with x fastest (XYT) or a
+ * single row (T).
  */
 #ifndef FILAMENT_ORACLE_H
 #define FILAMENT_ORACLE_H
@@ -22,7 +24,7 @@ extern "C" {
 
 #define ORC_MAXCOMP 4
 
-enum { ORC_GEOM_RT = 0, ORC_GEOM_T = 1 };
+enum { ORC_GEOM_RT = 0, ORC_GEOM_T = 1, ORC_GEOM_XYT = 2 };
 enum { ORC_MEDIUM_AIR_PECK_REEDER = 0, ORC_MEDIUM_CONSTANT = 1 };
 enum { ORC_OFF = 0, ORC_THG = 1, ORC_NOTHG = 2 };
 
@@ -51,6 +53,13 @@ typedef struct orc_config {
   int32_t comp_ntab[ORC_MAXCOMP];
   const double* comp_tab_x[ORC_MAXCOMP];   /* raw .tf columns: x [W/m^2]    */
   const double* comp_tab_y[ORC_MAXCOMP];   /*                  y [1/s]      */
+  /* XYT (Input.jl:151-178): grid in xu, yu units; guards; the reference's
+   * 3-D filter indexes its third factor with the y index (Guards.jl:314),
+   * reproduced when guard3d_bug != 0 */
+  int32_t Nx, Ny;
+  double xmin, xmax, ymin, ymax, xu, yu;
+  double xguard, yguard, kxguard, kyguard;
+  int32_t guard3d_bug;
 } orc_config;
 
 typedef struct orc_diag {
@@ -65,6 +74,7 @@ const char* orc_last_error(void);
 
 /* tables (always returned in FP64, after rounding to the working precision) */
 int orc_get_grid(void* h, double* r, double* dr, double* k, double* t, double* w);
+int orc_get_grid_xyt(void* h, double* x, double* y, double* kx, double* ky);
 int orc_get_scalars(void* h, double* out9);   /* w0, n0, Eu, vf, k0, phik, phip, dt, k2 */
 int orc_get_KZ(void* h, double* KZ);          /* [Nr][Nt] complex */
 int orc_get_QZ(void* h, double* QZ);          /* [Nr][Nt] complex */
@@ -80,8 +90,10 @@ int orc_get_plasma(void* h, double* rho, double* kdrho);
 /* the hot path */
 int orc_zstep(void* h, double z, double dz);            /* Models.jl:75-121 */
 int orc_plasma_solve(void* h);                          /* plasma_equations.jl */
+/* RT (FieldAnalyzers.jl:166-215): Fr [Nr]; XYT (:287-353): Fr = Fxy [rows],
+ * rfil = ax, rpl = ay, tau = 0. */
 int orc_analyze(void* h, double z, orc_diag* d, double* Fr, double* Ft,
-                double* rho_end, double* S);            /* FieldAnalyzers.jl:166-215 */
+                double* rho_end, double* S);
 /* One reference main-loop iteration without I/O: dz from AdaptiveSteps,
  * zstep, analyze (Filament.jl:132-143).  Returns dz used via *dz_out. */
 int orc_step_and_analyze(void* h, double* z, double dz0, double phimax,

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 MAXCOMP = 4
-GEOM_RT, GEOM_T = 0, 1
+GEOM_RT, GEOM_T, GEOM_XYT = 0, 1, 2
 MEDIUM_AIR, MEDIUM_CONST = 0, 1
 OFF, THG, NOTHG = 0, 1, 2
 
@@ -42,6 +42,12 @@ class OrcConfig(ctypes.Structure):
         ("comp_frac", ctypes.c_double * MAXCOMP), ("comp_Ui", ctypes.c_double * MAXCOMP),
         ("comp_ntab", ctypes.c_int32 * MAXCOMP),
         ("comp_tab_x", _dp * MAXCOMP), ("comp_tab_y", _dp * MAXCOMP),
+        ("Nx", ctypes.c_int32), ("Ny", ctypes.c_int32),
+        ("xmin", ctypes.c_double), ("xmax", ctypes.c_double), ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+        ("xu", ctypes.c_double), ("yu", ctypes.c_double),
+        ("xguard", ctypes.c_double), ("yguard", ctypes.c_double),
+        ("kxguard", ctypes.c_double), ("kyguard", ctypes.c_double),
+        ("guard3d_bug", ctypes.c_int32),
     ]
 
 
@@ -71,7 +77,7 @@ def lib():
         L.orc_create.argtypes = [ctypes.POINTER(OrcConfig), _dp, ctypes.c_int]
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_last_error.restype = ctypes.c_char_p
-        for name in ("orc_get_grid", "orc_get_scalars", "orc_get_KZ", "orc_get_QZ", "orc_get_guards",
+        for name in ("orc_get_grid", "orc_get_grid_xyt", "orc_get_scalars", "orc_get_KZ", "orc_get_QZ", "orc_get_guards",
                      "orc_get_dht", "orc_set_field", "orc_get_field", "orc_get_plasma",
                      "orc_plasma_solve", "orc_nl_rhs"):
             getattr(L, name).argtypes = None
@@ -142,7 +148,7 @@ class Oracle:
         c.ncomp = len(tabs)
         self.cfg = c
         self.geometry = c.geometry
-        self.Nr = c.Nr if c.geometry == GEOM_RT else 1
+        self.Nr = c.Nr if c.geometry == GEOM_RT else (c.Nx * c.Ny if c.geometry == GEOM_XYT else 1)
         self.Nt = c.Nt
         E0 = np.ascontiguousarray(E0, dtype=np.float64).reshape(self.Nr, self.Nt)
         self.h = lib().orc_create(ctypes.byref(c), _p(E0), precision)
@@ -157,6 +163,12 @@ class Oracle:
     @property
     def hp(self):
         return ctypes.c_void_p(self.h)
+
+    def grid_xyt(self):
+        x, y = np.zeros(self.cfg.Nx), np.zeros(self.cfg.Ny)
+        kx, ky = np.zeros(self.cfg.Nx), np.zeros(self.cfg.Ny)
+        _check(lib().orc_get_grid_xyt(self.hp, _p(x), _p(y), _p(kx), _p(ky)))
+        return dict(x=x, y=y, kx=kx, ky=ky)
 
     def grid(self):
         Nr, Nt = self.Nr, self.Nt

@@ -58,5 +58,20 @@ def oracle_config(p):
     )
 
 
+def oracle_config_xyt(p, guard3d_bug=False):
+    """The same XYT problem as an oracle configuration dict (tests only)."""
+    c = oracle_config(dict(p, rmax=0.0, Nr=0, ru=p["xu"], rguard=0.0, kguard=p["kxguard"]))
+    c.update(geometry=2, Nr=0, Nx=p["Nx"], Ny=p["Ny"], xmin=p["xmin"], xmax=p["xmax"], ymin=p["ymin"],
+             ymax=p["ymax"], xu=p["xu"], yu=p["yu"], xguard=p["xguard"], yguard=p["yguard"],
+             kxguard=p["kxguard"], kyguard=p["kyguard"], guard3d_bug=int(guard3d_bug))
+    return c
+
+
+def xyt_rows(E):
+    """[Nx, Ny, Nt] -> [Nx*Ny, Nt] with x fastest (the oracle's and the device's row order)."""
+    E = np.asarray(E)
+    return E.reshape(E.shape[0] * E.shape[1], E.shape[2], order="F")
+
+
 def response_name(init):
     return load_pkg().responses.response_kind(init)

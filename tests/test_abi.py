@@ -32,9 +32,10 @@ def test_library_exports_every_declared_symbol(pkg):
 
 
 def test_struct_layouts_match_header(pkg):
-    # fil_desc: 15 int32 fields; fil_diag: 9 doubles
-    assert ctypes.sizeof(pkg.lib.FilDesc) == 15 * 4
+    # fil_desc: 18 int32 fields; fil_diag: 9 doubles; fil_diag_xyt: 8 doubles
+    assert ctypes.sizeof(pkg.lib.FilDesc) == 18 * 4
     assert ctypes.sizeof(pkg.lib.FilDiag) == 9 * 8
+    assert ctypes.sizeof(pkg.lib.FilDiagXYT) == 8 * 8
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     body = text[text.index("typedef struct fil_desc"):text.index("} fil_desc;")]
     fields = re.findall(r"int32_t\s+([a-zA-Z0-9_, ]+);", body)
@@ -43,6 +44,14 @@ def test_struct_layouts_match_header(pkg):
     body = text[text.index("typedef struct fil_tables"):text.index("} fil_tables;")]
     names = re.findall(r"\b([a-zA-Z0-9_]+)(?:\[FIL_MAXCOMP\])?;", body)
     assert names == [n for n, _ in pkg.lib.FilTables._fields_]
+    body = text[text.index("typedef struct fil_tables_xyt"):text.index("} fil_tables_xyt;")]
+    names = re.findall(r"\b([a-zA-Z0-9_]+)(?:\[FIL_MAXCOMP\])?;", body)
+    names = [n for group in re.findall(r"(?:double\*?|int32_t)\s+([a-zA-Z0-9_, ]+);", body) for n in
+             (x.strip() for x in group.split(","))]
+    assert names == [n for n, _ in pkg.lib.FilTablesXYT._fields_]
+    body = text[text.index("typedef struct fil_diag_xyt"):text.index("} fil_diag_xyt;")]
+    names = [n.strip() for n in re.search(r"double ([^;]+);", body).group(1).split(",")]
+    assert names == [n for n, _ in pkg.lib.FilDiagXYT._fields_]
     body = text[text.index("typedef struct fil_corr_desc"):text.index("} fil_corr_desc;")]
     names = [f.strip() for group in re.findall(r"(?:int32_t|double)\s+([a-zA-Z0-9_, ]+);", body)
              for f in group.split(",")]
