@@ -78,9 +78,11 @@ double radius(const double* x, const double* y, int n) {
 struct fil_handle {
   fil_desc d{};
   int Nr = 0, Nrp = 0, Nt = 0, Nh = 0;
+  int B = 1;                 // batch: members side by side, rows (r, member) with r slowest
   bool fp64 = false, bf16x3 = false, scan64 = false;
   hipStream_t st = nullptr;
   bool tables = false, field = false;
+  std::vector<char> uploaded;   // members of a batch with a field
   bool pending_filter = false, plasma_valid = false;
   double P_dz = NAN;
   double dt = 0;
@@ -91,6 +93,12 @@ struct fil_handle {
   DevBuf tab_ly[FIL_MAXCOMP], tab_sl[FIL_MAXCOMP];
   DevBuf rho_end, rho_max, rho_full, kd_full;
   DevBuf aFr, aIt, aSw, aImax, aFt, aS, ardr, acol;
+  // host copy of the analyzer rows of every member, valid until the state changes
+  struct {
+    bool valid = false;
+    std::vector<double> fr, ft, im, re, rm, sp;
+    std::vector<char> served;         // members whose analysis was returned
+  } acache;
   // XYT geometry (rows are the Nx*Ny points, x fastest)
   bool xyt = false;
   int Nx = 0, Ny = 0;
@@ -127,7 +135,25 @@ struct fil_handle {
   int64_t count = 0;
 
   size_t rsz() const { return fp64 ? sizeof(double) : sizeof(float); }
+  int rows() const { return Nr * B; }                              // rows of the row-wise kernels
+  size_t ns() const { return (size_t)Nrp * B * Nh; }               // elements of the state S
   size_t csz() const { return 2 * rsz(); }
+};
+
+// Handles stepped together by one HIP graph per z-step (see fil_group_zstep).
+struct fil_group {
+  std::vector<fil_handle*> hs;
+  int device = 0;
+  hipStream_t st = nullptr;            // the graph is launched here
+  hipEvent_t fork = nullptr, post = nullptr;
+  std::vector<hipEvent_t> join, pre;
+  struct Entry {
+    double dz;
+    uint64_t pattern;                  // bit k: handle k had a pending field filter
+    hipGraphExec_t exec;
+  };
+  std::vector<Entry> cache;            // at most 4 instantiated graphs
+  int64_t captures = 0, replays = 0;
 };
 
 namespace {
@@ -197,15 +223,23 @@ int flush_timing(fil_handle* h) {
   return 0;
 }
 
+// Linear propagator P = exp(i KZ dz) * spectral guard for the step's dz (only
+// when dz changes).
+template <typename R>
+int update_propagator(fil_handle* h, double dz) {
+  if (h->P_dz == dz) return 0;
+  if (h->xyt)
+    HIPCHK(launch_xyt_propagator<R>(h->xprop, dz, h->P.as<cx<R>>(), h->st));
+  else
+    HIPCHK(launch_propagator<R>(h->KZ64.as<double>(), h->KG64.as<double>(), dz, h->P.as<cx<R>>(), h->Nrp, h->Nh, h->st));
+  h->P_dz = dz;
+  return 0;
+}
+
 template <typename R>
 int zstep_impl(fil_handle* h, double dz) {
-  if (!(h->P_dz == dz)) {
-    if (h->xyt)
-      HIPCHK(launch_xyt_propagator<R>(h->xprop, dz, h->P.as<cx<R>>(), h->st));
-    else
-      HIPCHK(launch_propagator<R>(h->KZ64.as<double>(), h->KG64.as<double>(), dz, h->P.as<cx<R>>(), h->Nrp, h->Nh, h->st));
-    h->P_dz = dz;
-  }
+  h->acache.valid = false;
+  if (int rc = update_propagator<R>(h, dz)) return rc;
   RowArgs<R> a{};
   a.S = h->S.as<cx<R>>();
   a.tw = h->tw.as<cx<R>>();
@@ -218,6 +252,7 @@ int zstep_impl(fil_handle* h, double dz) {
   a.apply_filter = h->pending_filter ? 1 : 0;
   a.nonlinearity = h->d.nonlinearity;
   a.plasma = h->d.plasma;
+  a.batch = h->B;
   a.responses = h->d.responses;
   a.losses_ereal = h->d.losses_ereal;
   if constexpr (sizeof(R) == 8) a.pp = h->pp64; else a.pp = h->pp32;
@@ -226,7 +261,7 @@ int zstep_impl(fil_handle* h, double dz) {
   a.rho_full = h->d.keep_plasma ? h->rho_full.as<R>() : nullptr;
   a.kd_full = h->d.keep_plasma ? h->kd_full.as<R>() : nullptr;
   if (int rc = record(h, 0)) return rc;
-  HIPCHK(launch_row<R>(a, h->Nr, h->Nt, h->scan64, h->st));
+  HIPCHK(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
   if (int rc = record(h, 1)) return rc;
   if (h->xyt) {
     // the three transform passes are timed together as the "dht1" phase
@@ -241,19 +276,21 @@ int zstep_impl(fil_handle* h, double dz) {
   if constexpr (sizeof(R) == 4) {
     if (h->bf16x3) {
       HIPCHK(launch_dht_gemm_bf16x3(h->M1h.as<uint16_t>(), h->M1l.as<uint16_t>(), h->S.as<cx<float>>(),
-                                    h->X.as<cx<float>>(), h->P.as<cx<float>>(), h->Nrp, h->Nh, h->st));
+                                    h->X.as<cx<float>>(), h->P.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh, h->st));
       if (int rc = record(h, 2)) return rc;
       HIPCHK(launch_dht_gemm_bf16x3(h->M2h.as<uint16_t>(), h->M2l.as<uint16_t>(), h->X.as<cx<float>>(),
-                                    h->S.as<cx<float>>(), nullptr, h->Nrp, h->Nh, h->st));
+                                    h->S.as<cx<float>>(), nullptr, h->Nrp, h->Nh * h->B, h->Nh, h->st));
       if (int rc = record(h, 3)) return rc;
       h->pending_filter = true;
       h->plasma_valid = h->d.plasma != 0;
       return 0;
     }
   }
-  HIPCHK(launch_dht_gemm<R>(h->M1.as<R>(), h->S.as<cx<R>>(), h->X.as<cx<R>>(), h->P.as<cx<R>>(), h->Nrp, h->Nh, h->st));
+  HIPCHK(launch_dht_gemm<R>(h->M1.as<R>(), h->S.as<cx<R>>(), h->X.as<cx<R>>(), h->P.as<cx<R>>(), h->Nrp, h->Nh * h->B,
+                            h->Nh, h->st));
   if (int rc = record(h, 2)) return rc;
-  HIPCHK(launch_dht_gemm<R>(h->M2.as<R>(), h->X.as<cx<R>>(), h->S.as<cx<R>>(), nullptr, h->Nrp, h->Nh, h->st));
+  HIPCHK(launch_dht_gemm<R>(h->M2.as<R>(), h->X.as<cx<R>>(), h->S.as<cx<R>>(), nullptr, h->Nrp, h->Nh * h->B, h->Nh,
+                            h->st));
   if (int rc = record(h, 3)) return rc;
   h->pending_filter = true;
   h->plasma_valid = h->d.plasma != 0;
@@ -264,6 +301,7 @@ int zstep_impl(fil_handle* h, double dz) {
 // solves the plasma equation for the current field (Models.jl:66).
 template <typename R>
 int plasma_only(fil_handle* h) {
+  h->acache.valid = false;
   RowArgs<R> a{};
   a.S = h->S.as<cx<R>>();
   a.tw = h->tw.as<cx<R>>();
@@ -276,14 +314,39 @@ int plasma_only(fil_handle* h) {
   a.nonlinearity = 0;
   a.plasma = 1;
   a.responses = 0;
+  a.batch = h->B;
   if constexpr (sizeof(R) == 8) a.pp = h->pp64; else a.pp = h->pp32;
   a.rho_end = h->rho_end.as<double>();
   a.rho_max = h->rho_max.as<double>();
   a.rho_full = h->d.keep_plasma ? h->rho_full.as<R>() : nullptr;
   a.kd_full = h->d.keep_plasma ? h->kd_full.as<R>() : nullptr;
-  HIPCHK(launch_row<R>(a, h->Nr, h->Nt, h->scan64, h->st));
+  HIPCHK(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
   h->pending_filter = false;
   h->plasma_valid = true;
+  return 0;
+}
+
+// Applies a pending field filter to every row (the row kernel without the
+// nonlinear step and the plasma), so that per-member operations of a batch
+// see a consistent state.
+template <typename R>
+int apply_pending_filter(fil_handle* h) {
+  if (!h->pending_filter) return 0;
+  h->acache.valid = false;
+  RowArgs<R> a{};
+  a.S = h->S.as<cx<R>>();
+  a.tw = h->tw.as<cx<R>>();
+  a.Tg = h->Tg.as<R>();
+  a.Rg = h->Rg.as<R>();
+  a.coef = h->coef.as<cx<R>>();
+  a.dt = (R)h->dt;
+  a.apply_filter = 1;
+  a.batch = h->B;
+  if constexpr (sizeof(R) == 8) a.pp = h->pp64; else a.pp = h->pp32;
+  a.rho_end = h->rho_end.as<double>();
+  a.rho_max = h->rho_max.as<double>();
+  HIPCHK(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
+  h->pending_filter = false;
   return 0;
 }
 
@@ -294,38 +357,78 @@ struct AnalyzerRows {
   std::vector<double> fr, ft, im, re, rm, sp;
 };
 
+// The first analysis after a state change runs over every member in one pass
+// and keeps the rows on the host; each member's first call is served from
+// there.  A repeated call for a member analyzes (and projects) that member
+// again, as the reference's analyze! does on a second call.
 template <typename R>
-int analyze_device(fil_handle* h, AnalyzerRows& v) {
-  const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh;
-  if (h->d.plasma && !h->plasma_valid)
-    if (int rc = plasma_only<R>(h)) return rc;
-  HIPCHK(launch_analyze_rows<R>(h->S.as<cx<R>>(), h->tw.as<cx<R>>(), h->Rg.as<R>(), h->Tg.as<R>(),
-                                h->pending_filter ? 1 : 0, (R)h->dt, h->aFr.as<double>(), h->aIt.as<float>(),
-                                h->aSw.as<float>(), h->aImax.as<double>(), Nr, Nt, h->st));
-  h->pending_filter = false;
-  HIPCHK(launch_colsum(h->aIt.as<float>(), Nr, Nt, h->ardr.as<double>(), h->acol.as<double>(), h->aFt.as<double>(),
-                       h->st));
-  HIPCHK(launch_colsum(h->aSw.as<float>(), Nr, Nh + 1, h->ardr.as<double>(), h->acol.as<double>(),
-                       h->aS.as<double>(), h->st));
-  v.fr.assign(Nr, 0.0); v.ft.assign(Nt, 0.0); v.im.assign(Nr, 0.0);
-  v.re.assign(Nr, 0.0); v.rm.assign(Nr, 0.0); v.sp.assign(Nh + 1, 0.0);
-  HIPCHK(hipMemcpyAsync(v.fr.data(), h->aFr.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipMemcpyAsync(v.ft.data(), h->aFt.p, Nt * 8, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipMemcpyAsync(v.im.data(), h->aImax.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
-  HIPCHK(hipMemcpyAsync(v.sp.data(), h->aS.p, (Nh + 1) * 8, hipMemcpyDeviceToHost, h->st));
-  if (h->d.plasma) {
-    HIPCHK(hipMemcpyAsync(v.re.data(), h->rho_end.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
-    HIPCHK(hipMemcpyAsync(v.rm.data(), h->rho_max.p, Nr * 8, hipMemcpyDeviceToHost, h->st));
+int analyze_device(fil_handle* h, AnalyzerRows& v, int member) {
+  const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh, B = h->B;
+  auto& c = h->acache;
+  if (c.valid && c.served[member]) {
+    HIPCHK(launch_analyze_rows<R>(h->S.as<cx<R>>(), h->tw.as<cx<R>>(), h->Rg.as<R>(), h->Tg.as<R>(), 0, (R)h->dt,
+                                  h->aFr.as<double>(), h->aIt.as<R>(), h->aSw.as<R>(), h->aImax.as<double>(), Nr, Nt,
+                                  B, member, 1, h->st));
+    HIPCHK(launch_colsum<R>(h->aIt.as<R>(), Nr, Nt, h->ardr.as<double>(), h->acol.as<double>(), h->aFt.as<double>(),
+                            h->st));
+    HIPCHK(launch_colsum<R>(h->aSw.as<R>(), Nr, Nh + 1, h->ardr.as<double>(), h->acol.as<double>(),
+                            h->aS.as<double>(), h->st));
+    v.fr.assign(Nr, 0.0); v.im.assign(Nr, 0.0); v.ft.assign(Nt, 0.0); v.sp.assign(Nh + 1, 0.0);
+    HIPCHK(hipMemcpyAsync(v.fr.data(), h->aFr.p, (size_t)Nr * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(v.im.data(), h->aImax.p, (size_t)Nr * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(v.ft.data(), h->aFt.p, (size_t)Nt * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(v.sp.data(), h->aS.p, (size_t)(Nh + 1) * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    v.re.resize(Nr); v.rm.resize(Nr);
+    for (int i = 0; i < Nr; i++) {
+      v.re[i] = c.re[(size_t)i * B + member];
+      v.rm[i] = c.rm[(size_t)i * B + member];
+    }
+    return 0;
   }
-  HIPCHK(hipStreamSynchronize(h->st));
+  if (!c.valid) {
+    // the per-member column sums are the column sums of [Nr][B x ncol]
+    if (h->d.plasma && !h->plasma_valid)
+      if (int rc = plasma_only<R>(h)) return rc;
+    const int n = h->rows();
+    HIPCHK(launch_analyze_rows<R>(h->S.as<cx<R>>(), h->tw.as<cx<R>>(), h->Rg.as<R>(), h->Tg.as<R>(),
+                                  h->pending_filter ? 1 : 0, (R)h->dt, h->aFr.as<double>(), h->aIt.as<R>(),
+                                  h->aSw.as<R>(), h->aImax.as<double>(), n, Nt, 1, 0, B, h->st));
+    h->pending_filter = false;
+    HIPCHK(launch_colsum<R>(h->aIt.as<R>(), Nr, B * Nt, h->ardr.as<double>(), h->acol.as<double>(),
+                            h->aFt.as<double>(), h->st));
+    HIPCHK(launch_colsum<R>(h->aSw.as<R>(), Nr, B * (Nh + 1), h->ardr.as<double>(), h->acol.as<double>(),
+                            h->aS.as<double>(), h->st));
+    c.fr.assign(n, 0.0); c.im.assign(n, 0.0); c.re.assign(n, 0.0); c.rm.assign(n, 0.0);
+    c.ft.assign((size_t)B * Nt, 0.0); c.sp.assign((size_t)B * (Nh + 1), 0.0);
+    HIPCHK(hipMemcpyAsync(c.fr.data(), h->aFr.p, (size_t)n * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(c.im.data(), h->aImax.p, (size_t)n * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(c.ft.data(), h->aFt.p, c.ft.size() * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(c.sp.data(), h->aS.p, c.sp.size() * 8, hipMemcpyDeviceToHost, h->st));
+    if (h->d.plasma) {
+      HIPCHK(hipMemcpyAsync(c.re.data(), h->rho_end.p, (size_t)n * 8, hipMemcpyDeviceToHost, h->st));
+      HIPCHK(hipMemcpyAsync(c.rm.data(), h->rho_max.p, (size_t)n * 8, hipMemcpyDeviceToHost, h->st));
+    }
+    HIPCHK(hipStreamSynchronize(h->st));
+    c.valid = true;
+    c.served.assign(B, 0);
+  }
+  v.fr.resize(Nr); v.im.resize(Nr); v.re.resize(Nr); v.rm.resize(Nr);
+  for (int i = 0; i < Nr; i++) {
+    const size_t q = (size_t)i * B + member;
+    v.fr[i] = c.fr[q]; v.im[i] = c.im[q]; v.re[i] = c.re[q]; v.rm[i] = c.rm[q];
+  }
+  v.ft.assign(c.ft.begin() + (size_t)member * Nt, c.ft.begin() + (size_t)(member + 1) * Nt);
+  v.sp.assign(c.sp.begin() + (size_t)member * (Nh + 1), c.sp.begin() + (size_t)(member + 1) * (Nh + 1));
+  c.served[member] = 1;
   return 0;
 }
 
 template <typename R>
-int analyze_impl(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
+int analyze_impl(fil_handle* h, int member, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
   const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh;
   AnalyzerRows v;
-  if (int rc = analyze_device<R>(h, v)) return rc;
+  if (int rc = analyze_device<R>(h, v, member)) return rc;
   // scalar diagnostics (FieldAnalyzers.jl:201-213)
   fil_diag o{};
   o.z = z;
@@ -356,7 +459,7 @@ template <typename R>
 int analyze_xyt_impl(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, double* Ft, double* rho_end, double* S) {
   const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh, Nx = h->Nx, Ny = h->Ny;
   AnalyzerRows v;
-  if (int rc = analyze_device<R>(h, v)) return rc;
+  if (int rc = analyze_device<R>(h, v, 0)) return rc;
   fil_diag_xyt o{};
   o.z = z;
   const int imax = (int)(std::max_element(v.fr.begin(), v.fr.end()) - v.fr.begin());   // findmax, x fastest
@@ -482,6 +585,9 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (xyt && (!pow2(d->Nx) || !pow2(d->Ny) || d->Nx < 16 || d->Ny < 16 || d->Nx > 1024 || d->Ny > 1024))
     return fail(FIL_EINVAL, "XYT: Nx and Ny must be powers of two in [16, 1024]");
   if (xyt && d->guard3d_bug && d->Ny > d->Nt) return fail(FIL_EINVAL, "guard3d_bug needs Ny <= Nt (the reference indexes the t window with y)");
+  const int batch = d->batch < 1 ? 1 : d->batch;
+  if (batch > 1 && xyt) return fail(FIL_EINVAL, "batches are defined for the RT geometry");
+  if (batch > 4096) return fail(FIL_EINVAL, "batch must be <= 4096");
   if (d->precision != FIL_FP32 && d->precision != FIL_FP64) return fail(FIL_EINVAL, "precision must be FIL_FP32 or FIL_FP64");
   const int Nt = d->Nt;
   if (Nt < 256 || Nt > 8192 || (Nt & (Nt - 1))) return fail(FIL_EINVAL, "Nt must be a power of two in [256, 8192]");
@@ -504,6 +610,7 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   h->Nx = xyt ? d->Nx : 0;
   h->Ny = xyt ? d->Ny : 0;
   h->Nr = xyt ? d->Nx * d->Ny : d->Nr;
+  h->B = batch;
   h->Nrp = (h->Nr + 63) / 64 * 64;
   h->Nt = Nt;
   h->Nh = Nt / 2;
@@ -513,25 +620,25 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   hipError_t e = hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking);
   if (e != hipSuccess) { delete h; return fail(FIL_EDEVICE, hipGetErrorString(e)); }
   const size_t cz = h->csz(), rz = h->rsz();
-  const size_t NS = (size_t)h->Nrp * h->Nh;
+  const size_t NS = h->ns(), NP = (size_t)h->Nrp * h->Nh, NR = (size_t)h->rows();
   bool ok = h->S.alloc(NS * cz) == hipSuccess && h->X.alloc(xyt ? 0 : NS * cz) == hipSuccess &&
-            h->P.alloc(NS * cz) == hipSuccess && h->Et.alloc((size_t)h->Nrp * Nt * cz) == hipSuccess &&
-            h->rho_end.alloc(h->Nr * 8) == hipSuccess && h->rho_max.alloc(h->Nr * 8) == hipSuccess &&
-            h->aFr.alloc(h->Nr * 8) == hipSuccess && h->aImax.alloc(h->Nr * 8) == hipSuccess &&
-            h->aIt.alloc((size_t)h->Nr * Nt * 4) == hipSuccess &&
-            h->aSw.alloc((size_t)h->Nr * (h->Nh + 1) * 4) == hipSuccess &&
-            h->aFt.alloc(Nt * 8) == hipSuccess && h->aS.alloc((h->Nh + 1) * 8) == hipSuccess &&
-            h->acol.alloc((size_t)COLSUM_MAXCHUNK * Nt * 8) == hipSuccess;
+            h->P.alloc(NP * cz) == hipSuccess && h->Et.alloc((size_t)h->Nrp * Nt * cz) == hipSuccess &&
+            h->rho_end.alloc(NR * 8) == hipSuccess && h->rho_max.alloc(NR * 8) == hipSuccess &&
+            h->aFr.alloc(NR * 8) == hipSuccess && h->aImax.alloc(NR * 8) == hipSuccess &&
+            h->aIt.alloc(NR * Nt * rz) == hipSuccess && h->aSw.alloc(NR * (h->Nh + 1) * rz) == hipSuccess &&
+            h->aFt.alloc((size_t)batch * Nt * 8) == hipSuccess &&
+            h->aS.alloc((size_t)batch * (h->Nh + 1) * 8) == hipSuccess &&
+            h->acol.alloc(std::min<size_t>(COLSUM_MAXCHUNK, (h->Nr + 63) / 64) * batch * Nt * 8) == hipSuccess;
   if (ok && d->keep_plasma)
-    ok = h->rho_full.alloc((size_t)h->Nr * Nt * rz) == hipSuccess && h->kd_full.alloc((size_t)h->Nr * Nt * rz) == hipSuccess;
+    ok = h->rho_full.alloc(NR * Nt * rz) == hipSuccess && h->kd_full.alloc(NR * Nt * rz) == hipSuccess;
   if (!ok) { delete h; return fail(FIL_ENOMEM, "device allocation failed"); }
   (void)hipMemset(h->S.p, 0, NS * cz);
   if (!xyt) (void)hipMemset(h->X.p, 0, NS * cz);
-  (void)hipMemset(h->rho_end.p, 0, h->Nr * 8);
-  (void)hipMemset(h->rho_max.p, 0, h->Nr * 8);
+  (void)hipMemset(h->rho_end.p, 0, NR * 8);
+  (void)hipMemset(h->rho_max.p, 0, NR * 8);
   if (d->keep_plasma) {
-    (void)hipMemset(h->rho_full.p, 0, (size_t)h->Nr * Nt * rz);
-    (void)hipMemset(h->kd_full.p, 0, (size_t)h->Nr * Nt * rz);
+    (void)hipMemset(h->rho_full.p, 0, NR * Nt * rz);
+    (void)hipMemset(h->kd_full.p, 0, NR * Nt * rz);
   }
   *out = h;
   return FIL_OK;
@@ -781,6 +888,7 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
     h->pp32 = p32;
   }
   h->tables = true;
+  h->acache.valid = false;
   return FIL_OK;
 }
 
@@ -802,11 +910,17 @@ int fil_set_tables_xyt(fil_handle* h, const fil_tables* t, const fil_tables_xyt*
   return set_tables_impl(h, t, x);
 }
 
-int fil_upload_field(fil_handle* h, const double* E, int64_t ld) {
+int fil_upload_member(fil_handle* h, int member, const double* E, int64_t ld) {
   if (!valid_handle(h) || !E) return fail(FIL_EINVAL, "null argument");
   if (!h->tables) return fail(FIL_ESTATE, "fil_set_tables first");
+  if (member < 0 || member >= h->B) return fail(FIL_EINVAL, "member out of range");
   if (ld < h->Nr) return fail(FIL_EINVAL, "ld < Nr");
   HIPCHK(hipSetDevice(h->d.device));
+  // the other members' pending filter is applied before this one is replaced
+  if (h->B > 1 && h->field) {
+    int rc = h->fp64 ? apply_pending_filter<double>(h) : apply_pending_filter<float>(h);
+    if (rc) return rc;
+  }
   const int Nr = h->Nr, Nt = h->Nt;
   HostArr a((size_t)h->Nrp * Nt, h->fp64, true);
   for (int j = 0; j < Nt; j++)
@@ -816,28 +930,42 @@ int fil_upload_field(fil_handle* h, const double* E, int64_t ld) {
     }
   HIPCHK(hipMemcpyAsync(h->Et.p, a.b.data(), a.b.size(), hipMemcpyHostToDevice, h->st));
   if (h->fp64)
-    HIPCHK(launch_to_spectrum<double>(h->Et.as<cx<double>>(), h->S.as<cx<double>>(), h->tw.as<cx<double>>(), Nr, Nt, h->st));
+    HIPCHK(launch_to_spectrum<double>(h->Et.as<cx<double>>(), h->S.as<cx<double>>(), h->tw.as<cx<double>>(), Nr, Nt,
+                                      h->B, member, h->st));
   else
-    HIPCHK(launch_to_spectrum<float>(h->Et.as<cx<float>>(), h->S.as<cx<float>>(), h->tw.as<cx<float>>(), Nr, Nt, h->st));
+    HIPCHK(launch_to_spectrum<float>(h->Et.as<cx<float>>(), h->S.as<cx<float>>(), h->tw.as<cx<float>>(), Nr, Nt,
+                                     h->B, member, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
-  h->field = true;
+  if (h->uploaded.size() != (size_t)h->B) h->uploaded.assign(h->B, 0);
+  h->uploaded[member] = 1;
+  h->acache.valid = false;
+  h->field = std::all_of(h->uploaded.begin(), h->uploaded.end(), [](char c) { return c != 0; });
   h->pending_filter = false;
   h->plasma_valid = false;
   return FIL_OK;
 }
 
-int fil_download_field(fil_handle* h, double* E, int64_t ld) {
+int fil_upload_field(fil_handle* h, const double* E, int64_t ld) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (h->B > 1) return fail(FIL_EINVAL, "batched handles take fil_upload_member");
+  return fil_upload_member(h, 0, E, ld);
+}
+
+int fil_download_member(fil_handle* h, int member, double* E, int64_t ld) {
   if (!valid_handle(h) || !E) return fail(FIL_EINVAL, "null argument");
   if (!h->field) return fail(FIL_ESTATE, "no field uploaded");
+  if (member < 0 || member >= h->B) return fail(FIL_EINVAL, "member out of range");
   if (ld < h->Nr) return fail(FIL_EINVAL, "ld < Nr");
   HIPCHK(hipSetDevice(h->d.device));
   const int Nr = h->Nr, Nt = h->Nt;
   if (h->fp64)
     HIPCHK(launch_to_time<double>(h->S.as<cx<double>>(), h->Et.as<cx<double>>(), h->tw.as<cx<double>>(),
-                                  h->Rg.as<double>(), h->Tg.as<double>(), h->pending_filter, Nr, Nt, h->st));
+                                  h->Rg.as<double>(), h->Tg.as<double>(), h->pending_filter, Nr, Nt, h->B, member,
+                                  h->st));
   else
     HIPCHK(launch_to_time<float>(h->S.as<cx<float>>(), h->Et.as<cx<float>>(), h->tw.as<cx<float>>(),
-                                 h->Rg.as<float>(), h->Tg.as<float>(), h->pending_filter, Nr, Nt, h->st));
+                                 h->Rg.as<float>(), h->Tg.as<float>(), h->pending_filter, Nr, Nt, h->B, member,
+                                 h->st));
   std::vector<unsigned char> buf((size_t)Nr * Nt * h->csz());
   HIPCHK(hipMemcpyAsync(buf.data(), h->Et.p, buf.size(), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
@@ -855,6 +983,12 @@ int fil_download_field(fil_handle* h, double* E, int64_t ld) {
   return FIL_OK;
 }
 
+int fil_download_field(fil_handle* h, double* E, int64_t ld) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (h->B > 1) return fail(FIL_EINVAL, "batched handles take fil_download_member");
+  return fil_download_member(h, 0, E, ld);
+}
+
 int fil_zstep(fil_handle* h, double z, double dz) {
   (void)z;
   if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
@@ -864,12 +998,165 @@ int fil_zstep(fil_handle* h, double z, double dz) {
   return h->fp64 ? zstep_impl<double>(h, dz) : zstep_impl<float>(h, dz);
 }
 
-int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
+int fil_group_create(fil_handle* const* hs, int n, fil_group** out) {
+  if (!hs || !out || n < 1) return fail(FIL_EINVAL, "null argument or empty group");
+  *out = nullptr;
+  if (n > FIL_GROUP_MAX) return fail(FIL_EINVAL, "too many handles in a group");
+  for (int k = 0; k < n; k++) {
+    if (!valid_handle(hs[k])) return fail(FIL_EINVAL, "null handle in group");
+    if (hs[k]->d.device != hs[0]->d.device) return fail(FIL_EINVAL, "group handles on different devices");
+    for (int q = 0; q < k; q++)
+      if (hs[q] == hs[k]) return fail(FIL_EINVAL, "a handle appears twice in the group");
+  }
+  HIPCHK(hipSetDevice(hs[0]->d.device));
+  auto* g = new fil_group();
+  g->hs.assign(hs, hs + n);
+  g->device = hs[0]->d.device;
+  bool ok = hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&g->post, hipEventDisableTiming) == hipSuccess;
+  g->join.assign(n, nullptr);
+  g->pre.assign(n, nullptr);
+  for (int k = 0; ok && k < n; k++)
+    ok = hipEventCreateWithFlags(&g->join[k], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&g->pre[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    fil_group_destroy(g);
+    return fail(FIL_EDEVICE, "stream or event creation failed");
+  }
+  *out = g;
+  return FIL_OK;
+}
+
+namespace {
+
+// Records one zstep of every handle of the group as a graph: a fork event on
+// the group's stream, each handle's kernels on its own stream (parallel
+// branches), joined back.  The handles' host flags are restored afterwards:
+// capturing runs nothing.
+int group_capture(fil_group* g, double dz, hipGraphExec_t* exec) {
+  const int n = (int)g->hs.size();
+  std::vector<char> pf(n), pv(n), tm(n);
+  for (int k = 0; k < n; k++) {
+    pf[k] = g->hs[k]->pending_filter;
+    pv[k] = g->hs[k]->plasma_valid;
+    tm[k] = g->hs[k]->timing;
+    g->hs[k]->timing = false;
+  }
+  int rc = 0;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(g->st, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) rc = fail(FIL_EDEVICE, hipGetErrorString(e));
+  if (!rc && hipEventRecord(g->fork, g->st) != hipSuccess) rc = fail(FIL_EDEVICE, "event record in capture failed");
+  for (int k = 0; !rc && k < n; k++) {
+    fil_handle* h = g->hs[k];
+    if (hipStreamWaitEvent(h->st, g->fork, 0) != hipSuccess) { rc = fail(FIL_EDEVICE, "stream join failed"); break; }
+    rc = h->fp64 ? zstep_impl<double>(h, dz) : zstep_impl<float>(h, dz);
+    if (!rc && (hipEventRecord(g->join[k], h->st) != hipSuccess || hipStreamWaitEvent(g->st, g->join[k], 0) != hipSuccess))
+      rc = fail(FIL_EDEVICE, "branch join failed");
+  }
+  if (e == hipSuccess) {
+    hipError_t e2 = hipStreamEndCapture(g->st, &graph);
+    if (!rc && e2 != hipSuccess) rc = fail(FIL_EDEVICE, hipGetErrorString(e2));
+  }
+  for (int k = 0; k < n; k++) {
+    g->hs[k]->pending_filter = pf[k];
+    g->hs[k]->plasma_valid = pv[k];
+    g->hs[k]->timing = tm[k];
+  }
+  if (!rc) {
+    hipError_t e3 = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    if (e3 != hipSuccess) rc = fail(FIL_EDEVICE, hipGetErrorString(e3));
+  }
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipGetLastError();
+  return rc;
+}
+
+}  // namespace
+
+int fil_group_zstep(fil_group* g, double z, double dz) {
+  (void)z;
+  if (!g) return fail(FIL_EINVAL, "null group");
+  if (!std::isfinite(dz)) return fail(FIL_EINVAL, "dz must be finite");
+  const int n = (int)g->hs.size();
+  for (auto* h : g->hs)
+    if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_group_zstep");
+  HIPCHK(hipSetDevice(g->device));
+  // the propagators for this dz, outside the graph
+  uint64_t pattern = 0;
+  for (int k = 0; k < n; k++) {
+    fil_handle* h = g->hs[k];
+    if (int rc = h->fp64 ? update_propagator<double>(h, dz) : update_propagator<float>(h, dz)) return rc;
+    if (h->pending_filter) pattern |= 1ull << k;
+  }
+  hipGraphExec_t exec = nullptr;
+  for (auto& c : g->cache)
+    if (c.dz == dz && c.pattern == pattern) exec = c.exec;
+  if (!exec) {
+    if (g->cache.size() >= 4) {
+      HIPCHK(hipStreamSynchronize(g->st));
+      (void)hipGraphExecDestroy(g->cache.front().exec);
+      g->cache.erase(g->cache.begin());
+    }
+    if (int rc = group_capture(g, dz, &exec)) return rc;
+    g->cache.push_back({dz, pattern, exec});
+    g->captures++;
+  }
+  // the graph runs after each handle's earlier work and before its later work
+  for (int k = 0; k < n; k++) {
+    HIPCHK(hipEventRecord(g->pre[k], g->hs[k]->st));
+    HIPCHK(hipStreamWaitEvent(g->st, g->pre[k], 0));
+  }
+  HIPCHK(hipGraphLaunch(exec, g->st));
+  HIPCHK(hipEventRecord(g->post, g->st));
+  for (auto* h : g->hs) {
+    HIPCHK(hipStreamWaitEvent(h->st, g->post, 0));
+    // the state after a zstep (as zstep_impl leaves it)
+    h->pending_filter = true;
+    h->plasma_valid = h->d.plasma != 0;
+    h->acache.valid = false;
+  }
+  g->replays++;
+  return FIL_OK;
+}
+
+int fil_group_stats(fil_group* g, int64_t* captures, int64_t* replays) {
+  if (!g) return fail(FIL_EINVAL, "null group");
+  if (captures) *captures = g->captures;
+  if (replays) *replays = g->replays;
+  return FIL_OK;
+}
+
+int fil_group_destroy(fil_group* g) {
+  if (!g) return FIL_OK;
+  (void)hipSetDevice(g->device);
+  if (g->st) (void)hipStreamSynchronize(g->st);
+  for (auto& c : g->cache) (void)hipGraphExecDestroy(c.exec);
+  for (auto e : g->join) if (e) (void)hipEventDestroy(e);
+  for (auto e : g->pre) if (e) (void)hipEventDestroy(e);
+  if (g->fork) (void)hipEventDestroy(g->fork);
+  if (g->post) (void)hipEventDestroy(g->post);
+  if (g->st) (void)hipStreamDestroy(g->st);
+  delete g;
+  return FIL_OK;
+}
+
+int fil_analyze_member(fil_handle* h, int member, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end,
+                       double* S) {
   if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
   if (h->xyt) return fail(FIL_EINVAL, "XYT handles take fil_analyze_xyt");
+  if (member < 0 || member >= h->B) return fail(FIL_EINVAL, "member out of range");
   if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_analyze");
   HIPCHK(hipSetDevice(h->d.device));
-  return h->fp64 ? analyze_impl<double>(h, z, d, Fr, Ft, rho_end, S) : analyze_impl<float>(h, z, d, Fr, Ft, rho_end, S);
+  return h->fp64 ? analyze_impl<double>(h, member, z, d, Fr, Ft, rho_end, S)
+                 : analyze_impl<float>(h, member, z, d, Fr, Ft, rho_end, S);
+}
+
+int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (h->B > 1) return fail(FIL_EINVAL, "batched handles take fil_analyze_member");
+  return fil_analyze_member(h, 0, z, d, Fr, Ft, rho_end, S);
 }
 
 int fil_analyze_xyt(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, double* Ft, double* rho_end, double* S) {
@@ -884,6 +1171,7 @@ int fil_analyze_xyt(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, doubl
 int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld) {
   if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
   if (!h->d.plasma || !h->d.keep_plasma) return fail(FIL_ESTATE, "create the handle with plasma and keep_plasma");
+  if (h->B > 1) return fail(FIL_EINVAL, "fil_download_plasma takes single-pulse handles");
   if (ld < h->Nr) return fail(FIL_EINVAL, "ld < Nr");
   HIPCHK(hipSetDevice(h->d.device));
   if (!h->plasma_valid) {
@@ -912,14 +1200,14 @@ int fil_state_save(fil_handle* h, int slot) {
   if (!h->field) return fail(FIL_ESTATE, "no field uploaded");
   HIPCHK(hipSetDevice(h->d.device));
   auto& s = h->slot[slot];
-  const size_t nb = (size_t)h->Nrp * h->Nh * h->csz();
+  const size_t nb = h->ns() * h->csz(), nr = (size_t)h->rows() * 8;
   if (!s.S.p) {
-    if (s.S.alloc(nb) != hipSuccess || s.rho_end.alloc(h->Nr * 8) != hipSuccess || s.rho_max.alloc(h->Nr * 8) != hipSuccess)
+    if (s.S.alloc(nb) != hipSuccess || s.rho_end.alloc(nr) != hipSuccess || s.rho_max.alloc(nr) != hipSuccess)
       return fail(FIL_ENOMEM, "device allocation failed");
   }
   HIPCHK(hipMemcpyAsync(s.S.p, h->S.p, nb, hipMemcpyDeviceToDevice, h->st));
-  HIPCHK(hipMemcpyAsync(s.rho_end.p, h->rho_end.p, h->Nr * 8, hipMemcpyDeviceToDevice, h->st));
-  HIPCHK(hipMemcpyAsync(s.rho_max.p, h->rho_max.p, h->Nr * 8, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(s.rho_end.p, h->rho_end.p, nr, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(s.rho_max.p, h->rho_max.p, nr, hipMemcpyDeviceToDevice, h->st));
   s.used = true;
   s.pending_filter = h->pending_filter;
   s.plasma_valid = h->plasma_valid;
@@ -930,12 +1218,12 @@ int fil_state_copy(fil_handle* dst, const fil_handle* src) {
   if (!valid_handle(dst) || !src) return fail(FIL_EINVAL, "null handle");
   if (dst == src) return FIL_OK;
   if (dst->d.device != src->d.device) return fail(FIL_EINVAL, "handles on different devices");
-  if (dst->Nr != src->Nr || dst->Nt != src->Nt) return fail(FIL_EINVAL, "handles of different grids");
+  if (dst->Nr != src->Nr || dst->Nt != src->Nt || dst->B != src->B) return fail(FIL_EINVAL, "handles of different grids");
   if (!src->field || !dst->tables) return fail(FIL_ESTATE, "source needs a field and destination its tables");
   HIPCHK(hipSetDevice(dst->d.device));
   // the source's pending work completes before the destination's stream reads it
   HIPCHK(hipStreamSynchronize(src->st));
-  const size_t n = (size_t)dst->Nrp * dst->Nh;
+  const size_t n = dst->ns();
   const dim3 grid((unsigned)((n + 255) / 256));
   if (dst->fp64 == src->fp64) {
     HIPCHK(hipMemcpyAsync(dst->S.p, src->S.p, n * dst->csz(), hipMemcpyDeviceToDevice, dst->st));
@@ -949,10 +1237,11 @@ int fil_state_copy(fil_handle* dst, const fil_handle* src) {
     HIPCHK(hipGetLastError());
   }
   if (src->d.plasma && dst->d.plasma) {
-    HIPCHK(hipMemcpyAsync(dst->rho_end.p, src->rho_end.p, dst->Nr * 8, hipMemcpyDeviceToDevice, dst->st));
-    HIPCHK(hipMemcpyAsync(dst->rho_max.p, src->rho_max.p, dst->Nr * 8, hipMemcpyDeviceToDevice, dst->st));
+    HIPCHK(hipMemcpyAsync(dst->rho_end.p, src->rho_end.p, (size_t)dst->rows() * 8, hipMemcpyDeviceToDevice, dst->st));
+    HIPCHK(hipMemcpyAsync(dst->rho_max.p, src->rho_max.p, (size_t)dst->rows() * 8, hipMemcpyDeviceToDevice, dst->st));
   }
   dst->field = true;
+  dst->acache.valid = false;
   dst->pending_filter = src->pending_filter;
   dst->plasma_valid = src->plasma_valid && src->d.plasma && dst->d.plasma;
   return FIL_OK;
@@ -964,12 +1253,13 @@ int fil_state_restore(fil_handle* h, int slot) {
   auto& s = h->slot[slot];
   if (!s.used) return fail(FIL_ESTATE, "slot is empty");
   HIPCHK(hipSetDevice(h->d.device));
-  const size_t nb = (size_t)h->Nrp * h->Nh * h->csz();
+  const size_t nb = h->ns() * h->csz(), nr = (size_t)h->rows() * 8;
   HIPCHK(hipMemcpyAsync(h->S.p, s.S.p, nb, hipMemcpyDeviceToDevice, h->st));
-  HIPCHK(hipMemcpyAsync(h->rho_end.p, s.rho_end.p, h->Nr * 8, hipMemcpyDeviceToDevice, h->st));
-  HIPCHK(hipMemcpyAsync(h->rho_max.p, s.rho_max.p, h->Nr * 8, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->rho_end.p, s.rho_end.p, nr, hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->rho_max.p, s.rho_max.p, nr, hipMemcpyDeviceToDevice, h->st));
   h->pending_filter = s.pending_filter;
   h->plasma_valid = s.plasma_valid;
+  h->acache.valid = false;
   return FIL_OK;
 }
 
@@ -980,12 +1270,22 @@ int fil_fluence(fil_handle* h, int slot, double* Fr) {
   HIPCHK(hipSetDevice(h->d.device));
   const void* S = slot < 0 ? h->S.p : h->slot[slot].S.p;
   const double scale = (double)h->Nt * h->dt;   // dt sum_t |E|^2 = dt Nt sum_w |S_w|^2
+  const int n = h->rows(), B = h->B, Nr = h->Nr;
   if (h->fp64)
-    HIPCHK(launch_rowsum_abs2<double>(static_cast<const cx<double>*>(S), h->Nr, h->Nh, scale, h->aFr.as<double>(), h->st));
+    HIPCHK(launch_rowsum_abs2<double>(static_cast<const cx<double>*>(S), n, h->Nh, scale, h->aFr.as<double>(), h->st));
   else
-    HIPCHK(launch_rowsum_abs2<float>(static_cast<const cx<float>*>(S), h->Nr, h->Nh, scale, h->aFr.as<double>(), h->st));
-  HIPCHK(hipMemcpyAsync(Fr, h->aFr.p, h->Nr * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(launch_rowsum_abs2<float>(static_cast<const cx<float>*>(S), n, h->Nh, scale, h->aFr.as<double>(), h->st));
+  if (B == 1) {
+    HIPCHK(hipMemcpyAsync(Fr, h->aFr.p, (size_t)n * 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return FIL_OK;
+  }
+  // device rows are (r, member) with r slowest; the caller gets Fr[member][r]
+  std::vector<double> t(n);
+  HIPCHK(hipMemcpyAsync(t.data(), h->aFr.p, (size_t)n * 8, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  for (int r = 0; r < Nr; r++)
+    for (int b = 0; b < B; b++) Fr[(size_t)b * Nr + r] = t[(size_t)r * B + b];
   return FIL_OK;
 }
 
@@ -1028,6 +1328,7 @@ int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
   if (!valid_handle(h) || !cd) return fail(FIL_EINVAL, "null argument");
   if (h->fp64) return fail(FIL_EINVAL, "the corrector needs an FP32 handle");
   if (h->xyt) return fail(FIL_EINVAL, "the corrector is defined for the RT geometry");
+  if (h->B > 1) return fail(FIL_EINVAL, "the corrector takes single-pulse handles");
   if (!h->tables) return fail(FIL_ESTATE, "fil_set_tables first");
   if (cd->pool_r < 1 || cd->pool_t < 1) return fail(FIL_EINVAL, "pool sizes must be >= 1");
   if (cd->nr < 1 || cd->nr > h->Nr || cd->nr % cd->pool_r) return fail(FIL_EINVAL, "nr must be in [1, Nr] and a multiple of pool_r");
@@ -1244,6 +1545,7 @@ int fil_corrector_apply(fil_handle* h) {
   const float scale = (float)(c.y_scale / (double)c.g.inv_as);
   HIPCHK(launch_corr_apply(h->S.as<cx<float>>(), h->tw.as<cx<float>>(), c.g, h->Nt, c.out.as<float>(), scale, h->st));
   h->plasma_valid = false;
+  h->acache.valid = false;
   return FIL_OK;
 }
 

@@ -33,7 +33,7 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 template <int BM, int BW, int BK, int WM, int WW>
 __global__ __launch_bounds__(256) void dht_gemm_f32_kernel(const float* __restrict__ MT, const cx<float>* __restrict__ In,
                                                            cx<float>* __restrict__ Out, const cx<float>* __restrict__ P,
-                                                           int Nr, int Nh) {
+                                                           int Nr, int Nh, int Pw) {
   constexpr int MT_ = WM / 16, WT = WW / 16;
   constexpr int WAVES_M = BM / WM;
   static_assert((BM / WM) * (BW / WW) == 4, "4 waves per workgroup");
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void dht_gemm_f32_kernel(const float* __restri
         const int w = w0 + ww + b * 16 + l16;
         cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
         const size_t o = (size_t)m * Nh + w;
-        if (P) v = v * P[o];
+        if (P) v = v * P[(size_t)m * Pw + (w & (Pw - 1))];
         Out[o] = v;
       }
 }
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void dht_gemm_f32_kernel(const float* __restri
 template <typename R>
 __global__ __launch_bounds__(256) void dht_gemm_valu_kernel(const R* __restrict__ MT, const cx<R>* __restrict__ In,
                                                             cx<R>* __restrict__ Out, const cx<R>* __restrict__ P,
-                                                            int Nr, int Nh) {
+                                                            int Nr, int Nh, int Pw) {
   constexpr int BM = 64, BW = 64, BK = 16;
   __shared__ R As[BK][BM + 1];
   __shared__ cx<R> Bs[BK][BW];
@@ -165,30 +165,37 @@ __global__ __launch_bounds__(256) void dht_gemm_valu_kernel(const R* __restrict_
   for (int x = 0; x < 4; x++)
 #pragma unroll
     for (int y = 0; y < 4; y++) {
-      const size_t o = (size_t)(m0 + tm + x) * Nh + w0 + tw + 16 * y;
+      const int w = w0 + tw + 16 * y;
+      const size_t o = (size_t)(m0 + tm + x) * Nh + w;
       cx<R> v = acc[x][y];
-      if (P) v = v * P[o];
+      if (P) v = v * P[(size_t)(m0 + tm + x) * Pw + (w & (Pw - 1))];
       Out[o] = v;
     }
 }
 
+// Tile shape by the size of one pulse's transform (Nr x Pw): a batch of
+// pulses takes the same kernel, hence the same summation order, as each
+// pulse on its own.
+static bool wide_tiles(int Nr, int Pw) { return Nr % 128 == 0 && (long)(Nr / 128) * (Pw / 64) >= 256; }
+
 template <typename R>
-hipError_t launch_dht_gemm(const R* MT, const cx<R>* In, cx<R>* Out, const cx<R>* P, int Nr, int Nh, hipStream_t st) {
-  if (Nr % 64 || Nh % 64) return hipErrorInvalidValue;
+hipError_t launch_dht_gemm(const R* MT, const cx<R>* In, cx<R>* Out, const cx<R>* P, int Nr, int Nh, int Pw,
+                           hipStream_t st) {
+  if (Nr % 64 || Nh % 64 || Pw < 1 || (Pw & (Pw - 1)) || Nh % Pw) return hipErrorInvalidValue;
   if (sizeof(R) == 8) {
-    hipLaunchKernelGGL((dht_gemm_valu_kernel<R>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st, MT, In, Out, P, Nr, Nh);
+    hipLaunchKernelGGL((dht_gemm_valu_kernel<R>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st, MT, In, Out, P, Nr, Nh, Pw);
     return hipGetLastError();
   }
   const float* A = (const float*)MT;
   const cx<float>* B = (const cx<float>*)In;
   cx<float>* C = (cx<float>*)Out;
   const cx<float>* PP = (const cx<float>*)P;
-  if (Nr % 128 == 0 && (long)(Nr / 128) * (Nh / 64) >= 256) {
+  if (wide_tiles(Nr, Pw)) {
     hipLaunchKernelGGL((dht_gemm_f32_kernel<128, 64, 32, 64, 32>), dim3(Nh / 64, Nr / 128), dim3(256), 0, st,
-                       A, B, C, PP, Nr, Nh);
+                       A, B, C, PP, Nr, Nh, Pw);
   } else {
     hipLaunchKernelGGL((dht_gemm_f32_kernel<64, 64, 16, 32, 32>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st,
-                       A, B, C, PP, Nr, Nh);
+                       A, B, C, PP, Nr, Nh, Pw);
   }
   return hipGetLastError();
 }
@@ -215,8 +222,8 @@ hipError_t launch_propagator(const double* KZ64, const double* KG64, double dz, 
   return hipGetLastError();
 }
 
-template hipError_t launch_dht_gemm<float>(const float*, const cx<float>*, cx<float>*, const cx<float>*, int, int, hipStream_t);
-template hipError_t launch_dht_gemm<double>(const double*, const cx<double>*, cx<double>*, const cx<double>*, int, int, hipStream_t);
+template hipError_t launch_dht_gemm<float>(const float*, const cx<float>*, cx<float>*, const cx<float>*, int, int, int, hipStream_t);
+template hipError_t launch_dht_gemm<double>(const double*, const cx<double>*, cx<double>*, const cx<double>*, int, int, int, hipStream_t);
 template hipError_t launch_propagator<float>(const double*, const double*, double, cx<float>*, int, int, hipStream_t);
 template hipError_t launch_propagator<double>(const double*, const double*, double, cx<double>*, int, int, hipStream_t);
 
@@ -244,7 +251,7 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __
                                                               const uint16_t* __restrict__ Mlo,
                                                               const cx<float>* __restrict__ In,
                                                               cx<float>* __restrict__ Out,
-                                                              const cx<float>* __restrict__ P, int Nr, int Nh) {
+                                                              const cx<float>* __restrict__ P, int Nr, int Nh, int Pw) {
   constexpr int BK = 32;
   constexpr int MT = BM / 32, WT = BW / 32;
   __shared__ __align__(16) uint16_t Ah[BM][BK + 8];
@@ -314,7 +321,7 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __
         const int w = w0 + ww + b * 16 + l16;
         cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
         const size_t o = (size_t)m * Nh + w;
-        if (P) v = v * P[o];
+        if (P) v = v * P[(size_t)m * Pw + (w & (Pw - 1))];
         Out[o] = v;
       }
 }
@@ -328,7 +335,7 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_v2_kernel(const uint16_t*
                                                                  const uint16_t* __restrict__ Mlo,
                                                                  const cx<float>* __restrict__ In,
                                                                  cx<float>* __restrict__ Out,
-                                                                 const cx<float>* __restrict__ P, int Nr, int Nh) {
+                                                                 const cx<float>* __restrict__ P, int Nr, int Nh, int Pw) {
   constexpr int SK = BK + 8;              // padded row: (BK + 8) bf16
   constexpr int WM = BM / 2, WW = BW / 2;
   constexpr int MT = WM / 16, WT = WW / 16;
@@ -428,20 +435,20 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_v2_kernel(const uint16_t*
         const int w = w0 + ww + b * 16 + l16;
         cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
         const size_t o = (size_t)m * Nh + w;
-        if (P) v = v * P[o];
+        if (P) v = v * P[(size_t)m * Pw + (w & (Pw - 1))];
         Out[o] = v;
       }
 }
 
 hipError_t launch_dht_gemm_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In, cx<float>* Out,
-                                  const cx<float>* P, int Nr, int Nh, hipStream_t st) {
-  if (Nr % 64 || Nh % 64) return hipErrorInvalidValue;
-  if (Nr % 128 == 0 && (long)(Nr / 128) * (Nh / 64) >= 256) {
+                                  const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st) {
+  if (Nr % 64 || Nh % 64 || Pw < 1 || (Pw & (Pw - 1)) || Nh % Pw) return hipErrorInvalidValue;
+  if (wide_tiles(Nr, Pw)) {
     hipLaunchKernelGGL((dht_gemm_bf16x3_v2_kernel<128, 64, 64>), dim3(Nh / 64, Nr / 128), dim3(256), 0, st,
-                       Mhi, Mlo, In, Out, P, Nr, Nh);
+                       Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
   } else {
     hipLaunchKernelGGL((dht_gemm_bf16x3_kernel<64, 64>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st,
-                       Mhi, Mlo, In, Out, P, Nr, Nh);
+                       Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
   }
   return hipGetLastError();
 }

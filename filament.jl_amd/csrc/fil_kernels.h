@@ -16,6 +16,7 @@ struct RowArgs {
   R dz, dt;
   int apply_filter;         // field filter of the previous step still pending
   int nonlinearity, plasma, responses, losses_ereal;
+  int batch;                // rows are (r, member) pairs, r slowest: Rg index = row / batch
   PlasmaParams<R> pp;
   double* rho_end;          // [Nr] rho(r, t_end)
   double* rho_max;          // [Nr] max_t rho(r, t)
@@ -25,18 +26,27 @@ struct RowArgs {
 
 template <typename R>
 hipError_t launch_row(const RowArgs<R>& a, int Nr, int N, bool scan64, hipStream_t st);
+// The per-member kernels below address spectrum row i (i < Nr) at S row i*rs + ro
+// (rs = batch size, ro = member); Et, Rg and the outputs are indexed by i.
+// Upload / download of one pulse: row i of Et is row (i rs + ro) of S (rs = batch,
+// ro = member; 1, 0 for a single pulse).
 template <typename R>
-hipError_t launch_to_spectrum(const cx<R>* Et, cx<R>* S, const cx<R>* tw, int Nr, int N, hipStream_t st);
+hipError_t launch_to_spectrum(const cx<R>* Et, cx<R>* S, const cx<R>* tw, int Nr, int N, int rs, int ro, hipStream_t st);
 template <typename R>
 hipError_t launch_to_time(const cx<R>* S, cx<R>* Et, const cx<R>* tw, const R* Rg, const R* Tg, int apply,
-                          int Nr, int N, hipStream_t st);
+                          int Nr, int N, int rs, int ro, hipStream_t st);
+// Analyzer rows: output row i (< Nr) reads row (i rs + ro) of S with the r guard
+// Rg[i / gdiv]: every row of a batch with (1, 0, batch), one member with
+// (batch, member, 1).
 template <typename R>
 hipError_t launch_analyze_rows(cx<R>* S, const cx<R>* tw, const R* Rg, const R* Tg, int apply, R dt, double* Fr,
-                               float* It, float* Sw, double* Imax_row, int Nr, int N, hipStream_t st);
+                               R* It, R* Sw, double* Imax_row, int Nr, int N, int rs, int ro, int gdiv,
+                               hipStream_t st);
 // out[j] = sum_i in[i][j] * wt[i] over Nr rows, FP64, in a fixed order (row chunks, then the
 // chunks in order); part: scratch of COLSUM_MAXCHUNK * ncol doubles
 constexpr int COLSUM_MAXCHUNK = 256;
-hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* wt, double* part, double* out,
+template <typename T>
+hipError_t launch_colsum(const T* in, int Nr, int ncol, const double* wt, double* part, double* out,
                          hipStream_t st);
 template <typename R>
 hipError_t launch_rowsum_abs2(const cx<R>* S, int Nr, int Nh, double scale, double* out, hipStream_t st);
@@ -127,10 +137,12 @@ hipError_t launch_propagator(const double* KZ64, const double* KG64, double dz, 
                              hipStream_t st);
 // Out[m][w] = sum_n M[m][n] In[n][w]  (optionally Out *= P), complex In/Out [Nr][Nh], real M [Nr][Nr].
 template <typename R>
-hipError_t launch_dht_gemm(const R* M, const cx<R>* In, cx<R>* Out, const cx<R>* P, int Nr, int Nh,
+// Nh: complex columns of In/Out (the batch's members side by side); the propagator P
+// [Nr][Pw] is shared by the members: Out[m][w] *= P[m][w mod Pw].
+hipError_t launch_dht_gemm(const R* M, const cx<R>* In, cx<R>* Out, const cx<R>* P, int Nr, int Nh, int Pw,
                            hipStream_t st);
 // split-BF16 3-term variant: M given as hi/lo bf16 planes.
 hipError_t launch_dht_gemm_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In, cx<float>* Out,
-                                  const cx<float>* P, int Nr, int Nh, hipStream_t st);
+                                  const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st);
 
 }  // namespace fil

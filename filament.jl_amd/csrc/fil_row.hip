@@ -350,7 +350,7 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   const int tid = threadIdx.x;
   const int i = blockIdx.x;
   cx<R>* Srow = a.S + (size_t)i * NH;
-  const R g_r = a.Rg[i];
+  const R g_r = a.Rg[i / a.batch];
 
   cx<R> S0[SP];
 #pragma unroll
@@ -424,7 +424,8 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
 // (forward_transform_time!, Models.jl:162-165).
 // ---------------------------------------------------------------------------
 template <typename R, int N, int NTH>
-__global__ __launch_bounds__(NTH) void to_spectrum_kernel(const cx<R>* __restrict__ Et, cx<R>* S, const cx<R>* tw) {
+__global__ __launch_bounds__(NTH) void to_spectrum_kernel(const cx<R>* __restrict__ Et, cx<R>* S, const cx<R>* tw,
+                                                          int rs, int ro) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   const int i = blockIdx.x, tid = threadIdx.x;
@@ -432,17 +433,20 @@ __global__ __launch_bounds__(NTH) void to_spectrum_kernel(const cx<R>* __restric
   __syncthreads();
   fft_lds<R, N, NTH, +1>(Z, tw);
   const R inv = (R)1 / (R)N;
-  for (int w = tid; w < N / 2; w += NTH) S[(size_t)i * (N / 2) + w] = inv * Z[lds_index(w)];
+  for (int w = tid; w < N / 2; w += NTH) S[((size_t)i * rs + ro) * (N / 2) + w] = inv * Z[lds_index(w)];
 }
 
 // Download: spectrum -> time rows, with the pending field filter.
 template <typename R, int N, int NTH>
 __global__ __launch_bounds__(NTH) void to_time_kernel(const cx<R>* __restrict__ S, cx<R>* Et, const cx<R>* tw,
-                                                      const R* Rg, const R* Tg, int apply_filter) {
+                                                      const R* Rg, const R* Tg, int apply_filter, int rs, int ro) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   const int i = blockIdx.x, tid = threadIdx.x;
-  for (int w = tid; w < N / 2; w += NTH) { Z[lds_index(w)] = S[(size_t)i * (N / 2) + w]; Z[lds_index(N / 2 + w)] = mk((R)0, (R)0); }
+  for (int w = tid; w < N / 2; w += NTH) {
+    Z[lds_index(w)] = S[((size_t)i * rs + ro) * (N / 2) + w];
+    Z[lds_index(N / 2 + w)] = mk((R)0, (R)0);
+  }
   __syncthreads();
   fft_lds<R, N, NTH, -1>(Z, tw);
   const R g = apply_filter ? Rg[i] : (R)1;
@@ -461,24 +465,25 @@ __global__ __launch_bounds__(NTH) void to_time_kernel(const cx<R>* __restrict__ 
 // ---------------------------------------------------------------------------
 template <typename R, int N, int NTH>
 __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>* tw, const R* Rg, const R* Tg,
-                                                           int apply_filter, R dt, double* Fr, float* It,
-                                                           float* Sw, double* Imax_row) {
+                                                           int apply_filter, R dt, double* Fr, R* It,
+                                                           R* Sw, double* Imax_row, int rs, int ro, int gdiv) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   double* red = reinterpret_cast<double*>(Z + lds_size(N));
   const int i = blockIdx.x, tid = threadIdx.x;
   constexpr int NH = N / 2;
-  for (int w = tid; w < NH; w += NTH) { Z[lds_index(w)] = S[(size_t)i * NH + w]; Z[lds_index(NH + w)] = mk((R)0, (R)0); }
+  cx<R>* Srow = S + ((size_t)i * rs + ro) * NH;
+  for (int w = tid; w < NH; w += NTH) { Z[lds_index(w)] = Srow[w]; Z[lds_index(NH + w)] = mk((R)0, (R)0); }
   __syncthreads();
   fft_lds<R, N, NTH, -1>(Z, tw);
-  const R g = apply_filter ? Rg[i] : (R)1;
+  const R g = apply_filter ? Rg[i / gdiv] : (R)1;
   double fr = 0;
   for (int t = tid; t < N; t += NTH) {
     cx<R> v = Z[lds_index(t)];
     if (apply_filter) { v = (g * Tg[t]) * v; Z[lds_index(t)] = v; }
     double I = (double)norm2(v);
     fr += I;
-    It[(size_t)i * N + t] = (float)I;
+    It[(size_t)i * N + t] = (R)I;
   }
   // block sum of fr
   const int lane = tid & 63, wv = tid >> 6;
@@ -495,7 +500,7 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
     cx<R> v = inv * Z[lds_index(w)];
     double re = (double)v.x, im = (double)v.y;
     if (w >= 1 && w < NH) { re *= 0.5; im *= 0.5; }   // aspec2rspec
-    Sw[(size_t)i * (NH + 1) + w] = (float)(c * c * (re * re + im * im));
+    Sw[(size_t)i * (NH + 1) + w] = (R)(c * c * (re * re + im * im));
   }
   __syncthreads();
   // projection: keep bins 0..N/2-1 as the new state
@@ -505,7 +510,7 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < NH / NTH; m++) {
-    S[(size_t)i * NH + tid + m * NTH] = keep[m];
+    Srow[tid + m * NTH] = keep[m];
     Z[lds_index(tid + m * NTH)] = keep[m];
     Z[lds_index(NH + tid + m * NTH)] = mk((R)0, (R)0);
   }
@@ -544,7 +549,8 @@ template hipError_t launch_rowsum_abs2<float>(const cx<float>*, int, int, double
 template hipError_t launch_rowsum_abs2<double>(const cx<double>*, int, int, double, double*, hipStream_t);
 
 // Column sums: out[j] = sum_i in[i][j] * wt[i] in FP64, in a fixed order.
-__global__ void colsum_part_kernel(const float* __restrict__ in, int Nr, int ncol, const double* __restrict__ wt,
+template <typename T>
+__global__ void colsum_part_kernel(const T* __restrict__ in, int Nr, int ncol, const double* __restrict__ wt,
                                    int rows, double* __restrict__ part) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ncol) return;
@@ -703,13 +709,13 @@ hipError_t launch_row(const RowArgs<R>& a, int Nr, int N, bool scan64, hipStream
 }
 
 template <typename R>
-hipError_t launch_to_spectrum(const cx<R>* Et, cx<R>* S, const cx<R>* tw, int Nr, int N, hipStream_t st) {
+hipError_t launch_to_spectrum(const cx<R>* Et, cx<R>* S, const cx<R>* tw, int Nr, int N, int rs, int ro, hipStream_t st) {
   FIL_DISPATCH_N(N, {
     constexpr int NTH = threads_for<NN>();
     size_t sm = (size_t)lds_size(NN) * sizeof(cx<R>);
     auto k = to_spectrum_kernel<R, NN, NTH>;
     { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
-    hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, Et, S, tw);
+    hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, Et, S, tw, rs, ro);
     return hipGetLastError();
   });
   return hipSuccess;
@@ -717,13 +723,13 @@ hipError_t launch_to_spectrum(const cx<R>* Et, cx<R>* S, const cx<R>* tw, int Nr
 
 template <typename R>
 hipError_t launch_to_time(const cx<R>* S, cx<R>* Et, const cx<R>* tw, const R* Rg, const R* Tg, int apply,
-                          int Nr, int N, hipStream_t st) {
+                          int Nr, int N, int rs, int ro, hipStream_t st) {
   FIL_DISPATCH_N(N, {
     constexpr int NTH = threads_for<NN>();
     size_t sm = (size_t)lds_size(NN) * sizeof(cx<R>);
     auto k = to_time_kernel<R, NN, NTH>;
     { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
-    hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, S, Et, tw, Rg, Tg, apply);
+    hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, S, Et, tw, Rg, Tg, apply, rs, ro);
     return hipGetLastError();
   });
   return hipSuccess;
@@ -731,24 +737,26 @@ hipError_t launch_to_time(const cx<R>* S, cx<R>* Et, const cx<R>* tw, const R* R
 
 template <typename R>
 hipError_t launch_analyze_rows(cx<R>* S, const cx<R>* tw, const R* Rg, const R* Tg, int apply, R dt, double* Fr,
-                               float* It, float* Sw, double* Imax_row, int Nr, int N, hipStream_t st) {
+                               R* It, R* Sw, double* Imax_row, int Nr, int N, int rs, int ro, int gdiv,
+                               hipStream_t st) {
   FIL_DISPATCH_N(N, {
     constexpr int NTH = threads_for<NN>();
     size_t sm = (size_t)lds_size(NN) * sizeof(cx<R>) + 16 * sizeof(double);
     auto k = analyze_rows_kernel<R, NN, NTH>;
     { hipError_t e0 = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); if (e0 != hipSuccess) return e0; }
-    hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, S, tw, Rg, Tg, apply, dt, Fr, It, Sw, Imax_row);
+    hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, S, tw, Rg, Tg, apply, dt, Fr, It, Sw, Imax_row, rs, ro, gdiv);
     return hipGetLastError();
   });
   return hipSuccess;
 }
 
-hipError_t launch_colsum(const float* in, int Nr, int ncol, const double* wt, double* part, double* out,
+template <typename T>
+hipError_t launch_colsum(const T* in, int Nr, int ncol, const double* wt, double* part, double* out,
                          hipStream_t st) {
   int nchunk = (Nr + 63) / 64;
   if (nchunk > COLSUM_MAXCHUNK) nchunk = COLSUM_MAXCHUNK;
   const int rows = (Nr + nchunk - 1) / nchunk;
-  hipLaunchKernelGGL(colsum_part_kernel, dim3((ncol + 255) / 256, nchunk), dim3(256), 0, st, in, Nr, ncol, wt, rows,
+  hipLaunchKernelGGL(colsum_part_kernel<T>, dim3((ncol + 255) / 256, nchunk), dim3(256), 0, st, in, Nr, ncol, wt, rows,
                      part);
   hipLaunchKernelGGL(colsum_final_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, part, nchunk, ncol, out);
   return hipGetLastError();
@@ -785,11 +793,13 @@ hipError_t launch_corr_apply(cx<float>* S, const cx<float>* tw, const CorrGeom& 
 
 template hipError_t launch_row<float>(const RowArgs<float>&, int, int, bool, hipStream_t);
 template hipError_t launch_row<double>(const RowArgs<double>&, int, int, bool, hipStream_t);
-template hipError_t launch_to_spectrum<float>(const cx<float>*, cx<float>*, const cx<float>*, int, int, hipStream_t);
-template hipError_t launch_to_spectrum<double>(const cx<double>*, cx<double>*, const cx<double>*, int, int, hipStream_t);
-template hipError_t launch_to_time<float>(const cx<float>*, cx<float>*, const cx<float>*, const float*, const float*, int, int, int, hipStream_t);
-template hipError_t launch_to_time<double>(const cx<double>*, cx<double>*, const cx<double>*, const double*, const double*, int, int, int, hipStream_t);
-template hipError_t launch_analyze_rows<float>(cx<float>*, const cx<float>*, const float*, const float*, int, float, double*, float*, float*, double*, int, int, hipStream_t);
-template hipError_t launch_analyze_rows<double>(cx<double>*, const cx<double>*, const double*, const double*, int, double, double*, float*, float*, double*, int, int, hipStream_t);
+template hipError_t launch_to_spectrum<float>(const cx<float>*, cx<float>*, const cx<float>*, int, int, int, int, hipStream_t);
+template hipError_t launch_to_spectrum<double>(const cx<double>*, cx<double>*, const cx<double>*, int, int, int, int, hipStream_t);
+template hipError_t launch_to_time<float>(const cx<float>*, cx<float>*, const cx<float>*, const float*, const float*, int, int, int, int, int, hipStream_t);
+template hipError_t launch_to_time<double>(const cx<double>*, cx<double>*, const cx<double>*, const double*, const double*, int, int, int, int, int, hipStream_t);
+template hipError_t launch_analyze_rows<float>(cx<float>*, const cx<float>*, const float*, const float*, int, float, double*, float*, float*, double*, int, int, int, int, int, hipStream_t);
+template hipError_t launch_analyze_rows<double>(cx<double>*, const cx<double>*, const double*, const double*, int, double, double*, double*, double*, double*, int, int, int, int, int, hipStream_t);
+template hipError_t launch_colsum<float>(const float*, int, int, const double*, double*, double*, hipStream_t);
+template hipError_t launch_colsum<double>(const double*, int, int, const double*, double*, double*, hipStream_t);
 
 }  // namespace fil

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libfilament_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAXCOMP = 4
 GEOM_RT, GEOM_XYT = 0, 1
 FP32, FP64 = 4, 8
@@ -27,7 +27,7 @@ class FilDesc(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "abi_version", "geometry", "Nr", "Nt", "precision", "form", "nonlinearity", "plasma",
         "responses", "losses_ereal", "plasma_ereal", "plasma_kdep", "ncomp", "keep_plasma", "device",
-        "Nx", "Ny", "guard3d_bug")]
+        "Nx", "Ny", "guard3d_bug", "batch")]
 
 
 class FilTables(ctypes.Structure):
@@ -73,9 +73,10 @@ class FilCorrDesc(ctypes.Structure):
 CORR_NPARAM = 20258
 
 EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_set_tables_xyt", "fil_upload_field",
-           "fil_download_field", "fil_zstep", "fil_analyze", "fil_analyze_xyt", "fil_download_plasma", "fil_synchronize", "fil_destroy",
+           "fil_download_field", "fil_upload_member", "fil_download_member", "fil_zstep", "fil_analyze",
+           "fil_analyze_member", "fil_analyze_xyt", "fil_download_plasma", "fil_synchronize", "fil_destroy",
            "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_state_copy",
-           "fil_fluence",
+           "fil_fluence", "fil_group_create", "fil_group_zstep", "fil_group_stats", "fil_group_destroy",
            "fil_corrector_init", "fil_corrector_params", "fil_corrector_scale", "fil_corrector_features",
            "fil_corrector_store_pair", "fil_corrector_train", "fil_corrector_eval", "fil_corrector_forward",
            "fil_corrector_apply")
@@ -102,6 +103,14 @@ def load():
     L.fil_analyze_xyt.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(FilDiagXYT), _dp, _dp, _dp, _dp]
     L.fil_upload_field.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
     L.fil_download_field.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64]
+    L.fil_upload_member.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64]
+    L.fil_download_member.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int64]
+    L.fil_analyze_member.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(FilDiag), _dp,
+                                     _dp, _dp, _dp]
+    L.fil_group_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.fil_group_zstep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+    L.fil_group_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.fil_group_destroy.argtypes = [ctypes.c_void_p]
     L.fil_zstep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
     L.fil_analyze.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(FilDiag), _dp, _dp, _dp, _dp]
     L.fil_download_plasma.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int64]

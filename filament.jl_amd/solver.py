@@ -228,8 +228,10 @@ class RTModel(_Model):
         self.E0 = ph.rsig2asig(E0)
         self.T, self.RJ, self.JV = ph.dht_plan(g)
 
-        d = self._desc(L.GEOM_RT, precision, form, keep_plasma, device)
-        d.Nr = g.Nr
+        self._start(self._desc(L.GEOM_RT, precision, form, keep_plasma, device))
+
+    def _start(self, d):
+        d.Nr = self.grid.Nr
         self._create(d)
         self._set_tables()
         self.upload(self.E0)
@@ -257,6 +259,78 @@ class RTModel(_Model):
             return out
         L.check(self._lib.fil_analyze(self.h, float(z), ctypes.byref(d), None, None, None, None))
         return d.as_dict()
+
+
+class RTEnsemble(RTModel):
+    """B pulses on one RT grid advanced together by one handle (fil_desc.batch
+    = B): the tables (grid, medium, guards, responses, plasma) are shared, each
+    member has its own field and plasma.  Equivalent to B runs of Filament.run
+    (src/Filament.jl:69-106) with the same input apart from the initial
+    condition, stepped with a common dz.
+
+    ``members``: parameter dicts that agree on everything but
+    ``initial_condition`` (e.g. from workloads.rt_ensemble); the first one
+    provides the shared tables."""
+
+    def __init__(self, members, precision=L.FP32, form=0, device=0, basedir="."):
+        self.B = len(members)
+        if self.B < 1:
+            raise ValueError("an ensemble needs at least one member")
+        for q in members[1:]:
+            for k in ("Nr", "Nt", "rmax", "tmin", "tmax", "lam0", "n2", "ru", "zu", "tu", "Iu", "rhou"):
+                if q[k] != members[0][k]:
+                    raise ValueError(f"ensemble members differ in {k}; group them by grid")
+        self.members = members
+        super().__init__(members[0], precision=precision, form=form, device=device, keep_plasma=False,
+                         basedir=basedir)
+
+    def _start(self, d):
+        d.Nr = self.grid.Nr
+        d.batch = self.B
+        self._create(d)
+        self._set_tables()
+        g, u = self.grid, self.unit
+        self.E0s = [ph.rsig2asig(np.asarray(q["initial_condition"](g.r, g.t, u.r, u.t, u.I), dtype=float))
+                    for q in self.members]
+        for b, E in enumerate(self.E0s):
+            self.upload(E, b)
+
+    def upload(self, E, member=0):
+        E = np.asfortranarray(self._to_rows(np.asarray(E, dtype=np.complex128)))
+        L.check(self._lib.fil_upload_member(self.h, int(member), L.ptr(E.ravel(order="F").view(np.float64)),
+                                            self.nrows))
+
+    def field(self, member=0):
+        n, Nt = self.nrows, self.grid.Nt
+        out = np.zeros(n * Nt, dtype=np.complex128)
+        L.check(self._lib.fil_download_member(self.h, int(member), L.ptr(out.view(np.float64)), n))
+        return out.reshape(Nt, n).T.copy()
+
+    def plasma_arrays(self):
+        raise NotImplementedError("plasma arrays are kept for single-pulse models")
+
+    def fluence(self, slot=-1):
+        """[B, Nr] fluence of every member (see _Model.fluence)."""
+        Fr = np.zeros(self.B * self.nrows)
+        L.check(self._lib.fil_fluence(self.h, int(slot), L.ptr(Fr)))
+        return Fr.reshape(self.B, self.nrows)
+
+    def analyze(self, z=0.0, arrays=False, member=0):
+        g = self.grid
+        d = L.FilDiag()
+        if arrays:
+            Fr, Ft, re, S = np.zeros(g.Nr), np.zeros(g.Nt), np.zeros(g.Nr), np.zeros(g.Nt // 2 + 1)
+            L.check(self._lib.fil_analyze_member(self.h, int(member), float(z), ctypes.byref(d), L.ptr(Fr),
+                                                 L.ptr(Ft), L.ptr(re), L.ptr(S)))
+            out = d.as_dict()
+            out.update(Fr=Fr, Ft=Ft, rho_end=re, S=S)
+            return out
+        L.check(self._lib.fil_analyze_member(self.h, int(member), float(z), ctypes.byref(d), None, None, None,
+                                             None))
+        return d.as_dict()
+
+    def analyze_all(self, z=0.0):
+        return [self.analyze(z, member=b) for b in range(self.B)]
 
 
 class XYTModel(_Model):
@@ -323,3 +397,40 @@ class XYTModel(_Model):
             return out
         L.check(self._lib.fil_analyze_xyt(self.h, float(z), ctypes.byref(d), None, None, None, None))
         return d.as_dict()
+
+
+class StepGroup:
+    """Models advanced together, one graph launch per z-step (fil_group_*):
+    e.g. the per-grid RTEnsembles of an ensemble of pulses of uneven size.
+    Close the group before its models."""
+
+    def __init__(self, models):
+        self._lib = L.load()
+        self.models = list(models)
+        arr = (ctypes.c_void_p * len(self.models))(*[m.h.value for m in self.models])
+        g = ctypes.c_void_p()
+        L.check(self._lib.fil_group_create(arr, len(self.models), ctypes.byref(g)))
+        self.g = g
+
+    def zstep(self, z, dz):
+        L.check(self._lib.fil_group_zstep(self.g, ctypes.c_double(z), ctypes.c_double(dz)))
+
+    def synchronize(self):
+        for m in self.models:
+            m.synchronize()
+
+    def stats(self):
+        c, r = ctypes.c_int64(), ctypes.c_int64()
+        L.check(self._lib.fil_group_stats(self.g, ctypes.byref(c), ctypes.byref(r)))
+        return dict(captures=c.value, replays=r.value)
+
+    def close(self):
+        if getattr(self, "g", None):
+            self._lib.fil_group_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -8,6 +8,9 @@ RT 256x2048  : air, 1 atm, 800 nm; Nr=256 on [0,10 mm] (Hankel nodes),
                free current, MPI losses; O2 multiphoton ionisation K=8,
                sigma_8 = 2.9e-99 s^-1 (cm^2/W)^8, neutrals 5.4e24 m^-3; dz=0.5 mm.
 RT 1024x4096 : same physics at Nr=1024, Nt=4096, 5 Pcr.
+RT ensemble  : 64 pulses of the same physics, Nr in {256, 512, 1024}, Nt in
+               {1024, 2048, 4096}, P ~ U[1, 10] Pcr, a0 ~ U[0.5, 2] mm,
+               tau0 ~ U[25, 100] fs (seeded).
 XYT 512^3    : air, 800 nm; x, y on [-10, 10] mm (512 points each), t on
                [-250, 250] fs (512); elliptical Gaussian 3 x 2 mm, 50 fs, 30 Pcr,
                5% amplitude noise (white noise smoothed by a Gaussian of 0.1 mm
@@ -155,3 +158,33 @@ def rt_1024x4096():
     p = rt_air(1024, 4096, 5.0)
     p["nsteps"] = 2000
     return p
+
+
+def rt_ensemble(n=64, seed=0, shapes_r=(256, 512, 1024), shapes_t=(1024, 2048, 4096)):
+    """The batched ensemble: n independent pulses of uneven size.  Nr is drawn
+    from shapes_r and Nt from shapes_t, power uniform on [1, 10] Pcr, a0 on
+    [0.5, 2] mm, tau0 on [25, 100] fs (seeded).  All pulses share the air
+    physics of rt_air, r on [0, 10] mm and t on [-0.5, 0.5] ps, so pulses of
+    equal (Nr, Nt) share a grid and batch together.  Returns the parameter
+    dicts in draw order; each carries its draw under 'draw'."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        Nr = int(rng.choice(shapes_r))
+        Nt = int(rng.choice(shapes_t))
+        P = float(rng.uniform(1.0, 10.0))
+        a0 = float(rng.uniform(0.5e-3, 2e-3))
+        tau0 = float(rng.uniform(25e-15, 100e-15))
+        p = rt_air(Nr, Nt, P, a0=a0, tau0=tau0, seed=seed * 1000 + k)
+        p["draw"] = dict(index=k, Nr=Nr, Nt=Nt, power_pcr=P, a0=a0, tau0=tau0)
+        out.append(p)
+    return out
+
+
+def group_by_grid(members):
+    """Members grouped by (Nr, Nt), in first-appearance order:
+    [((Nr, Nt), [indices])]."""
+    groups = {}
+    for k, p in enumerate(members):
+        groups.setdefault((p["Nr"], p["Nt"]), []).append(k)
+    return list(groups.items())

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define FIL_ABI_VERSION 2
+#define FIL_ABI_VERSION 3
 #define FIL_MAXCOMP 4
 
 typedef struct fil_handle fil_handle;
@@ -90,6 +90,12 @@ typedef struct fil_desc {
    * t (and w) factor with the y index (Guards.jl:314); 0 = the intended
    * filter X(x) Y(y) T(t) and KX(kx) KY(ky) W(w).  Needs Ny <= Nt. */
   int32_t guard3d_bug;
+  /* RT only: number of pulses B (0 or 1 = one) propagated together on one
+   * grid with one dz.  Replaces B independent runs of Filament.run
+   * (Filament.jl:69-106, main_loop :111-166) on the same Grid/Medium; each pulse has its own
+   * field and plasma, the tables are shared.  Batched handles take the
+   * *_member entry points and no corrector. */
+  int32_t batch;
 } fil_desc;
 
 /* Host-precomputed tables, FP64 (the library rounds them to the working
@@ -172,15 +178,37 @@ int fil_set_tables_xyt(fil_handle* h, const fil_tables* t, const fil_tables_xyt*
  * reference holds in field.E after the last zstep (guarded, time domain). */
 int fil_upload_field(fil_handle* h, const double* E, int64_t ld);
 int fil_download_field(fil_handle* h, double* E, int64_t ld);
+/* The same for pulse `member` of a batched handle (0 <= member < batch).  Every
+ * member is uploaded before the first zstep. */
+int fil_upload_member(fil_handle* h, int member, const double* E, int64_t ld);
+int fil_download_member(fil_handle* h, int member, double* E, int64_t ld);
 
 /* Models.zstep (Models.jl:75-121).  Asynchronous. */
 int fil_zstep(fil_handle* h, double z, double dz);
+
+/* Several handles (e.g. the per-grid batches of an ensemble of pulses of
+ * uneven size) advanced by one z-step each with ONE graph launch: the first
+ * call for a (dz, pending-filter pattern) records the handles' zstep kernels
+ * as a HIP graph with one branch per handle; later calls replay it.  Replaces
+ * the loop over independent runs' Models.zstep calls (Models.jl:75-121,
+ * Filament.jl:136).  Ordered after each handle's earlier work and before its
+ * later work; asynchronous.  Destroy the group before its handles. */
+#define FIL_GROUP_MAX 64
+typedef struct fil_group fil_group;
+int fil_group_create(fil_handle* const* hs, int n, fil_group** g);
+int fil_group_zstep(fil_group* g, double z, double dz);
+int fil_group_stats(fil_group* g, int64_t* captures, int64_t* replays);
+int fil_group_destroy(fil_group* g);
 
 /* FieldAnalyzers.analyze! RT (FieldAnalyzers.jl:166-215), including its side
  * effect of projecting the field onto its analytic part.  Fr [Nr], Ft [Nt],
  * rho_end [Nr] and S [Nt/2+1] may be NULL.  Blocks. */
 int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft,
                 double* rho_end, double* S);
+/* fil_analyze for pulse `member` of a batched handle (the projection applies
+ * to every member). */
+int fil_analyze_member(fil_handle* h, int member, double z, fil_diag* d, double* Fr, double* Ft,
+                       double* rho_end, double* S);
 
 /* FieldAnalyzerXYT (FieldAnalyzers.jl:262-353): Fxy [Nx x Ny] (x fastest),
  * Ft [Nt], rho_end [Nx x Ny], S [Nt/2+1] may be NULL.  ax, ay: radii of Fxy
@@ -213,7 +241,7 @@ int fil_state_copy(fil_handle* dst, const fil_handle* src);
 /* Fluence profile Fr(r) = dt * sum_t |E(r,t)|^2 (FieldAnalyzers.jl:171) of the
  * current state (slot < 0) or of a saved slot, computed on the device from
  * the spectrum by Parseval's identity, before the pending field filter.
- * Fr has Nr entries.  Blocks. */
+ * Fr has Nr entries (batch x Nr for a batched handle, member-major).  Blocks. */
 int fil_fluence(fil_handle* h, int slot, double* Fr);
 int fil_destroy(fil_handle* h);
 

@@ -32,8 +32,8 @@ def test_library_exports_every_declared_symbol(pkg):
 
 
 def test_struct_layouts_match_header(pkg):
-    # fil_desc: 18 int32 fields; fil_diag: 9 doubles; fil_diag_xyt: 8 doubles
-    assert ctypes.sizeof(pkg.lib.FilDesc) == 18 * 4
+    # fil_desc: 19 int32 fields; fil_diag: 9 doubles; fil_diag_xyt: 8 doubles
+    assert ctypes.sizeof(pkg.lib.FilDesc) == 19 * 4
     assert ctypes.sizeof(pkg.lib.FilDiag) == 9 * 8
     assert ctypes.sizeof(pkg.lib.FilDiagXYT) == 8 * 8
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
@@ -75,3 +75,13 @@ def test_errors_are_reported_without_a_device(pkg):
     assert L.fil_corrector_init(None, ctypes.byref(c), 0) == 1
     assert L.fil_corrector_apply(None) == 1 and b"null" in L.fil_last_error()
     assert L.fil_state_save(None, 0) == 1
+    assert L.fil_upload_member(None, 0, None, 0) == 1
+    assert L.fil_analyze_member(None, 0, 0.0, None, None, None, None, None) == 1
+    g = ctypes.c_void_p()
+    assert L.fil_group_create(None, 0, ctypes.byref(g)) == 1 and g.value is None
+    assert L.fil_group_zstep(None, 0.0, 1e-3) == 1
+    assert L.fil_group_destroy(None) == 0
+    # a batch is refused for the XYT geometry before any device call
+    d = pkg.lib.FilDesc()
+    d.abi_version, d.geometry, d.Nx, d.Ny, d.Nt, d.precision, d.ncomp, d.batch = pkg.lib.ABI_VERSION, 1, 16, 16, 256, 4, 1, 2
+    assert L.fil_create(ctypes.byref(d), ctypes.byref(h)) == 1 and b"batch" in L.fil_last_error()

@@ -110,3 +110,23 @@ def test_zstep_without_analyze_tracks_oracle(pkg, orc):
     err = rel_l2(m.field(), o.field())
     print(f"zstep-only rel_l2={err:.3e}")
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("precision,tol", [(4, 1e-4), (8, 1e-10)])
+def test_analyzer_arrays_match_oracle(pkg, orc, precision, tol):
+    """FieldAnalyzers.analyze! RT arrays (FieldAnalyzers.jl:166-215): fluence
+    Fr(r), power Ft(t) (2 pi r dr weighted column sums), the end-of-pulse
+    density rho_end(r) and the spectrum S(w), against the oracle's, after
+    steps with the analyzer's projection in between."""
+    p = small_rt(pkg, power=6.0)
+    o = oracle_for(orc, p)
+    dz = 5e-3
+    o.run(5, dz)
+    od = o.analyze(5 * dz)
+    m = run_gpu(pkg, p, 5, dz, precision, KZ=o.KZ())
+    d = m.analyze(5 * dz, arrays=True)
+    errs = {k: rel_l2(d[k], od[k]) for k in ("Fr", "Ft", "rho_end", "S")}
+    print(f"analyzer arrays precision={precision}: " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
+    assert od["rho_end"].max() > 0
+    for k, v in errs.items():
+        assert v < tol, k
