@@ -244,6 +244,27 @@ def main():
                     per_launch=f"{row_bytes:.4g} B (2*8*Nr*Nt/2)",
                     avg_launch_ms=kt["row"])
 
+    # whole-step roofline: T_roof = F_DHT / P_MFMA + max(B_step / BW_HBM, F_FFT / P_FP32) with
+    # F_DHT = 2 transforms x 4 Nr^2 Nh flop, B_step = 4 x 8 Nr Nh bytes (the c64 half-spectrum read
+    # and written once by the row phase and once by the Hankel phase), F_FFT = 14 x 5 Nt log2(Nt) Nr
+    # (the fewest t-transforms three responses with RK3 need); the split-BF16 Hankel products run at
+    # one third of the BF16 peak.  A coarse step of the chosen form is one z-step's work.
+    f_dht = 2 * 4.0 * Nr * Nr * Nh
+    b_step = 4 * 8.0 * Nr * Nh
+    f_fft = 14 * 5.0 * Nt * np.log2(Nt) * Nr
+    t_rest = max(b_step / (PEAK["hbm_gbs"] * 1e9), f_fft / (PEAK["fp32_mfma_tflops"] * 1e12))
+    t_plain = f_dht / (PEAK["fp32_mfma_tflops"] * 1e12) + t_rest
+    t_bf16 = f_dht / (PEAK["bf16_mfma_tflops"] * 1e12 / 3) + t_rest
+    step_roof = {
+        "plain_fp32": {"t_roof_us": t_plain * 1e6, "t_measured_us": forms["plain_fp32"]["ms_per_step"] * 1e3,
+                       "frac": t_plain * 1e6 / (forms["plain_fp32"]["ms_per_step"] * 1e3)},
+    }
+    if "chosen" in forms:
+        tc = forms["chosen"]["ms_per_fine_step_equivalent"] * args.factor * 1e3
+        step_roof["chosen"] = {"t_roof_us_per_coarse_step": t_bf16 * 1e6, "t_measured_us_per_coarse_step": tc,
+                               "frac": t_bf16 * 1e6 / tc,
+                               "note": "measured time includes the corrector and the amortised verification"}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pkg, p, args.cpu_steps, dz)
@@ -266,6 +287,7 @@ def main():
         "forms": forms,
         "speedup_chosen_vs_plain": forms["plain_fp32"]["ms_per_step"] / forms["chosen"]["ms_per_fine_step_equivalent"],
         "roofline": roof,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

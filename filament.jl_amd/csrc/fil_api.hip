@@ -86,10 +86,10 @@ struct fil_handle {
   bool pending_filter = false, plasma_valid = false;
   double P_dz = NAN;
   double dt = 0;
-  double scale[3] = {1, 1, 1};
+  double scale[4] = {1, 1, 1, 1};
   std::vector<double> r, rdr, t;
   // device
-  DevBuf S, X, Et, M1, M2, M1h, M1l, M2h, M2l, KZ64, KG64, P, tw, Tg, Rg, coef;
+  DevBuf S, X, Et, M1, M2, M1h, M1l, M2h, M2l, KZ64, KG64, P, tw, Tg, Rg, coef, hr;
   DevBuf tab_ly[FIL_MAXCOMP], tab_sl[FIL_MAXCOMP];
   DevBuf rho_end, rho_max, rho_full, kd_full;
   DevBuf aFr, aIt, aSw, aImax, aFt, aS, ardr, acol;
@@ -120,6 +120,7 @@ struct fil_handle {
     double y_scale = 0;
     int64_t adam_t = 0;
     DevBuf w, wbf, m, v, grad;        // flat parameters, BF16 copy, Adam moments, gradient
+    DevBuf wfm;                       // FP32 forward layout of the BF16 weights
     DevBuf X, Y;                      // stored pairs [cap][4][HW], [cap][2][HW]
     DevBuf Frow, feat, carrier;       // feature scratch, features [4][HW], exp(i w0 t_j) [Nt]
     DevBuf act[3], out, gA, gB, gout; // training activations and gradients [bmax][..][HW]
@@ -246,7 +247,8 @@ int zstep_impl(fil_handle* h, double dz) {
   a.Tg = h->Tg.as<R>();
   a.Rg = h->Rg.as<R>();
   a.coef = h->coef.as<cx<R>>();
-  for (int q = 0; q < 3; q++) a.sc[q] = (R)h->scale[q];
+  a.hr = h->hr.p ? h->hr.as<cx<R>>() : nullptr;
+  for (int q = 0; q < 4; q++) a.sc[q] = (R)h->scale[q];
   a.dz = (R)dz;
   a.dt = (R)h->dt;
   a.apply_filter = h->pending_filter ? 1 : 0;
@@ -275,11 +277,13 @@ int zstep_impl(fil_handle* h, double dz) {
   }
   if constexpr (sizeof(R) == 4) {
     if (h->bf16x3) {
-      HIPCHK(launch_dht_gemm_bf16x3(h->M1h.as<uint16_t>(), h->M1l.as<uint16_t>(), h->S.as<cx<float>>(),
-                                    h->X.as<cx<float>>(), h->P.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh, h->st));
+      // X travels as its four BF16 planes (same bytes as FP32 complex)
+      HIPCHK(launch_dht_gemm_bf16x3_to_planes(h->M1h.as<uint16_t>(), h->M1l.as<uint16_t>(), h->S.as<cx<float>>(),
+                                              h->X.as<uint16_t>(), h->P.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh,
+                                              h->st));
       if (int rc = record(h, 2)) return rc;
-      HIPCHK(launch_dht_gemm_bf16x3(h->M2h.as<uint16_t>(), h->M2l.as<uint16_t>(), h->X.as<cx<float>>(),
-                                    h->S.as<cx<float>>(), nullptr, h->Nrp, h->Nh * h->B, h->Nh, h->st));
+      HIPCHK(launch_dht_gemm_bf16x3_from_planes(h->M2h.as<uint16_t>(), h->M2l.as<uint16_t>(), h->X.as<uint16_t>(),
+                                                h->S.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh, h->st));
       if (int rc = record(h, 3)) return rc;
       h->pending_filter = true;
       h->plasma_valid = h->d.plasma != 0;
@@ -511,12 +515,12 @@ int corr_bias_off(int l) { return corr_woff(l) + corr_cout(l) * corr_cin(l) * 9;
 int corr_forward(fil_handle* h, const float* in, const int* idx, int B) {
   auto& c = h->corr;
   const float* w = c.w.as<float>();
-  const uint16_t* wb = c.wbf.as<uint16_t>();
+  const float* wf = c.wfm.as<float>();
   const float* src = in;
   const int* sidx = idx;
   for (int l = 0; l < CORR_LAYERS; l++) {
     float* dst = (l == CORR_LAYERS - 1) ? c.out.as<float>() : c.act[l].as<float>();
-    HIPCHK(launch_conv_fwd(l, src, sidx, wb + corr_woff(l), w + corr_bias_off(l), dst, B, c.g.H, c.g.W,
+    HIPCHK(launch_conv_fwd(l, src, sidx, wf + corr_wfm_off(l), w + corr_bias_off(l), dst, B, c.g.H, c.g.W,
                            l < CORR_LAYERS - 1 ? 1 : 0, h->st));
     src = dst;
     sidx = nullptr;
@@ -596,8 +600,8 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (d->plasma && (d->ncomp < 1 || d->ncomp > FIL_MAXCOMP)) return fail(FIL_EINVAL, "ncomp must be in [1, 4]");
   if ((d->form & FIL_FORM_DHT_BF16X3) && d->precision != FIL_FP32)
     return fail(FIL_EINVAL, "the split-BF16 Hankel form needs FIL_FP32");
-  if (d->nonlinearity && (d->responses & (FIL_RESP_RAMAN_THG | FIL_RESP_RAMAN_NOTHG)))
-    return fail(FIL_EINVAL, "the Raman response is not implemented");
+  if ((d->responses & FIL_RESP_RAMAN_THG) && (d->responses & FIL_RESP_RAMAN_NOTHG))
+    return fail(FIL_EINVAL, "choose one Raman response");
   if ((d->responses & FIL_RESP_CUBIC_THG) && (d->responses & FIL_RESP_CUBIC_NOTHG))
     return fail(FIL_EINVAL, "choose one cubic response");
   int ndev = 0;
@@ -649,9 +653,9 @@ int fil_create(const fil_desc* d, fil_handle** out) {
 namespace {
 
 // per-pass twiddle table of an N-point LDS FFT in the layout fft_lds reads (fil_fft.h)
-int build_twiddles(DevBuf& buf, int N, bool f64) {
-  HostArr tw(std::max(1, fft_tw_size(N)), f64, true);
-  size_t o = 0;
+// Per-pass twiddles of an N-point transform (fil_fft.h layout) appended to
+// tw starting at entry o; returns the next entry.
+size_t append_twiddles(HostArr& tw, size_t o, int N) {
   for (int s = 1; N / s > 1; s *= fft_radix(N, s)) {
     const int rad = fft_radix(N, s);
     if (s < N / rad) {
@@ -664,7 +668,24 @@ int build_twiddles(DevBuf& buf, int N, bool f64) {
         }
     }
   }
+  return o;
+}
+
+// ext: also the half-length table and the real-signal post-processing factors
+// the row kernel uses (fft_tw_ext_size layout).
+int build_twiddles(DevBuf& buf, int N, bool f64, bool ext) {
+  const int n = ext ? fft_tw_ext_size(N) : fft_tw_size(N);
+  HostArr tw(std::max(1, n), f64, true);
+  size_t o = append_twiddles(tw, 0, N);
   if ((int)o != fft_tw_size(N)) return fail(FIL_EINVAL, "internal: twiddle table size");
+  if (ext) {
+    o = append_twiddles(tw, o, N / 2);
+    if ((int)o != fft_tw_post_offset(N)) return fail(FIL_EINVAL, "internal: twiddle table size");
+    for (int w = 0; w < N / 2; w++) {
+      const double a = 2.0 * kPi * (double)w / N;
+      tw.setc(o++, std::cos(a), std::sin(a));
+    }
+  }
   HIPCHK(upload(buf, tw.b.data(), tw.b.size()));
   return 0;
 }
@@ -718,8 +739,8 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
     xp.zu = xt->zu;
     xp.norm = 1.0 / ((double)h->Nx * h->Ny);
     h->P_dz = NAN;
-    if (int rc = build_twiddles(h->twx, h->Nx, f64)) return rc;
-    if (int rc = build_twiddles(h->twy, h->Ny, f64)) return rc;
+    if (int rc = build_twiddles(h->twx, h->Nx, f64, false)) return rc;
+    if (int rc = build_twiddles(h->twy, h->Ny, f64, false)) return rc;
   }
 
   if (!h->xyt) {
@@ -778,7 +799,7 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
   // multiplies by T at the y index instead (Guards.jl:314), which the tables
   // express as Rg = X Y T(iy) and Tg = 1.
   {
-    if (int rc = build_twiddles(h->tw, Nt, f64)) return rc;
+    if (int rc = build_twiddles(h->tw, Nt, f64, true)) return rc;
     HostArr tg(Nt, f64, false), rg(Nr, f64, false);
     for (int j = 0; j < Nt; j++) tg.set(j, bug ? 1.0 : t->Tguard[j]);
     if (!h->xyt) {
@@ -793,22 +814,32 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
   }
   // response coefficients i QZ Rnl d(w) / (2 Nt), d(0) = 1, d(w>0) = 2
   {
-    HostArr cf((size_t)3 * Nh, f64, true);
-    const double* R[3] = {nullptr, nullptr, nullptr};
+    HostArr cf((size_t)4 * Nh, f64, true);
+    const double* R[4] = {nullptr, nullptr, nullptr, nullptr};
+    const bool raman = h->d.nonlinearity && (h->d.responses & (FIL_RESP_RAMAN_THG | FIL_RESP_RAMAN_NOTHG));
     if (h->d.nonlinearity) {
       if (h->d.responses & (FIL_RESP_CUBIC_THG | FIL_RESP_CUBIC_NOTHG)) R[0] = t->Rnl_cubic;
       if (h->d.responses & FIL_RESP_FREE) R[1] = t->Rnl_free;
       if (h->d.responses & FIL_RESP_LOSSES) R[2] = t->Rnl_losses;
+      if (raman) R[3] = t->Rnl_raman;
       if (!t->QZ) return fail(FIL_EINVAL, "missing QZ table");
       for (int q = 0; q < 3; q++)
         if ((q == 0 && (h->d.responses & (FIL_RESP_CUBIC_THG | FIL_RESP_CUBIC_NOTHG)) && !R[0]) ||
             (q == 1 && (h->d.responses & FIL_RESP_FREE) && !R[1]) ||
             (q == 2 && (h->d.responses & FIL_RESP_LOSSES) && !R[2]))
           return fail(FIL_EINVAL, "missing Rnl table");
+      if (raman && (!R[3] || !t->Hraman)) return fail(FIL_EINVAL, "missing Raman tables (Rnl_raman, Hraman)");
+    }
+    // Raman response spectrum (raman.jl:189-202), divided by Nt: the
+    // convolution (raman.jl:68-111) is FFT-(Hraman .* (1/Nt) FFT+(x))
+    if (raman) {
+      HostArr hv((size_t)Nt, f64, true);
+      for (int j = 0; j < Nt; j++) hv.setc(j, t->Hraman[2 * j] / Nt, t->Hraman[2 * j + 1] / Nt);
+      HIPCHK(upload(h->hr, hv.b.data(), hv.b.size()));
     }
     // Each response's real signal is scaled by s_q = median_w |i QZ Rnl| before
     // it shares a complex FFT with another one; the table holds coef / s_q.
-    for (int q = 0; q < 3; q++) {
+    for (int q = 0; q < 4; q++) {
       std::vector<std::complex<double>> c(Nh, std::complex<double>(0, 0));
       std::vector<double> mag;
       if (R[q]) {
@@ -1351,6 +1382,7 @@ int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
   c.bmax = cd->batch_max;
   const size_t HW = (size_t)H * W, NP = FIL_CORR_NPARAM, B = c.bmax;
   bool ok = c.w.alloc(NP * 4) == hipSuccess && c.wbf.alloc(NP * 2) == hipSuccess && c.m.alloc(NP * 4) == hipSuccess &&
+            c.wfm.alloc((size_t)corr_wfm_size() * 4) == hipSuccess &&
             c.v.alloc(NP * 4) == hipSuccess && c.grad.alloc(NP * 4) == hipSuccess &&
             c.X.alloc(std::max<size_t>(1, (size_t)c.cap) * 4 * HW * 4) == hipSuccess &&
             c.Y.alloc(std::max<size_t>(1, (size_t)c.cap) * 2 * HW * 4) == hipSuccess &&
@@ -1383,6 +1415,7 @@ int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
   HIPCHK(hipMemsetAsync(c.m.p, 0, NP * 4, h->st));
   HIPCHK(hipMemsetAsync(c.v.p, 0, NP * 4, h->st));
   HIPCHK(launch_round_bf16(c.w.as<float>(), c.wbf.as<uint16_t>(), (int)NP, h->st));
+  HIPCHK(launch_conv_weight_layout(c.wbf.as<uint16_t>(), c.wfm.as<float>(), h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   c.on = true;
   return FIL_OK;
@@ -1401,6 +1434,7 @@ int fil_corrector_params(fil_handle* h, float* w, int set) {
     HIPCHK(hipMemsetAsync(c.m.p, 0, NP * 4, h->st));
     HIPCHK(hipMemsetAsync(c.v.p, 0, NP * 4, h->st));
     HIPCHK(launch_round_bf16(c.w.as<float>(), c.wbf.as<uint16_t>(), (int)NP, h->st));
+    HIPCHK(launch_conv_weight_layout(c.wbf.as<uint16_t>(), c.wfm.as<float>(), h->st));
     c.adam_t = 0;
   } else {
     HIPCHK(hipMemcpyAsync(w, c.w.p, NP * 4, hipMemcpyDeviceToHost, h->st));
@@ -1495,6 +1529,7 @@ int fil_corrector_train(fil_handle* h, int iters, int batch, double lr, uint64_t
     const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)c.adam_t)));
     HIPCHK(launch_adam(c.w.as<float>(), c.grad.as<float>(), c.m.as<float>(), c.v.as<float>(), c.wbf.as<uint16_t>(),
                        FIL_CORR_NPARAM, (float)lr, b1, b2, eps, c1, c2, h->st));
+    HIPCHK(launch_conv_weight_layout(c.wbf.as<uint16_t>(), c.wfm.as<float>(), h->st));
   }
   if (int rc = corr_read_loss(h, nblk, batch, loss)) return rc;
   return FIL_OK;

@@ -5,9 +5,9 @@
 // Layer l maps [B][cin][H][W] -> [B][cout][H][W] with zero padding:
 //   out[co][y][x] = b[co] + sum_ci sum_{dy,dx in -1..1} w[co][ci][(dy+1)*3+(dx+1)] in[ci][y+dy][x+dx]
 // Weights are read as BF16 (the rounded copy of the FP32 master parameters)
-// and every product is accumulated in FP32.  One thread owns one pixel and all
-// its output channels; the layer's weights sit in LDS in the order the inner
-// loop reads them, so each LDS read feeds several FMAs.
+// and every product is accumulated in FP32.  The forward pass runs on the
+// matrix cores (FP32 MFMA); the backward passes own one pixel (data gradient)
+// or one weight (weight gradient) per thread on the vector ALUs.
 #include <algorithm>
 
 #include "fil_internal.h"
@@ -25,62 +25,103 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 constexpr int CT = 128;   // threads per block of the convolution kernels
 constexpr int CG = 8;     // channels per thread in the forward / data-gradient kernels
 
-// Forward: a block owns a TY x TX tile of output pixels of one sample and G
-// output channels (blockIdx.z); the input tile with its one-pixel halo is
-// staged in LDS eight input channels at a time, so each input value is read
-// from memory once per block instead of nine times per thread.
-constexpr int FTY = 4, FTX = 32;           // output tile of the forward kernel (FTY*FTX == CT)
-static_assert(FTY * FTX == CT, "one thread per output pixel of the tile");
+// Forward on the matrix cores: the layer is an implicit GEMM
+//   out[co][p] = b[co] + sum_k A[co][k] X[k][p],   k = tap * CIN + ci,
+//   A[co][k] = w[co][ci][tap] (BF16 values, exact in FP32),
+//   X[k][p]  = in[ci][p + d(tap)]  (zero outside the image),
+// on v_mfma_f32_16x16x4f32: FP32 operands, FP32 accumulation.  A block owns an
+// MY x MX pixel tile of one sample and 16 output channels (blockIdx.z): their
+// weight rows (copied as they are from the forward layout, whose row stride
+// puts the 16 rows of an MFMA operand in distinct LDS banks) and the input
+// tile with its one-pixel halo sit in LDS; wave w computes the 16 pixels of
+// row w / 2, columns (w % 2) * 16 .. + 15.  k runs tap-major so a 4-wide k
+// step is four consecutive input channels at one tap.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+constexpr int MY = 2, MX = 32;             // pixel tile of the forward kernel
+constexpr int MT = 64 * (MY * MX / 16);    // threads: one wave per 16 pixels
+// channel stride of the staged input tile: = 16 (mod 32), so the two k rows a
+// half-wave reads (adjacent channels) fall in different LDS banks
+constexpr int conv_x_stride() { return ((MY + 2) * (MX + 2) + 15) / 32 * 32 + 16; }
 
-template <int CIN, int COUT, int G>
-__global__ __launch_bounds__(CT) void conv_fwd_kernel(const float* __restrict__ in, const int* __restrict__ idx,
-                                                      const uint16_t* __restrict__ wbf, const float* __restrict__ bias,
+template <int CIN, int COUT>
+__global__ __launch_bounds__(MT) void conv_fwd_kernel(const float* __restrict__ in, const int* __restrict__ idx,
+                                                      const float* __restrict__ wfm, const float* __restrict__ bias,
                                                       float* __restrict__ out, int H, int W, int relu) {
-  static_assert(COUT % G == 0, "group size must divide the output channels");
-  constexpr int CC = CIN < 8 ? CIN : 8;     // input channels per staged chunk
-  static_assert(CIN % CC == 0, "chunk must divide the input channels");
-  constexpr int HY = FTY + 2, HX = FTX + 2;
-  __shared__ __align__(16) float wl[CIN * 9 * G];   // [ci][tap][co in group]
-  __shared__ float tin[CC][HY][HX];
-  const int co0 = blockIdx.z * G;
-  const int tid = threadIdx.x;
-  for (int q = tid; q < CIN * 9 * G; q += CT) {
-    const int co = q % G, r = q / G, tap = r % 9, ci = r / 9;
-    wl[q] = bf2f(wbf[((co0 + co) * CIN + ci) * 9 + tap]);
-  }
+  static_assert(CIN % 4 == 0, "a k step covers four input channels");
+  constexpr int K = 9 * CIN, AS = (K + 31) / 32 * 32 + 2;
+  constexpr int HY = MY + 2, HX = MX + 2, XS = conv_x_stride();
+  static_assert((16 * AS) % 4 == 0, "weight rows are copied as float4");
+  __shared__ __align__(16) float wl[16 * AS];
+  __shared__ float tin[CIN * XS];
+  const int tid = threadIdx.x, mb = blockIdx.z;
   const int HW = H * W;
   const int b = blockIdx.y;
-  const int ntx = (W + FTX - 1) / FTX;
-  const int y0 = (blockIdx.x / ntx) * FTY, x0 = (blockIdx.x % ntx) * FTX;
-  const int ty = tid / FTX, tx = tid % FTX;
-  const int y = y0 + ty, x = x0 + tx;
+  const int ntx = (W + MX - 1) / MX;
+  const int y0 = (blockIdx.x / ntx) * MY, x0 = (blockIdx.x % ntx) * MX;
   const float* inb = in + (size_t)(idx ? idx[b] : b) * CIN * HW;
-  float acc[G];
+  // staging: every global load of the thread is issued before the first LDS
+  // store, so their latencies overlap
+  constexpr int NW4 = 16 * AS / 4, PW = (NW4 + MT - 1) / MT;
+  constexpr int NIN = CIN * HY * HX, PI = (NIN + MT - 1) / MT;
+  const float4* src = reinterpret_cast<const float4*>(wfm + (size_t)mb * 16 * AS);
+  float4 wreg[PW];
+  float xv_[PI];
 #pragma unroll
-  for (int co = 0; co < G; co++) acc[co] = bias[co0 + co];
-#pragma unroll 1
-  for (int c0 = 0; c0 < CIN; c0 += CC) {
-    __syncthreads();
-    for (int e = tid; e < CC * HY * HX; e += CT) {
-      const int ci = e / (HY * HX), r = e % (HY * HX), yy = y0 - 1 + r / HX, xx = x0 - 1 + r % HX;
-      tin[ci][r / HX][r % HX] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? inb[(size_t)(c0 + ci) * HW + yy * W + xx] : 0.0f;
-    }
-    __syncthreads();
+  for (int j = 0; j < PW; j++) {
+    const int q = tid + j * MT;
+    wreg[j] = src[q < NW4 ? q : NW4 - 1];   // clamped: the extra copies are not stored
+  }
 #pragma unroll
-    for (int ci = 0; ci < CC; ci++) {
+  for (int j = 0; j < PI; j++) {
+    const int e = tid + j * MT;
+    const int ci = e / (HY * HX), r = e % (HY * HX), yy = y0 - 1 + r / HX, xx = x0 - 1 + r % HX;
+    xv_[j] = (e < NIN && yy >= 0 && yy < H && xx >= 0 && xx < W) ? inb[(size_t)ci * HW + yy * W + xx] : 0.0f;
+  }
 #pragma unroll
-      for (int tap = 0; tap < 9; tap++) {
-        const float v = tin[ci][ty + tap / 3][tx + tap % 3];
-        const float* wr = wl + ((c0 + ci) * 9 + tap) * G;
+  for (int j = 0; j < PW; j++) {
+    const int q = tid + j * MT;
+    if (q < NW4) reinterpret_cast<float4*>(wl)[q] = wreg[j];
+  }
 #pragma unroll
-        for (int co = 0; co < G; co++) acc[co] += wr[co] * v;
-      }
+  for (int j = 0; j < PI; j++) {
+    const int e = tid + j * MT;
+    if (e < NIN) tin[(e / (HY * HX)) * XS + e % (HY * HX)] = xv_[j];
+  }
+  __syncthreads();
+  const int wv = tid >> 6, lane = tid & 63, l16 = lane & 15, l4 = lane >> 4;
+  const int py = wv >> 1, px = (wv & 1) * 16 + l16;
+  floatx4 acc = {0, 0, 0, 0};
+  const float* ar = wl + l16 * AS + l4;
+#pragma unroll 9
+  for (int s = 0; s < K / 4; s++) {
+    const int tap = s / (CIN / 4), ci = (4 * s) % CIN + l4;
+    const float xv = tin[ci * XS + (py + tap / 3) * HX + px + tap % 3];
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[4 * s], xv, acc, 0, 0, 0);
+  }
+  const int y = y0 + py, x = x0 + px;
+  if (y >= H || x >= W) return;
+  float* ob = out + (size_t)b * COUT * HW + (size_t)y * W + x;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int co = mb * 16 + 4 * l4 + i;
+    if (co < COUT) {
+      const float v = acc[i] + bias[co];
+      ob[(size_t)co * HW] = relu ? fmaxf(v, 0.0f) : v;
     }
   }
-  if (y >= H || x >= W) return;
-  float* ob = out + ((size_t)b * COUT + co0) * HW + (size_t)y * W + x;
-#pragma unroll
-  for (int co = 0; co < G; co++) ob[(size_t)co * HW] = relu ? fmaxf(acc[co], 0.0f) : acc[co];
+}
+
+// wfm of every layer from the flat BF16 parameters (one thread per entry)
+__global__ void conv_weight_layout_kernel(const uint16_t* __restrict__ wbf, float* __restrict__ wfm) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= corr_wfm_size()) return;
+  int l = 0;
+  while (l + 1 < CORR_LAYERS && q >= corr_wfm_off(l + 1)) l++;
+  const int r = q - corr_wfm_off(l), AS = corr_a_stride(l), cin = corr_cin(l);
+  const int co = r / AS, k = r % AS, tap = k / cin, ci = k % cin;
+  float v = 0.0f;
+  if (co < corr_cout(l) && k < 9 * cin) v = bf2f(wbf[corr_woff(l) + (co * cin + ci) * 9 + tap]);
+  wfm[q] = v;
 }
 
 // din[ci][q] = sum_co sum_tap w[co][ci][tap] dout[co][q - d(tap)], times [act > 0];
@@ -250,13 +291,18 @@ __global__ void round_bf16_kernel(const float* __restrict__ w, uint16_t* wbf, in
   }
 static_assert(CORR_LAYERS == 4, "layer dispatch covers four layers");
 
-hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const uint16_t* wbf, const float* bias, float* out,
+hipError_t launch_conv_weight_layout(const uint16_t* wbf, float* wfm, hipStream_t st) {
+  constexpr int n = corr_wfm_size();
+  hipLaunchKernelGGL(conv_weight_layout_kernel, dim3((n + 255) / 256), dim3(256), 0, st, wbf, wfm);
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const float* wfm, const float* bias, float* out,
                            int B, int H, int W, int relu, hipStream_t st) {
   FIL_CORR_LAYER(l, {
-    constexpr int G = CO < 4 ? CO : 4;
-    const int tiles = ((H + FTY - 1) / FTY) * ((W + FTX - 1) / FTX);
-    const dim3 grid(tiles, B, CO / G);
-    hipLaunchKernelGGL((conv_fwd_kernel<CI, CO, G>), grid, dim3(CT), 0, st, in, idx, wbf, bias, out, H, W, relu);
+    const int tiles = ((H + MY - 1) / MY) * ((W + MX - 1) / MX);
+    hipLaunchKernelGGL((conv_fwd_kernel<CI, CO>), dim3(tiles, B, (CO + 15) / 16), dim3(MT), 0, st, in, idx, wfm, bias,
+                       out, H, W, relu);
   });
   return hipGetLastError();
 }

@@ -231,111 +231,49 @@ template hipError_t launch_propagator<double>(const double*, const double*, doub
 // Split-BF16 3-term form: with x = x_hi + x_lo (x_hi = bf16(x),
 // x_lo = bf16(x - x_hi)), M*S ~ M_hi S_hi + M_hi S_lo + M_lo S_hi, each on
 // v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  M is split once on the
-// host; the field is split in registers after the LDS read.
-// Operand layout (lane l): a[i] = A[l%16][8*(l/16)+i], b[i] = B[8*(l/16)+i][l%16],
-// c[i] = C[4*(l/16)+i][l%16].
+// host.  Operand layout (lane l): a[i] = A[l%16][8*(l/16)+i],
+// b[i] = B[8*(l/16)+i][l%16], c[i] = C[4*(l/16)+i][l%16].
+//
+// The two products of a step run as
+//   X = P .* (M1 S):  S (FP32, [k][w]) is split when its tile is staged; the
+//                     epilogue multiplies by P and writes X already split, as
+//                     four BF16 planes [re_hi, re_lo, im_hi, im_lo][w][m]
+//                     (m contiguous: the k of the second product);
+//   S = M2 X:         both operands arrive split and k-contiguous, so staging
+//                     is a plain copy and the kernel is MFMA work only.
+// The split values are the ones the FP32 X would give, so the result does not
+// depend on which form X travels in.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
 
-__device__ __forceinline__ void split8(const float* x, bf16x8& hi, bf16x8& lo) {
-#pragma unroll
-  for (int i = 0; i < 8; i++) {
-    __bf16 h = (__bf16)x[i];
-    hi[i] = h;
-    lo[i] = (__bf16)(x[i] - (float)h);
-  }
+// raw buffer resource over [p, p + bytes) (gfx9 word 3: raw 32-bit access);
+// loads outside the range return zero
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, size_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint4 load16(rsrc_t r, int voff, int soff) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ u32x2 load8(rsrc_t r, int voff, int soff) {
+  return __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
 }
 
-template <int BM, int BW>
+// BM (m) x BW (complex w) x BK (k) tiles, 4 waves of (BM/2) x (BW/2); B stored
+// [w][k] in LDS so each lane's MFMA operand (8 consecutive k) is one 16-byte
+// LDS read; register-prefetch double buffering.
+// BIN_PLANES: B is the four BF16 planes [4][Nh][Nr] (else FP32 complex [Nr][Nh]).
+// OUT_PLANES: the output is written as the four planes [4][Nh][Nr] (else FP32
+// complex [Nr][Nh]); P (optional) multiplies the output first.
+template <int BM, int BW, int BK, bool BIN_PLANES, bool OUT_PLANES>
 __global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __restrict__ Mhi,
                                                               const uint16_t* __restrict__ Mlo,
-                                                              const cx<float>* __restrict__ In,
-                                                              cx<float>* __restrict__ Out,
+                                                              const void* __restrict__ In_, void* __restrict__ Out_,
                                                               const cx<float>* __restrict__ P, int Nr, int Nh, int Pw) {
-  constexpr int BK = 32;
-  constexpr int MT = BM / 32, WT = BW / 32;
-  __shared__ __align__(16) uint16_t Ah[BM][BK + 8];
-  __shared__ __align__(16) uint16_t Al[BM][BK + 8];
-  __shared__ __align__(16) float Bre[BW][BK + 4];
-  __shared__ __align__(16) float Bim[BW][BK + 4];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int m0 = blockIdx.y * BM, w0 = blockIdx.x * BW;
-  const int wm = (wv >> 1) * (BM / 2), ww = (wv & 1) * (BW / 2);
-  const int l16 = lane & 15, l4 = lane >> 4;
-  floatx4 acc[MT][WT][2];
-#pragma unroll
-  for (int a = 0; a < MT; a++)
-#pragma unroll
-    for (int b = 0; b < WT; b++) { acc[a][b][0] = {0, 0, 0, 0}; acc[a][b][1] = {0, 0, 0, 0}; }
-
-  for (int k0 = 0; k0 < Nr; k0 += BK) {
-    // A planes: BM x BK bf16, 16-byte chunks
-    for (int e = tid; e < BM * (BK / 8); e += 256) {
-      const int row = e / (BK / 8), kq = (e % (BK / 8)) * 8;
-      const size_t g = (size_t)(m0 + row) * Nr + k0 + kq;
-      *reinterpret_cast<uint4*>(&Ah[row][kq]) = *reinterpret_cast<const uint4*>(Mhi + g);
-      *reinterpret_cast<uint4*>(&Al[row][kq]) = *reinterpret_cast<const uint4*>(Mlo + g);
-    }
-    // B tile: BK x BW complex, transposed into re / im planes [col][k]
-    for (int e = tid; e < BK * BW; e += 256) {
-      const int kk = e / BW, c = e % BW;
-      cx<float> v = In[(size_t)(k0 + kk) * Nh + w0 + c];
-      Bre[c][kk] = v.x;
-      Bim[c][kk] = v.y;
-    }
-    __syncthreads();
-    bf16x8 ah[MT], al[MT];
-#pragma unroll
-    for (int a = 0; a < MT; a++) {
-      ah[a] = *reinterpret_cast<const bf16x8*>(&Ah[wm + a * 16 + l16][8 * l4]);
-      al[a] = *reinterpret_cast<const bf16x8*>(&Al[wm + a * 16 + l16][8 * l4]);
-    }
-#pragma unroll
-    for (int b = 0; b < WT; b++) {
-      const int col = ww + b * 16 + l16;
-      float xr[8], xi[8];
-#pragma unroll
-      for (int i = 0; i < 8; i++) { xr[i] = Bre[col][8 * l4 + i]; xi[i] = Bim[col][8 * l4 + i]; }
-      bf16x8 rh, rl, ih, il;
-      split8(xr, rh, rl);
-      split8(xi, ih, il);
-#pragma unroll
-      for (int a = 0; a < MT; a++) {
-        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], rh, acc[a][b][0], 0, 0, 0);
-        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rl, acc[a][b][0], 0, 0, 0);
-        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rh, acc[a][b][0], 0, 0, 0);
-        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ih, acc[a][b][1], 0, 0, 0);
-        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], il, acc[a][b][1], 0, 0, 0);
-        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], ih, acc[a][b][1], 0, 0, 0);
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < MT; a++)
-#pragma unroll
-    for (int b = 0; b < WT; b++)
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int m = m0 + wm + a * 16 + 4 * l4 + i;
-        const int w = w0 + ww + b * 16 + l16;
-        cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
-        const size_t o = (size_t)m * Nh + w;
-        if (P) v = v * P[(size_t)m * Pw + (w & (Pw - 1))];
-        Out[o] = v;
-      }
-}
-
-// Second split-BF16 kernel: 128 (m) x 64 (complex w) x 32 (k) tiles, the
-// field split into four bf16 planes (re_hi, re_lo, im_hi, im_lo) once when a
-// tile is staged, stored [w][k] so each lane's MFMA operand (8 consecutive k)
-// is one 16-byte LDS read; register-prefetch double buffering.
-template <int BM, int BW, int BK>
-__global__ __launch_bounds__(256) void dht_gemm_bf16x3_v2_kernel(const uint16_t* __restrict__ Mhi,
-                                                                 const uint16_t* __restrict__ Mlo,
-                                                                 const cx<float>* __restrict__ In,
-                                                                 cx<float>* __restrict__ Out,
-                                                                 const cx<float>* __restrict__ P, int Nr, int Nh, int Pw) {
   constexpr int SK = BK + 8;              // padded row: (BK + 8) bf16
   constexpr int WM = BM / 2, WW = BW / 2;
   constexpr int MT = WM / 16, WT = WW / 16;
@@ -346,111 +284,190 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_v2_kernel(const uint16_t*
   const int wm = (wv >> 1) * WM, ww = (wv & 1) * WW;
   const int l16 = lane & 15, l4 = lane >> 4;
   constexpr int NA = BM * BK / 8 / 256;     // 16-byte chunks of each A plane per thread
-  constexpr int NB = BW * BK / 8 / 256;     // (column, 8 consecutive k) B chunks per thread
+  constexpr int NB = BW * BK / 8 / 256;     // 16-byte chunks (one w, 8 consecutive k) of each B plane per thread
   static_assert(NA >= 1 && NB >= 1 && BK % 32 == 0, "tile shape");
-  uint4 rah[NA], ral[NA];
-  cx<float> rb[NB][8];
-#define FIL_BF_GLOAD(k0_)                                                                       \
-  _Pragma("unroll") for (int u = 0; u < NA; u++) {                                               \
-    const int e = tid + 256 * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;                    \
-    const size_t g = (size_t)(m0 + row) * Nr + (k0_) + kq;                                        \
-    rah[u] = *reinterpret_cast<const uint4*>(Mhi + g);                                           \
-    ral[u] = *reinterpret_cast<const uint4*>(Mlo + g);                                           \
-  }                                                                                              \
-  _Pragma("unroll") for (int u = 0; u < NB; u++) {                                               \
-    const int e = tid + 256 * u, bw = e % BW, bkg = e / BW;                                      \
-    _Pragma("unroll") for (int i = 0; i < 8; i++)                                                \
-      rb[u][i] = In[(size_t)((k0_) + 8 * bkg + i) * Nh + w0 + bw];                               \
+  const size_t plane = (size_t)Nh * Nr;     // elements of one BF16 plane of X
+  // two sets of staging registers: tile kb + 2 is loaded while tile kb is
+  // multiplied, so each load has two MFMA phases to arrive
+  uint4 rah[2][NA], ral[2][NA];
+  // B staging registers: 8 complex values (FP32 input) or 4 planes x 8 bf16
+  uint4 rbp[2][BIN_PLANES ? NB : 1][4];
+  u32x2 rbf[2][BIN_PLANES ? 1 : NB][8];
+  // Buffer loads: a per-array resource (scalar registers) plus one 32-bit
+  // byte offset per thread and chunk, the k position in the scalar offset.
+  // Offsets stay below 2^31 (checked by the launcher).
+  const rsrc_t rMh = make_rsrc(Mhi, (size_t)Nr * Nr * 2), rMl = make_rsrc(Mlo, (size_t)Nr * Nr * 2);
+  rsrc_t rB[BIN_PLANES ? 4 : 1];
+  if constexpr (BIN_PLANES) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) rB[q] = make_rsrc(reinterpret_cast<const uint16_t*>(In_) + q * plane, plane * 2);
+  } else {
+    rB[0] = make_rsrc(In_, (size_t)Nr * Nh * 8);
   }
-#define FIL_BF_LSTORE(buf_)                                                                     \
-  _Pragma("unroll") for (int u = 0; u < NA; u++) {                                               \
-    const int e = tid + 256 * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;                    \
-    *reinterpret_cast<uint4*>(&As[buf_][0][row][kq]) = rah[u];                                   \
-    *reinterpret_cast<uint4*>(&As[buf_][1][row][kq]) = ral[u];                                   \
-  }                                                                                              \
-  _Pragma("unroll") for (int u = 0; u < NB; u++) {                                               \
-    const int e = tid + 256 * u, bw = e % BW, bkg = e / BW;                                      \
-    bf16x8 rh, rl, ih, il;                                                                       \
-    _Pragma("unroll") for (int i = 0; i < 8; i++) {                                              \
-      __bf16 hh = (__bf16)rb[u][i].x;                                                            \
-      rh[i] = hh; rl[i] = (__bf16)(rb[u][i].x - (float)hh);                                      \
-      hh = (__bf16)rb[u][i].y;                                                                   \
-      ih[i] = hh; il[i] = (__bf16)(rb[u][i].y - (float)hh);                                      \
-    }                                                                                            \
-    *reinterpret_cast<bf16x8*>(&Bs[buf_][0][bw][8 * bkg]) = rh;                                  \
-    *reinterpret_cast<bf16x8*>(&Bs[buf_][1][bw][8 * bkg]) = rl;                                  \
-    *reinterpret_cast<bf16x8*>(&Bs[buf_][2][bw][8 * bkg]) = ih;                                  \
-    *reinterpret_cast<bf16x8*>(&Bs[buf_][3][bw][8 * bkg]) = il;                                  \
+  int offA[NA], offB[NB];
+#pragma unroll
+  for (int u = 0; u < NA; u++) {
+    const int e = tid + 256 * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;
+    offA[u] = ((m0 + row) * Nr + kq) * 2;
+  }
+#pragma unroll
+  for (int u = 0; u < NB; u++) {
+    const int e = tid + 256 * u;
+    if constexpr (BIN_PLANES) {
+      const int bw = e / (BK / 8), kq = (e % (BK / 8)) * 8;
+      offB[u] = ((w0 + bw) * Nr + kq) * 2;
+    } else {
+      const int bw = e % BW, bkg = e / BW;
+      offB[u] = (8 * bkg * Nh + w0 + bw) * 8;
+    }
+  }
+#define FIL_BF_GLOAD(r_, k0_)                                                                     \
+  _Pragma("unroll") for (int u = 0; u < NA; u++) {                                                 \
+    rah[r_][u] = load16(rMh, offA[u], (k0_) * 2);                                                  \
+    ral[r_][u] = load16(rMl, offA[u], (k0_) * 2);                                                  \
+  }                                                                                                \
+  _Pragma("unroll") for (int u = 0; u < NB; u++) {                                                 \
+    if constexpr (BIN_PLANES) {                                                                    \
+      _Pragma("unroll") for (int q = 0; q < 4; q++) rbp[r_][u][q] = load16(rB[q], offB[u], (k0_) * 2); \
+    } else {                                                                                       \
+      _Pragma("unroll") for (int i = 0; i < 8; i++) rbf[r_][u][i] = load8(rB[0], offB[u], ((k0_) + i) * Nh * 8); \
+    }                                                                                              \
+  }
+#define FIL_BF_LSTORE(r_, buf_)                                                                   \
+  _Pragma("unroll") for (int u = 0; u < NA; u++) {                                                 \
+    const int e = tid + 256 * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;                      \
+    *reinterpret_cast<uint4*>(&As[buf_][0][row][kq]) = rah[r_][u];                                 \
+    *reinterpret_cast<uint4*>(&As[buf_][1][row][kq]) = ral[r_][u];                                 \
+  }                                                                                                \
+  _Pragma("unroll") for (int u = 0; u < NB; u++) {                                                 \
+    const int e = tid + 256 * u;                                                                   \
+    if constexpr (BIN_PLANES) {                                                                    \
+      const int bw = e / (BK / 8), kq = (e % (BK / 8)) * 8;                                        \
+      _Pragma("unroll") for (int q = 0; q < 4; q++) *reinterpret_cast<uint4*>(&Bs[buf_][q][bw][kq]) = rbp[r_][u][q]; \
+    } else {                                                                                       \
+      const int bw = e % BW, bkg = e / BW;                                                         \
+      bf16x8 rh, rl, ih, il;                                                                       \
+      _Pragma("unroll") for (int i = 0; i < 8; i++) {                                              \
+        const float xr = __uint_as_float(rbf[r_][u][i].x), xi = __uint_as_float(rbf[r_][u][i].y);  \
+        __bf16 hh = (__bf16)xr;                                                                    \
+        rh[i] = hh; rl[i] = (__bf16)(xr - (float)hh);                                              \
+        hh = (__bf16)xi;                                                                           \
+        ih[i] = hh; il[i] = (__bf16)(xi - (float)hh);                                              \
+      }                                                                                            \
+      *reinterpret_cast<bf16x8*>(&Bs[buf_][0][bw][8 * bkg]) = rh;                                  \
+      *reinterpret_cast<bf16x8*>(&Bs[buf_][1][bw][8 * bkg]) = rl;                                  \
+      *reinterpret_cast<bf16x8*>(&Bs[buf_][2][bw][8 * bkg]) = ih;                                  \
+      *reinterpret_cast<bf16x8*>(&Bs[buf_][3][bw][8 * bkg]) = il;                                  \
+    }                                                                                              \
   }
   floatx4 acc[MT][WT][2];
 #pragma unroll
   for (int a = 0; a < MT; a++)
 #pragma unroll
     for (int b = 0; b < WT; b++) { acc[a][b][0] = {0, 0, 0, 0}; acc[a][b][1] = {0, 0, 0, 0}; }
+#define FIL_BF_COMPUTE(buf_)                                                                      \
+  _Pragma("unroll") for (int ks = 0; ks < BK; ks += 32) {                                          \
+    bf16x8 ah[MT], al[MT];                                                                         \
+    _Pragma("unroll") for (int a = 0; a < MT; a++) {                                               \
+      ah[a] = *reinterpret_cast<const bf16x8*>(&As[buf_][0][wm + a * 16 + l16][ks + 8 * l4]);      \
+      al[a] = *reinterpret_cast<const bf16x8*>(&As[buf_][1][wm + a * 16 + l16][ks + 8 * l4]);      \
+    }                                                                                              \
+    _Pragma("unroll") for (int b = 0; b < WT; b++) {                                               \
+      const int col = ww + b * 16 + l16;                                                           \
+      const bf16x8 rh = *reinterpret_cast<const bf16x8*>(&Bs[buf_][0][col][ks + 8 * l4]);          \
+      const bf16x8 rl = *reinterpret_cast<const bf16x8*>(&Bs[buf_][1][col][ks + 8 * l4]);          \
+      const bf16x8 ih = *reinterpret_cast<const bf16x8*>(&Bs[buf_][2][col][ks + 8 * l4]);          \
+      const bf16x8 il = *reinterpret_cast<const bf16x8*>(&Bs[buf_][3][col][ks + 8 * l4]);          \
+      _Pragma("unroll") for (int a = 0; a < MT; a++) {                                             \
+        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], rh, acc[a][b][0], 0, 0, 0);  \
+        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rl, acc[a][b][0], 0, 0, 0);  \
+        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rh, acc[a][b][0], 0, 0, 0);  \
+        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ih, acc[a][b][1], 0, 0, 0);  \
+        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], il, acc[a][b][1], 0, 0, 0);  \
+        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], ih, acc[a][b][1], 0, 0, 0);  \
+      }                                                                                            \
+    }                                                                                              \
+  }
+  // LDS buffer b holds tile kb while register set b receives tile kb + 2;
+  // the prefetches past the last tile re-read valid tiles (kept unconditional
+  // so the registers stay out of scratch)
   const int nk = Nr / BK;
-  FIL_BF_GLOAD(0);
-  FIL_BF_LSTORE(0);
+  FIL_BF_GLOAD(0, 0);
+  FIL_BF_GLOAD(1, (1 % nk) * BK);
+  FIL_BF_LSTORE(0, 0);
   __syncthreads();
-  for (int kb = 0; kb < nk; kb++) {
-    const int buf = kb & 1;
-    FIL_BF_GLOAD(((kb + 1) % nk) * BK);
-#pragma unroll
-    for (int ks = 0; ks < BK; ks += 32) {
-    bf16x8 ah[MT], al[MT];
-#pragma unroll
-    for (int a = 0; a < MT; a++) {
-      ah[a] = *reinterpret_cast<const bf16x8*>(&As[buf][0][wm + a * 16 + l16][ks + 8 * l4]);
-      al[a] = *reinterpret_cast<const bf16x8*>(&As[buf][1][wm + a * 16 + l16][ks + 8 * l4]);
-    }
-#pragma unroll
-    for (int b = 0; b < WT; b++) {
-      const int col = ww + b * 16 + l16;
-      const bf16x8 rh = *reinterpret_cast<const bf16x8*>(&Bs[buf][0][col][ks + 8 * l4]);
-      const bf16x8 rl = *reinterpret_cast<const bf16x8*>(&Bs[buf][1][col][ks + 8 * l4]);
-      const bf16x8 ih = *reinterpret_cast<const bf16x8*>(&Bs[buf][2][col][ks + 8 * l4]);
-      const bf16x8 il = *reinterpret_cast<const bf16x8*>(&Bs[buf][3][col][ks + 8 * l4]);
-#pragma unroll
-      for (int a = 0; a < MT; a++) {
-        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], rh, acc[a][b][0], 0, 0, 0);
-        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rl, acc[a][b][0], 0, 0, 0);
-        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rh, acc[a][b][0], 0, 0, 0);
-        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ih, acc[a][b][1], 0, 0, 0);
-        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], il, acc[a][b][1], 0, 0, 0);
-        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], ih, acc[a][b][1], 0, 0, 0);
-      }
-    }
-    }
-    FIL_BF_LSTORE(buf ^ 1);
+  for (int kb = 0; kb < nk; kb += 2) {
+    FIL_BF_GLOAD(0, ((kb + 2) % nk) * BK);
+    FIL_BF_COMPUTE(0);
+    FIL_BF_LSTORE(1, 1);
+    __syncthreads();
+    if (kb + 1 >= nk) break;
+    FIL_BF_GLOAD(1, ((kb + 3) % nk) * BK);
+    FIL_BF_COMPUTE(1);
+    FIL_BF_LSTORE(0, 0);
     __syncthreads();
   }
+#undef FIL_BF_COMPUTE
 #undef FIL_BF_GLOAD
 #undef FIL_BF_LSTORE
 #pragma unroll
   for (int a = 0; a < MT; a++)
 #pragma unroll
-    for (int b = 0; b < WT; b++)
+    for (int b = 0; b < WT; b++) {
+      const int w = w0 + ww + b * 16 + l16;
+      const int mb = m0 + wm + a * 16 + 4 * l4;
+      if constexpr (OUT_PLANES) {
+        bf16x4 rh, rl, ih, il;
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int m = m0 + wm + a * 16 + 4 * l4 + i;
-        const int w = w0 + ww + b * 16 + l16;
-        cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
-        const size_t o = (size_t)m * Nh + w;
-        if (P) v = v * P[(size_t)m * Pw + (w & (Pw - 1))];
-        Out[o] = v;
+        for (int i = 0; i < 4; i++) {
+          cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
+          if (P) v = v * P[(size_t)(mb + i) * Pw + (w & (Pw - 1))];
+          __bf16 hh = (__bf16)v.x;
+          rh[i] = hh; rl[i] = (__bf16)(v.x - (float)hh);
+          hh = (__bf16)v.y;
+          ih[i] = hh; il[i] = (__bf16)(v.y - (float)hh);
+        }
+        __bf16* Xp = reinterpret_cast<__bf16*>(Out_) + (size_t)w * Nr + mb;
+        *reinterpret_cast<bf16x4*>(Xp) = rh;
+        *reinterpret_cast<bf16x4*>(Xp + plane) = rl;
+        *reinterpret_cast<bf16x4*>(Xp + 2 * plane) = ih;
+        *reinterpret_cast<bf16x4*>(Xp + 3 * plane) = il;
+      } else {
+        cx<float>* Out = reinterpret_cast<cx<float>*>(Out_);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          cx<float> v = mk(acc[a][b][0][i], acc[a][b][1][i]);
+          if (P) v = v * P[(size_t)(mb + i) * Pw + (w & (Pw - 1))];
+          Out[(size_t)(mb + i) * Nh + w] = v;
+        }
       }
+    }
 }
 
-hipError_t launch_dht_gemm_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In, cx<float>* Out,
-                                  const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st) {
+template <bool BIN_PLANES, bool OUT_PLANES>
+static hipError_t launch_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const void* In, void* Out,
+                                const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st) {
   if (Nr % 64 || Nh % 64 || Pw < 1 || (Pw & (Pw - 1)) || Nh % Pw) return hipErrorInvalidValue;
+  // the kernels address their operands with 32-bit byte offsets
+  if ((size_t)Nr * Nh * 8 >= ((size_t)1 << 31)) return hipErrorInvalidValue;
   if (wide_tiles(Nr, Pw)) {
-    hipLaunchKernelGGL((dht_gemm_bf16x3_v2_kernel<128, 64, 64>), dim3(Nh / 64, Nr / 128), dim3(256), 0, st,
-                       Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
+    hipLaunchKernelGGL((dht_gemm_bf16x3_kernel<128, 64, 64, BIN_PLANES, OUT_PLANES>), dim3(Nh / 64, Nr / 128),
+                       dim3(256), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
   } else {
-    hipLaunchKernelGGL((dht_gemm_bf16x3_kernel<64, 64>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st,
-                       Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
+    hipLaunchKernelGGL((dht_gemm_bf16x3_kernel<64, 64, 64, BIN_PLANES, OUT_PLANES>), dim3(Nh / 64, Nr / 64),
+                       dim3(256), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_dht_gemm_bf16x3_to_planes(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In,
+                                            uint16_t* Xp, const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st) {
+  return launch_bf16x3<false, true>(Mhi, Mlo, In, Xp, P, Nr, Nh, Pw, st);
+}
+
+hipError_t launch_dht_gemm_bf16x3_from_planes(const uint16_t* Mhi, const uint16_t* Mlo, const uint16_t* Xp,
+                                              cx<float>* Out, int Nr, int Nh, int Pw, hipStream_t st) {
+  return launch_bf16x3<true, false>(Mhi, Mlo, Xp, Out, nullptr, Nr, Nh, Pw, st);
 }
 
 }  // namespace fil

@@ -94,6 +94,12 @@ __host__ __device__ constexpr int fft_tw_offset(int N, int S) {
   return off;
 }
 __host__ __device__ constexpr int fft_tw_size(int N) { return fft_tw_offset(N, N); }
+// Extended table of the row transforms: after the N-point table the
+// N/2-point table (a real signal of length N transformed as N/2 complex
+// points), then the N/2 post-processing factors exp(+2 pi i w / N), w < N/2.
+__host__ __device__ constexpr int fft_tw_half_offset(int N) { return fft_tw_size(N); }
+__host__ __device__ constexpr int fft_tw_post_offset(int N) { return fft_tw_size(N) + fft_tw_size(N / 2); }
+__host__ __device__ constexpr int fft_tw_ext_size(int N) { return fft_tw_post_offset(N) + N / 2; }
 
 // LDS offset of element i0 + c for a compile-time c that is a multiple of a
 // power of two C with i0 mod C + c's carry staying below 16 (every access of

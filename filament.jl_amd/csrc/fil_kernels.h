@@ -11,8 +11,9 @@ struct RowArgs {
   const cx<R>* tw;          // [Nt] exp(-2 pi i j/Nt)
   const R* Tg;              // [Nt] Tguard
   const R* Rg;              // [Nr] Rguard
-  const cx<R>* coef;        // [3][Nt/2] i QZ Rnl d(w)/(2 Nt sc): cubic, free, losses
-  R sc[3];                  // per-response scale applied before the packed FFTs
+  const cx<R>* coef;        // [4][Nt/2] i QZ Rnl d(w)/(2 Nt sc): cubic, free, losses, Raman
+  const cx<R>* hr;          // [Nt] Raman response spectrum / Nt (null without Raman)
+  R sc[4];                  // per-response scale applied before the packed FFTs
   R dz, dt;
   int apply_filter;         // field filter of the previous step still pending
   int nonlinearity, plasma, responses, losses_ereal;
@@ -86,9 +87,24 @@ __host__ __device__ constexpr int corr_woff(int l) {
 }
 __host__ __device__ constexpr int corr_nparam() { return corr_woff(CORR_LAYERS); }
 
+// FP32 copy of the BF16 weights in the layout the forward kernel stages into
+// LDS: per layer [16 ceil(cout / 16)][corr_a_stride(l)] with k = tap * cin + ci
+// along a row (zero padding), so a block copies its 16 channel rows as they are.
+__host__ __device__ constexpr int corr_a_stride(int l) { return (9 * corr_cin(l) + 31) / 32 * 32 + 2; }
+__host__ __device__ constexpr int corr_a_rows(int l) { return 16 * ((corr_cout(l) + 15) / 16); }
+__host__ __device__ constexpr int corr_wfm_off(int l) {
+  int o = 0;
+  for (int k = 0; k < l; k++) o += corr_a_rows(k) * corr_a_stride(k);
+  return o;
+}
+__host__ __device__ constexpr int corr_wfm_size() { return corr_wfm_off(CORR_LAYERS); }
+// wfm <- the FP32 forward layout of every layer's BF16 weights (wbf: the flat
+// BF16 parameter copy; biases are not part of wfm)
+hipError_t launch_conv_weight_layout(const uint16_t* wbf, float* wfm, hipStream_t st);
+
 // forward of layer l: out = act(conv(in) + b); in batch element b is taken from
-// in + (idx ? idx[b] : b) * cin*H*W.
-hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const uint16_t* wbf, const float* bias, float* out,
+// in + (idx ? idx[b] : b) * cin*H*W; wfm: the layer's block of the forward layout.
+hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const float* wfm, const float* bias, float* out,
                            int B, int H, int W, int relu, hipStream_t st);
 // din = conv^T(dout) (* [act > 0] when act != null)
 hipError_t launch_conv_bwd_data(int l, const float* dout, const uint16_t* wbf, const float* act, float* din, int B,
@@ -141,8 +157,14 @@ template <typename R>
 // [Nr][Pw] is shared by the members: Out[m][w] *= P[m][w mod Pw].
 hipError_t launch_dht_gemm(const R* M, const cx<R>* In, cx<R>* Out, const cx<R>* P, int Nr, int Nh, int Pw,
                            hipStream_t st);
-// split-BF16 3-term variant: M given as hi/lo bf16 planes.
-hipError_t launch_dht_gemm_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In, cx<float>* Out,
-                                  const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st);
+// split-BF16 3-term variant, M given as hi/lo bf16 planes.  The first product
+// writes X = P .* (M1 In) as four BF16 planes Xp [re_hi, re_lo, im_hi, im_lo][Nh][Nr]
+// (the split of the FP32 X, k of the second product contiguous); the second
+// reads them: Out = M2 X.
+hipError_t launch_dht_gemm_bf16x3_to_planes(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In,
+                                            uint16_t* Xp, const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st);
+// (Pw: one pulse's columns, which picks the tile shape as in the first product)
+hipError_t launch_dht_gemm_bf16x3_from_planes(const uint16_t* Mhi, const uint16_t* Mlo, const uint16_t* Xp,
+                                              cx<float>* Out, int Nr, int Nh, int Pw, hipStream_t st);
 
 }  // namespace fil

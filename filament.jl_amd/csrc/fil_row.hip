@@ -102,9 +102,16 @@ __device__ __forceinline__ R block_max(R v, R* red) {
   return m;
 }
 
+// rho and kdrho rows in LDS carry one pad element per 128 bytes: the scan
+// writes them in per-thread chunks of N/NTH consecutive samples (a lane stride
+// of 32 or 64 bytes), which would put a wave's stores into 4 (FP32) or 2
+// (FP64) banks; padded, the lanes of a half-wave hit distinct banks.
+template <typename R> __host__ __device__ constexpr int rk_index(int j) { return j + j / (128 / (int)sizeof(R)); }
+template <typename R> __host__ __device__ constexpr int rk_size(int n) { return n + n / (128 / (int)sizeof(R)); }
+
 // ---------------------------------------------------------------------------
 // Plasma density of one row from the time-domain field Z (LDS).
-// Writes rho[t], kd[t] (LDS).  Without avalanche every component obeys the
+// Writes rho[rk_index(t)], kd[rk_index(t)] (LDS).  Without avalanche every component obeys the
 // linear ODE d rho/dt = W(t) (f - rho); one RK3 step of it (the reference's
 // ODEIntegrators.RK3, stages at t_j, t_j + dt/2, t_j + dt with the field
 // linearly interpolated) is exactly the affine map
@@ -124,7 +131,7 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
       R u[FIL_MAXCOMP], ut[FIL_MAXCOMP];
       for (int q = 0; q < nc; q++) { u[q] = (R)pp.rho0; ut[q] = (R)pp.rho0; }
       R s0 = 0; for (int q = 0; q < nc; q++) s0 += u[q];
-      rho[0] = s0;
+      rho[rk_index<R>(0)] = s0;
       for (int j = 0; j < N - 1; j++) {
         cx<R> E0 = Z[lds_index(j)], E1 = Z[lds_index(j + 1)];
         cx<R> Em = (R)0.5 * (E0 + E1);
@@ -152,7 +159,7 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
           u[q] = u[q] + dt * (k[0][q] / (R)6 + (R)2 * k[1][q] / (R)3 + k[2][q] / (R)6);
           s += u[q];
         }
-        rho[j + 1] = s;
+        rho[rk_index<R>(j + 1)] = s;
         // avalanche-free integrator for kdrho
         for (int st = 0; st < 3; st++) {
           for (int q = 0; q < nc; q++) {
@@ -170,9 +177,9 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
           R K = pp.kdep ? pw : (R)pp.K[q];
           kdv += K * (w1 * ((R)pp.frhont[q] - ut[q]));
         }
-        kd[j] = kdv;
+        kd[rk_index<R>(j)] = kdv;
       }
-      kd[N - 1] = 0;
+      kd[rk_index<R>(N - 1)] = 0;
     }
     __syncthreads();
     return;
@@ -217,7 +224,7 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
   }
   if (tid == 0) {
     R s0 = 0; for (int q = 0; q < nc; q++) s0 += (R)pp.rho0;
-    rho[0] = s0;
+    rho[rk_index<R>(0)] = s0;
   }
   // pass 2: walk the chunk again
   have_prev = false;
@@ -243,10 +250,10 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
       R K = pp.kdep ? pw : (R)pp.K[q];
       kdv += K * (W1 * (R)((A)pp.frhont[q] - r[q]));
     }
-    rho[j + 1] = (R)tot;
-    kd[j] = kdv;
+    rho[rk_index<R>(j + 1)] = (R)tot;
+    kd[rk_index<R>(j)] = kdv;
   }
-  if (tid == NTH - 1) kd[N - 1] = 0;
+  if (tid == NTH - 1) kd[rk_index<R>(N - 1)] = 0;
   __syncthreads();
 }
 
@@ -267,10 +274,11 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
 // without the scaling the FFT rounding of the large one would swamp the
 // small one, whose coefficient is correspondingly huge.
 // ---------------------------------------------------------------------------
-template <typename R, int N, int NTH>
+template <typename R, int N, int NTH, bool RAMAN>
 __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, const cx<R>* __restrict__ tw,
-                                          const cx<R>* __restrict__ coef, R sc0, R sc1, R sc2, R g_r,
-                                          const R* __restrict__ Tg, int resp, int losses_ereal, cx<R>* k) {
+                                          const cx<R>* __restrict__ coef, const cx<R>* __restrict__ hr,
+                                          const R* sc, R g_r, const R* __restrict__ Tg, int resp, int losses_ereal,
+                                          cx<R>* k) {
   constexpr int TP = N / NTH;
   constexpr int NH = N / 2;
   constexpr int SP = NH / NTH;
@@ -279,7 +287,11 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
   const bool thg = resp & FIL_RESP_CUBIC_THG;
   const bool fre = resp & FIL_RESP_FREE;
   const bool los = resp & FIL_RESP_LOSSES;
-  R Fl[TP];
+  const bool ram_thg = resp & FIL_RESP_RAMAN_THG;
+  // the Raman variant keeps Re(E) and the Raman intensity of its samples
+  // across the convolution; the kernel without Raman does not hold them
+  constexpr int TR = RAMAN ? TP : 1;
+  R Fl[TP], Re[TR], Ir[TR];
 #pragma unroll
   for (int u = 0; u < TP; u++) {
     const int t = tid + u * NTH;
@@ -288,14 +300,18 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
     const R g = g_r * Tg[t];
     R fc = 0, ff = 0, fl = 0;
     if (cub) fc = thg ? re * re * re : (R)0.75 * I * re;
-    if (fre) ff = rho[t] * re;
+    if (fre) ff = rho[rk_index<R>(t)] * re;
     if (los) {
       R fe = losses_ereal ? re * re : I;
       R inv = (fe >= (R)1e-30) ? recip<R>(fe) : (R)0;
-      fl = kd[t] * inv * re;
+      fl = kd[rk_index<R>(t)] * inv * re;
     }
-    Fl[u] = (g * sc2) * fl;
-    Z[lds_index(t)] = mk((g * sc0) * fc, (g * sc1) * ff);
+    Fl[u] = (g * sc[2]) * fl;
+    if constexpr (RAMAN) {
+      Re[u] = re;
+      Ir[u] = ram_thg ? re * re : (R)0.75 * I;    // calc_raman / calc_raman_nothg (raman.jl:46-65)
+    }
+    Z[lds_index(t)] = mk((g * sc[0]) * fc, (g * sc[1]) * ff);
   }
   __syncthreads();
 #pragma unroll
@@ -311,16 +327,56 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
     }
     __syncthreads();
   }
-  if (los) {
+  if constexpr (RAMAN) {
+    // Raman: F = conv(I, H) Re(E) with conv = FFT-(hr .* FFT+(I)), hr = Hraman / Nt
+    // (convolution!, raman.jl:68-111).  H comes from a real response, so the
+    // convolution of the real I is real; its real part is kept.
 #pragma unroll
-    for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = mk(Fl[u], (R)0);
+    for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = mk(Ir[u], (R)0);
+    __syncthreads();
+    fft_lds<R, N, NTH, +1>(Z, tw);
+    for (int j = tid; j < N; j += NTH) Z[lds_index(j)] = hr[j] * Z[lds_index(j)];
+    __syncthreads();
+    fft_lds<R, N, NTH, -1>(Z, tw);
+    // losses and Raman share one complex FFT
+#pragma unroll
+    for (int u = 0; u < TP; u++) {
+      const int t = tid + u * NTH;
+      const R g = g_r * Tg[t];
+      Ir[u] = (g * sc[3]) * (Z[lds_index(t)].x * Re[u]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = mk(Fl[u], Ir[u]);
     __syncthreads();
     fft_lds<R, N, NTH, +1>(Z, tw);
 #pragma unroll
     for (int m = 0; m < SP; m++) {
       const int w = tid + m * NTH;
       cx<R> zw = Z[lds_index(w)], zm = conj(Z[lds_index((N - w) & (N - 1))]);
-      k[m] = k[m] + coef[2 * NH + w] * (zw + zm);
+      k[m] = k[m] + coef[2 * NH + w] * (zw + zm) + coef[3 * NH + w] * mul_mi(zw - zm);
+    }
+    __syncthreads();
+  } else if (los) {
+    // the losses field is real: transform it as N/2 complex points
+    // y[n] = fl[2n] + i fl[2n+1], Y = FFT_{N/2}(y); then with
+    // A = Y[w] + conj(Y[-w]), B = Y[w] - conj(Y[-w]) (indices mod N/2),
+    // 2 FL[w] = A - i exp(+2 pi i w / N) B for w < N/2.
+    R* Zr = reinterpret_cast<R*>(Z);
+#pragma unroll
+    for (int u = 0; u < TP; u++) {
+      const int t = tid + u * NTH;
+      Zr[2 * lds_index(t >> 1) + (t & 1)] = Fl[u];
+    }
+    __syncthreads();
+    fft_lds<R, N / 2, NTH, +1>(Z, tw + fft_tw_half_offset(N));
+    const cx<R>* post = tw + fft_tw_post_offset(N);
+#pragma unroll
+    for (int m = 0; m < SP; m++) {
+      const int w = tid + m * NTH;
+      const cx<R> yw = Z[lds_index(w)], ym = conj(Z[lds_index((NH - w) & (NH - 1))]);
+      const cx<R> A = yw + ym, Bm = yw - ym;
+      k[m] = k[m] + coef[2 * NH + w] * (A + mul_mi(post[w] * Bm));
     }
     __syncthreads();
   }
@@ -338,13 +394,13 @@ __device__ __forceinline__ void store_padded(cx<R>* Z, const cx<R>* v) {
 // ---------------------------------------------------------------------------
 // The row kernel.  One workgroup per r row; LDS: Z (N complex), rho, kd (N R).
 // ---------------------------------------------------------------------------
-template <typename R, typename A, int N, int NTH>
+template <typename R, typename A, int N, int NTH, bool RAMAN>
 __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   R* rho = reinterpret_cast<R*>(Z + lds_size(N));
-  R* kd = rho + N;
-  A* red = reinterpret_cast<A*>(kd + N);
+  R* kd = rho + rk_size<R>(N);
+  A* red = reinterpret_cast<A*>(kd + rk_size<R>(N));
   constexpr int NH = N / 2;
   constexpr int SP = NH / NTH;
   const int tid = threadIdx.x;
@@ -375,17 +431,17 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   if (a.plasma) {
     plasma_row<R, A, N, NTH>(Z, rho, kd, a.pp, a.dt, red);
     R mx = 0;
-    for (int t = tid; t < N; t += NTH) mx = max(mx, rho[t]);
+    for (int t = tid; t < N; t += NTH) mx = max(mx, rho[rk_index<R>(t)]);
     mx = block_max<R>(mx, reinterpret_cast<R*>(red));
-    if (tid == 0) { a.rho_end[i] = (double)rho[N - 1]; a.rho_max[i] = (double)mx; }
+    if (tid == 0) { a.rho_end[i] = (double)rho[rk_index<R>(N - 1)]; a.rho_max[i] = (double)mx; }
     if (a.rho_full) {
       for (int t = tid; t < N; t += NTH) {
-        a.rho_full[(size_t)i * N + t] = rho[t];
-        a.kd_full[(size_t)i * N + t] = kd[t];
+        a.rho_full[(size_t)i * N + t] = rho[rk_index<R>(t)];
+        a.kd_full[(size_t)i * N + t] = kd[rk_index<R>(t)];
       }
     }
   } else {
-    for (int t = tid; t < N; t += NTH) { rho[t] = 0; kd[t] = 0; }
+    for (int t = tid; t < N; t += NTH) { rho[rk_index<R>(t)] = 0; kd[rk_index<R>(t)] = 0; }
     __syncthreads();
   }
 
@@ -400,17 +456,17 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   // RK3 (Kutta) in z: k1 = f(S0); k2 = f(S0 + dz/2 k1); k3 = f(S0 - dz k1 + 2 dz k2)
   const R dz = a.dz;
   cx<R> k1[SP], k2[SP], k[SP], U[SP];
-  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, a.sc[0], a.sc[1], a.sc[2], g_r, a.Tg, a.responses, a.losses_ereal, k1);
+  stage_rhs<R, N, NTH, RAMAN>(Z, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k1);
 #pragma unroll
   for (int m = 0; m < SP; m++) U[m] = S0[m] + ((R)0.5 * dz) * k1[m];
   store_padded<R, N, NTH>(Z, U);
   fft_lds<R, N, NTH, -1>(Z, a.tw);
-  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, a.sc[0], a.sc[1], a.sc[2], g_r, a.Tg, a.responses, a.losses_ereal, k2);
+  stage_rhs<R, N, NTH, RAMAN>(Z, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k2);
 #pragma unroll
   for (int m = 0; m < SP; m++) U[m] = S0[m] + dz * (k2[m] + k2[m] - k1[m]);
   store_padded<R, N, NTH>(Z, U);
   fft_lds<R, N, NTH, -1>(Z, a.tw);
-  stage_rhs<R, N, NTH>(Z, rho, kd, a.tw, a.coef, a.sc[0], a.sc[1], a.sc[2], g_r, a.Tg, a.responses, a.losses_ereal, k);
+  stage_rhs<R, N, NTH, RAMAN>(Z, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k);
   const R c16 = (R)1 / (R)6, c23 = (R)2 / (R)3;
 #pragma unroll
   for (int m = 0; m < SP; m++) {
@@ -582,8 +638,8 @@ __global__ __launch_bounds__(NTH) void corr_features_kernel(const cx<float>* __r
   extern __shared__ __align__(16) unsigned char smem[];
   cx<float>* Z = reinterpret_cast<cx<float>*>(smem);
   float* rho = reinterpret_cast<float*>(Z + lds_size(N));
-  float* kd = rho + N;
-  float* red = kd + N;
+  float* kd = rho + rk_size<float>(N);
+  float* red = kd + rk_size<float>(N);
   constexpr int NH = N / 2, SP = NH / NTH;
   const int tid = threadIdx.x, i = blockIdx.x;
   cx<float> v[SP];
@@ -603,7 +659,7 @@ __global__ __launch_bounds__(NTH) void corr_features_kernel(const cx<float>* __r
       sr += a.x;
       si += a.y;
       sI += norm2(e);
-      if (plasma) sn += rho[j];
+      if (plasma) sn += rho[rk_index<float>(j)];
     }
     out[b] = sr * inv * g.inv_as;
     out[g.W + b] = si * inv * g.inv_as;
@@ -673,14 +729,16 @@ template <int N> constexpr int threads_for() {
 
 template <typename R, typename A>
 static size_t row_smem(int N, int nth) {
-  return (size_t)lds_size(N) * sizeof(cx<R>) + (size_t)N * 2 * sizeof(R) + 2 * (nth / 64) * sizeof(A) + 64;
+  return (size_t)lds_size(N) * sizeof(cx<R>) + (size_t)rk_size<R>(N) * 2 * sizeof(R) + 2 * (nth / 64) * sizeof(A) + 64;
 }
 
 template <typename R, typename A, int N>
 static hipError_t launch_row_n(const RowArgs<R>& a, int Nr, hipStream_t st) {
   constexpr int NTH = threads_for<N>();
   size_t sm = row_smem<R, A>(N, NTH);
-  auto k = row_kernel<R, A, N, NTH>;
+  const bool raman = a.nonlinearity && (a.responses & (FIL_RESP_RAMAN_THG | FIL_RESP_RAMAN_NOTHG));
+  if (raman && !a.hr) return hipErrorInvalidValue;
+  auto k = raman ? row_kernel<R, A, N, NTH, true> : row_kernel<R, A, N, NTH, false>;
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, a);

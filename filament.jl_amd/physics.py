@@ -256,9 +256,33 @@ def qz_vector(unit, grid, medium, w0, Eu):
 # ---------------------------------------------------------------------------
 # Responses (medium_responses/*.jl)
 # ---------------------------------------------------------------------------
-def rnl_cubic(unit, medium, w0, Eu, Nt):
-    """cubic.jl:4-30 (scalar, broadcast over w)."""
-    return np.full(Nt, EPS0 * medium.chi3(w0) * Eu ** 3, dtype=complex)
+def rnl_cubic(unit, medium, w0, Eu, Nt, n2=None):
+    """cubic.jl:4-30 (scalar, broadcast over w); also the Raman Rnl
+    (raman.jl:4-14).  n2: the response's own nonlinear index (default: the
+    medium's)."""
+    n = float(np.real(medium.refractive_index(w0)))
+    chi3 = medium.chi3(w0) if n2 is None else 4.0 / 3.0 * n * n * EPS0 * C0 * n2
+    return np.full(Nt, EPS0 * chi3 * Eu ** 3, dtype=complex)
+
+
+def hraman_table(unit, grid, raman_response):
+    """Hraman(w) of init_raman (raman.jl:15-30): the response sampled at
+    (t - tshift) tu, tshift the window centre, times tu; ifftshift; ifft
+    (normalised, exp(+i w t)); times Nt dt."""
+    tshift = grid.tmin + 0.5 * (grid.tmax - grid.tmin)
+    h = np.array([raman_response((tt - tshift) * unit.t) for tt in grid.t], dtype=float) * unit.t
+    H = np.fft.ifft(np.fft.ifftshift(h.astype(complex)))
+    return H * grid.Nt * grid.dt
+
+
+def mlejnek_raman_response(t):
+    """Raman response of air (examples/RT/air.jl:47-58, Mlejnek, Wright and
+    Moloney, Opt. Lett. 23, 382 (1998)); t in seconds."""
+    if t < 0.0:
+        return 0.0
+    Om, Ga = 20.6e12, 26e12
+    La = np.sqrt(Om ** 2 - Ga ** 2 / 4.0)
+    return Om ** 2 * np.exp(-0.5 * Ga * t) * np.sin(La * t) / La
 
 
 def rnl_current_free(unit, grid, Eu, nuc, mr):

@@ -51,7 +51,7 @@ class _Model:
                 kind = response_kind(item["init"])
                 if kind == "cubic":
                     resp_flags |= L.RESP_CUBIC_THG if item["THG"] else L.RESP_CUBIC_NOTHG
-                    self.Rnl["cubic"] = ph.rnl_cubic(u, m, self.w0, self.Eu, g.Nt)
+                    self.Rnl["cubic"] = ph.rnl_cubic(u, m, self.w0, self.Eu, g.Nt, item.get("n2"))
                 elif kind == "current_free":
                     resp_flags |= L.RESP_FREE
                     self.Rnl["free"] = ph.rnl_current_free(u, g, self.Eu, item["nuc"], item["mr"])
@@ -60,7 +60,9 @@ class _Model:
                     self.losses_ereal = int(bool(item["EREAL"]))
                     self.Rnl["losses"] = ph.rnl_current_losses(u, g, self.Eu, self.w0)
                 elif kind == "raman":
-                    raise NotImplementedError("the Raman response is not implemented on the GPU path")
+                    resp_flags |= L.RESP_RAMAN_THG if item["THG"] else L.RESP_RAMAN_NOTHG
+                    self.Rnl["raman"] = ph.rnl_cubic(u, m, self.w0, self.Eu, g.Nt, item.get("n2"))
+                    self.Hraman = ph.hraman_table(u, g, item["raman_response"])
                 else:
                     raise ValueError(f"unknown response {kind}")
             # QPARAXIAL: QZ depends on w only, also in XYT (nonlinear_propagators.jl:184-242)
@@ -119,6 +121,8 @@ class _Model:
         t.Rnl_cubic = arr(self.Rnl.get("cubic", zero), True)
         t.Rnl_free = arr(self.Rnl.get("free", zero), True)
         t.Rnl_losses = arr(self.Rnl.get("losses", zero), True)
+        t.Rnl_raman = arr(self.Rnl.get("raman", zero), True)
+        t.Hraman = arr(getattr(self, "Hraman", zero), True)
         if self.pt is not None:
             t.rho0 = self.pt["rho0"]
             for q, c in enumerate(self.pt["components"]):

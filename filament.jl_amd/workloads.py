@@ -21,7 +21,8 @@ import math
 import numpy as np
 
 from . import physics as ph
-from .responses import RK3, init_cubic, init_current_free, init_current_losses, init_photoionization_avalanche
+from .responses import (RK3, init_cubic, init_current_free, init_current_losses, init_photoionization_avalanche,
+                        init_raman)
 
 
 def o2_rate_table(sigma8_cm=2.9e-99, K=8, xmin=1e9, xmax=1e20, n=12):
@@ -48,8 +49,10 @@ def gaussian_ic(a0, tau0, I0, lam0, noise=0.0, seed=0):
 
 
 def rt_air(Nr, Nt, power_pcr, rmax=10.0, tmin=-500.0, tmax=500.0, a0=1e-3, tau0=35e-15, noise=1e-3,
-           seed=0, n2=1e-23, nonlinearity=True, plasma=True, dz=5e-4):
-    """Parameter dict in the reference's input-file names (src/Input.jl:293-317)."""
+           seed=0, n2=1e-23, nonlinearity=True, plasma=True, dz=5e-4, raman=False):
+    """Parameter dict in the reference's input-file names (src/Input.jl:293-317).
+    raman: add the stimulated Raman response of air with the same n2 and no THG
+    (the response list of examples/RT/air.jl:88)."""
     lam0 = 800e-9
     medium = ph.Medium(ph.peck_reeder_air, ph.unit_permeability, n2)
     w0 = 2 * math.pi * ph.C0 / lam0
@@ -68,7 +71,8 @@ def rt_air(Nr, Nt, power_pcr, rmax=10.0, tmin=-500.0, tmax=500.0, a0=1e-3, tau0=
             {"init": init_cubic, "THG": False, "n2": n2},
             {"init": init_current_free, "nuc": nuc, "mr": mr},
             {"init": init_current_losses, "EREAL": False},
-        ],
+        ] + ([{"init": init_raman, "THG": False, "n2": n2, "raman_response": ph.mlejnek_raman_response}]
+             if raman else []),
         plasma_equation={
             "init": init_photoionization_avalanche, "ALG": RK3(), "EREAL": False, "KDEP": False,
             "rho0": 0.0, "rho_nt": 5.4e24, "nuc": 0.0, "mr": mr,

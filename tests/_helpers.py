@@ -38,6 +38,12 @@ def oracle_config(p):
     if medium is None:
         raise ValueError("the oracle restates only Peck-Reeder air")
     cub = resp.get("cubic")
+    ram = resp.get("raman")
+    if ram is not None and ram["raman_response"] is not load_pkg().physics.mlejnek_raman_response:
+        raise ValueError("the oracle restates only the Mlejnek Raman response of air")
+    for r in resp.values():
+        if "n2" in r and r["n2"] != p["n2"]:
+            raise ValueError("the oracle takes one n2 for the cubic and Raman responses")
     return dict(
         geometry=0, Nr=p["Nr"], Nt=p["Nt"], rmax=p["rmax"], tmin=p["tmin"], tmax=p["tmax"],
         ru=p["ru"], zu=p["zu"], tu=p["tu"], Iu=p["Iu"], rhou=p["rhou"], lam0=p["lam0"],
@@ -45,7 +51,7 @@ def oracle_config(p):
         nonlinearity=int(p["NONLINEARITY"]), plasma=int(p["PLASMA"]),
         kparaxial=int(p["KPARAXIAL"]), qparaxial=int(p["QPARAXIAL"]),
         rguard=p["rguard"], tguard=p["tguard"], kguard=p["kguard"], wguard=p["wguard"],
-        cubic=(0 if cub is None else (1 if cub["THG"] else 2)), raman=0,
+        cubic=(0 if cub is None else (1 if cub["THG"] else 2)), raman=(0 if ram is None else (1 if ram["THG"] else 2)),
         current_free=int("current_free" in resp),
         nuc=resp["current_free"]["nuc"] if "current_free" in resp else p["nuc"],
         mr=resp["current_free"]["mr"] if "current_free" in resp else p["mr"],

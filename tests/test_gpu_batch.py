@@ -118,10 +118,26 @@ def test_ensemble_64_groups_match_single_pulses(pkg):
     for _, _, e in ens:
         e.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / n
-    print(f"64-pulse ensemble ({len(groups)} grids): {ms:.2f} ms per ensemble z-step, "
+    print(f"64-pulse ensemble ({len(groups)} grids), eager: {ms:.2f} ms per ensemble z-step, "
           f"{64 * 1e3 / ms:.0f} pulse z-steps/s; setup-to-now {time.perf_counter() - t_setup:.1f} s")
+    # the same steps as one graph launch per z-step
+    grp = pkg.StepGroup([e for _, _, e in ens])
+    z = (nsteps + n) * dz
+    for k in range(2):
+        z += dz
+        grp.zstep(z, dz)
+    grp.synchronize()
+    t0 = time.perf_counter()
+    for k in range(n):
+        z += dz
+        grp.zstep(z, dz)
+    grp.synchronize()
+    msg = (time.perf_counter() - t0) * 1e3 / n
+    print(f"64-pulse ensemble, one graph launch per z-step: {msg:.2f} ms per ensemble z-step, "
+          f"{64 * 1e3 / msg:.0f} pulse z-steps/s ({grp.stats()})")
+    grp.close()
     for _, _, e in ens:
-        d = e.analyze_all((nsteps + n) * dz)
+        d = e.analyze_all(z)
         assert all(np.isfinite(x["W"]) and x["W"] > 0 for x in d)
         e.close()
 

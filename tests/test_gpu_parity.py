@@ -130,3 +130,22 @@ def test_analyzer_arrays_match_oracle(pkg, orc, precision, tol):
     assert od["rho_end"].max() > 0
     for k, v in errs.items():
         assert v < tol, k
+
+
+@pytest.mark.parametrize("precision,tol,oracle_kz", [(4, 1e-4, False), (8, 1e-10, True)])
+def test_raman_steps_match_oracle(pkg, orc, precision, tol, oracle_kz):
+    """The full response set of examples/RT/air.jl (cubic, stimulated Raman,
+    free current, losses) with plasma: the device's Raman convolution (two
+    extra t-transforms per stage) against the oracle's."""
+    p = pkg.workloads.rt_air(64, 512, 6.0, tmin=-125.0, tmax=125.0, tau0=20e-15, raman=True)
+    o = oracle_for(orc, p)
+    dz = 5e-3
+    o.run(20, dz)
+    m = run_gpu(pkg, p, 20, dz, precision, KZ=o.KZ() if oracle_kz else None)
+    err = rel_l2(m.field(), o.field())
+    # the Raman term matters: without it the field differs far beyond the gate
+    q = pkg.workloads.rt_air(64, 512, 6.0, tmin=-125.0, tmax=125.0, tau0=20e-15, raman=False)
+    m0 = run_gpu(pkg, q, 20, dz, precision, KZ=o.KZ() if oracle_kz else None)
+    effect = rel_l2(m0.field(), o.field())
+    print(f"raman precision={precision} rel_l2={err:.3e} (without Raman {effect:.3e})")
+    assert err < tol and effect > 100 * tol

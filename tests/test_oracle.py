@@ -189,6 +189,71 @@ def test_kerr_rhs_is_self_phase_modulation(orc):
     assert np.max(np.abs(dE)) < 1e-12 * abs(expect)
 
 
+def _mlejnek(t_s):
+    """examples/RT/air.jl:47-58 (t in seconds), vectorised."""
+    Om, Ga = 20.6e12, 26e12
+    La = np.sqrt(Om ** 2 - Ga ** 2 / 4)
+    return np.where(t_s >= 0, Om ** 2 * np.exp(-0.5 * Ga * np.maximum(t_s, 0)) * np.sin(La * t_s) / La, 0.0)
+
+
+def test_raman_rhs_constant_intensity_is_scaled_kerr(orc):
+    """With a constant |E| the Raman convolution of the intensity is the
+    intensity times H0 = dt sum_j h(t_j - tshift) tu (the discrete integral of
+    the response), so the Raman right-hand side is H0 times the cubic one
+    (same Rnl, raman.jl:4-14 and cubic.jl:4-12)."""
+    Nr, Nt = 2, 1024
+    tmin, tmax = -1000.0, 1000.0
+    c0 = base_cfg(Nr=Nr, Nt=Nt, tmin=tmin, tmax=tmax, nonlinearity=1, medium=1, n_const=1.0, rguard=0.0, tguard=0.0)
+    oc = orc.Oracle(dict(c0, cubic=2), np.zeros((Nr, Nt)))
+    orm = orc.Oracle(dict(c0, raman=2), np.zeros((Nr, Nt)))
+    t = np.linspace(tmin, tmax, Nt)
+    dt = t[1] - t[0]
+    H0 = dt * np.sum(_mlejnek((t - 0.5 * (tmin + tmax)) * 1e-15) * 1e-15)
+    assert abs(H0 - 1) < 1e-2        # the response is normalised: int h dt = Omega^2 / (Gamma^2/4 + Lambda^2) = 1
+    Es = np.zeros((Nr, Nt), dtype=complex)
+    Es[:, 300] = 0.6
+    dc, dr = oc.nl_rhs(Es), orm.nl_rhs(Es)
+    assert np.max(np.abs(dr - H0 * dc)) < 1e-12 * np.max(np.abs(dc))
+
+
+def test_raman_rhs_matches_brute_force_convolution(orc):
+    """Raman right-hand side of a short pulse plus a weak continuous probe
+    against a direct evaluation of its definition (raman.jl:57-65,68-111,
+    nonlinear_propagators.jl:38-67) in a constant-index medium: the circular
+    convolution summed directly, dt sum_k hs[j-k] 3/4 |E_k|^2 with hs the
+    ifftshift of h((t - tshift) tu) tu, times Re E, then the analytic
+    spectrum (1/N) sum_j F_j exp(+i w t_j) with positive bins doubled and
+    dE = i (4/3)(w/c) n2 Iu x that (the factor pinned by the Kerr test above).
+    A reversed (anti-causal) convolution or a response shifted by half the
+    window fails by O(1)."""
+    Nr, Nt = 2, 512
+    tmin, tmax = -600.0, 600.0
+    c0 = base_cfg(Nr=Nr, Nt=Nt, tmin=tmin, tmax=tmax, nonlinearity=1, medium=1, n_const=1.0, rguard=0.0,
+                  tguard=0.0, raman=2)
+    o = orc.Oracle(c0, np.zeros((Nr, Nt)))
+    t = np.linspace(tmin, tmax, Nt)
+    dt = t[1] - t[0]
+    w = 2 * np.pi * np.fft.fftfreq(Nt, dt)              # 1/fs, the oracle's frequency grid
+    # field: Gaussian pulse (10 fs) at -200 fs on a 1e-2 probe, carriers on grid bins
+    k1, k2 = 60, 75
+    E = (1.0 * np.exp(-((t + 200.0) / 10.0) ** 2) * np.exp(-1j * w[k1] * (t - tmin))
+         + 1e-2 * np.exp(-1j * w[k2] * (t - tmin)))
+    Es = np.tile(np.fft.ifft(E), (Nr, 1))              # E_j = sum_k Es_k exp(-2 pi i jk/N)
+    I = 0.75 * np.abs(E) ** 2
+    hs = np.fft.ifftshift(_mlejnek((t - 0.5 * (tmin + tmax)) * 1e-15) * 1e-15)
+    conv = np.array([dt * np.sum(hs[(j - np.arange(Nt)) % Nt] * I) for j in range(Nt)])
+    F = conv * E.real
+    A = np.fft.ifft(F)
+    A[1:Nt // 2] *= 2
+    A[Nt // 2:] = 0
+    expect = 1j * (4.0 / 3.0) * (w * 1e15 / 299792458.0) * 1e-23 * 1e16 * A
+    got = o.nl_rhs(Es)[0]
+    assert np.max(np.abs(got - expect)) < 1e-9 * np.max(np.abs(expect))
+    # causality, read off the direct convolution: the pulse's response is behind it
+    j0 = np.argmin(np.abs(t + 200.0))
+    assert conv[j0 + 60] > 10 * abs(conv[j0 - 60])
+
+
 def test_linear_step_gaussian_beam_diffraction(orc):
     """Restates test/test_diffraction_rt.jl on the oracle at reduced Nt and
     step count: a(z) = a0 sqrt(1 + (z/zd)^2), I = (a0/a)^2 on axis."""
