@@ -158,7 +158,7 @@ def main():
     ap.add_argument("--corr-draws", type=int, default=4)
     ap.add_argument("--corr-pairs", type=int, default=24)
     ap.add_argument("--corr-iters", type=int, default=1500)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=30, help="oracle z-steps timed for the CPU baseline (~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,plain,chosen",
                     help="comma list of forms (fp64, plain, chosen); anything but all three is a profiling run "

@@ -101,6 +101,7 @@ struct fil_handle {
   } acache;
   // XYT geometry (rows are the Nx*Ny points, x fastest)
   bool xyt = false;
+  bool tgeo = false;                 // T geometry: one row per pulse, diagonal linear step
   int Nx = 0, Ny = 0;
   double dx = 0, dy = 0;
   std::vector<double> x, y;
@@ -265,6 +266,15 @@ int zstep_impl(fil_handle* h, double dz) {
   if (int rc = record(h, 0)) return rc;
   HIPCHK(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
   if (int rc = record(h, 1)) return rc;
+  if (h->tgeo) {
+    // propagate!(E, LP, dz) for T (spectral.jl:16-26): E(w) *= exp(i KZ dz) Wguard
+    HIPCHK(launch_scale_rows<R>(h->S.as<cx<R>>(), h->P.as<cx<R>>(), h->rows(), h->Nh, h->st));
+    if (int rc = record(h, 2)) return rc;
+    if (int rc = record(h, 3)) return rc;
+    h->pending_filter = true;
+    h->plasma_valid = h->d.plasma != 0;
+    return 0;
+  }
   if (h->xyt) {
     // the three transform passes are timed together as the "dht1" phase
     HIPCHK(launch_xyt_linear<R>(h->S.as<cx<R>>(), h->P.as<cx<R>>(), h->twx.as<cx<R>>(), h->twy.as<cx<R>>(), h->Nx,
@@ -433,9 +443,23 @@ int analyze_impl(fil_handle* h, int member, double z, fil_diag* d, double* Fr, d
   const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh;
   AnalyzerRows v;
   if (int rc = analyze_device<R>(h, v, member)) return rc;
-  // scalar diagnostics (FieldAnalyzers.jl:201-213)
   fil_diag o{};
   o.z = z;
+  if (h->tgeo) {
+    // FieldAnalyzerT (FieldAnalyzers.jl:99-114): Imax, rhomax, duration = radius(t, I),
+    // F = sum(I) dt; reported as tau = duration and W = Fmax = F
+    o.Imax = v.im[0];
+    o.rhomax = v.rm[0];
+    o.tau = radius(h->t.data(), v.ft.data(), Nt);
+    o.W = o.Fmax = v.fr[0];
+    if (d) *d = o;
+    if (Fr) Fr[0] = v.fr[0];
+    if (Ft) std::memcpy(Ft, v.ft.data(), Nt * 8);
+    if (rho_end) rho_end[0] = v.re[0];
+    if (S) std::memcpy(S, v.sp.data(), (Nh + 1) * 8);
+    return 0;
+  }
+  // scalar diagnostics (FieldAnalyzers.jl:201-213)
   double Fmax = 0, Imax = 0, rhomax = 0, De = 0, W = 0;
   for (int i = 0; i < Nr; i++) {
     Fmax = std::max(Fmax, v.fr[i]);
@@ -583,8 +607,10 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (!d || !out) return fail(FIL_EINVAL, "null argument");
   *out = nullptr;
   if (d->abi_version != FIL_ABI_VERSION) return fail(FIL_EINVAL, "ABI version mismatch");
-  if (d->geometry != FIL_GEOM_RT && d->geometry != FIL_GEOM_XYT) return fail(FIL_EINVAL, "geometry must be RT or XYT");
+  if (d->geometry != FIL_GEOM_RT && d->geometry != FIL_GEOM_XYT && d->geometry != FIL_GEOM_T)
+    return fail(FIL_EINVAL, "geometry must be RT, XYT or T");
   const bool xyt = d->geometry == FIL_GEOM_XYT;
+  const bool tgeo = d->geometry == FIL_GEOM_T;
   auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
   if (xyt && (!pow2(d->Nx) || !pow2(d->Ny) || d->Nx < 16 || d->Ny < 16 || d->Nx > 1024 || d->Ny > 1024))
     return fail(FIL_EINVAL, "XYT: Nx and Ny must be powers of two in [16, 1024]");
@@ -596,7 +622,8 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   const int Nt = d->Nt;
   if (Nt < 256 || Nt > 8192 || (Nt & (Nt - 1))) return fail(FIL_EINVAL, "Nt must be a power of two in [256, 8192]");
   if (d->precision == FIL_FP64 && Nt > 4096) return fail(FIL_EINVAL, "FP64 supports Nt <= 4096");
-  if (!xyt && (d->Nr < 2 || d->Nr > 16384)) return fail(FIL_EINVAL, "Nr must be in [2, 16384]");
+  if (!xyt && !tgeo && (d->Nr < 2 || d->Nr > 16384)) return fail(FIL_EINVAL, "Nr must be in [2, 16384]");
+  if (tgeo && (d->form & FIL_FORM_DHT_BF16X3)) return fail(FIL_EINVAL, "the T geometry has no Hankel transform");
   if (d->plasma && (d->ncomp < 1 || d->ncomp > FIL_MAXCOMP)) return fail(FIL_EINVAL, "ncomp must be in [1, 4]");
   if ((d->form & FIL_FORM_DHT_BF16X3) && d->precision != FIL_FP32)
     return fail(FIL_EINVAL, "the split-BF16 Hankel form needs FIL_FP32");
@@ -611,11 +638,12 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   fil_handle* h = new fil_handle();
   h->d = *d;
   h->xyt = xyt;
+  h->tgeo = tgeo;
   h->Nx = xyt ? d->Nx : 0;
   h->Ny = xyt ? d->Ny : 0;
-  h->Nr = xyt ? d->Nx * d->Ny : d->Nr;
+  h->Nr = xyt ? d->Nx * d->Ny : (tgeo ? 1 : d->Nr);
   h->B = batch;
-  h->Nrp = (h->Nr + 63) / 64 * 64;
+  h->Nrp = tgeo ? 1 : (h->Nr + 63) / 64 * 64;
   h->Nt = Nt;
   h->Nh = Nt / 2;
   h->fp64 = d->precision == FIL_FP64;
@@ -625,7 +653,7 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (e != hipSuccess) { delete h; return fail(FIL_EDEVICE, hipGetErrorString(e)); }
   const size_t cz = h->csz(), rz = h->rsz();
   const size_t NS = h->ns(), NP = (size_t)h->Nrp * h->Nh, NR = (size_t)h->rows();
-  bool ok = h->S.alloc(NS * cz) == hipSuccess && h->X.alloc(xyt ? 0 : NS * cz) == hipSuccess &&
+  bool ok = h->S.alloc(NS * cz) == hipSuccess && h->X.alloc(xyt || tgeo ? 0 : NS * cz) == hipSuccess &&
             h->P.alloc(NP * cz) == hipSuccess && h->Et.alloc((size_t)h->Nrp * Nt * cz) == hipSuccess &&
             h->rho_end.alloc(NR * 8) == hipSuccess && h->rho_max.alloc(NR * 8) == hipSuccess &&
             h->aFr.alloc(NR * 8) == hipSuccess && h->aImax.alloc(NR * 8) == hipSuccess &&
@@ -637,7 +665,7 @@ int fil_create(const fil_desc* d, fil_handle** out) {
     ok = h->rho_full.alloc(NR * Nt * rz) == hipSuccess && h->kd_full.alloc(NR * Nt * rz) == hipSuccess;
   if (!ok) { delete h; return fail(FIL_ENOMEM, "device allocation failed"); }
   (void)hipMemset(h->S.p, 0, NS * cz);
-  if (!xyt) (void)hipMemset(h->X.p, 0, NS * cz);
+  if (!xyt && !tgeo) (void)hipMemset(h->X.p, 0, NS * cz);
   (void)hipMemset(h->rho_end.p, 0, NR * 8);
   (void)hipMemset(h->rho_max.p, 0, NR * 8);
   if (d->keep_plasma) {
@@ -699,7 +727,10 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
   h->dt = t->dt;
   h->t.assign(t->t, t->t + Nt);
   std::vector<double> wt(Nr);    // analyzer weights of the rows
-  if (!h->xyt) {
+  if (h->tgeo) {
+    if (!t->KZ) return fail(FIL_EINVAL, "missing linear-propagator table");
+    wt[0] = 1.0;                  // the analyzer's Ft is then I(t)
+  } else if (!h->xyt) {
     if (!t->r || !t->dr || !t->KZ || !t->Kguard || !t->Rguard || !t->dht_T || !t->dht_RJ || !t->dht_JV)
       return fail(FIL_EINVAL, "missing linear-propagator table");
     h->r.assign(t->r, t->r + Nr);
@@ -743,7 +774,20 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
     if (int rc = build_twiddles(h->twy, h->Ny, f64, false)) return rc;
   }
 
-  if (!h->xyt) {
+  if (h->tgeo) {
+    // T: KZ(w) and Wguard(w) (spectral.jl:55-75, Guards.jl:78-89); the linear
+    // step is the diagonal multiply by exp(i KZ dz) Wguard
+    std::vector<double> kz((size_t)Nh * 2), kg(Nh);
+    for (int w = 0; w < Nh; w++) {
+      kz[2 * w] = t->KZ[2 * w];
+      kz[2 * w + 1] = t->KZ[2 * w + 1];
+      kg[w] = t->Wguard[w];
+    }
+    HIPCHK(upload(h->KZ64, kz.data(), kz.size() * 8));
+    HIPCHK(upload(h->KG64, kg.data(), kg.size() * 8));
+    h->P_dz = NAN;
+  }
+  if (!h->xyt && !h->tgeo) {
   // folded QDHT matrices M1 = T diag(RJ), M2 = diag(1/RJ) T, zero padded
   std::vector<double> M1((size_t)Nrp * Nrp, 0.0), M2((size_t)Nrp * Nrp, 0.0);
   for (int m = 0; m < Nr; m++)
@@ -802,7 +846,9 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
     if (int rc = build_twiddles(h->tw, Nt, f64, true)) return rc;
     HostArr tg(Nt, f64, false), rg(Nr, f64, false);
     for (int j = 0; j < Nt; j++) tg.set(j, bug ? 1.0 : t->Tguard[j]);
-    if (!h->xyt) {
+    if (h->tgeo) {
+      rg.set(0, 1.0);             // Guard1D: the field filter is Tguard only
+    } else if (!h->xyt) {
       for (int i = 0; i < Nr; i++) rg.set(i, t->Rguard[i]);
     } else {
       for (int iy = 0; iy < h->Ny; iy++)
@@ -1358,7 +1404,7 @@ int fil_timing_read(fil_handle* h, double* ms_row, double* ms_dht1, double* ms_d
 int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
   if (!valid_handle(h) || !cd) return fail(FIL_EINVAL, "null argument");
   if (h->fp64) return fail(FIL_EINVAL, "the corrector needs an FP32 handle");
-  if (h->xyt) return fail(FIL_EINVAL, "the corrector is defined for the RT geometry");
+  if (h->xyt || h->tgeo) return fail(FIL_EINVAL, "the corrector is defined for the RT geometry");
   if (h->B > 1) return fail(FIL_EINVAL, "the corrector takes single-pulse handles");
   if (!h->tables) return fail(FIL_ESTATE, "fil_set_tables first");
   if (cd->pool_r < 1 || cd->pool_t < 1) return fail(FIL_EINVAL, "pool sizes must be >= 1");

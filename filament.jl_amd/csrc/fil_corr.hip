@@ -90,14 +90,19 @@ __global__ __launch_bounds__(MT) void conv_fwd_kernel(const float* __restrict__ 
   __syncthreads();
   const int wv = tid >> 6, lane = tid & 63, l16 = lane & 15, l4 = lane >> 4;
   const int py = wv >> 1, px = (wv & 1) * 16 + l16;
-  floatx4 acc = {0, 0, 0, 0};
+  // two interleaved accumulation chains (even and odd k steps), fully
+  // unrolled, so the LDS operand reads run ahead of the MFMA chain
+  floatx4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
   const float* ar = wl + l16 * AS + l4;
-#pragma unroll 9
+  const float* xr = tin + l4 * XS + py * HX + px;
+#pragma unroll
   for (int s = 0; s < K / 4; s++) {
-    const int tap = s / (CIN / 4), ci = (4 * s) % CIN + l4;
-    const float xv = tin[ci * XS + (py + tap / 3) * HX + px + tap % 3];
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[4 * s], xv, acc, 0, 0, 0);
+    const int tap = s / (CIN / 4), c4 = (4 * s) % CIN;
+    const float xv = xr[c4 * XS + (tap / 3) * HX + tap % 3];
+    if (s & 1) acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[4 * s], xv, acc1, 0, 0, 0);
+    else acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[4 * s], xv, acc0, 0, 0, 0);
   }
+  const floatx4 acc = acc0 + acc1;
   const int y = y0 + py, x = x0 + px;
   if (y >= H || x >= W) return;
   float* ob = out + (size_t)b * COUT * HW + (size_t)y * W + x;

@@ -372,20 +372,21 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __
       ah[a] = *reinterpret_cast<const bf16x8*>(&As[buf_][0][wm + a * 16 + l16][ks + 8 * l4]);      \
       al[a] = *reinterpret_cast<const bf16x8*>(&As[buf_][1][wm + a * 16 + l16][ks + 8 * l4]);      \
     }                                                                                              \
+    bf16x8 bp[WT][4];                                                                              \
     _Pragma("unroll") for (int b = 0; b < WT; b++) {                                               \
       const int col = ww + b * 16 + l16;                                                           \
-      const bf16x8 rh = *reinterpret_cast<const bf16x8*>(&Bs[buf_][0][col][ks + 8 * l4]);          \
-      const bf16x8 rl = *reinterpret_cast<const bf16x8*>(&Bs[buf_][1][col][ks + 8 * l4]);          \
-      const bf16x8 ih = *reinterpret_cast<const bf16x8*>(&Bs[buf_][2][col][ks + 8 * l4]);          \
-      const bf16x8 il = *reinterpret_cast<const bf16x8*>(&Bs[buf_][3][col][ks + 8 * l4]);          \
-      _Pragma("unroll") for (int a = 0; a < MT; a++) {                                             \
-        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], rh, acc[a][b][0], 0, 0, 0);  \
-        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rl, acc[a][b][0], 0, 0, 0);  \
-        acc[a][b][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], rh, acc[a][b][0], 0, 0, 0);  \
-        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], ih, acc[a][b][1], 0, 0, 0);  \
-        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], il, acc[a][b][1], 0, 0, 0);  \
-        acc[a][b][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], ih, acc[a][b][1], 0, 0, 0);  \
-      }                                                                                            \
+      _Pragma("unroll") for (int q = 0; q < 4; q++)                                                \
+        bp[b][q] = *reinterpret_cast<const bf16x8*>(&Bs[buf_][q][col][ks + 8 * l4]);               \
+    }                                                                                              \
+    /* the three terms in turn over all accumulators: consecutive MFMAs are   */                  \
+    /* independent; each accumulator still sums lo*hi, hi*lo, hi*hi in order  */                  \
+    _Pragma("unroll") for (int term = 0; term < 3; term++)                                         \
+    _Pragma("unroll") for (int b = 0; b < WT; b++)                                                 \
+    _Pragma("unroll") for (int a = 0; a < MT; a++)                                                 \
+    _Pragma("unroll") for (int c = 0; c < 2; c++) {                                                \
+      const bf16x8 av = term == 0 ? al[a] : ah[a];                                                 \
+      const bf16x8 bv = term == 1 ? bp[b][2 * c + 1] : bp[b][2 * c];                               \
+      acc[a][b][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[a][b][c], 0, 0, 0);       \
     }                                                                                              \
   }
   // LDS buffer b holds tile kb while register set b receives tile kb + 2;
@@ -396,17 +397,18 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __
   FIL_BF_GLOAD(1, (1 % nk) * BK);
   FIL_BF_LSTORE(0, 0);
   __syncthreads();
-  for (int kb = 0; kb < nk; kb += 2) {
+  const int nk2 = nk & ~1;
+  for (int kb = 0; kb < nk2; kb += 2) {
     FIL_BF_GLOAD(0, ((kb + 2) % nk) * BK);
     FIL_BF_COMPUTE(0);
     FIL_BF_LSTORE(1, 1);
     __syncthreads();
-    if (kb + 1 >= nk) break;
     FIL_BF_GLOAD(1, ((kb + 3) % nk) * BK);
     FIL_BF_COMPUTE(1);
     FIL_BF_LSTORE(0, 0);
     __syncthreads();
   }
+  if (nk & 1) FIL_BF_COMPUTE(0);   // the last tile of an odd count sits in buffer 0
 #undef FIL_BF_COMPUTE
 #undef FIL_BF_GLOAD
 #undef FIL_BF_LSTORE

@@ -27,6 +27,9 @@ struct RowArgs {
 
 template <typename R>
 hipError_t launch_row(const RowArgs<R>& a, int Nr, int N, bool scan64, hipStream_t st);
+// T geometry linear step: S[i][w] *= P[w], i < rows, w < Nh
+template <typename R>
+hipError_t launch_scale_rows(cx<R>* S, const cx<R>* P, int rows, int Nh, hipStream_t st);
 // The per-member kernels below address spectrum row i (i < Nr) at S row i*rs + ro
 // (rs = batch size, ro = member); Et, Rg and the outputs are indexed by i.
 // Upload / download of one pulse: row i of Et is row (i rs + ro) of S (rs = batch,

@@ -581,6 +581,22 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
   if (tid == 0) { double m = 0; for (int w = 0; w < NTH / 64; w++) m = fmax(m, red[w]); Imax_row[i] = m; }
 }
 
+// Diagonal linear step of the T geometry: S[i][w] *= P[w] for every row.
+template <typename R>
+__global__ void scale_rows_kernel(cx<R>* S, const cx<R>* __restrict__ P, int Nh, size_t n) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) S[q] = S[q] * P[q % Nh];
+}
+
+template <typename R>
+hipError_t launch_scale_rows(cx<R>* S, const cx<R>* P, int rows, int Nh, hipStream_t st) {
+  const size_t n = (size_t)rows * Nh;
+  hipLaunchKernelGGL(scale_rows_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, S, P, Nh, n);
+  return hipGetLastError();
+}
+template hipError_t launch_scale_rows<float>(cx<float>*, const cx<float>*, int, int, hipStream_t);
+template hipError_t launch_scale_rows<double>(cx<double>*, const cx<double>*, int, int, hipStream_t);
+
 // Parseval fluence of each row from its half-spectrum: out[i] = scale * sum_w |S[i][w]|^2.
 template <typename R>
 __global__ __launch_bounds__(256) void rowsum_abs2_kernel(const cx<R>* __restrict__ S, int Nh, double scale,

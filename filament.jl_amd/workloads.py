@@ -111,9 +111,9 @@ def gaussian_ic_xyt(ax, ay, tau0, I0, lam0, noise=0.0, corr=1e-4, seed=0):
 
 
 def xyt_air(Nx, Ny, Nt, power_pcr, xmax=10.0, ymax=10.0, tmin=-250.0, tmax=250.0, ax=3e-3, ay=2e-3, tau0=50e-15,
-            noise=0.05, seed=0, n2=1e-23, nonlinearity=True, plasma=True, dz=2e-4):
+            noise=0.05, seed=0, n2=1e-23, nonlinearity=True, plasma=True, dz=2e-4, raman=False):
     """XYT parameter dict in the reference's input-file names (src/Input.jl:151-178,
-    examples/XYT/input.jl)."""
+    examples/XYT/input.jl); raman as in rt_air."""
     lam0 = 800e-9
     medium = ph.Medium(ph.peck_reeder_air, ph.unit_permeability, n2)
     w0 = 2 * math.pi * ph.C0 / lam0
@@ -132,7 +132,8 @@ def xyt_air(Nx, Ny, Nt, power_pcr, xmax=10.0, ymax=10.0, tmin=-250.0, tmax=250.0
             {"init": init_cubic, "THG": False, "n2": n2},
             {"init": init_current_free, "nuc": nuc, "mr": mr},
             {"init": init_current_losses, "EREAL": False},
-        ],
+        ] + ([{"init": init_raman, "THG": False, "n2": n2, "raman_response": ph.mlejnek_raman_response}]
+             if raman else []),
         plasma_equation={
             "init": init_photoionization_avalanche, "ALG": RK3(), "EREAL": False, "KDEP": False,
             "rho0": 0.0, "rho_nt": 5.4e24, "nuc": 0.0, "mr": mr,

@@ -46,7 +46,7 @@ enum {
   FIL_ENOMEM = 4
 };
 
-enum { FIL_GEOM_RT = 0, FIL_GEOM_XYT = 1 };
+enum { FIL_GEOM_RT = 0, FIL_GEOM_XYT = 1, FIL_GEOM_T = 2 };
 
 enum { FIL_FP32 = 4, FIL_FP64 = 8 };
 
@@ -70,7 +70,7 @@ enum {
  * Filament.run, src/Filament.jl:69-75). */
 typedef struct fil_desc {
   int32_t abi_version;     /* FIL_ABI_VERSION */
-  int32_t geometry;        /* FIL_GEOM_RT */
+  int32_t geometry;        /* FIL_GEOM_RT, FIL_GEOM_XYT or FIL_GEOM_T (Nr ignored: one row) */
   int32_t Nr, Nt;          /* Nt: power of two, 256..8192 (FP64: ..4096) */
   int32_t precision;       /* FIL_FP32 (the reference's FloatGPU) or FIL_FP64 */
   int32_t form;            /* FIL_FORM_* flags */
@@ -202,7 +202,10 @@ int fil_group_destroy(fil_group* g);
 
 /* FieldAnalyzers.analyze! RT (FieldAnalyzers.jl:166-215), including its side
  * effect of projecting the field onto its analytic part.  Fr [Nr], Ft [Nt],
- * rho_end [Nr] and S [Nt/2+1] may be NULL.  Blocks. */
+ * rho_end [Nr] and S [Nt/2+1] may be NULL.  Blocks.
+ * T handles: FieldAnalyzerT (FieldAnalyzers.jl:99-114) in the same struct:
+ * Imax, rhomax, tau = duration = radius(t, I), W = Fmax = F = sum(I) dt, the
+ * other fields 0; Ft = I(t), Fr [1] = F. */
 int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft,
                 double* rho_end, double* S);
 /* fil_analyze for pulse `member` of a batched handle (the projection applies

@@ -149,3 +149,32 @@ def test_raman_steps_match_oracle(pkg, orc, precision, tol, oracle_kz):
     effect = rel_l2(m0.field(), o.field())
     print(f"raman precision={precision} rel_l2={err:.3e} (without Raman {effect:.3e})")
     assert err < tol and effect > 100 * tol
+
+
+def air_plasma(p):
+    """The plasma equation of examples/RT/air.jl:95-120: N2 and O2 with the
+    reference's multiphoton tables, K depending on intensity, avalanche on."""
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference")
+    tab = {g: tuple(np.loadtxt(os.path.join(gold, f"multiphoton_{g}.tf")).T) for g in ("N2", "O2")}
+    q = dict(p)
+    q["plasma_equation"] = dict(p["plasma_equation"], KDEP=True, nuc=5e12, mr=1.0, rho_nt=2.5e25, components=[
+        {"fraction": 0.79, "ionization_energy": 15.576, "ionization_rate": tab["N2"]},
+        {"fraction": 0.21, "ionization_energy": 12.063, "ionization_rate": tab["O2"]}])
+    return q
+
+
+@pytest.mark.parametrize("precision,tol", [(4, 2e-5), (8, 1e-10)])
+def test_plasma_air_avalanche_matches_oracle(pkg, orc, precision, tol):
+    """Two components, intensity-dependent K (tfpower) and avalanche
+    ionization: the device walks each row with the reference's RK3
+    (plasma_equations.jl:60-69) in one lane."""
+    p = air_plasma(small_rt(pkg, power=8.0, nonlinearity=False, plasma=True))
+    o = oracle_for(orc, p)
+    rho_o, kd_o = o.plasma()
+    m = pkg.RTModel(p, precision=precision, keep_plasma=True)
+    rho, kd = m.plasma_arrays()
+    e1, e2 = rel_l2(rho, rho_o), rel_l2(kd, kd_o)
+    print(f"air plasma precision={precision} rho {e1:.3e} kdrho {e2:.3e} max rho {rho_o.max():.3e}")
+    assert rho_o.max() > 0
+    assert e1 < tol and e2 < tol

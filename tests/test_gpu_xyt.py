@@ -17,9 +17,10 @@ import _helpers as H
 pytestmark = pytest.mark.gpu
 
 
-def small(pkg, nonlinearity=True, plasma=True, power=3.0, N=32, Nt=256):
+def small(pkg, nonlinearity=True, plasma=True, power=3.0, N=32, Nt=256, raman=False):
     return pkg.workloads.xyt_air(N, N, Nt, power, xmax=4.0, ymax=4.0, tmin=-120.0, tmax=120.0, ax=0.8e-3,
-                                 ay=0.6e-3, tau0=20e-15, noise=0.05, nonlinearity=nonlinearity, plasma=plasma)
+                                 ay=0.6e-3, tau0=20e-15, noise=0.05, nonlinearity=nonlinearity, plasma=plasma,
+                                 raman=raman)
 
 
 def oracle_for(pkg, orc, p, bug=False):
@@ -123,3 +124,16 @@ def test_xyt_512_cube_energy_and_timing(pkg):
           f"xy transforms {kt['dht1_ms'] / kt['count']:.2f} ms); W {d0['W']:.6e} -> {d['W']:.6e}, "
           f"Imax {d0['Imax']:.4f} -> {d['Imax']:.4f}")
     assert np.isfinite(d["W"]) and 0.99 < d["W"] / d0["W"] <= 1.0 + 1e-6
+
+
+@pytest.mark.parametrize("precision,tol,oracle_vf", [(4, 1e-4, False), (8, 1e-10, True)])
+def test_xyt_raman_steps_match_oracle(pkg, orc, precision, tol, oracle_vf):
+    """The XYT row kernel with the stimulated Raman response."""
+    p = small(pkg, power=6.0, raman=True)
+    o = oracle_for(pkg, orc, p)
+    dz = 2e-4
+    o.run(6, dz)
+    m = run(pkg, p, 6, dz, precision, vf=o.scalars()["vf"] if oracle_vf else None)
+    err = rel(H.xyt_rows(m.field()), o.field())
+    print(f"xyt raman precision={precision}: rel {err:.3e}")
+    assert err < tol

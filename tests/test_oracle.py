@@ -148,6 +148,40 @@ def test_plasma_rk3_closed_form_constant_intensity(orc):
     assert kd[0, -1] == 0.0
 
 
+def test_plasma_avalanche_closed_form_constant_field(orc):
+    """With avalanche (photoionization_avalanche.jl:120-160) and a constant
+    real field E = A: d rho/dt = (f - rho)(W + a rho), a = Rava A^2 / f, a
+    Riccati equation whose solution is rho = f - 1/v,
+    v(t) = a/(W + a f) + (1/f - a/(W + a f)) exp((W + a f) t).  Rava is the
+    inverse-bremsstrahlung rate sigma_B I / Ui written through the cycle
+    average Re(E)^2 = |E|^2 / 2 and I = (n0 eps0 c / 2) Eu^2 |E|^2:
+    sigma_B = e^2 nu / (n0 c eps0 m (nu^2 + w0^2)) gives Rava = e^2 nu /
+    (m (nu^2 + w0^2)) Eu^2 / Ui in 1/s, times tu.  The RK3 solution converges
+    at third order."""
+    A, nu = 10.0, 5e12
+    e, me, eps0, c0 = 1.602176634e-19, 9.1093837015e-31, 8.8541878128e-12, 299792458.0
+    errs = []
+    for Nt in (256, 512, 1024):
+        o = orc.Oracle(_power_law_cfg(Nr=2, Nt=Nt, tmin=-100.0, tmax=100.0, pe_nuc=nu), np.zeros((2, Nt)))
+        o.set_field(np.full((2, Nt), A + 0j))
+        o.plasma_solve()
+        rho, _ = o.plasma()
+        s = o.scalars()
+        n0, Eu, w0 = s["n0"], s["Eu"], s["w0"]
+        W = 2.9e-131 * (A * A * 1e16) ** 8 * 1e-15
+        Rava = e ** 2 * nu / (me * (nu ** 2 + w0 ** 2)) * Eu ** 2 / (12.063 * e) * 1e-15
+        f = 5.4e24 / 2.5e25
+        a = Rava * A * A / f
+        t = s["dt"] * np.arange(Nt)
+        lam = W + a * f
+        v = a / lam + (1 / f - a / lam) * np.exp(lam * t)
+        ref = f - 1 / v
+        assert ref[-1] > 0.5 * f and a * f > 0.2 * W      # both terms matter
+        errs.append(np.max(np.abs(rho[0] - ref)) / f)
+        assert n0 > 1.0 and abs(Eu ** 2 * n0 * eps0 * c0 / 2 - 1e16) < 1e-6 * 1e16    # Eu from Iu = 1e16 W/m^2
+    assert errs[0] < 1e-4 and errs[0] / errs[1] > 6 and errs[1] / errs[2] > 6
+
+
 def test_plasma_converges_to_exact_solution(orc):
     """rho = f (1 - exp(-int W dt)) for a smooth carrier-free Gaussian field:
     second-order convergence (linear interpolation of E at the RK3 midpoints)."""
