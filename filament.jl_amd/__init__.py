@@ -6,11 +6,11 @@ construction, the run driver and plot.dat output.  Device side
 (csrc/, built into libfilament_hip.so): the z-step and the field analyzer.
 """
 from . import lib, physics, responses, workloads  # noqa: F401
-from .solver import RTEnsemble, RTModel, StepGroup, XYTModel  # noqa: F401
+from .solver import RTEnsemble, RTModel, StepGroup, TModel, XYTModel  # noqa: F401
 from . import corrector, driver, stepping  # noqa: F401
 from .stepping import CoarseStepper  # noqa: F401
 from .driver import prepare, run  # noqa: F401
 from .responses import (RK3, init_cubic, init_current_free, init_current_losses,  # noqa: F401
                         init_photoionization_avalanche, init_raman)
 
-__all__ = ["RTModel", "RTEnsemble", "StepGroup", "XYTModel", "CoarseStepper", "lib", "physics", "responses", "workloads"]
+__all__ = ["RTModel", "RTEnsemble", "StepGroup", "TModel", "XYTModel", "CoarseStepper", "lib", "physics", "responses", "workloads"]

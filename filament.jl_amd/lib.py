@@ -14,7 +14,7 @@ LIBPATH = os.path.join(HERE, "libfilament_hip.so")
 
 ABI_VERSION = 3
 MAXCOMP = 4
-GEOM_RT, GEOM_XYT = 0, 1
+GEOM_RT, GEOM_XYT, GEOM_T = 0, 1, 2
 FP32, FP64 = 4, 8
 FORM_DHT_BF16X3, FORM_SCAN_FP64 = 1, 2
 RESP_CUBIC_THG, RESP_CUBIC_NOTHG, RESP_FREE, RESP_LOSSES = 1, 2, 4, 8

@@ -35,6 +35,12 @@ class UnitXYT:
         self.z, self.t, self.w, self.lam, self.I, self.rho = zu, tu, 1.0 / tu, tu, Iu, rhou
 
 
+class UnitT:
+    """Units.jl:47-61 (T): time, intensity and density units only."""
+    def __init__(self, zu, tu, Iu, rhou):
+        self.z, self.t, self.w, self.lam, self.I, self.rho = zu, tu, 1.0 / tu, tu, Iu, rhou
+
+
 def field_unit(unit, n0):
     """Units.E (Units.jl:339-343)."""
     return math.sqrt(unit.I / (n0 * EPS0 * C0 / 2))
@@ -61,6 +67,15 @@ class GridRT:
         self.r = j[:Nr] * rmax / j[Nr]
         self.k = j[:Nr] / rmax
         self.dr = _step(self.r)
+        self.tmin, self.tmax, self.Nt = tmin, tmax, Nt
+        self.t = np.linspace(tmin, tmax, Nt)
+        self.dt = (tmax - tmin) / (Nt - 1)
+        self.w = 2 * np.pi * np.fft.fftfreq(Nt, self.dt)
+
+
+class GridT:
+    """Grids.jl:78-91: t = range(tmin, tmax, Nt), w = 2 pi fftfreq(Nt, 1/dt)."""
+    def __init__(self, tmin, tmax, Nt):
         self.tmin, self.tmax, self.Nt = tmin, tmax, Nt
         self.t = np.linspace(tmin, tmax, Nt)
         self.dt = (tmax - tmin) / (Nt - 1)
@@ -188,6 +203,13 @@ class GuardRT:
         self.Wguard = np.exp(-(((grid.w * unit.w) ** 2 / wguard ** 2) ** 20))
 
 
+class GuardT:
+    """Guards.jl:78-89 (Guard1D): the field window T(t) and the spectral W(w)."""
+    def __init__(self, unit, grid, tguard, wguard):
+        self.Tguard = guard_window_both(grid.t, tguard)
+        self.Wguard = np.exp(-(((grid.w * unit.w) ** 2 / wguard ** 2) ** 20))
+
+
 class GuardXYT:
     """Guards.jl:162-195: field windows X(x), Y(y), T(t); the spectral KX, KY
     guards are ones in the reference (its kmax lines are commented out,
@@ -239,6 +261,15 @@ def kz_table(unit, grid, medium, w0, paraxial):
     else:
         K = np.sqrt(beta ** 2 - kt ** 2 + 0j)
     return K * unit.z - (w / vf * unit.z)[None, :]
+
+
+def kz_table_t(unit, grid, medium, w0):
+    """KZ T (spectral.jl:55-75): kt = 0 and paraxial (Input.jl:285-291 forces
+    it for T), i.e. (beta(w) - w / vf) zu."""
+    vf = medium.group_velocity(w0)
+    w = grid.w * unit.w
+    beta = np.asarray(medium.beta(w), dtype=complex)
+    return beta * unit.z - w / vf * unit.z
 
 
 # ---------------------------------------------------------------------------

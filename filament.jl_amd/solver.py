@@ -337,6 +337,55 @@ class RTEnsemble(RTModel):
         return [self.analyze(z, member=b) for b in range(self.B)]
 
 
+class TModel(_Model):
+    """The T geometry (Fields.jl:70-93, Guards.jl:78-89, spectral.jl:55-75,
+    FieldAnalyzers.jl:74-114): one time row, the linear step a diagonal
+    multiply by exp(i KZ dz) Wguard.  ``p`` carries ``zu, tu, Iu, rhou, tmin,
+    tmax, Nt, tguard, wguard`` and ``initial_condition(t, tu, Iu)``."""
+
+    def __init__(self, p, precision=L.FP64, form=0, device=0, keep_plasma=False, basedir=".", KZ=None):
+        """``KZ`` overrides the host-computed KZ(w) table (Nt complex)."""
+        if form & L.FORM_DHT_BF16X3:
+            raise ValueError("the T geometry has no Hankel transform")
+        self.p = p
+        self.unit = u = ph.UnitT(p["zu"], p["tu"], p["Iu"], p["rhou"])
+        self.grid = g = ph.GridT(p["tmin"], p["tmax"], p["Nt"])
+        self.nrows = 1
+        self.precision, self.form = precision, form
+        self._physics(p, basedir)
+        self.guard = ph.GuardT(u, g, p["tguard"], p["wguard"])
+        self.KZ = ph.kz_table_t(u, g, self.medium, self.w0) if KZ is None else np.asarray(KZ).ravel()
+        E0 = np.asarray(p["initial_condition"](g.t, u.t, u.I), dtype=float)
+        self.E0 = ph.rsig2asig(E0[None, :])
+        d = self._desc(L.GEOM_T, precision, form, keep_plasma, device)
+        d.Nr = 1
+        self._create(d)
+        self._set_tables()
+        self.upload(self.E0)
+
+    def _from_rows(self, A):
+        return A[0]
+
+    def _to_rows(self, E):
+        return np.asarray(E).reshape(1, self.grid.Nt)
+
+    def _set_tables(self):
+        keep = []
+        t, arr = self._common_tables(keep)
+        t.KZ = arr(self.KZ, True)
+        L.check(self._lib.fil_set_tables(self.h, ctypes.byref(t)))
+
+    def analyze(self, z=0.0, arrays=False):
+        """FieldAnalyzerT: Imax, rhomax, duration = radius(t, I), F = sum(I) dt."""
+        d = L.FilDiag()
+        It = np.zeros(self.grid.Nt)
+        L.check(self._lib.fil_analyze(self.h, float(z), ctypes.byref(d), None, L.ptr(It), None, None))
+        out = dict(z=d.z, Imax=d.Imax, rhomax=d.rhomax, duration=d.tau, F=d.W)
+        if arrays:
+            out["I"] = It
+        return out
+
+
 class XYTModel(_Model):
     def __init__(self, p, precision=L.FP32, form=0, device=0, keep_plasma=False, basedir=".", guard3d_bug=False,
                  vf=None):

@@ -193,3 +193,31 @@ def group_by_grid(members):
     for k, p in enumerate(members):
         groups.setdefault((p["Nr"], p["Nt"]), []).append(k)
     return list(groups.items())
+
+
+def t_air(Nt, I0=1e16, tmin=-100.0, tmax=100.0, tau0=20e-15, nonlinearity=True, plasma=True, n2=1e-23,
+          tguard=10.0, dz=1e-2, raman=False):
+    """T-geometry parameter dict (test/dispersion/T/input.jl, gauss.jl): a
+    Gaussian pulse sqrt(I0/Iu) exp(-t^2 / (2 tau0^2)) cos(w0 t) in air with the
+    responses and plasma of rt_air."""
+    lam0 = 800e-9
+    nuc, mr = 5e12, 1.0
+
+    def initial_condition(t, tu, Iu):
+        w0 = 2 * math.pi * ph.C0 / lam0
+        tt = np.asarray(t) * tu
+        return math.sqrt(I0 / Iu) * np.exp(-0.5 * tt ** 2 / tau0 ** 2) * np.cos(w0 * tt)
+
+    q = rt_air(2, Nt, 1.0, tmin=tmin, tmax=tmax, n2=n2, nonlinearity=nonlinearity, plasma=plasma, dz=dz, raman=raman)
+    p = {k: v for k, v in q.items() if k not in ("ru", "rmax", "Nr", "rguard", "kguard", "I0", "Pcr", "power_pcr")}
+    p.update(geometry="T", tguard=tguard, initial_condition=initial_condition, I0=I0, nuc=nuc, mr=mr)
+    return p
+
+
+def t_dispersion():
+    """test/dispersion/T/input.jl: linear, Nt = 2048 on [-100, 100] fs, a 20 fs
+    pulse at 1e12 W/cm^2, 100 steps of 0.2 m."""
+    p = t_air(2048, I0=1e16, tmin=-100.0, tmax=100.0, tau0=20e-15, nonlinearity=False, plasma=False,
+              tguard=10.0, dz=0.2)
+    p["nsteps"] = 100
+    return p

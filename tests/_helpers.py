@@ -64,6 +64,13 @@ def oracle_config(p):
     )
 
 
+def oracle_config_t(p):
+    """The same T problem as an oracle configuration dict (tests only)."""
+    c = oracle_config(dict(p, rmax=0.0, Nr=0, ru=1.0, rguard=0.0, kguard=0.0))
+    c.update(geometry=1, Nr=0)
+    return c
+
+
 def oracle_config_xyt(p, guard3d_bug=False):
     """The same XYT problem as an oracle configuration dict (tests only)."""
     c = oracle_config(dict(p, rmax=0.0, Nr=0, ru=p["xu"], rguard=0.0, kguard=p["kxguard"]))

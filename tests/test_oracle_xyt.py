@@ -141,3 +141,26 @@ def test_xyt_nonlinear_step_agrees_with_rt(pkg, orc):
     assert gi_r > 0.01 and abs(gi_x - gi_r) < 0.1 * gi_r
     assert gr_r > 0.05 and abs(gr_x - gr_r) < 0.1 * gr_r
     assert abs(a["W"] / a0["W"] - 1) < 1e-6 and abs(b["W"] / b0["W"] - 1) < 1e-6
+
+
+def test_t_mode_reproduces_reference_dispersion_test(pkg, orc):
+    """The oracle's T geometry (spectral.jl:55-75, Guards.jl:78-89,
+    FieldAnalyzers.jl:99-114) on the reference's test/test_dispersion_t.jl
+    case: tau(z) = tau0 sqrt(1 + (z/zd)^2), zd = tau0^2/k2 with the golden k2,
+    within the reference's tolerances (0.1 fs, 1e-3); the host's KZ(w) agrees
+    with the oracle's to the frame-velocity rounding."""
+    p = pkg.workloads.t_dispersion()
+    g = pkg.physics.GridT(p["tmin"], p["tmax"], p["Nt"])
+    E0 = p["initial_condition"](g.t, p["tu"], p["Iu"])
+    o = orc.Oracle(H.oracle_config_t(p), E0[None, :], 8)
+    rows = o.run(p["nsteps"], p["dz_initial"])
+    k2, tau0 = 2.131584978375e-29, 20e-15
+    zd = tau0 ** 2 / k2
+    for d in rows[1:]:
+        tauth = tau0 * np.sqrt(1 + (d["z"] / zd) ** 2)
+        assert abs(d["tau"] * 1e-15 - tauth) <= 0.1e-15
+        assert abs(d["Imax"] / rows[0]["Imax"] - tau0 / tauth) <= 1e-3
+    ph = pkg.physics
+    kz = ph.kz_table_t(ph.UnitT(p["zu"], p["tu"], p["Iu"], p["rhou"]), g,
+                       ph.Medium(p["permittivity"], p["permeability"], p["n2"]), 2 * np.pi * ph.C0 / p["lam0"])
+    assert np.max(np.abs(kz - o.KZ().ravel())) <= 1e-8 * np.max(np.abs(kz))
