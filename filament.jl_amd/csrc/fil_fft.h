@@ -49,10 +49,32 @@ struct Dft {
     Dft<T, H, SIGN>::run(o);
 #pragma unroll
     for (int k = 0; k < H; k++) {
-      cx<T> w;
-      if (k == 0) w = mk((T)1, (T)0);
-      else w = mk((T)c16(k * (16 / RAD)), (T)(SIGN * s16(k * (16 / RAD))));
-      cx<T> t = (k == 0) ? o[k] : w * o[k];
+      const cx<T> t = twiddle16(k * (16 / RAD), o[k]);
+      a[k] = e[k] + t;
+      a[k + H] = e[k] - t;
+    }
+  }
+  // o * exp(SIGN 2 pi i m / 16); m = 0 and m = 4 (a factor SIGN i) need no multiply
+  static __device__ __forceinline__ cx<T> twiddle16(int m, cx<T> o) {
+    if (m == 0) return o;
+    if (m == 4) return (SIGN > 0) ? mul_i(o) : mul_mi(o);
+    return mk((T)c16(m), (T)(SIGN * s16(m))) * o;
+  }
+  // the same DFT when a[j] = 0 for j >= RAD/2 (the upper half of the input is
+  // not read): both half-size DFTs see only their lower half
+  static __device__ __forceinline__ void run_half(cx<T>* a) {
+    constexpr int H = RAD / 2;
+    cx<T> e[H], o[H];
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+      e[j] = (j < H / 2) ? a[2 * j] : mk((T)0, (T)0);
+      o[j] = (j < H / 2) ? a[2 * j + 1] : mk((T)0, (T)0);
+    }
+    Dft<T, H, SIGN>::run_half(e);
+    Dft<T, H, SIGN>::run_half(o);
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+      const cx<T> t = twiddle16(k * (16 / RAD), o[k]);
       a[k] = e[k] + t;
       a[k + H] = e[k] - t;
     }
@@ -65,6 +87,7 @@ struct Dft<T, 2, SIGN> {
     a[0] = t + a[1];
     a[1] = t - a[1];
   }
+  static __device__ __forceinline__ void run_half(cx<T>* a) { a[1] = a[0]; }
 };
 template <typename T, int SIGN>
 struct Dft<T, 4, SIGN> {
@@ -75,6 +98,15 @@ struct Dft<T, 4, SIGN> {
     a[1] = amc + jb;
     a[2] = apc - bpd;
     a[3] = amc - jb;
+  }
+  // a[2] = a[3] = 0
+  static __device__ __forceinline__ void run_half(cx<T>* a) {
+    const cx<T> jb = (SIGN > 0) ? mul_i(a[1]) : mul_mi(a[1]);
+    const cx<T> a0 = a[0], a1 = a[1];
+    a[0] = a0 + a1;
+    a[1] = a0 + jb;
+    a[2] = a0 - a1;
+    a[3] = a0 - jb;
   }
 };
 
@@ -198,6 +230,103 @@ __device__ __forceinline__ void fft_lds(cx<T>* z, const cx<T>* __restrict__ tw, 
 template <typename T, int N, int NTH, int SIGN>
 __device__ __forceinline__ void fft_lds(cx<T>* z, const cx<T>* __restrict__ tw) {
   fft_lds<T, N, NTH, SIGN>(z, tw, (int)threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Transforms whose input or output stays in registers (NTH = N/8, N >= 512).
+// With N/8 threads the first pass (radix 8, stride 1) reads x[tid + k N/8]
+// and the last pass writes y[tid + j N/8], j = 0..7: exactly the elements
+// t = tid + j NTH that the elementwise code of the row kernel owns.  Taking
+// the input from registers saves the LDS store + barrier + load in front of
+// the first pass; leaving the output in registers saves the store + barrier +
+// load behind the last one.
+// ---------------------------------------------------------------------------
+template <int N, int NTH>
+__host__ __device__ constexpr bool fft_regs_ok() { return NTH * 8 == N && N >= 512; }
+
+// stride of the last pass
+__host__ __device__ constexpr int fft_last_stride(int N) {
+  int s = 1;
+  while (s * fft_radix(N, s) < N) s *= fft_radix(N, s);
+  return s;
+}
+
+// First pass from registers: in[k] = x[tid + k N/8]; HALF: x[i] = 0 for
+// i >= N/2 (in[4..7] are not read).  The caller guarantees that no thread
+// still reads z; ends with a barrier.
+template <typename T, int N, int NTH, int SIGN, bool HALF>
+__device__ __forceinline__ void fft_first_pass_regs(const cx<T>* in, cx<T>* z, const cx<T>* __restrict__ tw, int tid) {
+  static_assert(fft_regs_ok<N, NTH>(), "register passes need NTH = N/8");
+  asm volatile("" : "+v"(tid));
+  cx<T> a[8], w[8];
+  const cx<T>* t = tw + fft_tw_offset(N, 1) + tid * FIL_TW_STRIDE;
+#pragma unroll
+  for (int k = 1; k < 8; k++) w[k] = t[k];
+#pragma unroll
+  for (int k = 0; k < 8; k++) a[k] = (HALF && k >= 4) ? mk((T)0, (T)0) : in[k];
+  if constexpr (HALF) Dft<T, 8, SIGN>::run_half(a);
+  else Dft<T, 8, SIGN>::run(a);
+#pragma unroll
+  for (int k = 1; k < 8; k++) a[k] = a[k] * ((SIGN > 0) ? conj(w[k]) : w[k]);
+  cx<T>* zw = z + lds_index(8 * tid);
+#pragma unroll
+  for (int k = 0; k < 8; k++) zw[lds_step(k)] = a[k];
+  __syncthreads();
+}
+
+// the passes between the first (stride 1) and the last
+template <typename T, int N, int NTH, int SIGN, int S>
+struct FftMidPasses {
+  static __device__ __forceinline__ void run(cx<T>* z, const cx<T>* tw, int tid) {
+    if constexpr (S < fft_last_stride(N)) {
+      stockham_pass<T, N, NTH, SIGN, fft_radix(N, S), S>(z, tw, tid);
+      FftMidPasses<T, N, NTH, SIGN, S * fft_radix(N, S)>::run(z, tw, tid);
+    }
+  }
+};
+
+// Last pass into registers: out[j] = y[tid + j N/8].  Reads z, barriers (z is
+// free afterwards), then transforms.
+template <typename T, int N, int NTH, int SIGN>
+__device__ __forceinline__ void fft_last_pass_regs(const cx<T>* z, cx<T>* out, int tid) {
+  static_assert(fft_regs_ok<N, NTH>(), "register passes need NTH = N/8");
+  constexpr int S = fft_last_stride(N);
+  constexpr int RAD = N / S;
+  constexpr int NB = N / RAD;          // = S
+  constexpr int BPT = NB / NTH;
+  asm volatile("" : "+v"(tid));
+  cx<T> a[BPT][RAD];
+#pragma unroll
+  for (int u = 0; u < BPT; u++) {
+    const cx<T>* zr = z + lds_index(tid + u * NTH);
+#pragma unroll
+    for (int k = 0; k < RAD; k++) a[u][k] = zr[lds_step(k * NB)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < BPT; u++) {
+    Dft<T, RAD, SIGN>::run(a[u]);
+    // output q + S k with q = tid + u NTH: element tid + NTH (u + BPT k)
+#pragma unroll
+    for (int k = 0; k < RAD; k++) out[u + BPT * k] = a[u][k];
+  }
+}
+
+// registers -> z (natural order, like fft_lds)
+template <typename T, int N, int NTH, int SIGN, bool HALF>
+__device__ __forceinline__ void fft_regs_to_lds(const cx<T>* in, cx<T>* z, const cx<T>* __restrict__ tw, int tid) {
+  fft_first_pass_regs<T, N, NTH, SIGN, HALF>(in, z, tw, tid);
+  FftMidPasses<T, N, NTH, SIGN, 8>::run(z, tw, tid);
+  constexpr int S = fft_last_stride(N);
+  stockham_pass<T, N, NTH, SIGN, N / S, S>(z, tw, tid);
+}
+// registers -> registers, z as the exchange buffer (free on return)
+template <typename T, int N, int NTH, int SIGN, bool HALF>
+__device__ __forceinline__ void fft_regs_to_regs(const cx<T>* in, cx<T>* out, cx<T>* z, const cx<T>* __restrict__ tw,
+                                                 int tid) {
+  fft_first_pass_regs<T, N, NTH, SIGN, HALF>(in, z, tw, tid);
+  FftMidPasses<T, N, NTH, SIGN, 8>::run(z, tw, tid);
+  fft_last_pass_regs<T, N, NTH, SIGN>(z, out, tid);
 }
 
 }  // namespace fil

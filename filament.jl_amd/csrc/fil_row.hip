@@ -188,11 +188,18 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
   constexpr int C = N / NTH;   // contiguous chunk of t per thread
   const int j0 = tid * C;
   const A dtA = (A)dt;
+  // One component: pass 1 leaves the gain h_j of every sample (rounded to R,
+  // and used rounded by both passes, so the scan stays consistent) in rho[j+1]
+  // and K W1(t_j) in kd[j], the slots pass 2 overwrites with rho and kdrho
+  // (each thread owns the slots of its chunk in both passes); pass 2 then
+  // evaluates no rates.
+  const bool cache = (nc == 1);
   // pass 1: this thread's composed affine map for each component
   Aff<A> m[FIL_MAXCOMP];
 #pragma unroll
   for (int q = 0; q < FIL_MAXCOMP; q++) { m[q].a = (A)1; m[q].b = (A)0; }
   R Wprev[FIL_MAXCOMP];
+  R Pprev = 0;
   bool have_prev = false;
   for (int c = 0; c < C; c++) {
     const int j = j0 + c;
@@ -200,15 +207,24 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
     cx<R> E0 = Z[lds_index(j)], E1 = Z[lds_index(j + 1)];
     cx<R> Em = (R)0.5 * (E0 + E1);
     R I0 = fiarg(E0, pp.ereal), Im = fiarg(Em, pp.ereal), I1 = fiarg(E1, pp.ereal);
+    const bool slope = cache && pp.kdep;
 #pragma unroll
     for (int q = 0; q < FIL_MAXCOMP; q++) {
       if (q >= nc) break;
-      R W1 = have_prev ? Wprev[q] : tf_eval<R>(pp, q, I0, nullptr);
+      R pw1 = Pprev, pw3 = 0;
+      R W1 = have_prev ? Wprev[q] : tf_eval<R>(pp, q, I0, slope ? &pw1 : nullptr);
       R W2 = tf_eval<R>(pp, q, Im, nullptr);
-      R W3 = tf_eval<R>(pp, q, I1, nullptr);
+      R W3 = tf_eval<R>(pp, q, I1, slope ? &pw3 : nullptr);
       Wprev[q] = W3;
+      Pprev = pw3;
       A w1 = W1, w2 = W2, w3 = W3;
-      const A h = rk3_gain<A>(dtA, w1, w2, w3);
+      A h = rk3_gain<A>(dtA, w1, w2, w3);
+      if (cache) {
+        const R hr = (R)h;
+        h = (A)hr;
+        rho[rk_index<R>(j + 1)] = hr;
+        kd[rk_index<R>(j)] = (pp.kdep ? pw1 : (R)pp.K[0]) * W1;
+      }
       Aff<A> st; st.a = (A)1 - h; st.b = h * (A)pp.frhont[q];
       m[q] = compose(st, m[q]);
     }
@@ -227,6 +243,22 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
     rho[rk_index<R>(0)] = s0;
   }
   // pass 2: walk the chunk again
+  if (cache) {
+    const A f = (A)pp.frhont[0];
+    A r0 = r[0];
+    for (int c = 0; c < C; c++) {
+      const int j = j0 + c;
+      if (j >= N - 1) break;
+      const A h = (A)rho[rk_index<R>(j + 1)];
+      const R kw1 = kd[rk_index<R>(j)];
+      r0 = ((A)1 - h) * r0 + h * f;
+      rho[rk_index<R>(j + 1)] = (R)r0;
+      kd[rk_index<R>(j)] = kw1 * (R)(f - r0);
+    }
+    if (tid == NTH - 1) kd[rk_index<R>(N - 1)] = 0;
+    __syncthreads();
+    return;
+  }
   have_prev = false;
   for (int c = 0; c < C; c++) {
     const int j = j0 + c;
@@ -258,8 +290,11 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
 }
 
 // ---------------------------------------------------------------------------
-// Responses of one RK stage.  Z holds the stage field in time (LDS); on exit
-// k[m] (spectral bins w = tid + NTH*m, w < N/2) holds func!'s dE.
+// Responses of one RK stage.  ev[u] holds the stage field in time at
+// t = tid + u NTH (registers); on exit k[m] (spectral bins w = tid + NTH*m,
+// w < N/2) holds func!'s dE.  Z (LDS) is the transforms' work buffer: no
+// thread may still read it on entry, except the caller's own reads of the
+// elements it passes in ev when the register passes are not used.
 //   F_cubic = 3/4 |E|^2 Re E  or Re(E)^3          cubic.jl:36,44
 //   F_free  = rho Re E                             current_free.jl:35
 //   F_loss  = kdrho Re E / |E|^2 (0 below 1e-30)   current_losses.jl:131-133, :71-83
@@ -275,7 +310,8 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
 // small one, whose coefficient is correspondingly huge.
 // ---------------------------------------------------------------------------
 template <typename R, int N, int NTH, bool RAMAN>
-__device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, const cx<R>* __restrict__ tw,
+__device__ __forceinline__ void stage_rhs(cx<R>* Z, const cx<R>* ev, const R* rho, const R* kd,
+                                          const cx<R>* __restrict__ tw,
                                           const cx<R>* __restrict__ coef, const cx<R>* __restrict__ hr,
                                           const R* sc, R g_r, const R* __restrict__ Tg, int resp, int losses_ereal,
                                           cx<R>* k) {
@@ -291,11 +327,13 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
   // the Raman variant keeps Re(E) and the Raman intensity of its samples
   // across the convolution; the kernel without Raman does not hold them
   constexpr int TR = RAMAN ? TP : 1;
+  constexpr bool REGS = fft_regs_ok<N, NTH>();
   R Fl[TP], Re[TR], Ir[TR];
+  cx<R> zin[TP];
 #pragma unroll
   for (int u = 0; u < TP; u++) {
     const int t = tid + u * NTH;
-    cx<R> e = Z[lds_index(t)];
+    const cx<R> e = ev[u];
     const R re = e.x, I = norm2(e);
     const R g = g_r * Tg[t];
     R fc = 0, ff = 0, fl = 0;
@@ -311,13 +349,21 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
       Re[u] = re;
       Ir[u] = ram_thg ? re * re : (R)0.75 * I;    // calc_raman / calc_raman_nothg (raman.jl:46-65)
     }
-    Z[lds_index(t)] = mk((g * sc[0]) * fc, (g * sc[1]) * ff);
+    zin[u] = mk((g * sc[0]) * fc, (g * sc[1]) * ff);
   }
-  __syncthreads();
+  // without register passes the caller's reads of ev came from Z
+  if constexpr (!REGS) __syncthreads();
 #pragma unroll
   for (int m = 0; m < SP; m++) k[m] = mk((R)0, (R)0);
   if (cub || fre) {
-    fft_lds<R, N, NTH, +1>(Z, tw);
+    if constexpr (REGS) {
+      fft_regs_to_lds<R, N, NTH, +1, false>(zin, Z, tw, tid);
+    } else {
+#pragma unroll
+      for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = zin[u];
+      __syncthreads();
+      fft_lds<R, N, NTH, +1>(Z, tw);
+    }
 #pragma unroll
     for (int m = 0; m < SP; m++) {
       const int w = tid + m * NTH;
@@ -331,25 +377,41 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const R* rho, const R* kd, c
     // Raman: F = conv(I, H) Re(E) with conv = FFT-(hr .* FFT+(I)), hr = Hraman / Nt
     // (convolution!, raman.jl:68-111).  H comes from a real response, so the
     // convolution of the real I is real; its real part is kept.
+    if constexpr (REGS) {
 #pragma unroll
-    for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = mk(Ir[u], (R)0);
-    __syncthreads();
-    fft_lds<R, N, NTH, +1>(Z, tw);
-    for (int j = tid; j < N; j += NTH) Z[lds_index(j)] = hr[j] * Z[lds_index(j)];
-    __syncthreads();
-    fft_lds<R, N, NTH, -1>(Z, tw);
+      for (int u = 0; u < TP; u++) zin[u] = mk(Ir[u], (R)0);
+      fft_regs_to_regs<R, N, NTH, +1, false>(zin, zin, Z, tw, tid);
+#pragma unroll
+      for (int u = 0; u < TP; u++) zin[u] = hr[tid + u * NTH] * zin[u];
+      fft_regs_to_regs<R, N, NTH, -1, false>(zin, zin, Z, tw, tid);
+    } else {
+#pragma unroll
+      for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = mk(Ir[u], (R)0);
+      __syncthreads();
+      fft_lds<R, N, NTH, +1>(Z, tw);
+      for (int j = tid; j < N; j += NTH) Z[lds_index(j)] = hr[j] * Z[lds_index(j)];
+      __syncthreads();
+      fft_lds<R, N, NTH, -1>(Z, tw);
+#pragma unroll
+      for (int u = 0; u < TP; u++) zin[u] = Z[lds_index(tid + u * NTH)];
+      __syncthreads();
+    }
     // losses and Raman share one complex FFT
 #pragma unroll
     for (int u = 0; u < TP; u++) {
       const int t = tid + u * NTH;
       const R g = g_r * Tg[t];
-      Ir[u] = (g * sc[3]) * (Z[lds_index(t)].x * Re[u]);
+      Ir[u] = (g * sc[3]) * (zin[u].x * Re[u]);
+      zin[u] = mk(Fl[u], Ir[u]);
     }
-    __syncthreads();
+    if constexpr (REGS) {
+      fft_regs_to_lds<R, N, NTH, +1, false>(zin, Z, tw, tid);
+    } else {
 #pragma unroll
-    for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = mk(Fl[u], Ir[u]);
-    __syncthreads();
-    fft_lds<R, N, NTH, +1>(Z, tw);
+      for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = zin[u];
+      __syncthreads();
+      fft_lds<R, N, NTH, +1>(Z, tw);
+    }
 #pragma unroll
     for (int m = 0; m < SP; m++) {
       const int w = tid + m * NTH;
@@ -391,6 +453,21 @@ __device__ __forceinline__ void store_padded(cx<R>* Z, const cx<R>* v) {
   __syncthreads();
 }
 
+// e[u] <- time-domain field at t = tid + u NTH of the half spectrum U
+// (bins tid + m NTH, m < N/2/NTH; the negative frequencies are zero).  Z must
+// be free on entry and is free on return.
+template <typename R, int N, int NTH>
+__device__ __forceinline__ void stage_field(const cx<R>* U, cx<R>* e, cx<R>* Z, const cx<R>* __restrict__ tw) {
+  if constexpr (fft_regs_ok<N, NTH>()) {
+    fft_regs_to_regs<R, N, NTH, -1, true>(U, e, Z, tw, (int)threadIdx.x);
+  } else {
+    store_padded<R, N, NTH>(Z, U);
+    fft_lds<R, N, NTH, -1>(Z, tw);
+#pragma unroll
+    for (int u = 0; u < N / NTH; u++) e[u] = Z[lds_index(threadIdx.x + u * NTH)];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The row kernel.  One workgroup per r row; LDS: Z (N complex), rho, kd (N R).
 // ---------------------------------------------------------------------------
@@ -412,21 +489,37 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
 #pragma unroll
   for (int m = 0; m < SP; m++) S0[m] = Srow[tid + m * NTH];
 
+  constexpr bool REGS = fft_regs_ok<N, NTH>();
+  constexpr int TP = N / NTH;
+  cx<R> e[TP];   // time-domain field at t = tid + u NTH
   if (a.apply_filter) {
     // field filter deferred from the end of the previous zstep
-    store_padded<R, N, NTH>(Z, S0);
-    fft_lds<R, N, NTH, -1>(Z, a.tw);
-    for (int t = tid; t < N; t += NTH) Z[lds_index(t)] = (g_r * a.Tg[t]) * Z[lds_index(t)];
-    __syncthreads();
-    fft_lds<R, N, NTH, +1>(Z, a.tw);
     const R inv = (R)1 / (R)N;
+    if constexpr (REGS) {
+      fft_regs_to_regs<R, N, NTH, -1, true>(S0, e, Z, a.tw, tid);
 #pragma unroll
-    for (int m = 0; m < SP; m++) S0[m] = inv * Z[lds_index(tid + m * NTH)];
-    __syncthreads();
+      for (int u = 0; u < TP; u++) e[u] = (g_r * a.Tg[tid + u * NTH]) * e[u];
+      fft_regs_to_regs<R, N, NTH, +1, false>(e, e, Z, a.tw, tid);
+#pragma unroll
+      for (int m = 0; m < SP; m++) S0[m] = inv * e[m];
+    } else {
+      store_padded<R, N, NTH>(Z, S0);
+      fft_lds<R, N, NTH, -1>(Z, a.tw);
+      for (int t = tid; t < N; t += NTH) Z[lds_index(t)] = (g_r * a.Tg[t]) * Z[lds_index(t)];
+      __syncthreads();
+      fft_lds<R, N, NTH, +1>(Z, a.tw);
+#pragma unroll
+      for (int m = 0; m < SP; m++) S0[m] = inv * Z[lds_index(tid + m * NTH)];
+      __syncthreads();
+    }
   }
   // time-domain field of the step's starting spectrum
-  store_padded<R, N, NTH>(Z, S0);
-  fft_lds<R, N, NTH, -1>(Z, a.tw);
+  if constexpr (REGS) {
+    fft_regs_to_lds<R, N, NTH, -1, true>(S0, Z, a.tw, tid);
+  } else {
+    store_padded<R, N, NTH>(Z, S0);
+    fft_lds<R, N, NTH, -1>(Z, a.tw);
+  }
 
   if (a.plasma) {
     plasma_row<R, A, N, NTH>(Z, rho, kd, a.pp, a.dt, red);
@@ -456,17 +549,19 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   // RK3 (Kutta) in z: k1 = f(S0); k2 = f(S0 + dz/2 k1); k3 = f(S0 - dz k1 + 2 dz k2)
   const R dz = a.dz;
   cx<R> k1[SP], k2[SP], k[SP], U[SP];
-  stage_rhs<R, N, NTH, RAMAN>(Z, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k1);
+  // stage 1 starts from the field the plasma scan read (Z)
+#pragma unroll
+  for (int u = 0; u < TP; u++) e[u] = Z[lds_index(tid + u * NTH)];
+  if constexpr (REGS) __syncthreads();
+  stage_rhs<R, N, NTH, RAMAN>(Z, e, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k1);
 #pragma unroll
   for (int m = 0; m < SP; m++) U[m] = S0[m] + ((R)0.5 * dz) * k1[m];
-  store_padded<R, N, NTH>(Z, U);
-  fft_lds<R, N, NTH, -1>(Z, a.tw);
-  stage_rhs<R, N, NTH, RAMAN>(Z, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k2);
+  stage_field<R, N, NTH>(U, e, Z, a.tw);
+  stage_rhs<R, N, NTH, RAMAN>(Z, e, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k2);
 #pragma unroll
   for (int m = 0; m < SP; m++) U[m] = S0[m] + dz * (k2[m] + k2[m] - k1[m]);
-  store_padded<R, N, NTH>(Z, U);
-  fft_lds<R, N, NTH, -1>(Z, a.tw);
-  stage_rhs<R, N, NTH, RAMAN>(Z, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k);
+  stage_field<R, N, NTH>(U, e, Z, a.tw);
+  stage_rhs<R, N, NTH, RAMAN>(Z, e, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k);
   const R c16 = (R)1 / (R)6, c23 = (R)2 / (R)3;
 #pragma unroll
   for (int m = 0; m < SP; m++) {
@@ -661,8 +756,12 @@ __global__ __launch_bounds__(NTH) void corr_features_kernel(const cx<float>* __r
   cx<float> v[SP];
 #pragma unroll
   for (int m = 0; m < SP; m++) v[m] = S[(size_t)i * NH + tid + m * NTH];
-  store_padded<float, N, NTH>(Z, v);
-  fft_lds<float, N, NTH, -1>(Z, tw);
+  if constexpr (fft_regs_ok<N, NTH>()) {
+    fft_regs_to_lds<float, N, NTH, -1, true>(v, Z, tw, tid);
+  } else {
+    store_padded<float, N, NTH>(Z, v);
+    fft_lds<float, N, NTH, -1>(Z, tw);
+  }
   if (plasma) plasma_row<float, float, N, NTH>(Z, rho, kd, pp, g.dt, red);
   float* out = Frow + (size_t)i * 4 * g.W;
   const float inv = 1.0f / (float)g.pt;
@@ -717,22 +816,37 @@ __global__ __launch_bounds__(NTH) void corr_apply_kernel(cx<float>* S, const cx<
   cx<float> v[SP];
 #pragma unroll
   for (int m = 0; m < SP; m++) v[m] = S[(size_t)i * NH + tid + m * NTH];
-  store_padded<float, N, NTH>(Z, v);
-  fft_lds<float, N, NTH, -1>(Z, tw);
   // pooled block h covers rows [h pr, (h+1) pr); its centre is at row (h + 1/2) pr - 1/2
   const float y = ((float)i + 0.5f) / (float)g.pr - 0.5f;
   const int HW = g.H * g.W;
-  for (int j = g.t0 + tid; j < g.t0 + g.nt; j += NTH) {
+  // (dre + i dim) exp(-i w0 t) at sample j of the window
+  auto corr = [&](int j) {
     const float x = ((float)(j - g.t0) + 0.5f) / (float)g.pt - 0.5f;
     const cx<float> d = mk(scale * bilinear(D, g.H, g.W, y, x), scale * bilinear(D + HW, g.H, g.W, y, x));
-    // (dre + i dim) exp(-i w0 t)
-    Z[lds_index(j)] = Z[lds_index(j)] + d * conj(g.carrier[j]);
-  }
-  __syncthreads();
-  fft_lds<float, N, NTH, +1>(Z, tw);
+    return d * conj(g.carrier[j]);
+  };
   const float inv = 1.0f / (float)N;
+  if constexpr (fft_regs_ok<N, NTH>()) {
+    constexpr int TP = N / NTH;
+    cx<float> e[TP];
+    fft_regs_to_regs<float, N, NTH, -1, true>(v, e, Z, tw, tid);
 #pragma unroll
-  for (int m = 0; m < SP; m++) S[(size_t)i * NH + tid + m * NTH] = inv * Z[lds_index(tid + m * NTH)];
+    for (int u = 0; u < TP; u++) {
+      const int j = tid + u * NTH;
+      if (j >= g.t0 && j < g.t0 + g.nt) e[u] = e[u] + corr(j);
+    }
+    fft_regs_to_regs<float, N, NTH, +1, false>(e, e, Z, tw, tid);
+#pragma unroll
+    for (int m = 0; m < SP; m++) S[(size_t)i * NH + tid + m * NTH] = inv * e[m];
+  } else {
+    store_padded<float, N, NTH>(Z, v);
+    fft_lds<float, N, NTH, -1>(Z, tw);
+    for (int j = g.t0 + tid; j < g.t0 + g.nt; j += NTH) Z[lds_index(j)] = Z[lds_index(j)] + corr(j);
+    __syncthreads();
+    fft_lds<float, N, NTH, +1>(Z, tw);
+#pragma unroll
+    for (int m = 0; m < SP; m++) S[(size_t)i * NH + tid + m * NTH] = inv * Z[lds_index(tid + m * NTH)];
+  }
 }
 
 // ---------------------------------------------------------------------------
