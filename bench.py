@@ -63,16 +63,17 @@ def kernel_ms(kt):
     return dict(row=kt["row_ms"] / n, dht1=kt["dht1_ms"] / n, dht2=kt["dht2_ms"] / n)
 
 
-def run_fine(pkg, p, precision, form, W, K, dz):
-    """W warm-up and K timed fine steps; per-kernel HIP-event times over the
-    timed steps (events on the library's stream)."""
+def run_fine(pkg, p, precision, form, W, K, dz, every):
+    """W warm-up and K timed fine steps; per-kernel HIP-event times over every
+    ``every``-th timed step (events on the library's stream, carried by the
+    kernel dispatches)."""
     m = pkg.RTModel(p, precision=precision, form=form)
     z = 0.0
     for _ in range(W):
         z += dz
         m.zstep(z, dz)
     m.synchronize()
-    m.timing(True)
+    m.timing(every)
     t0 = time.perf_counter()
     for _ in range(K):
         z += dz
@@ -86,7 +87,7 @@ def run_fine(pkg, p, precision, form, W, K, dz):
     return ms, kernel_ms(kt), Fr
 
 
-def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geom):
+def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geom, every):
     """The chosen form: coarse steps (+ corrector) with verification, W/K in
     fine-step equivalents."""
     m = pkg.RTModel(p, precision=pkg.lib.FP32, form=form)
@@ -98,7 +99,7 @@ def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geo
     z = st.advance(0.0, dz, W)
     m.synchronize()
     s0 = st.stats()
-    m.timing(True)
+    m.timing(every)
     t0 = time.perf_counter()
     z = st.advance(z, dz, K)
     m.synchronize()
@@ -160,6 +161,8 @@ def main():
     ap.add_argument("--corr-iters", type=int, default=1500)
     ap.add_argument("--cpu-steps", type=int, default=30, help="oracle z-steps timed for the CPU baseline (~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--timing-every", type=int, default=8,
+                    help="per-kernel HIP-event timing on every n-th z-step of the timed region (1: every step)")
     ap.add_argument("--forms", default="fp64,plain,chosen",
                     help="comma list of forms (fp64, plain, chosen); anything but all three is a profiling run "
                          "that prints per-form times only")
@@ -187,12 +190,12 @@ def main():
     Fr64 = None
     if "fp64" in want:
         log("fp64: fine steps")
-        ms64, kt64, Fr64 = run_fine(pkg, p, L.FP64, 0, W, K, dz)
+        ms64, kt64, Fr64 = run_fine(pkg, p, L.FP64, 0, W, K, dz, args.timing_every)
         forms["fp64"] = dict(ms_per_step=ms64, z_steps_per_s=1e3 / ms64, kernel_ms=kt64)
         log(f"fp64: {ms64:.4f} ms/step")
     if "plain" in want:
         log("plain FP32: fine steps")
-        msp, ktp, Frp = run_fine(pkg, p, L.FP32, 0, W, K, dz)
+        msp, ktp, Frp = run_fine(pkg, p, L.FP32, 0, W, K, dz, args.timing_every)
         forms["plain_fp32"] = dict(ms_per_step=msp, z_steps_per_s=1e3 / msp, kernel_ms=ktp,
                                    fluence_err_vs_fp64=fluence_error(Frp, Fr64) if Fr64 is not None else None)
         log(f"plain FP32: {msp:.4f} ms/step")
@@ -206,7 +209,7 @@ def main():
             cst, cinfo = train_corrector(pkg, p, chosen_form, dz, F, draws, args.corr_pairs, args.corr_iters, geom)
         log(f"chosen: coarse x{F}, corrector {'off' if cst is None else 'on'}")
         msc, ktc, Frc, timed = run_coarse(pkg, p, chosen_form, W, K, dz, F, args.check_every, args.check_tol,
-                                          cst, geom)
+                                          cst, geom, args.timing_every)
         forms["chosen"] = dict(
             ms_per_fine_step_equivalent=msc, z_steps_per_s=1e3 / msc, kernel_ms_per_zstep=ktc,
             fluence_err_vs_fp64=fluence_error(Frc, Fr64) if Fr64 is not None else None,
@@ -243,6 +246,8 @@ def main():
                     frac=ach / PEAK["hbm_gbs"], traffic=TRAFFIC_BYTES["row_kernel"],
                     per_launch=f"{row_bytes:.4g} B (2*8*Nr*Nt/2)",
                     avg_launch_ms=kt["row"])
+    roof["timing"] = (f"HIP events carried by the kernel dispatches on the library's stream, every "
+                      f"{args.timing_every}th z-step of the timed region")
 
     # whole-step roofline: T_roof = F_DHT / P_MFMA + max(B_step / BW_HBM, F_FFT / P_FP32) with
     # F_DHT = 2 transforms x 4 Nr^2 Nh flop, B_step = 4 x 8 Nr Nh bytes (the c64 half-spectrum read

@@ -131,7 +131,10 @@ struct fil_handle {
   PlasmaParams<double> pp64{};
   // timing
   bool timing = false;
-  std::vector<hipEvent_t> ev;   // 4 per recorded step
+  int timing_period = 1;        // time every n-th z-step
+  int64_t timing_tick = 0;
+  bool timing_now = false;      // the current z-step is timed
+  std::vector<hipEvent_t> ev;   // 6 per recorded step: (start, stop) of the row phase and of the two linear phases
   size_t ev_used = 0;
   double ms[3] = {0, 0, 0};
   int64_t count = 0;
@@ -196,28 +199,56 @@ uint16_t to_bf16(float f, float* back) {
   return h;
 }
 
-int record(fil_handle* h, int slot) {
-  if (!h->timing) return 0;
-  size_t need = h->ev_used + 1;
-  if (need > h->ev.size()) {
-    hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
-    h->ev.push_back(e);
+} // namespace
+
+namespace fil {
+thread_local LaunchEvents fil_launch_events;
+}
+
+namespace {
+
+// The (start, stop) events of the next timed phase; none while timing is off.
+int next_pair(fil_handle* h, hipEvent_t* s, hipEvent_t* e) {
+  *s = *e = nullptr;
+  if (!h->timing || !h->timing_now) return 0;
+  while (h->ev.size() < h->ev_used + 2) {
+    hipEvent_t x;
+    HIPCHK(hipEventCreate(&x));
+    h->ev.push_back(x);
   }
-  (void)slot;
-  HIPCHK(hipEventRecord(h->ev[h->ev_used], h->st));
-  h->ev_used++;
+  *s = h->ev[h->ev_used];
+  *e = h->ev[h->ev_used + 1];
+  h->ev_used += 2;
+  return 0;
+}
+// A phase of one kernel: the next fil_launch carries the pair on its dispatch.
+int arm(fil_handle* h) {
+  hipEvent_t s, e;
+  if (int rc = next_pair(h, &s, &e)) return rc;
+  fil_launch_events.start = s;
+  fil_launch_events.stop = e;
+  return 0;
+}
+// A phase of several kernels (or none): event records around it.
+int phase_begin(fil_handle* h, hipEvent_t* stop) {
+  hipEvent_t s;
+  if (int rc = next_pair(h, &s, stop)) return rc;
+  if (s) HIPCHK(hipEventRecord(s, h->st));
+  return 0;
+}
+int phase_end(fil_handle* h, hipEvent_t stop) {
+  if (stop) HIPCHK(hipEventRecord(stop, h->st));
   return 0;
 }
 
 int flush_timing(fil_handle* h) {
   if (!h->ev_used) return 0;
   HIPCHK(hipStreamSynchronize(h->st));
-  for (size_t s = 0; s + 4 <= h->ev_used; s += 4) {
+  for (size_t s = 0; s + 6 <= h->ev_used; s += 6) {
     float a, b, c;
     HIPCHK(hipEventElapsedTime(&a, h->ev[s], h->ev[s + 1]));
-    HIPCHK(hipEventElapsedTime(&b, h->ev[s + 1], h->ev[s + 2]));
-    HIPCHK(hipEventElapsedTime(&c, h->ev[s + 2], h->ev[s + 3]));
+    HIPCHK(hipEventElapsedTime(&b, h->ev[s + 2], h->ev[s + 3]));
+    HIPCHK(hipEventElapsedTime(&c, h->ev[s + 4], h->ev[s + 5]));
     h->ms[0] += a; h->ms[1] += b; h->ms[2] += c;
     h->count++;
   }
@@ -263,49 +294,49 @@ int zstep_impl(fil_handle* h, double dz) {
   a.rho_max = h->rho_max.as<double>();
   a.rho_full = h->d.keep_plasma ? h->rho_full.as<R>() : nullptr;
   a.kd_full = h->d.keep_plasma ? h->kd_full.as<R>() : nullptr;
-  if (int rc = record(h, 0)) return rc;
-  HIPCHK(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
-  if (int rc = record(h, 1)) return rc;
+  // Timed phases (row, linear 1, linear 2): a failed launch drops the step's events.
+  const size_t ev0 = h->ev_used;
+  h->timing_now = h->timing && (h->timing_tick++ % h->timing_period) == 0;
+#define FIL_TIMED(launch_)                                                                          \
+  do {                                                                                              \
+    if (int rc_ = arm(h)) return rc_;                                                               \
+    const hipError_t e_ = (launch_);                                                                \
+    fil_launch_events = LaunchEvents{};                                                             \
+    if (e_ != hipSuccess) {                                                                         \
+      h->ev_used = ev0;                                                                             \
+      return fail(FIL_EDEVICE, std::string(#launch_) + ": " + hipGetErrorString(e_));               \
+    }                                                                                               \
+  } while (0)
+  FIL_TIMED(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
   if (h->tgeo) {
     // propagate!(E, LP, dz) for T (spectral.jl:16-26): E(w) *= exp(i KZ dz) Wguard
-    HIPCHK(launch_scale_rows<R>(h->S.as<cx<R>>(), h->P.as<cx<R>>(), h->rows(), h->Nh, h->st));
-    if (int rc = record(h, 2)) return rc;
-    if (int rc = record(h, 3)) return rc;
-    h->pending_filter = true;
-    h->plasma_valid = h->d.plasma != 0;
-    return 0;
-  }
-  if (h->xyt) {
-    // the three transform passes are timed together as the "dht1" phase
+    FIL_TIMED(launch_scale_rows<R>(h->S.as<cx<R>>(), h->P.as<cx<R>>(), h->rows(), h->Nh, h->st));
+    hipEvent_t e2;
+    if (int rc = phase_begin(h, &e2)) return rc;
+    if (int rc = phase_end(h, e2)) return rc;
+  } else if (h->xyt) {
+    // the three transform passes are timed together as the first linear phase
+    hipEvent_t e1, e2;
+    if (int rc = phase_begin(h, &e1)) return rc;
     HIPCHK(launch_xyt_linear<R>(h->S.as<cx<R>>(), h->P.as<cx<R>>(), h->twx.as<cx<R>>(), h->twy.as<cx<R>>(), h->Nx,
                                 h->Ny, h->Nh, h->st));
-    if (int rc = record(h, 2)) return rc;
-    if (int rc = record(h, 3)) return rc;
-    h->pending_filter = true;
-    h->plasma_valid = h->d.plasma != 0;
-    return 0;
+    if (int rc = phase_end(h, e1)) return rc;
+    if (int rc = phase_begin(h, &e2)) return rc;
+    if (int rc = phase_end(h, e2)) return rc;
+  } else if (sizeof(R) == 4 && h->bf16x3) {
+    // X travels as its four BF16 planes (same bytes as FP32 complex)
+    FIL_TIMED(launch_dht_gemm_bf16x3_to_planes(h->M1h.as<uint16_t>(), h->M1l.as<uint16_t>(), h->S.as<cx<float>>(),
+                                               h->X.as<uint16_t>(), h->P.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh,
+                                               h->st));
+    FIL_TIMED(launch_dht_gemm_bf16x3_from_planes(h->M2h.as<uint16_t>(), h->M2l.as<uint16_t>(), h->X.as<uint16_t>(),
+                                                 h->S.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh, h->st));
+  } else {
+    FIL_TIMED(launch_dht_gemm<R>(h->M1.as<R>(), h->S.as<cx<R>>(), h->X.as<cx<R>>(), h->P.as<cx<R>>(), h->Nrp,
+                                 h->Nh * h->B, h->Nh, h->st));
+    FIL_TIMED(launch_dht_gemm<R>(h->M2.as<R>(), h->X.as<cx<R>>(), h->S.as<cx<R>>(), nullptr, h->Nrp, h->Nh * h->B,
+                                 h->Nh, h->st));
   }
-  if constexpr (sizeof(R) == 4) {
-    if (h->bf16x3) {
-      // X travels as its four BF16 planes (same bytes as FP32 complex)
-      HIPCHK(launch_dht_gemm_bf16x3_to_planes(h->M1h.as<uint16_t>(), h->M1l.as<uint16_t>(), h->S.as<cx<float>>(),
-                                              h->X.as<uint16_t>(), h->P.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh,
-                                              h->st));
-      if (int rc = record(h, 2)) return rc;
-      HIPCHK(launch_dht_gemm_bf16x3_from_planes(h->M2h.as<uint16_t>(), h->M2l.as<uint16_t>(), h->X.as<uint16_t>(),
-                                                h->S.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh, h->st));
-      if (int rc = record(h, 3)) return rc;
-      h->pending_filter = true;
-      h->plasma_valid = h->d.plasma != 0;
-      return 0;
-    }
-  }
-  HIPCHK(launch_dht_gemm<R>(h->M1.as<R>(), h->S.as<cx<R>>(), h->X.as<cx<R>>(), h->P.as<cx<R>>(), h->Nrp, h->Nh * h->B,
-                            h->Nh, h->st));
-  if (int rc = record(h, 2)) return rc;
-  HIPCHK(launch_dht_gemm<R>(h->M2.as<R>(), h->X.as<cx<R>>(), h->S.as<cx<R>>(), nullptr, h->Nrp, h->Nh * h->B, h->Nh,
-                            h->st));
-  if (int rc = record(h, 3)) return rc;
+#undef FIL_TIMED
   h->pending_filter = true;
   h->plasma_valid = h->d.plasma != 0;
   return 0;
@@ -1385,7 +1416,10 @@ int fil_destroy(fil_handle* h) {
 int fil_timing_enable(fil_handle* h, int enable) {
   if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
   if (int rc = flush_timing(h)) return rc;
+  if (enable < 0) return fail(FIL_EINVAL, "timing period must be >= 0");
   h->timing = enable != 0;
+  h->timing_period = enable > 1 ? enable : 1;
+  h->timing_tick = 0;
   h->ms[0] = h->ms[1] = h->ms[2] = 0;
   h->count = 0;
   return FIL_OK;

@@ -31,12 +31,12 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 // LDS row strides are padded to 16 (mod 64) floats so that the four k rows
 // read by one MFMA operand load fall in disjoint bank groups.
 template <int BM, int BW, int BK, int WM, int WW>
-__global__ __launch_bounds__(256) void dht_gemm_f32_kernel(const float* __restrict__ MT, const cx<float>* __restrict__ In,
-                                                           cx<float>* __restrict__ Out, const cx<float>* __restrict__ P,
-                                                           int Nr, int Nh, int Pw) {
+__global__ __launch_bounds__(64 * (BM / WM) * (BW / WW)) void dht_gemm_f32_kernel(
+    const float* __restrict__ MT, const cx<float>* __restrict__ In, cx<float>* __restrict__ Out,
+    const cx<float>* __restrict__ P, int Nr, int Nh, int Pw) {
   constexpr int MT_ = WM / 16, WT = WW / 16;
   constexpr int WAVES_M = BM / WM;
-  static_assert((BM / WM) * (BW / WW) == 4, "4 waves per workgroup");
+  constexpr int NT = 64 * (BM / WM) * (BW / WW);
   constexpr int SA = BM + 16, SB = BW + 16;
   __shared__ __align__(16) float As[2][BK][SA];
   __shared__ __align__(16) float Bre[2][BK][SB];
@@ -45,28 +45,28 @@ __global__ __launch_bounds__(256) void dht_gemm_f32_kernel(const float* __restri
   const int m0 = blockIdx.y * BM, w0 = blockIdx.x * BW;
   const int wm = (wv % WAVES_M) * WM, ww = (wv / WAVES_M) * WW;
   const int l16 = lane & 15, l4 = lane >> 4;
-  constexpr int NA = BK * BM / 4 / 256;   // float4 of A per thread
-  constexpr int NB = BK * BW / 2 / 256;   // float4 (two complex) of B per thread
-  static_assert(NA >= 1 && NB >= 1, "tile too small for 256 threads");
+  constexpr int NA = BK * BM / 4 / NT;    // float4 of A per thread
+  constexpr int NB = BK * BW / 2 / NT;    // float4 (two complex) of B per thread
+  static_assert(NA >= 1 && NB >= 1 && NA * 4 * NT == BK * BM && NB * 2 * NT == BK * BW, "tile shape");
   float4 ra[NA], rb[NB];
   // global -> registers (prefetch of the next k tile) and registers -> LDS,
   // written as macros so the prefetch arrays stay in registers
 #define FIL_F32_GLOAD(k0_)                                                                      \
   _Pragma("unroll") for (int u = 0; u < NA; u++) {                                               \
-    const int e = tid + 256 * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;                     \
+    const int e = tid + NT * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;                     \
     ra[u] = *reinterpret_cast<const float4*>(MT + (size_t)((k0_) + kk) * Nr + m0 + m4);           \
   }                                                                                              \
   _Pragma("unroll") for (int u = 0; u < NB; u++) {                                               \
-    const int e = tid + 256 * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;                     \
+    const int e = tid + NT * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;                     \
     rb[u] = *reinterpret_cast<const float4*>(In + (size_t)((k0_) + kk) * Nh + w0 + c2);           \
   }
 #define FIL_F32_LSTORE(buf_)                                                                    \
   _Pragma("unroll") for (int u = 0; u < NA; u++) {                                               \
-    const int e = tid + 256 * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;                     \
+    const int e = tid + NT * u, kk = e / (BM / 4), m4 = (e % (BM / 4)) * 4;                     \
     *reinterpret_cast<float4*>(&As[buf_][kk][m4]) = ra[u];                                       \
   }                                                                                              \
   _Pragma("unroll") for (int u = 0; u < NB; u++) {                                               \
-    const int e = tid + 256 * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;                     \
+    const int e = tid + NT * u, kk = e / (BW / 2), c2 = (e % (BW / 2)) * 2;                     \
     *reinterpret_cast<float2*>(&Bre[buf_][kk][c2]) = make_float2(rb[u].x, rb[u].z);              \
     *reinterpret_cast<float2*>(&Bim[buf_][kk][c2]) = make_float2(rb[u].y, rb[u].w);              \
   }
@@ -183,7 +183,7 @@ hipError_t launch_dht_gemm(const R* MT, const cx<R>* In, cx<R>* Out, const cx<R>
                            hipStream_t st) {
   if (Nr % 64 || Nh % 64 || Pw < 1 || (Pw & (Pw - 1)) || Nh % Pw) return hipErrorInvalidValue;
   if (sizeof(R) == 8) {
-    hipLaunchKernelGGL((dht_gemm_valu_kernel<R>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st, MT, In, Out, P, Nr, Nh, Pw);
+    fil_launch(dht_gemm_valu_kernel<R>, dim3(Nh / 64, Nr / 64), dim3(256), 0, st, MT, In, Out, P, Nr, Nh, Pw);
     return hipGetLastError();
   }
   const float* A = (const float*)MT;
@@ -191,11 +191,12 @@ hipError_t launch_dht_gemm(const R* MT, const cx<R>* In, cx<R>* Out, const cx<R>
   cx<float>* C = (cx<float>*)Out;
   const cx<float>* PP = (const cx<float>*)P;
   if (wide_tiles(Nr, Pw)) {
-    hipLaunchKernelGGL((dht_gemm_f32_kernel<128, 64, 32, 64, 32>), dim3(Nh / 64, Nr / 128), dim3(256), 0, st,
-                       A, B, C, PP, Nr, Nh, Pw);
+    // 8 waves of 32 x 32 (two per SIMD: one workgroup per CU at Nr = 1024)
+    fil_launch(dht_gemm_f32_kernel<128, 64, 32, 32, 32>, dim3(Nh / 64, Nr / 128), dim3(512), 0, st, A, B, C, PP, Nr,
+               Nh, Pw);
   } else {
-    hipLaunchKernelGGL((dht_gemm_f32_kernel<64, 64, 16, 32, 32>), dim3(Nh / 64, Nr / 64), dim3(256), 0, st,
-                       A, B, C, PP, Nr, Nh, Pw);
+    fil_launch(dht_gemm_f32_kernel<64, 64, 16, 32, 32>, dim3(Nh / 64, Nr / 64), dim3(256), 0, st, A, B, C, PP, Nr, Nh,
+               Pw);
   }
   return hipGetLastError();
 }
@@ -263,29 +264,31 @@ __device__ __forceinline__ u32x2 load8(rsrc_t r, int voff, int soff) {
   return __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
 }
 
-// BM (m) x BW (complex w) x BK (k) tiles, 4 waves of (BM/2) x (BW/2); B stored
+// BM (m) x BW (complex w) x BK (k) tiles, NWM x NWW waves of (BM/NWM) x (BW/NWW); B stored
 // [w][k] in LDS so each lane's MFMA operand (8 consecutive k) is one 16-byte
 // LDS read; register-prefetch double buffering.
 // BIN_PLANES: B is the four BF16 planes [4][Nh][Nr] (else FP32 complex [Nr][Nh]).
 // OUT_PLANES: the output is written as the four planes [4][Nh][Nr] (else FP32
 // complex [Nr][Nh]); P (optional) multiplies the output first.
-template <int BM, int BW, int BK, bool BIN_PLANES, bool OUT_PLANES>
-__global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __restrict__ Mhi,
+template <int BM, int BW, int BK, int NWM, int NWW, bool BIN_PLANES, bool OUT_PLANES>
+__global__ __launch_bounds__(64 * NWM * NWW) void dht_gemm_bf16x3_kernel(const uint16_t* __restrict__ Mhi,
                                                               const uint16_t* __restrict__ Mlo,
                                                               const void* __restrict__ In_, void* __restrict__ Out_,
                                                               const cx<float>* __restrict__ P, int Nr, int Nh, int Pw) {
   constexpr int SK = BK + 8;              // padded row: (BK + 8) bf16
-  constexpr int WM = BM / 2, WW = BW / 2;
+  constexpr int NT = 64 * NWM * NWW;
+  constexpr int WM = BM / NWM, WW = BW / NWW;
   constexpr int MT = WM / 16, WT = WW / 16;
   __shared__ __align__(16) uint16_t As[2][2][BM][SK];      // [buf][hi/lo][m][k]
   __shared__ __align__(16) uint16_t Bs[2][4][BW][SK];      // [buf][re_hi,re_lo,im_hi,im_lo][w][k]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * BM, w0 = blockIdx.x * BW;
-  const int wm = (wv >> 1) * WM, ww = (wv & 1) * WW;
+  const int wm = (wv / NWW) * WM, ww = (wv % NWW) * WW;
   const int l16 = lane & 15, l4 = lane >> 4;
-  constexpr int NA = BM * BK / 8 / 256;     // 16-byte chunks of each A plane per thread
-  constexpr int NB = BW * BK / 8 / 256;     // 16-byte chunks (one w, 8 consecutive k) of each B plane per thread
-  static_assert(NA >= 1 && NB >= 1 && BK % 32 == 0, "tile shape");
+  constexpr int NA = BM * BK / 8 / NT;      // 16-byte chunks of each A plane per thread
+  constexpr int NB = BW * BK / 8 / NT;      // 16-byte chunks (one w, 8 consecutive k) of each B plane per thread
+  static_assert(NA >= 1 && NB >= 1 && NA * 8 * NT == BM * BK && NB * 8 * NT == BW * BK && BK % 32 == 0 &&
+                WM % 16 == 0 && WW % 16 == 0, "tile shape");
   const size_t plane = (size_t)Nh * Nr;     // elements of one BF16 plane of X
   // two sets of staging registers: tile kb + 2 is loaded while tile kb is
   // multiplied, so each load has two MFMA phases to arrive
@@ -307,12 +310,12 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __
   int offA[NA], offB[NB];
 #pragma unroll
   for (int u = 0; u < NA; u++) {
-    const int e = tid + 256 * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;
+    const int e = tid + NT * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;
     offA[u] = ((m0 + row) * Nr + kq) * 2;
   }
 #pragma unroll
   for (int u = 0; u < NB; u++) {
-    const int e = tid + 256 * u;
+    const int e = tid + NT * u;
     if constexpr (BIN_PLANES) {
       const int bw = e / (BK / 8), kq = (e % (BK / 8)) * 8;
       offB[u] = ((w0 + bw) * Nr + kq) * 2;
@@ -335,12 +338,12 @@ __global__ __launch_bounds__(256) void dht_gemm_bf16x3_kernel(const uint16_t* __
   }
 #define FIL_BF_LSTORE(r_, buf_)                                                                   \
   _Pragma("unroll") for (int u = 0; u < NA; u++) {                                                 \
-    const int e = tid + 256 * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;                      \
+    const int e = tid + NT * u, row = e / (BK / 8), kq = (e % (BK / 8)) * 8;                      \
     *reinterpret_cast<uint4*>(&As[buf_][0][row][kq]) = rah[r_][u];                                 \
     *reinterpret_cast<uint4*>(&As[buf_][1][row][kq]) = ral[r_][u];                                 \
   }                                                                                                \
   _Pragma("unroll") for (int u = 0; u < NB; u++) {                                                 \
-    const int e = tid + 256 * u;                                                                   \
+    const int e = tid + NT * u;                                                                   \
     if constexpr (BIN_PLANES) {                                                                    \
       const int bw = e / (BK / 8), kq = (e % (BK / 8)) * 8;                                        \
       _Pragma("unroll") for (int q = 0; q < 4; q++) *reinterpret_cast<uint4*>(&Bs[buf_][q][bw][kq]) = rbp[r_][u][q]; \
@@ -452,12 +455,16 @@ static hipError_t launch_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const 
   if (Nr % 64 || Nh % 64 || Pw < 1 || (Pw & (Pw - 1)) || Nh % Pw) return hipErrorInvalidValue;
   // the kernels address their operands with 32-bit byte offsets
   if ((size_t)Nr * Nh * 8 >= ((size_t)1 << 31)) return hipErrorInvalidValue;
+  // 8 waves (two per SIMD) in 4 x 2 over the tile: one workgroup per CU at
+  // Nr = 1024 (the LDS stages fill the CU), so the second wave per SIMD is
+  // what hides the LDS and MFMA latencies
+  constexpr int NWM = 4, NWW = 2;
   if (wide_tiles(Nr, Pw)) {
-    hipLaunchKernelGGL((dht_gemm_bf16x3_kernel<128, 64, 64, BIN_PLANES, OUT_PLANES>), dim3(Nh / 64, Nr / 128),
-                       dim3(256), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
+    fil_launch(dht_gemm_bf16x3_kernel<128, 64, 64, NWM, NWW, BIN_PLANES, OUT_PLANES>, dim3(Nh / 64, Nr / 128),
+               dim3(64 * NWM * NWW), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
   } else {
-    hipLaunchKernelGGL((dht_gemm_bf16x3_kernel<64, 64, 64, BIN_PLANES, OUT_PLANES>), dim3(Nh / 64, Nr / 64),
-                       dim3(256), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
+    fil_launch(dht_gemm_bf16x3_kernel<64, 64, 64, NWM, NWW, BIN_PLANES, OUT_PLANES>, dim3(Nh / 64, Nr / 64),
+               dim3(64 * NWM * NWW), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
   }
   return hipGetLastError();
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 
 #include "../../include/filament_hip.h"
@@ -26,6 +27,25 @@ template <typename R> __host__ __device__ __forceinline__ cx<R> conj(cx<R> a) { 
 template <typename R> __host__ __device__ __forceinline__ cx<R> mul_i(cx<R> a) { return mk(-a.y, a.x); }
 template <typename R> __host__ __device__ __forceinline__ cx<R> mul_mi(cx<R> a) { return mk(a.y, -a.x); }
 template <typename R> __host__ __device__ __forceinline__ R norm2(cx<R> a) { return a.x * a.x + a.y * a.y; }
+
+// Kernel timing without marker packets: a launcher of a timed phase starts its
+// kernel through fil_launch, which attaches the armed (start, stop) event pair
+// to the dispatch itself (hipExtLaunchKernel) and disarms it.  Event records
+// between kernels would each leave a gap of several microseconds in the stream.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+extern thread_local LaunchEvents fil_launch_events;
+template <typename F, typename... Args>
+inline void fil_launch(F kernel, const dim3& grid, const dim3& block, uint32_t smem, hipStream_t st, Args... args) {
+  if (fil_launch_events.start) {
+    const LaunchEvents ev = fil_launch_events;
+    fil_launch_events = LaunchEvents{};
+    hipExtLaunchKernelGGL(kernel, grid, block, smem, st, ev.start, ev.stop, 0, args...);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, smem, st, args...);
+  }
+}
 
 // Plasma-equation parameters passed by value to the row kernel
 // (photoionization_avalanche.jl:126-183), in the working precision.

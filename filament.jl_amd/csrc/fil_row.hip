@@ -686,7 +686,7 @@ __global__ void scale_rows_kernel(cx<R>* S, const cx<R>* __restrict__ P, int Nh,
 template <typename R>
 hipError_t launch_scale_rows(cx<R>* S, const cx<R>* P, int rows, int Nh, hipStream_t st) {
   const size_t n = (size_t)rows * Nh;
-  hipLaunchKernelGGL(scale_rows_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, S, P, Nh, n);
+  fil_launch(scale_rows_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, S, P, Nh, n);
   return hipGetLastError();
 }
 template hipError_t launch_scale_rows<float>(cx<float>*, const cx<float>*, int, int, hipStream_t);
@@ -871,7 +871,7 @@ static hipError_t launch_row_n(const RowArgs<R>& a, int Nr, hipStream_t st) {
   auto k = raman ? row_kernel<R, A, N, NTH, true> : row_kernel<R, A, N, NTH, false>;
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3(Nr), dim3(NTH), sm, st, a);
+  fil_launch(k, dim3(Nr), dim3(NTH), sm, st, a);
   return hipGetLastError();
 }
 

@@ -191,6 +191,8 @@ class _Model:
         return Fr
 
     def timing(self, enable=True):
+        """Per-kernel timing: False off, True every z-step, an integer n > 1
+        every n-th z-step."""
         L.check(self._lib.fil_timing_enable(self.h, int(enable)))
 
     def timing_read(self):

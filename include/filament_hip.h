@@ -306,7 +306,11 @@ int fil_corrector_apply(fil_handle* h);
 /* Per-phase device timing with HIP events on the handle's stream (the
  * reference's TimerOutputs sections "plasma/nonlinearity" and "linear",
  * Models.jl:84-119).  names: "row" (plasma + t-FFTs + RK), "dht1", "dht2".
- * fil_timing_read returns accumulated milliseconds and launch counts. */
+ * The events ride on the kernel dispatches.  enable: 0 off, 1 every z-step,
+ * n > 1 every n-th z-step (timed dispatches are separated by gaps of several
+ * microseconds, so sampling keeps the timing from slowing the stream).
+ * fil_timing_read returns accumulated milliseconds and the number of timed
+ * z-steps. */
 int fil_timing_enable(fil_handle* h, int enable);
 int fil_timing_read(fil_handle* h, double* ms_row, double* ms_dht1, double* ms_dht2, int64_t* count);
 
