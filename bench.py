@@ -21,7 +21,9 @@ equivalents) from the same initial field:
 
 Prints one JSON line.  `value` is the chosen form's fine-equivalent z-steps/s
 when its fluence profile stays within 1e-3 relative L2 of the FP64 build's at
-the end of the trajectory, otherwise the plain port's z-steps/s.
+the end of the trajectory, otherwise the plain port's z-steps/s.  By default
+the trajectory is the workload's whole 2000 z-steps (1 m), so the error is the
+one at its far end.
 """
 import argparse
 import json
@@ -149,9 +151,10 @@ def cpu_baseline(pkg, p, sample_steps, dz):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400, help="timed fine-step equivalents")
+    ap.add_argument("--steps", type=int, default=2000,
+                    help="timed fine-step equivalents (2000: the workload's full 1 m at dz = 0.5 mm)")
     ap.add_argument("--warmup", type=int, default=16, help="warm-up fine-step equivalents")
-    ap.add_argument("--factor", type=int, default=8, help="coarse step = factor * dz in the chosen form")
+    ap.add_argument("--factor", type=int, default=16, help="coarse step = factor * dz in the chosen form")
     ap.add_argument("--check-every", type=int, default=50, help="coarse steps between fine-step verifications")
     ap.add_argument("--check-tol", type=float, default=1e-4,
                     help="relative fluence distance of one coarse step from its fine steps that triggers the fallback")
