@@ -264,26 +264,31 @@ __device__ __forceinline__ u32x2 load8(rsrc_t r, int voff, int soff) {
   return __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
 }
 
-// BM (m) x BW (complex w) x BK (k) tiles, NWM x NWW waves of (BM/NWM) x (BW/NWW); B stored
+// BM (m) x BW (complex w) x BK (k) tiles, NWM x NWW waves of (BM/NWM) x (BW/NWW)
+// per k group; with KS = 2 two groups of waves take alternate 32-k halves of
+// every k tile (two waves per SIMD at the LDS traffic of one group's register
+// tiles) and add their partial sums once at the end; B stored
 // [w][k] in LDS so each lane's MFMA operand (8 consecutive k) is one 16-byte
 // LDS read; register-prefetch double buffering.
 // BIN_PLANES: B is the four BF16 planes [4][Nh][Nr] (else FP32 complex [Nr][Nh]).
 // OUT_PLANES: the output is written as the four planes [4][Nh][Nr] (else FP32
 // complex [Nr][Nh]); P (optional) multiplies the output first.
-template <int BM, int BW, int BK, int NWM, int NWW, bool BIN_PLANES, bool OUT_PLANES>
-__global__ __launch_bounds__(64 * NWM * NWW) void dht_gemm_bf16x3_kernel(const uint16_t* __restrict__ Mhi,
+template <int BM, int BW, int BK, int NWM, int NWW, int KS, bool BIN_PLANES, bool OUT_PLANES>
+__global__ __launch_bounds__(64 * NWM * NWW * KS) void dht_gemm_bf16x3_kernel(const uint16_t* __restrict__ Mhi,
                                                               const uint16_t* __restrict__ Mlo,
                                                               const void* __restrict__ In_, void* __restrict__ Out_,
                                                               const cx<float>* __restrict__ P, int Nr, int Nh, int Pw) {
   constexpr int SK = BK + 8;              // padded row: (BK + 8) bf16
-  constexpr int NT = 64 * NWM * NWW;
+  constexpr int NT = 64 * NWM * NWW * KS;
   constexpr int WM = BM / NWM, WW = BW / NWW;
   constexpr int MT = WM / 16, WT = WW / 16;
+  static_assert((KS == 1 || KS == 2) && BK % (32 * KS) == 0 && (KS == 1 || MT % 2 == 0), "k groups");
   __shared__ __align__(16) uint16_t As[2][2][BM][SK];      // [buf][hi/lo][m][k]
   __shared__ __align__(16) uint16_t Bs[2][4][BW][SK];      // [buf][re_hi,re_lo,im_hi,im_lo][w][k]
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * BM, w0 = blockIdx.x * BW;
-  const int wm = (wv / NWW) * WM, ww = (wv % NWW) * WW;
+  const int kg = wv / (NWM * NWW), wq = wv % (NWM * NWW);   // k group, wave within the group
+  const int wm = (wq / NWW) * WM, ww = (wq % NWW) * WW;
   const int l16 = lane & 15, l4 = lane >> 4;
   constexpr int NA = BM * BK / 8 / NT;      // 16-byte chunks of each A plane per thread
   constexpr int NB = BW * BK / 8 / NT;      // 16-byte chunks (one w, 8 consecutive k) of each B plane per thread
@@ -369,7 +374,8 @@ __global__ __launch_bounds__(64 * NWM * NWW) void dht_gemm_bf16x3_kernel(const u
 #pragma unroll
     for (int b = 0; b < WT; b++) { acc[a][b][0] = {0, 0, 0, 0}; acc[a][b][1] = {0, 0, 0, 0}; }
 #define FIL_BF_COMPUTE(buf_)                                                                      \
-  _Pragma("unroll") for (int ks = 0; ks < BK; ks += 32) {                                          \
+  _Pragma("unroll") for (int s_ = 0; s_ < BK / (32 * KS); s_++) {                                 \
+    const int ks = (s_ * KS + kg) * 32;                                                            \
     bf16x8 ah[MT], al[MT];                                                                         \
     _Pragma("unroll") for (int a = 0; a < MT; a++) {                                               \
       ah[a] = *reinterpret_cast<const bf16x8*>(&As[buf_][0][wm + a * 16 + l16][ks + 8 * l4]);      \
@@ -415,10 +421,46 @@ __global__ __launch_bounds__(64 * NWM * NWW) void dht_gemm_bf16x3_kernel(const u
 #undef FIL_BF_COMPUTE
 #undef FIL_BF_GLOAD
 #undef FIL_BF_LSTORE
+  if constexpr (KS == 2) {
+    // The k groups add their partial sums: group 0 finalizes the row blocks
+    // a < MT/2, group 1 the others, each handing the other group's wave with
+    // the same (wm, ww) its half through LDS (the tiles are no longer read).
+    constexpr int H = MT / 2, NV = H * WT * 2 * 4;
+    static_assert(2 * NWM * NWW * NV * 64 * sizeof(float) <= sizeof(As), "exchange buffer");
+    __syncthreads();
+    float* xr = reinterpret_cast<float*>(&As[0][0][0][0]);
+    float* mine = xr + (size_t)(kg * (NWM * NWW) + wq) * NV * 64 + lane;
+    const float* other = xr + (size_t)((1 - kg) * (NWM * NWW) + wq) * NV * 64 + lane;
+#pragma unroll
+    for (int a = 0; a < H; a++)
+#pragma unroll
+      for (int b = 0; b < WT; b++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int v = ((a * WT + b) * 2 + c) * 4 + i;
+            mine[v * 64] = kg == 0 ? acc[H + a][b][c][i] : acc[a][b][c][i];
+          }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < H; a++)
+#pragma unroll
+      for (int b = 0; b < WT; b++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const int v = ((a * WT + b) * 2 + c) * 4 + i;
+            if (kg == 0) acc[a][b][c][i] += other[v * 64];
+            else acc[H + a][b][c][i] += other[v * 64];
+          }
+  }
 #pragma unroll
   for (int a = 0; a < MT; a++)
 #pragma unroll
     for (int b = 0; b < WT; b++) {
+      if (KS == 2 && ((a < MT / 2) != (kg == 0))) continue;   // the other group's row blocks
       const int w = w0 + ww + b * 16 + l16;
       const int mb = m0 + wm + a * 16 + 4 * l4;
       if constexpr (OUT_PLANES) {
@@ -455,16 +497,18 @@ static hipError_t launch_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const 
   if (Nr % 64 || Nh % 64 || Pw < 1 || (Pw & (Pw - 1)) || Nh % Pw) return hipErrorInvalidValue;
   // the kernels address their operands with 32-bit byte offsets
   if ((size_t)Nr * Nh * 8 >= ((size_t)1 << 31)) return hipErrorInvalidValue;
-  // 8 waves (two per SIMD) in 4 x 2 over the tile: one workgroup per CU at
-  // Nr = 1024 (the LDS stages fill the CU), so the second wave per SIMD is
-  // what hides the LDS and MFMA latencies
-  constexpr int NWM = 4, NWW = 2;
+  // 8 waves (two per SIMD): one workgroup per CU at Nr = 1024 (the LDS stages
+  // fill the CU), so the second wave per SIMD is what hides the LDS and MFMA
+  // latencies; as two k groups of 2 x 2 waves their fragment reads from LDS
+  // are those of 4 waves (43.0/34.3 -> 40.3/30.6 us at 1024 x 4096 against
+  // one group of 4 x 2 waves)
+  constexpr int KS = 2, NWM = 2, NWW = 2;
   if (wide_tiles(Nr, Pw)) {
-    fil_launch(dht_gemm_bf16x3_kernel<128, 64, 64, NWM, NWW, BIN_PLANES, OUT_PLANES>, dim3(Nh / 64, Nr / 128),
-               dim3(64 * NWM * NWW), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
+    fil_launch(dht_gemm_bf16x3_kernel<128, 64, 64, NWM, NWW, KS, BIN_PLANES, OUT_PLANES>, dim3(Nh / 64, Nr / 128),
+               dim3(64 * NWM * NWW * KS), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
   } else {
-    fil_launch(dht_gemm_bf16x3_kernel<64, 64, 64, NWM, NWW, BIN_PLANES, OUT_PLANES>, dim3(Nh / 64, Nr / 64),
-               dim3(64 * NWM * NWW), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
+    fil_launch(dht_gemm_bf16x3_kernel<64, 64, 64, NWM, NWW, KS, BIN_PLANES, OUT_PLANES>, dim3(Nh / 64, Nr / 64),
+               dim3(64 * NWM * NWW * KS), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
   }
   return hipGetLastError();
 }
