@@ -84,6 +84,10 @@ struct fil_handle {
   bool tables = false, field = false;
   std::vector<char> uploaded;   // members of a batch with a field
   bool pending_filter = false, plasma_valid = false;
+  // a corrector output (corr.out, times corr_scale) not yet added to S: the
+  // next z-step's row kernel adds it; any other reader of S adds it first
+  bool corr_pending = false;
+  float corr_scale = 0;
   double P_dz = NAN;
   double dt = 0;
   double scale[4] = {1, 1, 1, 1};
@@ -241,6 +245,19 @@ int phase_end(fil_handle* h, hipEvent_t stop) {
   return 0;
 }
 
+// Adds a pending corrector output to S now (every reader or writer of the
+// state other than the next z-step sees the corrected field).
+int corr_materialize(fil_handle* h) {
+  if (!h->corr_pending) return 0;
+  h->corr_pending = false;
+  auto& c = h->corr;
+  HIPCHK(launch_corr_apply(h->S.as<cx<float>>(), h->tw.as<cx<float>>(), c.g, h->Nt, c.out.as<float>(),
+                           h->corr_scale, h->st));
+  h->plasma_valid = false;
+  h->acache.valid = false;
+  return 0;
+}
+
 int flush_timing(fil_handle* h) {
   if (!h->ev_used) return 0;
   HIPCHK(hipStreamSynchronize(h->st));
@@ -294,6 +311,14 @@ int zstep_impl(fil_handle* h, double dz) {
   a.rho_max = h->rho_max.as<double>();
   a.rho_full = h->d.keep_plasma ? h->rho_full.as<R>() : nullptr;
   a.kd_full = h->d.keep_plasma ? h->kd_full.as<R>() : nullptr;
+  if constexpr (sizeof(R) == 4) {
+    if (h->corr_pending) {
+      // the row kernel adds the corrector output before the field filter
+      a.corr_D = h->corr.out.as<float>();
+      a.corr_g = h->corr.g;
+      a.corr_scale = h->corr_scale;
+    }
+  }
   // Timed phases (row, linear 1, linear 2): a failed launch drops the step's events.
   const size_t ev0 = h->ev_used;
   h->timing_now = h->timing && (h->timing_tick++ % h->timing_period) == 0;
@@ -308,6 +333,7 @@ int zstep_impl(fil_handle* h, double dz) {
     }                                                                                               \
   } while (0)
   FIL_TIMED(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
+  h->corr_pending = false;
   if (h->tgeo) {
     // propagate!(E, LP, dz) for T (spectral.jl:16-26): E(w) *= exp(i KZ dz) Wguard
     FIL_TIMED(launch_scale_rows<R>(h->S.as<cx<R>>(), h->P.as<cx<R>>(), h->rows(), h->Nh, h->st));
@@ -376,6 +402,7 @@ int plasma_only(fil_handle* h) {
 // see a consistent state.
 template <typename R>
 int apply_pending_filter(fil_handle* h) {
+  if (int rc = corr_materialize(h)) return rc;
   if (!h->pending_filter) return 0;
   h->acache.valid = false;
   RowArgs<R> a{};
@@ -471,6 +498,7 @@ int analyze_device(fil_handle* h, AnalyzerRows& v, int member) {
 
 template <typename R>
 int analyze_impl(fil_handle* h, int member, double z, fil_diag* d, double* Fr, double* Ft, double* rho_end, double* S) {
+  if (int rc = corr_materialize(h)) return rc;
   const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh;
   AnalyzerRows v;
   if (int rc = analyze_device<R>(h, v, member)) return rc;
@@ -622,9 +650,15 @@ int corr_read_loss(fil_handle* h, int nblk, int B, double* loss) {
   return 0;
 }
 
+// Corrector entry points start here: the previous apply's output (corr.out)
+// is added to the state before anything can overwrite it.
 int corr_ready(fil_handle* h) {
   if (!h) return fail(FIL_EINVAL, "null handle");
   if (!h->corr.on) return fail(FIL_ESTATE, "fil_corrector_init first");
+  if (h->corr_pending) {
+    HIPCHK(hipSetDevice(h->d.device));
+    if (int rc = corr_materialize(h)) return rc;
+  }
   return 0;
 }
 
@@ -1050,6 +1084,7 @@ int fil_upload_member(fil_handle* h, int member, const double* E, int64_t ld) {
   h->field = std::all_of(h->uploaded.begin(), h->uploaded.end(), [](char c) { return c != 0; });
   h->pending_filter = false;
   h->plasma_valid = false;
+  h->corr_pending = false;
   return FIL_OK;
 }
 
@@ -1065,6 +1100,7 @@ int fil_download_member(fil_handle* h, int member, double* E, int64_t ld) {
   if (member < 0 || member >= h->B) return fail(FIL_EINVAL, "member out of range");
   if (ld < h->Nr) return fail(FIL_EINVAL, "ld < Nr");
   HIPCHK(hipSetDevice(h->d.device));
+  if (int rc = corr_materialize(h)) return rc;
   const int Nr = h->Nr, Nt = h->Nt;
   if (h->fp64)
     HIPCHK(launch_to_time<double>(h->S.as<cx<double>>(), h->Et.as<cx<double>>(), h->tw.as<cx<double>>(),
@@ -1191,6 +1227,9 @@ int fil_group_zstep(fil_group* g, double z, double dz) {
   for (auto* h : g->hs)
     if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_group_zstep");
   HIPCHK(hipSetDevice(g->device));
+  // pending corrector outputs are added outside the graph
+  for (auto* h : g->hs)
+    if (int rc = corr_materialize(h)) return rc;
   // the propagators for this dz, outside the graph
   uint64_t pattern = 0;
   for (int k = 0; k < n; k++) {
@@ -1282,6 +1321,7 @@ int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld) {
   if (h->B > 1) return fail(FIL_EINVAL, "fil_download_plasma takes single-pulse handles");
   if (ld < h->Nr) return fail(FIL_EINVAL, "ld < Nr");
   HIPCHK(hipSetDevice(h->d.device));
+  if (int rc = corr_materialize(h)) return rc;
   if (!h->plasma_valid) {
     int rc = h->fp64 ? plasma_only<double>(h) : plasma_only<float>(h);
     if (rc) return rc;
@@ -1307,6 +1347,7 @@ int fil_state_save(fil_handle* h, int slot) {
   if (slot < 0 || slot >= FIL_NSLOTS) return fail(FIL_EINVAL, "slot out of range");
   if (!h->field) return fail(FIL_ESTATE, "no field uploaded");
   HIPCHK(hipSetDevice(h->d.device));
+  if (int rc = corr_materialize(h)) return rc;
   auto& s = h->slot[slot];
   const size_t nb = h->ns() * h->csz(), nr = (size_t)h->rows() * 8;
   if (!s.S.p) {
@@ -1329,6 +1370,10 @@ int fil_state_copy(fil_handle* dst, const fil_handle* src) {
   if (dst->Nr != src->Nr || dst->Nt != src->Nt || dst->B != src->B) return fail(FIL_EINVAL, "handles of different grids");
   if (!src->field || !dst->tables) return fail(FIL_ESTATE, "source needs a field and destination its tables");
   HIPCHK(hipSetDevice(dst->d.device));
+  // the source's pending corrector output belongs to its state (adding it
+  // leaves that state unchanged); the destination's is replaced
+  if (int rc = corr_materialize(const_cast<fil_handle*>(src))) return rc;
+  dst->corr_pending = false;
   // the source's pending work completes before the destination's stream reads it
   HIPCHK(hipStreamSynchronize(src->st));
   const size_t n = dst->ns();
@@ -1367,6 +1412,7 @@ int fil_state_restore(fil_handle* h, int slot) {
   HIPCHK(hipMemcpyAsync(h->rho_max.p, s.rho_max.p, nr, hipMemcpyDeviceToDevice, h->st));
   h->pending_filter = s.pending_filter;
   h->plasma_valid = s.plasma_valid;
+  h->corr_pending = false;
   h->acache.valid = false;
   return FIL_OK;
 }
@@ -1376,6 +1422,8 @@ int fil_fluence(fil_handle* h, int slot, double* Fr) {
   if (!h->field) return fail(FIL_ESTATE, "no field uploaded");
   if (slot >= FIL_NSLOTS || (slot >= 0 && !h->slot[slot].used)) return fail(FIL_EINVAL, "bad slot");
   HIPCHK(hipSetDevice(h->d.device));
+  if (slot < 0)
+    if (int rc = corr_materialize(h)) return rc;
   const void* S = slot < 0 ? h->S.p : h->slot[slot].S.p;
   const double scale = (double)h->Nt * h->dt;   // dt sum_t |E|^2 = dt Nt sum_w |S_w|^2
   const int n = h->rows(), B = h->B, Nr = h->Nr;
@@ -1399,6 +1447,8 @@ int fil_fluence(fil_handle* h, int slot, double* Fr) {
 
 int fil_synchronize(fil_handle* h) {
   if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  HIPCHK(hipSetDevice(h->d.device));
+  if (int rc = corr_materialize(h)) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
   return FIL_OK;
 }
@@ -1441,6 +1491,11 @@ int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
   if (h->xyt || h->tgeo) return fail(FIL_EINVAL, "the corrector is defined for the RT geometry");
   if (h->B > 1) return fail(FIL_EINVAL, "the corrector takes single-pulse handles");
   if (!h->tables) return fail(FIL_ESTATE, "fil_set_tables first");
+  if (h->corr_pending) {
+    // the output of the previous network is added before its buffers are replaced
+    HIPCHK(hipSetDevice(h->d.device));
+    if (int rc = corr_materialize(h)) return rc;
+  }
   if (cd->pool_r < 1 || cd->pool_t < 1) return fail(FIL_EINVAL, "pool sizes must be >= 1");
   if (cd->nr < 1 || cd->nr > h->Nr || cd->nr % cd->pool_r) return fail(FIL_EINVAL, "nr must be in [1, Nr] and a multiple of pool_r");
   if (cd->t0 < 0 || cd->nt < 1 || cd->t0 + cd->nt > h->Nt || cd->nt % cd->pool_t)
@@ -1657,8 +1712,10 @@ int fil_corrector_apply(fil_handle* h) {
   HIPCHK(launch_corr_features(h->S.as<cx<float>>(), h->tw.as<cx<float>>(), h->pp32, h->d.plasma, c.g, h->Nt,
                               c.Frow.as<float>(), c.feat.as<float>(), h->st));
   if (int rc = corr_forward(h, c.feat.as<float>(), nullptr, 1)) return rc;
-  const float scale = (float)(c.y_scale / (double)c.g.inv_as);
-  HIPCHK(launch_corr_apply(h->S.as<cx<float>>(), h->tw.as<cx<float>>(), c.g, h->Nt, c.out.as<float>(), scale, h->st));
+  // the output is added to the field by the next z-step's row kernel (before
+  // its field filter), or by corr_materialize when anything else comes first
+  h->corr_scale = (float)(c.y_scale / (double)c.g.inv_as);
+  h->corr_pending = true;
   h->plasma_valid = false;
   h->acache.valid = false;
   return FIL_OK;

@@ -5,6 +5,16 @@
 
 namespace fil {
 
+// Learned coarse-step corrector (fil_row.hip: features / apply; fil_corr.hip: network).
+// Window rows [0, nr), t samples [t0, t0 + nt); pooled grid H = nr/pr by W = nt/pt.
+struct CorrGeom {
+  int nr, t0, nt, pr, pt, H, W;
+  const cx<float>* carrier;  // [Nt] exp(i w0 t_j) (rounded from FP64): envelope A = E carrier
+  float dt;        // t step (plasma scan)
+  float inv_as;    // 1 / field scale
+  float inv_rs;    // 1 / density scale
+};
+
 template <typename R>
 struct RowArgs {
   cx<R>* S;                 // [Nr][Nt/2] positive-frequency spectrum (in/out)
@@ -23,6 +33,12 @@ struct RowArgs {
   double* rho_max;          // [Nr] max_t rho(r, t)
   R* rho_full;              // optional [Nr][Nt]
   R* kd_full;               // optional [Nr][Nt]
+  // pending corrector output (FP32 single-pulse handles; null: none): added
+  // in time to the window rows before the field filter, as launch_corr_apply
+  // would add it
+  const float* corr_D;      // [2][H][W]
+  CorrGeom corr_g;
+  float corr_scale;
 };
 
 template <typename R>
@@ -55,15 +71,6 @@ hipError_t launch_colsum(const T* in, int Nr, int ncol, const double* wt, double
 template <typename R>
 hipError_t launch_rowsum_abs2(const cx<R>* S, int Nr, int Nh, double scale, double* out, hipStream_t st);
 
-// Learned coarse-step corrector (fil_row.hip: features / apply; fil_corr.hip: network).
-// Window rows [0, nr), t samples [t0, t0 + nt); pooled grid H = nr/pr by W = nt/pt.
-struct CorrGeom {
-  int nr, t0, nt, pr, pt, H, W;
-  const cx<float>* carrier;  // [Nt] exp(i w0 t_j) (rounded from FP64): envelope A = E carrier
-  float dt;        // t step (plasma scan)
-  float inv_as;    // 1 / field scale
-  float inv_rs;    // 1 / density scale
-};
 // Features of the current state, X[4][H][W]: Re A, Im A (/a_s), |E|^2 (/a_s^2),
 // rho (/rho_s), each averaged over its pr x pt block.  Frow: scratch [nr][4][W].
 hipError_t launch_corr_features(const cx<float>* S, const cx<float>* tw, const PlasmaParams<float>& pp, int plasma,

@@ -39,6 +39,30 @@ __device__ __forceinline__ R tf_eval(const PlasmaParams<R>& pp, int q, R I, R* p
   return (sizeof(R) == 8) ? (R)exp2((double)y) : (R)exp2f((float)y);
 }
 
+// bilinear interpolation of one pooled plane at pooled coordinates (y, x), clamped
+__device__ __forceinline__ float bilinear(const float* P, int H, int W, float y, float x) {
+  y = fminf(fmaxf(y, 0.0f), (float)(H - 1));
+  x = fminf(fmaxf(x, 0.0f), (float)(W - 1));
+  const int y0 = (int)y, x0 = (int)x;
+  const int y1 = min(y0 + 1, H - 1), x1 = min(x0 + 1, W - 1);
+  const float fy = y - (float)y0, fx = x - (float)x0;
+  const float a = P[y0 * W + x0] + fx * (P[y0 * W + x1] - P[y0 * W + x0]);
+  const float b = P[y1 * W + x0] + fx * (P[y1 * W + x1] - P[y1 * W + x0]);
+  return a + fy * (b - a);
+}
+
+// Corrector output at row i, sample j of the window: scale (Dre + i Dim)
+// exp(-i w0 t_j), D bilinearly interpolated from the pooled-block centres
+// (block h covers rows [h pr, (h+1) pr), centre at row (h + 1/2) pr - 1/2).
+__device__ __forceinline__ cx<float> corr_value(const CorrGeom& g, const float* __restrict__ D, float scale, int i,
+                                                int j) {
+  const float y = ((float)i + 0.5f) / (float)g.pr - 0.5f;
+  const float x = ((float)(j - g.t0) + 0.5f) / (float)g.pt - 0.5f;
+  const int HW = g.H * g.W;
+  const cx<float> d = mk(scale * bilinear(D, g.H, g.W, y, x), scale * bilinear(D + HW, g.H, g.W, y, x));
+  return d * conj(g.carrier[j]);
+}
+
 template <typename R>
 __device__ __forceinline__ R fiarg(cx<R> e, int ereal) { return ereal ? e.x * e.x : norm2(e); }
 
@@ -492,20 +516,60 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   constexpr bool REGS = fft_regs_ok<N, NTH>();
   constexpr int TP = N / NTH;
   cx<R> e[TP];   // time-domain field at t = tid + u NTH
-  if (a.apply_filter) {
-    // field filter deferred from the end of the previous zstep
+  // A corrector output still to be added (window rows of an FP32 single-pulse
+  // handle): D is interpolated along r once per row into Dy [2][W] (LDS, in
+  // the rho area, which the plasma scan fills only later; the transform's
+  // barriers order these stores before the reads), then along t per sample.
+  bool corr_row = false;
+  if constexpr (sizeof(R) == 4) corr_row = a.corr_D != nullptr && i < a.corr_g.nr;
+  float* Dy = reinterpret_cast<float*>(rho);
+  if constexpr (sizeof(R) == 4) {
+    if (corr_row) {
+      const CorrGeom& g = a.corr_g;
+      const float y = fminf(fmaxf(((float)i + 0.5f) / (float)g.pr - 0.5f, 0.0f), (float)(g.H - 1));
+      const int y0 = (int)y, y1 = min(y0 + 1, g.H - 1);
+      const float fy = y - (float)y0;
+      for (int q = tid; q < 2 * g.W; q += NTH) {
+        const float* P = a.corr_D + (q / g.W) * g.H * g.W + q % g.W;
+        const float d0 = P[y0 * g.W], d1 = P[y1 * g.W];
+        Dy[q] = d0 + fy * (d1 - d0);
+      }
+    }
+  }
+  auto corr_at = [&](int t, cx<R> v) {
+    if constexpr (sizeof(R) == 4) {
+      const CorrGeom& g = a.corr_g;
+      if (corr_row && t >= g.t0 && t < g.t0 + g.nt) {
+        const float x = fminf(fmaxf(((float)(t - g.t0) + 0.5f) / (float)g.pt - 0.5f, 0.0f), (float)(g.W - 1));
+        const int x0 = (int)x, x1 = min(x0 + 1, g.W - 1);
+        const float fx = x - (float)x0;
+        const cx<float> d = mk(a.corr_scale * (Dy[x0] + fx * (Dy[x1] - Dy[x0])),
+                               a.corr_scale * (Dy[g.W + x0] + fx * (Dy[g.W + x1] - Dy[g.W + x0])));
+        v = v + d * conj(g.carrier[t]);
+      }
+    }
+    return v;
+  };
+  if (a.apply_filter || corr_row) {
+    // field filter deferred from the end of the previous zstep (and the
+    // corrector's addition, which comes before it)
     const R inv = (R)1 / (R)N;
+    const R gr = a.apply_filter ? g_r : (R)1;
     if constexpr (REGS) {
       fft_regs_to_regs<R, N, NTH, -1, true>(S0, e, Z, a.tw, tid);
 #pragma unroll
-      for (int u = 0; u < TP; u++) e[u] = (g_r * a.Tg[tid + u * NTH]) * e[u];
+      for (int u = 0; u < TP; u++) {
+        const int t = tid + u * NTH;
+        e[u] = (gr * (a.apply_filter ? a.Tg[t] : (R)1)) * corr_at(t, e[u]);
+      }
       fft_regs_to_regs<R, N, NTH, +1, false>(e, e, Z, a.tw, tid);
 #pragma unroll
       for (int m = 0; m < SP; m++) S0[m] = inv * e[m];
     } else {
       store_padded<R, N, NTH>(Z, S0);
       fft_lds<R, N, NTH, -1>(Z, a.tw);
-      for (int t = tid; t < N; t += NTH) Z[lds_index(t)] = (g_r * a.Tg[t]) * Z[lds_index(t)];
+      for (int t = tid; t < N; t += NTH)
+        Z[lds_index(t)] = (gr * (a.apply_filter ? a.Tg[t] : (R)1)) * corr_at(t, Z[lds_index(t)]);
       __syncthreads();
       fft_lds<R, N, NTH, +1>(Z, a.tw);
 #pragma unroll
@@ -539,7 +603,7 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   }
 
   if (!a.nonlinearity) {
-    if (a.apply_filter) {
+    if (a.apply_filter || corr_row) {
 #pragma unroll
       for (int m = 0; m < SP; m++) Srow[tid + m * NTH] = S0[m];
     }
@@ -794,18 +858,6 @@ __global__ void corr_pool_rows_kernel(const float* __restrict__ Frow, CorrGeom g
   X[q] = s / (float)g.pr;
 }
 
-// bilinear interpolation of one pooled plane at pooled coordinates (y, x), clamped
-__device__ __forceinline__ float bilinear(const float* P, int H, int W, float y, float x) {
-  y = fminf(fmaxf(y, 0.0f), (float)(H - 1));
-  x = fminf(fmaxf(x, 0.0f), (float)(W - 1));
-  const int y0 = (int)y, x0 = (int)x;
-  const int y1 = min(y0 + 1, H - 1), x1 = min(x0 + 1, W - 1);
-  const float fy = y - (float)y0, fx = x - (float)x0;
-  const float a = P[y0 * W + x0] + fx * (P[y0 * W + x1] - P[y0 * W + x0]);
-  const float b = P[y1 * W + x0] + fx * (P[y1 * W + x1] - P[y1 * W + x0]);
-  return a + fy * (b - a);
-}
-
 template <int N, int NTH>
 __global__ __launch_bounds__(NTH) void corr_apply_kernel(cx<float>* S, const cx<float>* tw, CorrGeom g,
                                                          const float* __restrict__ D, float scale) {
@@ -816,15 +868,7 @@ __global__ __launch_bounds__(NTH) void corr_apply_kernel(cx<float>* S, const cx<
   cx<float> v[SP];
 #pragma unroll
   for (int m = 0; m < SP; m++) v[m] = S[(size_t)i * NH + tid + m * NTH];
-  // pooled block h covers rows [h pr, (h+1) pr); its centre is at row (h + 1/2) pr - 1/2
-  const float y = ((float)i + 0.5f) / (float)g.pr - 0.5f;
-  const int HW = g.H * g.W;
-  // (dre + i dim) exp(-i w0 t) at sample j of the window
-  auto corr = [&](int j) {
-    const float x = ((float)(j - g.t0) + 0.5f) / (float)g.pt - 0.5f;
-    const cx<float> d = mk(scale * bilinear(D, g.H, g.W, y, x), scale * bilinear(D + HW, g.H, g.W, y, x));
-    return d * conj(g.carrier[j]);
-  };
+  auto corr = [&](int j) { return corr_value(g, D, scale, i, j); };
   const float inv = 1.0f / (float)N;
   if constexpr (fft_regs_ok<N, NTH>()) {
     constexpr int TP = N / NTH;

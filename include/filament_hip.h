@@ -300,7 +300,11 @@ int fil_corrector_train(fil_handle* h, int iters, int batch, double lr, uint64_t
 int fil_corrector_eval(fil_handle* h, int first, int count, double* loss, float* grad);
 /* Network output Y [2][H][W] for features X [4][H][W] from the host.  Blocks. */
 int fil_corrector_forward(fil_handle* h, const float* X, float* Y);
-/* Adds the network's correction to the current state.  Asynchronous. */
+/* Adds the network's correction to the current state.  Asynchronous.  The
+ * network runs now; its output is added in time by the next fil_zstep's row
+ * kernel (before the pending field filter), or first by any other call that
+ * reads or replaces the state (download, analyze, fluence, state save/copy,
+ * synchronize, corrector calls). */
 int fil_corrector_apply(fil_handle* h);
 
 /* Per-phase device timing with HIP events on the handle's stream (the

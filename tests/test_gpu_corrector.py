@@ -129,6 +129,34 @@ def test_apply_matches_oracle(pkg, oc):
     assert e < 1e-3
 
 
+def test_deferred_apply_in_zstep(pkg):
+    """fil_corrector_apply leaves its output to the next z-step's row kernel,
+    which adds it before the pending field filter.  That step must give the
+    state that adding the correction first (fil_synchronize does) and then
+    stepping gives; a step without the correction differs by far more."""
+    p = small(pkg)
+    dz = p["dz_initial"]
+    out = {}
+    for mode in ("fused", "materialized", "none"):
+        m, c = make(pkg, p, rho_scale=1e-17)
+        rng = np.random.default_rng(5)
+        c.set_params((rng.standard_normal(pkg.lib.CORR_NPARAM) * 0.1).astype(np.float32))
+        c.y_scale = 1e-3
+        m.zstep(dz, dz)                  # leaves its field filter pending
+        if mode != "none":
+            c.apply()
+        if mode == "materialized":
+            m.synchronize()              # adds the correction now
+        m.zstep(2 * dz, dz)
+        out[mode] = m.field()
+        m.close()
+    d_corr = rel(out["fused"], out["none"])
+    d_path = rel(out["fused"], out["materialized"])
+    print(f"deferred apply: correction changes the step by {d_corr:.2e}, fused vs materialized {d_path:.2e}")
+    assert d_corr > 1e-6
+    assert d_path < 1e-3 * d_corr
+
+
 def test_training_reduces_loss(pkg):
     p = small(pkg)
     m, c = make(pkg, p, capacity=32, batch_max=8)
