@@ -37,7 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (fetch x 2.00, write x 1.00,
 # calibrated on a 16 MiB device copy), RT 1024x4096: profiles/rt1024x4096_hbm_traffic_pmc.txt
-TRAFFIC_BYTES = {"row_kernel": 38.2e6, "dht_gemm_bf16x3": 78.8e6}
+TRAFFIC_BYTES = {"row_kernel": 39.9e6, "dht_gemm_bf16x3": 83.3e6}
 GATE = 1e-3
 
 
