@@ -155,7 +155,9 @@ def main():
                     help="timed fine-step equivalents (2000: the workload's full 1 m at dz = 0.5 mm)")
     ap.add_argument("--warmup", type=int, default=16, help="warm-up fine-step equivalents")
     ap.add_argument("--factor", type=int, default=16, help="coarse step = factor * dz in the chosen form")
-    ap.add_argument("--check-every", type=int, default=50, help="coarse steps between fine-step verifications")
+    ap.add_argument("--check-every", type=int, default=800,
+                    help="fine-step equivalents (z distance in dz) between fine-step verifications; "
+                         "a factor F checks every max(1, N/F) coarse steps")
     ap.add_argument("--check-tol", type=float, default=1e-4,
                     help="relative fluence distance of one coarse step from its fine steps that triggers the fallback")
     ap.add_argument("--no-corrector", action="store_true")
@@ -164,6 +166,8 @@ def main():
     ap.add_argument("--corr-iters", type=int, default=1500)
     ap.add_argument("--cpu-steps", type=int, default=30, help="oracle z-steps timed for the CPU baseline (~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--factor-sweep", default="4,8",
+                    help="other coarse factors run (own corrector each) and reported beside the headline factor")
     ap.add_argument("--timing-every", type=int, default=8,
                     help="per-kernel HIP-event timing on every n-th z-step of the timed region (1: every step)")
     ap.add_argument("--forms", default="fp64,plain,chosen",
@@ -211,18 +215,33 @@ def main():
             draws = pkg.corrector.ensemble_draws(args.corr_draws, seed=1)
             cst, cinfo = train_corrector(pkg, p, chosen_form, dz, F, draws, args.corr_pairs, args.corr_iters, geom)
         log(f"chosen: coarse x{F}, corrector {'off' if cst is None else 'on'}")
-        msc, ktc, Frc, timed = run_coarse(pkg, p, chosen_form, W, K, dz, F, args.check_every, args.check_tol,
+        msc, ktc, Frc, timed = run_coarse(pkg, p, chosen_form, W, K, dz, F, max(1, args.check_every // F), args.check_tol,
                                           cst, geom, args.timing_every)
         forms["chosen"] = dict(
             ms_per_fine_step_equivalent=msc, z_steps_per_s=1e3 / msc, kernel_ms_per_zstep=ktc,
             fluence_err_vs_fp64=fluence_error(Frc, Fr64) if Fr64 is not None else None,
             description=f"FP32 + split-BF16 Hankel MFMA + FP64 plasma scan, coarse steps of {F} dz"
                         f"{'' if cst is None else ' + learned corrector'}, fine-step verification every "
-                        f"{args.check_every} coarse steps (tol {args.check_tol:g})",
+                        f"{max(1, args.check_every // F)} coarse steps ({args.check_every} dz, tol {args.check_tol:g})",
             timed_region=timed, corrector_training=cinfo)
         log(f"chosen: {msc:.4f} ms per fine-step equivalent, {timed}")
         if Fr64 is not None:
             log(f"chosen: fluence err vs fp64 {forms['chosen']['fluence_err_vs_fp64']:.3e}")
+        # the same form at other coarse factors (4 dz is the learned part's design point), each with
+        # a corrector trained for that factor; reported only, the headline is --factor
+        sweep = []
+        for f in [int(x) for x in args.factor_sweep.split(",") if x.strip()]:
+            if f == F or K % f or W % f:
+                continue
+            cs = None
+            if not args.no_corrector:
+                cs, _ = train_corrector(pkg, p, chosen_form, dz, f, draws, args.corr_pairs, args.corr_iters, geom)
+            msf, _, Frf, tf = run_coarse(pkg, p, chosen_form, W, K, dz, f, max(1, args.check_every // f), args.check_tol, cs, geom,
+                                         args.timing_every)
+            ef = fluence_error(Frf, Fr64) if Fr64 is not None else None
+            sweep.append(dict(factor=f, z_steps_per_s=1e3 / msf, fluence_err_vs_fp64=ef, timed_region=tf))
+            log(f"chosen x{f}: {msf:.4f} ms per fine-step equivalent, fluence err {ef}")
+        forms["chosen"]["factor_sweep"] = sweep
     if not full:
         print(json.dumps({"profiling_forms": forms}))
         return

@@ -595,7 +595,9 @@ double normal01(uint64_t& s) {
 int corr_bias_off(int l) { return corr_woff(l) + corr_cout(l) * corr_cin(l) * 9; }
 
 // network forward over B samples of `in` (gathered by idx when given) into corr.out
-int corr_forward(fil_handle* h, const float* in, const int* idx, int B) {
+// in: pooled features [B][4][H][W] (idx selects stored samples), or with
+// from_rows the per-row features Frow of one sample (layer 0 pools them)
+int corr_forward(fil_handle* h, const float* in, const int* idx, int B, bool from_rows = false) {
   auto& c = h->corr;
   const float* w = c.w.as<float>();
   const float* wf = c.wfm.as<float>();
@@ -603,8 +605,11 @@ int corr_forward(fil_handle* h, const float* in, const int* idx, int B) {
   const int* sidx = idx;
   for (int l = 0; l < CORR_LAYERS; l++) {
     float* dst = (l == CORR_LAYERS - 1) ? c.out.as<float>() : c.act[l].as<float>();
-    HIPCHK(launch_conv_fwd(l, src, sidx, wf + corr_wfm_off(l), w + corr_bias_off(l), dst, B, c.g.H, c.g.W,
-                           l < CORR_LAYERS - 1 ? 1 : 0, h->st));
+    if (l == 0 && from_rows)
+      HIPCHK(launch_conv_fwd_pooled(src, c.g.pr, wf, w + corr_bias_off(0), dst, c.g.H, c.g.W, h->st));
+    else
+      HIPCHK(launch_conv_fwd(l, src, sidx, wf + corr_wfm_off(l), w + corr_bias_off(l), dst, B, c.g.H, c.g.W,
+                             l < CORR_LAYERS - 1 ? 1 : 0, h->st));
     src = dst;
     sidx = nullptr;
   }
@@ -1710,8 +1715,8 @@ int fil_corrector_apply(fil_handle* h) {
   if (!(c.y_scale > 0)) return fail(FIL_ESTATE, "the corrector is untrained (y_scale is 0)");
   HIPCHK(hipSetDevice(h->d.device));
   HIPCHK(launch_corr_features(h->S.as<cx<float>>(), h->tw.as<cx<float>>(), h->pp32, h->d.plasma, c.g, h->Nt,
-                              c.Frow.as<float>(), c.feat.as<float>(), h->st));
-  if (int rc = corr_forward(h, c.feat.as<float>(), nullptr, 1)) return rc;
+                              c.Frow.as<float>(), nullptr, h->st));
+  if (int rc = corr_forward(h, c.Frow.as<float>(), nullptr, 1, true)) return rc;
   // the output is added to the field by the next z-step's row kernel (before
   // its field filter), or by corr_materialize when anything else comes first
   h->corr_scale = (float)(c.y_scale / (double)c.g.inv_as);

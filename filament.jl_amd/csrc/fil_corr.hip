@@ -43,10 +43,13 @@ constexpr int MT = 64 * (MY * MX / 16);    // threads: one wave per 16 pixels
 // half-wave reads (adjacent channels) fall in different LDS banks
 constexpr int conv_x_stride() { return ((MY + 2) * (MX + 2) + 15) / 32 * 32 + 16; }
 
-template <int CIN, int COUT>
+// POOL: the input is the per-row feature array Frow [H pr][CIN][W] of one
+// sample, averaged over the pr rows of each pooled row while it is staged
+// (the same sums, in the same order, as corr_pool_rows_kernel).
+template <int CIN, int COUT, bool POOL = false>
 __global__ __launch_bounds__(MT) void conv_fwd_kernel(const float* __restrict__ in, const int* __restrict__ idx,
                                                       const float* __restrict__ wfm, const float* __restrict__ bias,
-                                                      float* __restrict__ out, int H, int W, int relu) {
+                                                      float* __restrict__ out, int H, int W, int relu, int pr = 1) {
   static_assert(CIN % 4 == 0, "a k step covers four input channels");
   constexpr int K = 9 * CIN, AS = (K + 31) / 32 * 32 + 2;
   constexpr int HY = MY + 2, HX = MX + 2, XS = conv_x_stride();
@@ -75,7 +78,18 @@ __global__ __launch_bounds__(MT) void conv_fwd_kernel(const float* __restrict__ 
   for (int j = 0; j < PI; j++) {
     const int e = tid + j * MT;
     const int ci = e / (HY * HX), r = e % (HY * HX), yy = y0 - 1 + r / HX, xx = x0 - 1 + r % HX;
-    xv_[j] = (e < NIN && yy >= 0 && yy < H && xx >= 0 && xx < W) ? inb[(size_t)ci * HW + yy * W + xx] : 0.0f;
+    const bool inside = e < NIN && yy >= 0 && yy < H && xx >= 0 && xx < W;
+    if constexpr (POOL) {
+      float s = 0.0f;
+      if (inside) {
+        const float* src = in + ((size_t)yy * pr * CIN + ci) * W + xx;
+#pragma unroll 8
+        for (int k = 0; k < pr; k++) s += src[(size_t)k * CIN * W];
+      }
+      xv_[j] = inside ? s / (float)pr : 0.0f;
+    } else {
+      xv_[j] = inside ? inb[(size_t)ci * HW + yy * W + xx] : 0.0f;
+    }
   }
 #pragma unroll
   for (int j = 0; j < PW; j++) {
@@ -299,6 +313,15 @@ static_assert(CORR_LAYERS == 4, "layer dispatch covers four layers");
 hipError_t launch_conv_weight_layout(const uint16_t* wbf, float* wfm, hipStream_t st) {
   constexpr int n = corr_wfm_size();
   hipLaunchKernelGGL(conv_weight_layout_kernel, dim3((n + 255) / 256), dim3(256), 0, st, wbf, wfm);
+  return hipGetLastError();
+}
+
+hipError_t launch_conv_fwd_pooled(const float* Frow, int pr, const float* wfm, const float* bias, float* out, int H,
+                                  int W, hipStream_t st) {
+  static_assert(corr_cin(0) == CORR_IN && corr_cout(0) == CORR_CH, "layer 0");
+  const int tiles = ((H + MY - 1) / MY) * ((W + MX - 1) / MX);
+  hipLaunchKernelGGL((conv_fwd_kernel<CORR_IN, CORR_CH, true>), dim3(tiles, 1, (CORR_CH + 15) / 16), dim3(MT), 0, st,
+                     Frow, nullptr, wfm, bias, out, H, W, 1, pr);
   return hipGetLastError();
 }
 

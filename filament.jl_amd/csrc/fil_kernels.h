@@ -72,7 +72,9 @@ template <typename R>
 hipError_t launch_rowsum_abs2(const cx<R>* S, int Nr, int Nh, double scale, double* out, hipStream_t st);
 
 // Features of the current state, X[4][H][W]: Re A, Im A (/a_s), |E|^2 (/a_s^2),
-// rho (/rho_s), each averaged over its pr x pt block.  Frow: scratch [nr][4][W].
+// rho (/rho_s), each averaged over its pr x pt block.  Frow: [nr][4][W], the
+// per-row block averages; X null: Frow only (the pooling over rows is left to
+// launch_conv_fwd_pooled).
 hipError_t launch_corr_features(const cx<float>* S, const cx<float>* tw, const PlasmaParams<float>& pp, int plasma,
                                 const CorrGeom& g, int N, float* Frow, float* X, hipStream_t st);
 // S rows [0, nr): E += scale * D(r, t) exp(-i w0 t) with D[2][H][W] bilinearly
@@ -116,6 +118,10 @@ hipError_t launch_conv_weight_layout(const uint16_t* wbf, float* wfm, hipStream_
 // in + (idx ? idx[b] : b) * cin*H*W; wfm: the layer's block of the forward layout.
 hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const float* wfm, const float* bias, float* out,
                            int B, int H, int W, int relu, hipStream_t st);
+// layer 0 of one sample straight from the per-row features Frow [H pr][4][W]
+// (pooled over pr rows as it is staged; ReLU)
+hipError_t launch_conv_fwd_pooled(const float* Frow, int pr, const float* wfm, const float* bias, float* out, int H,
+                                  int W, hipStream_t st);
 // din = conv^T(dout) (* [act > 0] when act != null)
 hipError_t launch_conv_bwd_data(int l, const float* dout, const uint16_t* wbf, const float* act, float* din, int B,
                                 int H, int W, hipStream_t st);

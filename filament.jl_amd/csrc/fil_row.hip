@@ -829,9 +829,16 @@ __global__ __launch_bounds__(NTH) void corr_features_kernel(const cx<float>* __r
   if (plasma) plasma_row<float, float, N, NTH>(Z, rho, kd, pp, g.dt, red);
   float* out = Frow + (size_t)i * 4 * g.W;
   const float inv = 1.0f / (float)g.pt;
-  for (int b = tid; b < g.W; b += NTH) {
+  // tpb consecutive lanes share a block (a power of two dividing pt, at most
+  // NTH / W): each sums pt / tpb consecutive samples, then the group adds its
+  // partial sums by lane exchanges
+  int tpb = 1;
+  while (tpb < 64 && 2 * tpb * g.W <= NTH && g.pt % (2 * tpb) == 0) tpb *= 2;
+  const int per = g.pt / tpb;
+  for (int q = tid; q < g.W * tpb; q += NTH) {
+    const int b = q / tpb, s0 = (q % tpb) * per;
     float sr = 0, si = 0, sI = 0, sn = 0;
-    for (int s = 0; s < g.pt; s++) {
+    for (int s = s0; s < s0 + per; s++) {
       const int j = g.t0 + b * g.pt + s;
       const cx<float> e = Z[lds_index(j)];
       const cx<float> a = e * g.carrier[j];
@@ -840,10 +847,18 @@ __global__ __launch_bounds__(NTH) void corr_features_kernel(const cx<float>* __r
       sI += norm2(e);
       if (plasma) sn += rho[rk_index<float>(j)];
     }
-    out[b] = sr * inv * g.inv_as;
-    out[g.W + b] = si * inv * g.inv_as;
-    out[2 * g.W + b] = sI * inv * g.inv_as * g.inv_as;
-    out[3 * g.W + b] = sn * inv * g.inv_rs;
+    for (int off = 1; off < tpb; off <<= 1) {
+      sr += __shfl_xor(sr, off, 64);
+      si += __shfl_xor(si, off, 64);
+      sI += __shfl_xor(sI, off, 64);
+      sn += __shfl_xor(sn, off, 64);
+    }
+    if (q % tpb == 0) {
+      out[b] = sr * inv * g.inv_as;
+      out[g.W + b] = si * inv * g.inv_as;
+      out[2 * g.W + b] = sI * inv * g.inv_as * g.inv_as;
+      out[3 * g.W + b] = sn * inv * g.inv_rs;
+    }
   }
 }
 
@@ -1005,6 +1020,7 @@ hipError_t launch_corr_features(const cx<float>* S, const cx<float>* tw, const P
     hipError_t e1 = hipGetLastError();
     if (e1 != hipSuccess) return e1;
   });
+  if (!X) return hipSuccess;
   const int n = 4 * g.H * g.W;
   hipLaunchKernelGGL(corr_pool_rows_kernel, dim3((n + 255) / 256), dim3(256), 0, st, Frow, g, X);
   return hipGetLastError();
