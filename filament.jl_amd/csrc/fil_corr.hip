@@ -37,7 +37,7 @@ constexpr int CG = 8;     // channels per thread in the forward / data-gradient 
 // row w / 2, columns (w % 2) * 16 .. + 15.  k runs tap-major so a 4-wide k
 // step is four consecutive input channels at one tap.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-constexpr int MY = 2, MX = 32;             // pixel tile of the forward kernel
+constexpr int MY = 2, MX = 32;             // pixel tile of the forward kernel (one-row tiles measured slower)
 constexpr int MT = 64 * (MY * MX / 16);    // threads: one wave per 16 pixels
 // channel stride of the staged input tile: = 16 (mod 32), so the two k rows a
 // half-wave reads (adjacent channels) fall in different LDS banks
