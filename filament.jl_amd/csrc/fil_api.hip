@@ -88,6 +88,7 @@ struct fil_handle {
   // next z-step's row kernel adds it; any other reader of S adds it first
   bool corr_pending = false;
   float corr_scale = 0;
+  int tiling = FIL_TILING_RULE;  // MFMA tile shape of the Hankel products (fil_set_tiling)
   double P_dz = NAN;
   double dt = 0;
   double scale[4] = {1, 1, 1, 1};
@@ -159,6 +160,7 @@ struct fil_group {
   struct Entry {
     double dz;
     uint64_t pattern;                  // bit k: handle k had a pending field filter
+    std::vector<int> tiling;           // the handles' tile shapes when recorded
     hipGraphExec_t exec;
   };
   std::vector<Entry> cache;            // at most 4 instantiated graphs
@@ -353,14 +355,14 @@ int zstep_impl(fil_handle* h, double dz) {
     // X travels as its four BF16 planes (same bytes as FP32 complex)
     FIL_TIMED(launch_dht_gemm_bf16x3_to_planes(h->M1h.as<uint16_t>(), h->M1l.as<uint16_t>(), h->S.as<cx<float>>(),
                                                h->X.as<uint16_t>(), h->P.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh,
-                                               h->st));
+                                               h->st, h->tiling));
     FIL_TIMED(launch_dht_gemm_bf16x3_from_planes(h->M2h.as<uint16_t>(), h->M2l.as<uint16_t>(), h->X.as<uint16_t>(),
-                                                 h->S.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh, h->st));
+                                                 h->S.as<cx<float>>(), h->Nrp, h->Nh * h->B, h->Nh, h->st, h->tiling));
   } else {
     FIL_TIMED(launch_dht_gemm<R>(h->M1.as<R>(), h->S.as<cx<R>>(), h->X.as<cx<R>>(), h->P.as<cx<R>>(), h->Nrp,
-                                 h->Nh * h->B, h->Nh, h->st));
+                                 h->Nh * h->B, h->Nh, h->st, h->tiling));
     FIL_TIMED(launch_dht_gemm<R>(h->M2.as<R>(), h->X.as<cx<R>>(), h->S.as<cx<R>>(), nullptr, h->Nrp, h->Nh * h->B,
-                                 h->Nh, h->st));
+                                 h->Nh, h->st, h->tiling));
   }
 #undef FIL_TIMED
   h->pending_filter = true;
@@ -1237,14 +1239,16 @@ int fil_group_zstep(fil_group* g, double z, double dz) {
     if (int rc = corr_materialize(h)) return rc;
   // the propagators for this dz, outside the graph
   uint64_t pattern = 0;
+  std::vector<int> tiling(n);
   for (int k = 0; k < n; k++) {
     fil_handle* h = g->hs[k];
     if (int rc = h->fp64 ? update_propagator<double>(h, dz) : update_propagator<float>(h, dz)) return rc;
     if (h->pending_filter) pattern |= 1ull << k;
+    tiling[k] = h->tiling;
   }
   hipGraphExec_t exec = nullptr;
   for (auto& c : g->cache)
-    if (c.dz == dz && c.pattern == pattern) exec = c.exec;
+    if (c.dz == dz && c.pattern == pattern && c.tiling == tiling) exec = c.exec;
   if (!exec) {
     if (g->cache.size() >= 4) {
       HIPCHK(hipStreamSynchronize(g->st));
@@ -1252,7 +1256,7 @@ int fil_group_zstep(fil_group* g, double z, double dz) {
       g->cache.erase(g->cache.begin());
     }
     if (int rc = group_capture(g, dz, &exec)) return rc;
-    g->cache.push_back({dz, pattern, exec});
+    g->cache.push_back({dz, pattern, tiling, exec});
     g->captures++;
   }
   // the graph runs after each handle's earlier work and before its later work
@@ -1465,6 +1469,21 @@ int fil_destroy(fil_handle* h) {
   for (auto e : h->ev) (void)hipEventDestroy(e);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
+  return FIL_OK;
+}
+
+int fil_set_tiling(fil_handle* h, int tiling) {
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (tiling != FIL_TILING_RULE && tiling != FIL_TILING_128 && tiling != FIL_TILING_64)
+    return fail(FIL_EINVAL, "tiling must be FIL_TILING_RULE, FIL_TILING_128 or FIL_TILING_64");
+  if (tiling == FIL_TILING_128 && h->Nrp % 128) return fail(FIL_EINVAL, "128-row tiles need Nr (padded to 64) % 128 == 0");
+  h->tiling = tiling;
+  return FIL_OK;
+}
+
+int fil_get_tiling(fil_handle* h, int* tiling) {
+  if (!valid_handle(h) || !tiling) return fail(FIL_EINVAL, "null argument");
+  *tiling = h->tiling;
   return FIL_OK;
 }
 

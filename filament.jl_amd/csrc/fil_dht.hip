@@ -176,12 +176,17 @@ __global__ __launch_bounds__(256) void dht_gemm_valu_kernel(const R* __restrict_
 // Tile shape by the size of one pulse's transform (Nr x Pw): a batch of
 // pulses takes the same kernel, hence the same summation order, as each
 // pulse on its own.
-static bool wide_tiles(int Nr, int Pw) { return Nr % 128 == 0 && (long)(Nr / 128) * (Pw / 64) >= 256; }
+static bool wide_tiles(int Nr, int Pw, int tiling) {
+  if (tiling == FIL_TILING_128) return true;
+  if (tiling == FIL_TILING_64) return false;
+  return Nr % 128 == 0 && (long)(Nr / 128) * (Pw / 64) >= 256;
+}
 
 template <typename R>
 hipError_t launch_dht_gemm(const R* MT, const cx<R>* In, cx<R>* Out, const cx<R>* P, int Nr, int Nh, int Pw,
-                           hipStream_t st) {
+                           hipStream_t st, int tiling) {
   if (Nr % 64 || Nh % 64 || Pw < 1 || (Pw & (Pw - 1)) || Nh % Pw) return hipErrorInvalidValue;
+  if (tiling == FIL_TILING_128 && Nr % 128) return hipErrorInvalidValue;
   if (sizeof(R) == 8) {
     fil_launch(dht_gemm_valu_kernel<R>, dim3(Nh / 64, Nr / 64), dim3(256), 0, st, MT, In, Out, P, Nr, Nh, Pw);
     return hipGetLastError();
@@ -190,7 +195,7 @@ hipError_t launch_dht_gemm(const R* MT, const cx<R>* In, cx<R>* Out, const cx<R>
   const cx<float>* B = (const cx<float>*)In;
   cx<float>* C = (cx<float>*)Out;
   const cx<float>* PP = (const cx<float>*)P;
-  if (wide_tiles(Nr, Pw)) {
+  if (wide_tiles(Nr, Pw, tiling)) {
     // 8 waves of 32 x 32 (two per SIMD: one workgroup per CU at Nr = 1024)
     fil_launch(dht_gemm_f32_kernel<128, 64, 32, 32, 32>, dim3(Nh / 64, Nr / 128), dim3(512), 0, st, A, B, C, PP, Nr,
                Nh, Pw);
@@ -223,8 +228,10 @@ hipError_t launch_propagator(const double* KZ64, const double* KG64, double dz, 
   return hipGetLastError();
 }
 
-template hipError_t launch_dht_gemm<float>(const float*, const cx<float>*, cx<float>*, const cx<float>*, int, int, int, hipStream_t);
-template hipError_t launch_dht_gemm<double>(const double*, const cx<double>*, cx<double>*, const cx<double>*, int, int, int, hipStream_t);
+template hipError_t launch_dht_gemm<float>(const float*, const cx<float>*, cx<float>*, const cx<float>*, int, int, int,
+                                           hipStream_t, int);
+template hipError_t launch_dht_gemm<double>(const double*, const cx<double>*, cx<double>*, const cx<double>*, int, int,
+                                            int, hipStream_t, int);
 template hipError_t launch_propagator<float>(const double*, const double*, double, cx<float>*, int, int, hipStream_t);
 template hipError_t launch_propagator<double>(const double*, const double*, double, cx<double>*, int, int, hipStream_t);
 
@@ -493,8 +500,9 @@ __global__ __launch_bounds__(64 * NWM * NWW * KS) void dht_gemm_bf16x3_kernel(co
 
 template <bool BIN_PLANES, bool OUT_PLANES>
 static hipError_t launch_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const void* In, void* Out,
-                                const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st) {
+                                const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st, int tiling) {
   if (Nr % 64 || Nh % 64 || Pw < 1 || (Pw & (Pw - 1)) || Nh % Pw) return hipErrorInvalidValue;
+  if (tiling == FIL_TILING_128 && Nr % 128) return hipErrorInvalidValue;
   // the kernels address their operands with 32-bit byte offsets
   if ((size_t)Nr * Nh * 8 >= ((size_t)1 << 31)) return hipErrorInvalidValue;
   // 8 waves (two per SIMD): one workgroup per CU at Nr = 1024 (the LDS stages
@@ -503,7 +511,7 @@ static hipError_t launch_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const 
   // are those of 4 waves (43.0/34.3 -> 40.3/30.6 us at 1024 x 4096 against
   // one group of 4 x 2 waves)
   constexpr int KS = 2, NWM = 2, NWW = 2;
-  if (wide_tiles(Nr, Pw)) {
+  if (wide_tiles(Nr, Pw, tiling)) {
     fil_launch(dht_gemm_bf16x3_kernel<128, 64, 64, NWM, NWW, KS, BIN_PLANES, OUT_PLANES>, dim3(Nh / 64, Nr / 128),
                dim3(64 * NWM * NWW * KS), 0, st, Mhi, Mlo, In, Out, P, Nr, Nh, Pw);
   } else {
@@ -514,13 +522,14 @@ static hipError_t launch_bf16x3(const uint16_t* Mhi, const uint16_t* Mlo, const 
 }
 
 hipError_t launch_dht_gemm_bf16x3_to_planes(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In,
-                                            uint16_t* Xp, const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st) {
-  return launch_bf16x3<false, true>(Mhi, Mlo, In, Xp, P, Nr, Nh, Pw, st);
+                                            uint16_t* Xp, const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st,
+                                            int tiling) {
+  return launch_bf16x3<false, true>(Mhi, Mlo, In, Xp, P, Nr, Nh, Pw, st, tiling);
 }
 
 hipError_t launch_dht_gemm_bf16x3_from_planes(const uint16_t* Mhi, const uint16_t* Mlo, const uint16_t* Xp,
-                                              cx<float>* Out, int Nr, int Nh, int Pw, hipStream_t st) {
-  return launch_bf16x3<true, false>(Mhi, Mlo, Xp, Out, nullptr, Nr, Nh, Pw, st);
+                                              cx<float>* Out, int Nr, int Nh, int Pw, hipStream_t st, int tiling) {
+  return launch_bf16x3<true, false>(Mhi, Mlo, Xp, Out, nullptr, Nr, Nh, Pw, st, tiling);
 }
 
 }  // namespace fil

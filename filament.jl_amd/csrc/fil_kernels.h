@@ -172,15 +172,21 @@ template <typename R>
 // Nh: complex columns of In/Out (the batch's members side by side); the propagator P
 // [Nr][Pw] is shared by the members: Out[m][w] *= P[m][w mod Pw].
 hipError_t launch_dht_gemm(const R* M, const cx<R>* In, cx<R>* Out, const cx<R>* P, int Nr, int Nh, int Pw,
-                           hipStream_t st);
+                           hipStream_t st, int tiling = 0);
+// Tile shapes of the MFMA products (M rows per workgroup): FIL_TILING_RULE picks
+// 128-row tiles when they still give >= 256 workgroups for one pulse, else 64;
+// the other two force one shape (128 needs Nr % 128 == 0).  The k summation
+// order of every output is the same for all shapes, so results do not depend
+// on the choice.
 // split-BF16 3-term variant, M given as hi/lo bf16 planes.  The first product
 // writes X = P .* (M1 In) as four BF16 planes Xp [re_hi, re_lo, im_hi, im_lo][Nh][Nr]
 // (the split of the FP32 X, k of the second product contiguous); the second
 // reads them: Out = M2 X.
 hipError_t launch_dht_gemm_bf16x3_to_planes(const uint16_t* Mhi, const uint16_t* Mlo, const cx<float>* In,
-                                            uint16_t* Xp, const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st);
+                                            uint16_t* Xp, const cx<float>* P, int Nr, int Nh, int Pw, hipStream_t st,
+                                            int tiling = 0);
 // (Pw: one pulse's columns, which picks the tile shape as in the first product)
 hipError_t launch_dht_gemm_bf16x3_from_planes(const uint16_t* Mhi, const uint16_t* Mlo, const uint16_t* Xp,
-                                              cx<float>* Out, int Nr, int Nh, int Pw, hipStream_t st);
+                                              cx<float>* Out, int Nr, int Nh, int Pw, hipStream_t st, int tiling = 0);
 
 }  // namespace fil

@@ -19,6 +19,7 @@ FP32, FP64 = 4, 8
 FORM_DHT_BF16X3, FORM_SCAN_FP64 = 1, 2
 RESP_CUBIC_THG, RESP_CUBIC_NOTHG, RESP_FREE, RESP_LOSSES = 1, 2, 4, 8
 RESP_RAMAN_THG, RESP_RAMAN_NOTHG = 16, 32
+TILING_RULE, TILING_128, TILING_64 = 0, 1, 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -77,6 +78,7 @@ EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_set_tables_xyt
            "fil_analyze_member", "fil_analyze_xyt", "fil_download_plasma", "fil_synchronize", "fil_destroy",
            "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_state_copy",
            "fil_fluence", "fil_group_create", "fil_group_zstep", "fil_group_stats", "fil_group_destroy",
+           "fil_set_tiling", "fil_get_tiling",
            "fil_corrector_init", "fil_corrector_params", "fil_corrector_scale", "fil_corrector_features",
            "fil_corrector_store_pair", "fil_corrector_train", "fil_corrector_eval", "fil_corrector_forward",
            "fil_corrector_apply")
@@ -111,6 +113,8 @@ def load():
     L.fil_group_zstep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
     L.fil_group_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
     L.fil_group_destroy.argtypes = [ctypes.c_void_p]
+    L.fil_set_tiling.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.fil_get_tiling.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     L.fil_zstep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
     L.fil_analyze.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(FilDiag), _dp, _dp, _dp, _dp]
     L.fil_download_plasma.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int64]

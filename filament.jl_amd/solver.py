@@ -190,6 +190,17 @@ class _Model:
         L.check(self._lib.fil_fluence(self.h, int(slot), L.ptr(Fr)))
         return Fr
 
+    def set_tiling(self, tiling):
+        """MFMA tile shape of the Hankel products (lib.TILING_*, see
+        fil_set_tiling); the result does not depend on it."""
+        L.check(self._lib.fil_set_tiling(self.h, int(tiling)))
+
+    @property
+    def tiling(self):
+        t = ctypes.c_int()
+        L.check(self._lib.fil_get_tiling(self.h, ctypes.byref(t)))
+        return t.value
+
     def timing(self, enable=True):
         """Per-kernel timing: False off, True every z-step, an integer n > 1
         every n-th z-step."""

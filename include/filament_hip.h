@@ -200,6 +200,19 @@ int fil_group_zstep(fil_group* g, double z, double dz);
 int fil_group_stats(fil_group* g, int64_t* captures, int64_t* replays);
 int fil_group_destroy(fil_group* g);
 
+/* MFMA tile shape of a handle's Hankel-transform products (rows of the QDHT
+ * matrix per workgroup).  No counterpart in the reference (CUBLAS chooses its
+ * own); it is what the ensemble's autotuner (filament_amd.tuning) sets from its
+ * tile table.  FIL_TILING_RULE (the default): 128-row tiles when they still give
+ * >= 256 workgroups for one pulse, else 64; FIL_TILING_128 needs Nr (padded to
+ * 64) divisible by 128.  Every shape sums each output over k in the same order,
+ * so the choice changes the speed, never the result. */
+#define FIL_TILING_RULE 0
+#define FIL_TILING_128 1
+#define FIL_TILING_64 2
+int fil_set_tiling(fil_handle* h, int tiling);
+int fil_get_tiling(fil_handle* h, int* tiling);
+
 /* FieldAnalyzers.analyze! RT (FieldAnalyzers.jl:166-215), including its side
  * effect of projecting the field onto its analytic part.  Fr [Nr], Ft [Nt],
  * rho_end [Nr] and S [Nt/2+1] may be NULL.  Blocks.

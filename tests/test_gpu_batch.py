@@ -85,6 +85,34 @@ def test_batch_member_errors(pkg):
     ens.close()
 
 
+@pytest.mark.parametrize("form", [0, 3])
+def test_tile_shapes_give_identical_steps(pkg, form):
+    """128-row and 64-row tiles of the Hankel products sum every output over
+    k in the same order: the z-steps agree to the bit (FP32 and split-BF16)."""
+    p = pkg.workloads.rt_air(256, 1024, 3.0, seed=2)
+    dz = p["dz_initial"]
+    out = {}
+    for t in (pkg.lib.TILING_128, pkg.lib.TILING_64):
+        m = pkg.RTModel(p, form=form)
+        m.set_tiling(t)
+        assert m.tiling == t
+        for k in range(3):
+            m.zstep((k + 1) * dz, dz)
+        out[t] = m.field()
+        m.close()
+    assert np.array_equal(out[pkg.lib.TILING_128], out[pkg.lib.TILING_64])
+
+
+def test_tiling_rejects_bad_shapes(pkg):
+    m = pkg.RTModel(pkg.workloads.rt_air(64, 512, 1.0))     # 64 rows: no 128-row tiles
+    with pytest.raises(pkg.lib.FilamentError):
+        m.set_tiling(pkg.lib.TILING_128)
+    with pytest.raises(pkg.lib.FilamentError):
+        m.set_tiling(7)
+    m.set_tiling(pkg.lib.TILING_64)
+    m.close()
+
+
 def test_ensemble_64_groups_match_single_pulses(pkg):
     W = pkg.workloads
     ps = W.rt_ensemble(64, seed=0)
@@ -93,12 +121,15 @@ def test_ensemble_64_groups_match_single_pulses(pkg):
     dz = 5e-4
     nsteps = 4
     ens = [(shape, ix, pkg.RTEnsemble([ps[k] for k in ix], form=3)) for shape, ix in groups]
+    # tile table of this ensemble, timed on its own handles (state restored)
+    table = pkg.tuning.autotune([e for _, _, e in ens], dz, steps=3, log=print)
+    assert len(table) == len(ens)
     t_setup = time.perf_counter()
     for k in range(nsteps):
         for _, _, e in ens:
             e.zstep((k + 1) * dz, dz)
     for shape, ix, e in ens:
-        single = pkg.RTModel(ps[ix[-1]], form=3)
+        single = pkg.RTModel(ps[ix[-1]], form=3)    # the rule's tiles: same result as the table's
         for k in range(nsteps):
             single.zstep((k + 1) * dz, dz)
         err = rel(e.field(len(ix) - 1), single.field())
