@@ -60,6 +60,30 @@ def fluence_error(F, Fref):
     return float(np.linalg.norm(F - Fref) / np.linalg.norm(Fref))
 
 
+def diagnostics(m, z):
+    """End-of-run quantities compared with the FP64 build: the analyzer's
+    fluence profile, peak intensity, peak electron density and pulse energy
+    (FieldAnalyzers.jl:166-215; its sums in FP64), and the on-axis spectrum
+    |E(r=0, w)|^2 of the downloaded field (positive frequencies)."""
+    d = m.analyze(z, arrays=True)
+    E0 = m.field()[0]
+    sp = np.abs(np.fft.ifft(E0)) ** 2
+    return dict(Fr=d["Fr"], Imax=d["Imax"], rhomax=d["rhomax"], W=d["W"], onaxis=sp[: len(sp) // 2 + 1])
+
+
+def errors_vs(ref, d):
+    """Relative errors of a form's end-of-run diagnostics against the FP64 build's."""
+    def rl2(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    def rs(a, b):
+        return float(abs(a - b) / abs(b)) if b != 0 else float(abs(a))
+
+    return dict(fluence=rl2(d["Fr"], ref["Fr"]), peak_intensity=rs(d["Imax"], ref["Imax"]),
+                max_electron_density=rs(d["rhomax"], ref["rhomax"]), onaxis_spectrum=rl2(d["onaxis"], ref["onaxis"]),
+                energy=rs(d["W"], ref["W"]))
+
+
 def kernel_ms(kt):
     n = max(1, kt["count"])
     return dict(row=kt["row_ms"] / n, dht1=kt["dht1_ms"] / n, dht2=kt["dht2_ms"] / n)
@@ -70,6 +94,7 @@ def run_fine(pkg, p, precision, form, W, K, dz, every):
     ``every``-th timed step (events on the library's stream, carried by the
     kernel dispatches)."""
     m = pkg.RTModel(p, precision=precision, form=form)
+    W0 = m.analyze(0.0)["W"]
     z = 0.0
     for _ in range(W):
         z += dz
@@ -84,15 +109,17 @@ def run_fine(pkg, p, precision, form, W, K, dz, every):
     ms = (time.perf_counter() - t0) * 1e3 / K
     kt = m.timing_read()
     m.timing(False)
-    Fr = m.analyze(z, arrays=True)["Fr"]
+    dg = diagnostics(m, z)
+    dg["energy_ratio"] = dg["W"] / W0
     m.close()
-    return ms, kernel_ms(kt), Fr
+    return ms, kernel_ms(kt), dg
 
 
 def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geom, every):
     """The chosen form: coarse steps (+ corrector) with verification, W/K in
     fine-step equivalents."""
     m = pkg.RTModel(p, precision=pkg.lib.FP32, form=form)
+    W0 = m.analyze(0.0)["W"]
     corr = None
     if corr_state is not None:
         corr = pkg.corrector.Corrector(m, geom=geom, capacity=1, batch_max=1)
@@ -109,12 +136,13 @@ def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geo
     kt = m.timing_read()
     m.timing(False)
     s1 = st.stats()
-    Fr = m.analyze(z, arrays=True)["Fr"]
+    dg = diagnostics(m, z)
+    dg["energy_ratio"] = dg["W"] / W0
     m.close()
     timed = dict(coarse_steps=s1["coarse_steps"] - s0["coarse_steps"], fine_steps=s1["fine_steps"] - s0["fine_steps"],
                  checks=s1["checks"] - s0["checks"], failed_checks=s1["failed_checks"] - s0["failed_checks"],
                  last_check_distance=s1["last_distance"])
-    return ms, kernel_ms(kt), Fr, timed
+    return ms, kernel_ms(kt), dg, timed
 
 
 def train_corrector(pkg, p, form, dz, factor, draws, npairs, iters, geom):
@@ -194,17 +222,24 @@ def main():
     want = set(args.forms.split(","))
     full = want == {"fp64", "plain", "chosen"}
     forms = {}
-    Fr64 = None
+    dg64 = Fr64 = None
+
+    def errs(dg):
+        return errors_vs(dg64, dg) if dg64 is not None else None
+
     if "fp64" in want:
         log("fp64: fine steps")
-        ms64, kt64, Fr64 = run_fine(pkg, p, L.FP64, 0, W, K, dz, args.timing_every)
-        forms["fp64"] = dict(ms_per_step=ms64, z_steps_per_s=1e3 / ms64, kernel_ms=kt64)
+        ms64, kt64, dg64 = run_fine(pkg, p, L.FP64, 0, W, K, dz, args.timing_every)
+        Fr64 = dg64["Fr"]
+        forms["fp64"] = dict(ms_per_step=ms64, z_steps_per_s=1e3 / ms64, kernel_ms=kt64,
+                             energy_ratio_end_to_start=dg64["energy_ratio"])
         log(f"fp64: {ms64:.4f} ms/step")
     if "plain" in want:
         log("plain FP32: fine steps")
-        msp, ktp, Frp = run_fine(pkg, p, L.FP32, 0, W, K, dz, args.timing_every)
+        msp, ktp, dgp = run_fine(pkg, p, L.FP32, 0, W, K, dz, args.timing_every)
         forms["plain_fp32"] = dict(ms_per_step=msp, z_steps_per_s=1e3 / msp, kernel_ms=ktp,
-                                   fluence_err_vs_fp64=fluence_error(Frp, Fr64) if Fr64 is not None else None)
+                                   fluence_err_vs_fp64=fluence_error(dgp["Fr"], Fr64) if Fr64 is not None else None,
+                                   errors_vs_fp64=errs(dgp), energy_ratio_end_to_start=dgp["energy_ratio"])
         log(f"plain FP32: {msp:.4f} ms/step")
     if "chosen" in want:
         m0 = pkg.RTModel(p, precision=L.FP32, form=chosen_form)
@@ -215,11 +250,12 @@ def main():
             draws = pkg.corrector.ensemble_draws(args.corr_draws, seed=1)
             cst, cinfo = train_corrector(pkg, p, chosen_form, dz, F, draws, args.corr_pairs, args.corr_iters, geom)
         log(f"chosen: coarse x{F}, corrector {'off' if cst is None else 'on'}")
-        msc, ktc, Frc, timed = run_coarse(pkg, p, chosen_form, W, K, dz, F, max(1, args.check_every // F), args.check_tol,
+        msc, ktc, dgc, timed = run_coarse(pkg, p, chosen_form, W, K, dz, F, max(1, args.check_every // F), args.check_tol,
                                           cst, geom, args.timing_every)
         forms["chosen"] = dict(
             ms_per_fine_step_equivalent=msc, z_steps_per_s=1e3 / msc, kernel_ms_per_zstep=ktc,
-            fluence_err_vs_fp64=fluence_error(Frc, Fr64) if Fr64 is not None else None,
+            fluence_err_vs_fp64=fluence_error(dgc["Fr"], Fr64) if Fr64 is not None else None,
+            errors_vs_fp64=errs(dgc), energy_ratio_end_to_start=dgc["energy_ratio"],
             description=f"FP32 + split-BF16 Hankel MFMA + FP64 plasma scan, coarse steps of {F} dz"
                         f"{'' if cst is None else ' + learned corrector'}, fine-step verification every "
                         f"{max(1, args.check_every // F)} coarse steps ({args.check_every} dz, tol {args.check_tol:g})",
@@ -236,10 +272,11 @@ def main():
             cs = None
             if not args.no_corrector:
                 cs, _ = train_corrector(pkg, p, chosen_form, dz, f, draws, args.corr_pairs, args.corr_iters, geom)
-            msf, _, Frf, tf = run_coarse(pkg, p, chosen_form, W, K, dz, f, max(1, args.check_every // f), args.check_tol, cs, geom,
-                                         args.timing_every)
-            ef = fluence_error(Frf, Fr64) if Fr64 is not None else None
-            sweep.append(dict(factor=f, z_steps_per_s=1e3 / msf, fluence_err_vs_fp64=ef, timed_region=tf))
+            msf, _, dgf, tf = run_coarse(pkg, p, chosen_form, W, K, dz, f, max(1, args.check_every // f),
+                                         args.check_tol, cs, geom, args.timing_every)
+            ef = fluence_error(dgf["Fr"], Fr64) if Fr64 is not None else None
+            sweep.append(dict(factor=f, z_steps_per_s=1e3 / msf, fluence_err_vs_fp64=ef, errors_vs_fp64=errs(dgf),
+                              timed_region=tf))
             log(f"chosen x{f}: {msf:.4f} ms per fine-step equivalent, fluence err {ef}")
         forms["chosen"]["factor_sweep"] = sweep
     if not full:
