@@ -146,7 +146,9 @@ __host__ __device__ constexpr int lds_step(int c) { return c + (c >> 4); }
 // multiple of 16 (N >= 128); for writes lds_index(w0 + S k) = lds_index(w0) + lds_step(S k) with
 // w0 = q + S RAD p, because S, RAD and 16 are powers of two and q < S keeps
 // the low four bits of w0 + S k from carrying.
-template <typename T, int N, int NTH, int SIGN, int RAD, int S>
+// NAT: the outputs go to z in the natural (unpadded) order, z[i] = y[i]; only
+// for the last pass of a transform whose reader wants that layout.
+template <typename T, int N, int NTH, int SIGN, int RAD, int S, bool NAT = false>
 __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict__ tw, int tid) {
   constexpr int NB = N / RAD;
   constexpr int BPT = (NB + NTH - 1) / NTH;
@@ -195,23 +197,23 @@ __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict_
 #pragma unroll
         for (int k = 1; k < RAD; k++) a[u][k] = a[u][k] * ((SIGN > 0) ? conj(w[u][k]) : w[u][k]);
       }
-      cx<T>* zw = z + lds_index(q + S * RAD * p);
+      cx<T>* zw = z + (NAT ? q + S * RAD * p : lds_index(q + S * RAD * p));
 #pragma unroll
-      for (int k = 0; k < RAD; k++) zw[lds_step(S * k)] = a[u][k];
+      for (int k = 0; k < RAD; k++) zw[NAT ? S * k : lds_step(S * k)] = a[u][k];
     }
   }
   __syncthreads();
 }
 
-template <typename T, int N, int NTH, int SIGN, int S>
+template <typename T, int N, int NTH, int SIGN, int S, bool NAT = false>
 struct FftPasses {
   static __device__ __forceinline__ void run(cx<T>* z, const cx<T>* tw, int tid) {
     constexpr int REM = N / S;
     if constexpr (REM >= FIL_FFT_MAXRAD) {
-      stockham_pass<T, N, NTH, SIGN, FIL_FFT_MAXRAD, S>(z, tw, tid);
-      FftPasses<T, N, NTH, SIGN, S * FIL_FFT_MAXRAD>::run(z, tw, tid);
+      stockham_pass<T, N, NTH, SIGN, FIL_FFT_MAXRAD, S, NAT && REM == FIL_FFT_MAXRAD>(z, tw, tid);
+      FftPasses<T, N, NTH, SIGN, S * FIL_FFT_MAXRAD, NAT>::run(z, tw, tid);
     } else if constexpr (REM > 1) {
-      stockham_pass<T, N, NTH, SIGN, REM, S>(z, tw, tid);
+      stockham_pass<T, N, NTH, SIGN, REM, S, NAT>(z, tw, tid);
     }
   }
 };
@@ -222,14 +224,15 @@ struct FftPasses {
 // tid: the thread's index among the NTH threads working on this transform
 // (several transforms may share a workgroup; every pass ends in a
 // workgroup-wide barrier, so all threads of the workgroup must call it).
-template <typename T, int N, int NTH, int SIGN>
+// NAT_OUT: the result is left in natural order (z[i], no padding) instead.
+template <typename T, int N, int NTH, int SIGN, bool NAT_OUT = false>
 __device__ __forceinline__ void fft_lds(cx<T>* z, const cx<T>* __restrict__ tw, int tid) {
   static_assert((N & (N - 1)) == 0 && N >= 16, "N must be a power of two >= 16");
-  FftPasses<T, N, NTH, SIGN, 1>::run(z, tw, tid);
+  FftPasses<T, N, NTH, SIGN, 1, NAT_OUT>::run(z, tw, tid);
 }
-template <typename T, int N, int NTH, int SIGN>
+template <typename T, int N, int NTH, int SIGN, bool NAT_OUT = false>
 __device__ __forceinline__ void fft_lds(cx<T>* z, const cx<T>* __restrict__ tw) {
-  fft_lds<T, N, NTH, SIGN>(z, tw, (int)threadIdx.x);
+  fft_lds<T, N, NTH, SIGN, NAT_OUT>(z, tw, (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -312,13 +315,13 @@ __device__ __forceinline__ void fft_last_pass_regs(const cx<T>* z, cx<T>* out, i
   }
 }
 
-// registers -> z (natural order, like fft_lds)
-template <typename T, int N, int NTH, int SIGN, bool HALF>
+// registers -> z (stored like fft_lds; NAT_OUT: unpadded natural order)
+template <typename T, int N, int NTH, int SIGN, bool HALF, bool NAT_OUT = false>
 __device__ __forceinline__ void fft_regs_to_lds(const cx<T>* in, cx<T>* z, const cx<T>* __restrict__ tw, int tid) {
   fft_first_pass_regs<T, N, NTH, SIGN, HALF>(in, z, tw, tid);
   FftMidPasses<T, N, NTH, SIGN, 8>::run(z, tw, tid);
   constexpr int S = fft_last_stride(N);
-  stockham_pass<T, N, NTH, SIGN, N / S, S>(z, tw, tid);
+  stockham_pass<T, N, NTH, SIGN, N / S, S, NAT_OUT>(z, tw, tid);
 }
 // registers -> registers, z as the exchange buffer (free on return)
 template <typename T, int N, int NTH, int SIGN, bool HALF>

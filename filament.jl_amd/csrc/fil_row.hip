@@ -381,17 +381,17 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const cx<R>* ev, const R* rh
   for (int m = 0; m < SP; m++) k[m] = mk((R)0, (R)0);
   if (cub || fre) {
     if constexpr (REGS) {
-      fft_regs_to_lds<R, N, NTH, +1, false>(zin, Z, tw, tid);
+      fft_regs_to_lds<R, N, NTH, +1, false, true>(zin, Z, tw, tid);
     } else {
 #pragma unroll
       for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = zin[u];
       __syncthreads();
-      fft_lds<R, N, NTH, +1>(Z, tw);
+      fft_lds<R, N, NTH, +1, true>(Z, tw);
     }
 #pragma unroll
     for (int m = 0; m < SP; m++) {
       const int w = tid + m * NTH;
-      cx<R> zw = Z[lds_index(w)], zm = conj(Z[lds_index((N - w) & (N - 1))]);
+      cx<R> zw = Z[w], zm = conj(Z[(N - w) & (N - 1)]);   // natural order
       cx<R> a2 = zw + zm, b2 = mul_mi(zw - zm);
       k[m] = coef[w] * a2 + coef[NH + w] * b2;
     }
@@ -429,17 +429,17 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const cx<R>* ev, const R* rh
       zin[u] = mk(Fl[u], Ir[u]);
     }
     if constexpr (REGS) {
-      fft_regs_to_lds<R, N, NTH, +1, false>(zin, Z, tw, tid);
+      fft_regs_to_lds<R, N, NTH, +1, false, true>(zin, Z, tw, tid);
     } else {
 #pragma unroll
       for (int u = 0; u < TP; u++) Z[lds_index(tid + u * NTH)] = zin[u];
       __syncthreads();
-      fft_lds<R, N, NTH, +1>(Z, tw);
+      fft_lds<R, N, NTH, +1, true>(Z, tw);
     }
 #pragma unroll
     for (int m = 0; m < SP; m++) {
       const int w = tid + m * NTH;
-      cx<R> zw = Z[lds_index(w)], zm = conj(Z[lds_index((N - w) & (N - 1))]);
+      cx<R> zw = Z[w], zm = conj(Z[(N - w) & (N - 1)]);   // natural order
       k[m] = k[m] + coef[2 * NH + w] * (zw + zm) + coef[3 * NH + w] * mul_mi(zw - zm);
     }
     __syncthreads();
@@ -455,12 +455,12 @@ __device__ __forceinline__ void stage_rhs(cx<R>* Z, const cx<R>* ev, const R* rh
       Zr[2 * lds_index(t >> 1) + (t & 1)] = Fl[u];
     }
     __syncthreads();
-    fft_lds<R, N / 2, NTH, +1>(Z, tw + fft_tw_half_offset(N));
+    fft_lds<R, N / 2, NTH, +1, true>(Z, tw + fft_tw_half_offset(N));
     const cx<R>* post = tw + fft_tw_post_offset(N);
 #pragma unroll
     for (int m = 0; m < SP; m++) {
       const int w = tid + m * NTH;
-      const cx<R> yw = Z[lds_index(w)], ym = conj(Z[lds_index((NH - w) & (NH - 1))]);
+      const cx<R> yw = Z[w], ym = conj(Z[(NH - w) & (NH - 1)]);   // natural order
       const cx<R> A = yw + ym, Bm = yw - ym;
       k[m] = k[m] + coef[2 * NH + w] * (A + mul_mi(post[w] * Bm));
     }
