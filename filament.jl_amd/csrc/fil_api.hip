@@ -134,6 +134,8 @@ struct fil_handle {
   } corr;
   PlasmaParams<float> pp32{};
   PlasmaParams<double> pp64{};
+  // device z loop (fil_run_adaptive): loop state, log, the state at the last host check
+  DevBuf ad_state, ad_log, ad_S, ad_re, ad_rm;
   // timing
   bool timing = false;
   int timing_period = 1;        // time every n-th z-step
@@ -288,10 +290,21 @@ int update_propagator(fil_handle* h, double dz) {
   return 0;
 }
 
+// dz_dev: dz is read from device memory by the propagator and the row kernel
+// (the device z loop); dz is then unused.
 template <typename R>
-int zstep_impl(fil_handle* h, double dz) {
+int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr) {
   h->acache.valid = false;
-  if (int rc = update_propagator<R>(h, dz)) return rc;
+  if (dz_dev) {
+    if (h->xyt)
+      HIPCHK(launch_xyt_propagator<R>(h->xprop, 0.0, h->P.as<cx<R>>(), h->st, dz_dev));
+    else
+      HIPCHK(launch_propagator<R>(h->KZ64.as<double>(), h->KG64.as<double>(), 0.0, h->P.as<cx<R>>(), h->Nrp, h->Nh,
+                                  h->st, dz_dev));
+    h->P_dz = NAN;
+  } else if (int rc = update_propagator<R>(h, dz)) {
+    return rc;
+  }
   RowArgs<R> a{};
   a.S = h->S.as<cx<R>>();
   a.tw = h->tw.as<cx<R>>();
@@ -301,6 +314,7 @@ int zstep_impl(fil_handle* h, double dz) {
   a.hr = h->hr.p ? h->hr.as<cx<R>>() : nullptr;
   for (int q = 0; q < 4; q++) a.sc[q] = (R)h->scale[q];
   a.dz = (R)dz;
+  a.dz_dev = dz_dev;
   a.dt = (R)h->dt;
   a.apply_filter = h->pending_filter ? 1 : 0;
   a.nonlinearity = h->d.nonlinearity;
@@ -495,6 +509,99 @@ int analyze_device(fil_handle* h, AnalyzerRows& v, int member) {
   v.ft.assign(c.ft.begin() + (size_t)member * Nt, c.ft.begin() + (size_t)(member + 1) * Nt);
   v.sp.assign(c.sp.begin() + (size_t)member * (Nh + 1), c.sp.begin() + (size_t)(member + 1) * (Nh + 1));
   c.served[member] = 1;
+  return 0;
+}
+
+// The analyzer's row pass over the whole state as analyze_device's first pass
+// runs it (the plasma solve when stale, the pending filter, the projection,
+// the Imax rows), without the column sums and host copies.
+template <typename R>
+int analysis_rows(fil_handle* h) {
+  if (h->d.plasma && !h->plasma_valid)
+    if (int rc = plasma_only<R>(h)) return rc;
+  HIPCHK(launch_analyze_rows<R>(h->S.as<cx<R>>(), h->tw.as<cx<R>>(), h->Rg.as<R>(), h->Tg.as<R>(),
+                                h->pending_filter ? 1 : 0, (R)h->dt, h->aFr.as<double>(), h->aIt.as<R>(),
+                                h->aSw.as<R>(), h->aImax.as<double>(), h->rows(), h->Nt, 1, 0, h->B, h->st));
+  h->pending_filter = false;
+  h->acache.valid = false;
+  return 0;
+}
+
+// Filament.run's z loop (Filament.jl:129-164) on the device, checked by the
+// host every `chunk` steps.  Steps past the stop within a chunk are undone:
+// the state at the chunk's start is restored and the chunk's logged steps are
+// run again with their dz from the host, which gives the same bits.
+template <typename R>
+int run_adaptive_impl(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log, int* nsteps,
+                      int* reason) {
+  if (int rc = corr_materialize(h)) return rc;
+  const size_t nb = h->ns() * h->csz(), nr = (size_t)h->rows() * 8, nlog = ((size_t)max_steps + 1) * 4 * 8;
+  if (!h->ad_state.p && h->ad_state.alloc(sizeof(AdaptState)) != hipSuccess)
+    return fail(FIL_ENOMEM, "device allocation failed");
+  if (h->ad_log.n < nlog && h->ad_log.alloc(nlog) != hipSuccess) return fail(FIL_ENOMEM, "device allocation failed");
+  if (!h->ad_S.p && (h->ad_S.alloc(nb) != hipSuccess || h->ad_re.alloc(nr) != hipSuccess ||
+                     h->ad_rm.alloc(nr) != hipSuccess))
+    return fail(FIL_ENOMEM, "device allocation failed");
+  AdaptRule rule{};
+  rule.dz0 = p->dz0; rule.phimax = p->phimax; rule.phik = p->phik; rule.phip = p->phip; rule.Istop = p->Istop;
+  rule.zmax = (float)p->zmax;
+  rule.nonlinearity = p->nonlinearity ? 1 : 0;
+  rule.plasma = h->d.plasma ? 1 : 0;
+  rule.nrows = h->rows();
+  rule.max_steps = max_steps;
+  const int chunk = p->chunk > 0 ? p->chunk : 32;
+  AdaptState* dst = h->ad_state.as<AdaptState>();
+  double* dlog = h->ad_log.as<double>();
+  const double* dz_dev = &dst->dz;
+  const double* rm = h->d.plasma ? h->rho_max.as<double>() : nullptr;
+  AdaptState s{};
+  s.z = (float)*z;
+  const double head[2] = {(double)s.z, 0.0};    // log row 0: (z, dz) of the start
+  // the analysis before the first step (Filament.jl:112), then the first rule evaluation
+  if (int rc = analysis_rows<R>(h)) return rc;
+  HIPCHK(hipMemcpyAsync(dst, &s, sizeof s, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(dlog, head, sizeof head, hipMemcpyHostToDevice, h->st));
+  HIPCHK(launch_adapt(rule, dst, h->aImax.as<double>(), rm, dlog, h->st));
+  HIPCHK(hipMemcpyAsync(&s, dst, sizeof s, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  while (!s.done) {
+    const int k0 = s.steps - 1;                    // steps completed before this chunk
+    const int n = std::min(chunk, max_steps - k0);
+    const bool plasma_valid0 = h->plasma_valid;
+    HIPCHK(hipMemcpyAsync(h->ad_S.p, h->S.p, nb, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->ad_re.p, h->rho_end.p, nr, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->ad_rm.p, h->rho_max.p, nr, hipMemcpyDeviceToDevice, h->st));
+    for (int i = 0; i < n; i++) {
+      if (int rc = zstep_impl<R>(h, 0.0, dz_dev)) return rc;
+      if (int rc = analysis_rows<R>(h)) return rc;
+      HIPCHK(launch_adapt(rule, dst, h->aImax.as<double>(), rm, dlog, h->st));
+    }
+    HIPCHK(hipMemcpyAsync(&s, dst, sizeof s, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    // steps the chunk ran that count: s.steps - k0 while running (the last one
+    // proposed but not taken), s.steps - k0 once stopped
+    const int taken = s.done ? s.steps - k0 : s.steps - 1 - k0;
+    if (taken < n) {
+      // the stop came inside the chunk: back to its start, then its counted steps again
+      std::vector<double> lg((size_t)taken * 4);
+      if (taken)
+        HIPCHK(hipMemcpy(lg.data(), dlog + 4 * ((size_t)k0 + 1), lg.size() * 8, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpyAsync(h->S.p, h->ad_S.p, nb, hipMemcpyDeviceToDevice, h->st));
+      HIPCHK(hipMemcpyAsync(h->rho_end.p, h->ad_re.p, nr, hipMemcpyDeviceToDevice, h->st));
+      HIPCHK(hipMemcpyAsync(h->rho_max.p, h->ad_rm.p, nr, hipMemcpyDeviceToDevice, h->st));
+      h->pending_filter = false;
+      h->plasma_valid = plasma_valid0;
+      for (int i = 0; i < taken; i++) {
+        if (int rc = zstep_impl<R>(h, lg[4 * (size_t)i + 1])) return rc;
+        if (int rc = analysis_rows<R>(h)) return rc;
+      }
+    }
+  }
+  if (log) HIPCHK(hipMemcpy(log, dlog, ((size_t)s.steps + 1) * 4 * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipStreamSynchronize(h->st));
+  *z = (double)s.z;
+  *nsteps = s.steps;
+  *reason = s.done;
   return 0;
 }
 
@@ -1147,6 +1254,18 @@ int fil_zstep(fil_handle* h, double z, double dz) {
   if (!std::isfinite(dz)) return fail(FIL_EINVAL, "dz must be finite");
   HIPCHK(hipSetDevice(h->d.device));
   return h->fp64 ? zstep_impl<double>(h, dz) : zstep_impl<float>(h, dz);
+}
+
+int fil_run_adaptive(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log, int* nsteps,
+                     int* reason) {
+  if (!valid_handle(h) || !p || !z || !nsteps || !reason) return fail(FIL_EINVAL, "null argument");
+  if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_run_adaptive");
+  if (h->B != 1) return fail(FIL_EINVAL, "fil_run_adaptive takes single-pulse handles");
+  if (max_steps < 0 || p->chunk < 0) return fail(FIL_EINVAL, "max_steps and chunk must be >= 0");
+  if (!std::isfinite(p->dz0) || !std::isfinite(*z)) return fail(FIL_EINVAL, "dz0 and z must be finite");
+  HIPCHK(hipSetDevice(h->d.device));
+  return h->fp64 ? run_adaptive_impl<double>(h, p, z, max_steps, log, nsteps, reason)
+                 : run_adaptive_impl<float>(h, p, z, max_steps, log, nsteps, reason);
 }
 
 int fil_group_create(fil_handle* const* hs, int n, fil_group** out) {

@@ -208,10 +208,11 @@ hipError_t launch_dht_gemm(const R* MT, const cx<R>* In, cx<R>* Out, const cx<R>
 
 // P = exp(i KZ dz) * KG, FP64 arithmetic, rounded to R.
 template <typename R>
-__global__ void propagator_kernel(const double2* __restrict__ KZ, const double* __restrict__ KG, double dz,
-                                  cx<R>* __restrict__ P, size_t n) {
+__global__ void propagator_kernel(const double2* __restrict__ KZ, const double* __restrict__ KG, double dz_arg,
+                                  const double* __restrict__ dz_dev, cx<R>* __restrict__ P, size_t n) {
   size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (q >= n) return;
+  const double dz = dz_dev ? *dz_dev : dz_arg;
   double2 kz = KZ[q];
   double amp = exp(-kz.y * dz) * KG[q];
   double s, c;
@@ -221,10 +222,10 @@ __global__ void propagator_kernel(const double2* __restrict__ KZ, const double* 
 
 template <typename R>
 hipError_t launch_propagator(const double* KZ64, const double* KG64, double dz, cx<R>* P, int Nr, int Nh,
-                             hipStream_t st) {
+                             hipStream_t st, const double* dz_dev) {
   size_t n = (size_t)Nr * Nh;
   hipLaunchKernelGGL(propagator_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     reinterpret_cast<const double2*>(KZ64), KG64, dz, P, n);
+                     reinterpret_cast<const double2*>(KZ64), KG64, dz, dz_dev, P, n);
   return hipGetLastError();
 }
 
@@ -232,8 +233,10 @@ template hipError_t launch_dht_gemm<float>(const float*, const cx<float>*, cx<fl
                                            hipStream_t, int);
 template hipError_t launch_dht_gemm<double>(const double*, const cx<double>*, cx<double>*, const cx<double>*, int, int,
                                             int, hipStream_t, int);
-template hipError_t launch_propagator<float>(const double*, const double*, double, cx<float>*, int, int, hipStream_t);
-template hipError_t launch_propagator<double>(const double*, const double*, double, cx<double>*, int, int, hipStream_t);
+template hipError_t launch_propagator<float>(const double*, const double*, double, cx<float>*, int, int, hipStream_t,
+                                             const double*);
+template hipError_t launch_propagator<double>(const double*, const double*, double, cx<double>*, int, int, hipStream_t,
+                                              const double*);
 
 // ---------------------------------------------------------------------------
 // Split-BF16 3-term form: with x = x_hi + x_lo (x_hi = bf16(x),

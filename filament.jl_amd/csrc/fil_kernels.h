@@ -39,6 +39,7 @@ struct RowArgs {
   const float* corr_D;      // [2][H][W]
   CorrGeom corr_g;
   float corr_scale;
+  const double* dz_dev;     // non-null: dz is read from device memory (the adaptive run)
 };
 
 template <typename R>
@@ -156,17 +157,19 @@ struct XytProp {
   double norm;          // 1 / (Nx Ny), the inverse transform's normalisation
 };
 template <typename R>
-hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStream_t st);
+hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStream_t st,
+                                 const double* dz_dev = nullptr);
 // S <- IFFT_xy[P .* FFT_xy[S]] for every w column of S [(ix + Nx iy)][Nh]
 template <typename R>
 hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const cx<R>* twx, const cx<R>* twy, int Nx, int Ny, int Nh,
                              hipStream_t st);
 
 // Hankel-transform phase (fil_dht.hip)
-// Propagator P[k][w] = exp(i KZ dz) Kguard Wguard computed in FP64.
+// Propagator P[k][w] = exp(i KZ dz) Kguard Wguard computed in FP64 (dz_dev: dz
+// read from device memory instead).
 template <typename R>
 hipError_t launch_propagator(const double* KZ64, const double* KG64, double dz, cx<R>* P, int Nr, int Nh,
-                             hipStream_t st);
+                             hipStream_t st, const double* dz_dev = nullptr);
 // Out[m][w] = sum_n M[m][n] In[n][w]  (optionally Out *= P), complex In/Out [Nr][Nh], real M [Nr][Nr].
 template <typename R>
 // Nh: complex columns of In/Out (the batch's members side by side); the propagator P
@@ -188,5 +191,29 @@ hipError_t launch_dht_gemm_bf16x3_to_planes(const uint16_t* Mhi, const uint16_t*
 // (Pw: one pulse's columns, which picks the tile shape as in the first product)
 hipError_t launch_dht_gemm_bf16x3_from_planes(const uint16_t* Mhi, const uint16_t* Mlo, const uint16_t* Xp,
                                               cx<float>* Out, int Nr, int Nh, int Pw, hipStream_t st, int tiling = 0);
+
+// Adaptive z-step on the device (fil_row.hip): the step rule of AStepKerrPlasma
+// (AdaptiveSteps.jl:96-107) and the loop tests of Filament.run
+// (Filament.jl:129-164), evaluated after each step's analysis.
+struct AdaptRule {
+  double dz0, phimax, phik, phip, Istop;
+  float zmax;             // z and zmax are single precision, as the RT run keeps them
+  int nonlinearity, plasma;
+  int nrows;              // analyzer rows reduced for Imax and rhomax
+  int max_steps;          // steps the log holds
+};
+struct AdaptState {
+  double dz;              // dz of the next step, read by the propagator and the row kernel
+  float z;                // z after `steps` steps
+  int steps;
+  int done;               // 0 running, else an FIL_STOP_* reason
+  int pad;
+};
+// One workgroup: reduces Imax_row (and rhomax_row with plasma) over the rows,
+// logs them for step `steps`, then stops or sets dz, z and steps for the next
+// step.  After `done` is set it does nothing.  log: [max_steps + 1][4] of
+// (z, dz, Imax, rhomax).
+hipError_t launch_adapt(const AdaptRule& rule, AdaptState* state, const double* Imax_row, const double* rhomax_row,
+                        double* log, hipStream_t st);
 
 }  // namespace fil

@@ -611,7 +611,7 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   }
 
   // RK3 (Kutta) in z: k1 = f(S0); k2 = f(S0 + dz/2 k1); k3 = f(S0 - dz k1 + 2 dz k2)
-  const R dz = a.dz;
+  const R dz = a.dz_dev ? (R)*a.dz_dev : a.dz;
   cx<R> k1[SP], k2[SP], k[SP], U[SP];
   // stage 1 starts from the field the plasma scan read (Z)
 #pragma unroll
@@ -1037,6 +1037,65 @@ hipError_t launch_corr_apply(cx<float>* S, const cx<float>* tw, const CorrGeom& 
     return hipGetLastError();
   });
   return hipSuccess;
+}
+
+// Max over rows with the analyzer's fold (start at 0, a NaN row never wins).
+__device__ double adapt_rowmax(const double* v, int n, double* red) {
+  double m = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, v[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  m = red[0];
+  __syncthreads();
+  return m;
+}
+
+__global__ __launch_bounds__(256) void adapt_kernel(AdaptRule rule, AdaptState* state, const double* Imax_row,
+                                                   const double* rhomax_row, double* log) {
+  __shared__ double red[256];
+  if (state->done) return;
+  const double Imax = adapt_rowmax(Imax_row, rule.nrows, red);
+  const double rhomax = rule.plasma ? adapt_rowmax(rhomax_row, rule.nrows, red) : 0.0;
+  if (threadIdx.x != 0) return;
+  const int s = state->steps;
+  log[4 * s + 2] = Imax;
+  log[4 * s + 3] = rhomax;
+  // Filament.jl:160-164: stop after the step whose analysis exceeds Istop;
+  // Filament.jl:129: loop while z < zmax
+  int done = 0;
+  if (s > 0 && Imax > rule.Istop) done = FIL_STOP_ISTOP;
+  else if (!(state->z < rule.zmax)) done = FIL_STOP_ZMAX;
+  else if (s >= rule.max_steps) done = FIL_STOP_MAXSTEPS;
+  if (done) {
+    state->done = done;
+    return;
+  }
+  // AdaptiveSteps.jl:96-107, min(dz0, dzk, dzp) in FP64, then single precision
+  double dz = rule.dz0;
+  if (rule.nonlinearity) {
+    const double pk = rule.phik * Imax, pp = rule.phip * rhomax;
+    const double dzk = pk != 0 ? rule.phimax / pk : INFINITY;
+    const double dzp = pp != 0 ? rule.phimax / pp : INFINITY;
+    if (dzk < dz) dz = dzk;
+    if (dzp < dz) dz = dzp;
+  }
+  const float dzf = (float)dz;
+  const float zn = state->z + dzf;
+  state->dz = (double)dzf;
+  state->z = zn;
+  state->steps = s + 1;
+  log[4 * (s + 1) + 0] = (double)zn;
+  log[4 * (s + 1) + 1] = (double)dzf;
+}
+
+hipError_t launch_adapt(const AdaptRule& rule, AdaptState* state, const double* Imax_row, const double* rhomax_row,
+                        double* log, hipStream_t st) {
+  hipLaunchKernelGGL(adapt_kernel, dim3(1), dim3(256), 0, st, rule, state, Imax_row, rhomax_row, log);
+  return hipGetLastError();
 }
 
 template hipError_t launch_row<float>(const RowArgs<float>&, int, int, bool, hipStream_t);

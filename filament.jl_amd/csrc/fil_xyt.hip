@@ -44,10 +44,12 @@ __device__ __forceinline__ cd csqrt(cd z) {
 // P[(ix + Nx iy) Nh + w] for w < Nh (Kfunc spectral.jl:160-184, KZ XYT :131-156,
 // spectral filter Guards.jl:250-261 with kernel! :301-318).
 template <typename R>
-__global__ void xyt_propagator_kernel(XytProp pp, double dz, cx<R>* __restrict__ P) {
+__global__ void xyt_propagator_kernel(XytProp pp, double dz_arg, const double* __restrict__ dz_dev,
+                                      cx<R>* __restrict__ P) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t n = (size_t)pp.Nx * pp.Ny * pp.Nh;
   if (q >= n) return;
+  const double dz = dz_dev ? *dz_dev : dz_arg;
   const int w = (int)(q % pp.Nh);
   const size_t row = q / pp.Nh;
   const int ix = (int)(row % pp.Nx), iy = (int)(row / pp.Nx);
@@ -74,9 +76,10 @@ __global__ void xyt_propagator_kernel(XytProp pp, double dz, cx<R>* __restrict__
 }
 
 template <typename R>
-hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStream_t st) {
+hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStream_t st, const double* dz_dev) {
   const size_t n = (size_t)pp.Nx * pp.Ny * pp.Nh;
-  hipLaunchKernelGGL(xyt_propagator_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pp, dz, P);
+  hipLaunchKernelGGL(xyt_propagator_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pp, dz, dz_dev,
+                     P);
   return hipGetLastError();
 }
 
@@ -170,8 +173,8 @@ hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const cx<R>* twx, const c
   return e;
 }
 
-template hipError_t launch_xyt_propagator<float>(const XytProp&, double, cx<float>*, hipStream_t);
-template hipError_t launch_xyt_propagator<double>(const XytProp&, double, cx<double>*, hipStream_t);
+template hipError_t launch_xyt_propagator<float>(const XytProp&, double, cx<float>*, hipStream_t, const double*);
+template hipError_t launch_xyt_propagator<double>(const XytProp&, double, cx<double>*, hipStream_t, const double*);
 template hipError_t launch_xyt_linear<float>(cx<float>*, const cx<float>*, const cx<float>*, const cx<float>*, int, int,
                                              int, hipStream_t);
 template hipError_t launch_xyt_linear<double>(cx<double>*, const cx<double>*, const cx<double>*, const cx<double>*, int,

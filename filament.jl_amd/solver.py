@@ -173,6 +173,22 @@ class _Model:
             dzp = phimax / (self.phip * rhomax) if self.phip * rhomax != 0 else math.inf
         return min(dz0, dzk, dzp)
 
+    def run_adaptive(self, z, zmax, dz0, phimax, Istop, max_steps, chunk=0):
+        """Filament.run's z loop (Filament.jl:129-164) with the adaptive step
+        (AdaptiveSteps.jl:96-107) kept on the device (fil_run_adaptive): z and
+        dz in single precision, stop when z >= zmax, after a step whose Imax >
+        Istop, or after ``max_steps`` steps.  Returns (z, nsteps, reason, log)
+        with log[s] = (z, dz, Imax, rhomax) after step s (row 0: the start)."""
+        p = L.FilAdaptParams(zmax=float(zmax), dz0=float(dz0), phimax=float(phimax), phik=float(self.phik),
+                             phip=float(self.phip), Istop=float(Istop), nonlinearity=int(bool(self.nonlinearity)),
+                             chunk=int(chunk))
+        zz = np.array([float(z)])
+        log = np.zeros((int(max_steps) + 1, 4))
+        n, reason = ctypes.c_int(), ctypes.c_int()
+        L.check(self._lib.fil_run_adaptive(self.h, ctypes.byref(p), L.ptr(zz), int(max_steps), L.ptr(log),
+                                           ctypes.byref(n), ctypes.byref(reason)))
+        return float(zz[0]), n.value, reason.value, log[:n.value + 1]
+
     def state_save(self, slot):
         L.check(self._lib.fil_state_save(self.h, int(slot)))
 

@@ -236,6 +236,32 @@ typedef struct fil_diag_xyt {
 int fil_analyze_xyt(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, double* Ft,
                     double* rho_end, double* S);
 
+/* The z loop of Filament.run (Filament.jl:129-164) with the adaptive step of
+ * AStepKerrPlasma (AdaptiveSteps.jl:96-107; the Kerr-only rule is phip = 0)
+ * kept on the device: per step, dz = min(dz0, phimax/(phik Imax),
+ * phimax/(phip rhomax)) from the previous analysis (FP64, then rounded to
+ * single precision), z += dz in single precision, zstep, then the analysis
+ * (its projection included); the loop ends when z >= zmax, or after a step
+ * whose Imax > Istop, or after max_steps steps.  The host checks the loop
+ * every `chunk` steps (0: 32) instead of after every step.  Steps and their
+ * analyses are the kernels fil_zstep and fil_analyze run, so the state after
+ * fil_run_adaptive equals the host loop's (fil_analyze, then zstep(z + dz, dz)
+ * with the same dz, ...) bit for bit.  *z: the start on entry, the end on
+ * return.  log (may be NULL): [max_steps + 1][4] of (z, dz, Imax, rhomax),
+ * row 0 the analysis before the first step.  *nsteps: steps taken; *reason:
+ * FIL_STOP_*.  Single-pulse handles (any geometry and form); a pending
+ * corrector output is added first.  Blocks. */
+#define FIL_STOP_ZMAX 1
+#define FIL_STOP_ISTOP 2
+#define FIL_STOP_MAXSTEPS 3
+typedef struct fil_adapt_params {
+  double zmax, dz0, phimax, phik, phip, Istop;
+  int nonlinearity;            /* 0: dz = dz0 (AStepKerrPlasma.NONLINEARITY) */
+  int chunk;
+} fil_adapt_params;
+int fil_run_adaptive(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log,
+                     int* nsteps, int* reason);
+
 /* field.rho and field.kdrho (Fields.jl:357-358) as the last plasma solve left
  * them, column-major [i + ld*j].  Blocks. */
 int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld);

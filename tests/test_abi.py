@@ -57,6 +57,13 @@ def test_struct_layouts_match_header(pkg):
              for f in group.split(",")]
     assert names == [n for n, _ in pkg.lib.FilCorrDesc._fields_]
     assert ctypes.sizeof(pkg.lib.FilCorrDesc) == 7 * 4 + 4 + 5 * 8   # 4 bytes of padding before w0
+    body = text[text.index("typedef struct fil_adapt_params"):text.index("} fil_adapt_params;")]
+    names = [f.strip() for group in re.findall(r"(?:int|double)\s+([a-zA-Z0-9_, ]+);", body)
+             for f in group.split(",")]
+    assert names == [n for n, _ in pkg.lib.FilAdaptParams._fields_]
+    assert ctypes.sizeof(pkg.lib.FilAdaptParams) == 6 * 8 + 2 * 4
+    for name in ("ZMAX", "ISTOP", "MAXSTEPS"):
+        assert int(re.search(rf"#define FIL_STOP_{name} (\d+)", text).group(1)) == getattr(pkg.lib, f"STOP_{name}")
     nparam = int(re.search(r"#define FIL_CORR_NPARAM (\d+)", text).group(1))
     # 4 -> 32 -> 32 -> 32 -> 2 channels of 3x3 weights plus biases
     assert nparam == pkg.lib.CORR_NPARAM == (32 * 4 * 9 + 32) + 2 * (32 * 32 * 9 + 32) + (2 * 32 * 9 + 2)
@@ -75,6 +82,8 @@ def test_errors_are_reported_without_a_device(pkg):
     assert L.fil_corrector_init(None, ctypes.byref(c), 0) == 1
     assert L.fil_corrector_apply(None) == 1 and b"null" in L.fil_last_error()
     assert L.fil_state_save(None, 0) == 1
+    n, r = ctypes.c_int(), ctypes.c_int()
+    assert L.fil_run_adaptive(None, None, None, 0, None, ctypes.byref(n), ctypes.byref(r)) == 1
     assert L.fil_upload_member(None, 0, None, 0) == 1
     assert L.fil_analyze_member(None, 0, 0.0, None, None, None, None, None) == 1
     g = ctypes.c_void_p()

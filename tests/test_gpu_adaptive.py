@@ -1,0 +1,116 @@
+"""Filament.run's z loop kept on the device (fil_run_adaptive) against the same
+loop driven from the host (Filament.jl:129-164 as filament_amd.driver.run runs
+it: analyze, dz = AStepKerrPlasma(Imax, rhomax) (AdaptiveSteps.jl:96-107), z
+and dz in single precision, zstep, analyze, Istop test).  The device loop runs
+the same step and analysis kernels, so the gate is equality: the same number
+of steps and stop reason, the same z, dz, Imax and rhomax after every step and
+the same final field, bit for bit.  The host checks the device loop every
+`chunk` steps; chunks that the stop ends early (steps taken past it are undone
+and the counted ones run again) are covered by picking a chunk that does not
+divide the step count."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FORMS = [("fp32", 4, 0), ("fp32-bf16x3-scan64", 4, 3), ("fp64", 8, 0)]
+
+
+def host_loop(pkg, m, z, zmax, dz0, phimax, Istop, max_steps):
+    L = pkg.lib
+    f32 = np.float32
+    z, zmax = f32(z), f32(zmax)
+    d = m.analyze(float(z))
+    log = [(float(z), 0.0, d["Imax"], d["rhomax"])]
+    reason = L.STOP_ZMAX
+    while z < zmax:
+        if len(log) - 1 >= max_steps:
+            reason = L.STOP_MAXSTEPS
+            break
+        dz = f32(m.dz_adaptive(dz0, phimax, d["Imax"], d["rhomax"]))
+        z = f32(z + dz)
+        m.zstep(float(z), float(dz))
+        d = m.analyze(float(z))
+        log.append((float(z), float(dz), d["Imax"], d["rhomax"]))
+        if d["Imax"] > Istop:
+            reason = L.STOP_ISTOP
+            break
+    return float(z), len(log) - 1, reason, np.array(log)
+
+
+def workload(pkg):
+    return pkg.workloads.rt_air(128, 512, 8.0, a0=1e-3, dz=4e-3, seed=3)
+
+
+def kerr_phimax(pkg, p, precision=4, form=0):
+    """phimax for which the Kerr rule sets dz = 0.9 dz0 at the start, from a
+    model of its own (every analysis projects the field again, so the models
+    compared must see the same calls)."""
+    m = pkg.RTModel(p, precision=precision, form=form)
+    try:
+        return 0.9 * p["dz_initial"] * m.phik * m.analyze(0.0)["Imax"]
+    finally:
+        m.close()
+
+
+@pytest.mark.parametrize("name,precision,form", FORMS)
+@pytest.mark.parametrize("stop", ["zmax", "istop", "maxsteps"])
+def test_device_loop_equals_host_loop(pkg, name, precision, form, stop):
+    p = workload(pkg)
+    dz0 = p["dz_initial"]
+    # the Kerr rule sets dz from the first step on and shrinks it as the beam focuses
+    phimax = kerr_phimax(pkg, p, precision, form)
+    zmax, Istop, max_steps = 40 * dz0, np.inf, 1000
+    if stop == "istop":
+        # a threshold the run crosses halfway
+        m = pkg.RTModel(p, precision=precision, form=form)
+        try:
+            _, _, _, lg0 = host_loop(pkg, m, 0.0, zmax, dz0, phimax, np.inf, max_steps)
+        finally:
+            m.close()
+        assert lg0[-1, 2] > lg0[0, 2], "the beam does not focus: no Istop stop to test"
+        Istop = 0.5 * (lg0[0, 2] + lg0[-1, 2])
+    a = pkg.RTModel(p, precision=precision, form=form)
+    b = pkg.RTModel(p, precision=precision, form=form)
+    try:
+        if stop == "maxsteps":
+            max_steps = 17
+        zh, nh, rh, lh = host_loop(pkg, a, 0.0, zmax, dz0, phimax, Istop, max_steps)
+        chunk = next(c for c in (5, 7, 6) if nh % c)
+        zd, nd, rd, ld = b.run_adaptive(0.0, zmax, dz0, phimax, Istop, max_steps, chunk=chunk)
+        print(f"{name} {stop}: {nd} steps (host {nh}), chunk {chunk}, z {zd:.6g}, reason {rd}, "
+              f"dz {ld[1, 1]:.4g}..{ld[-1, 1]:.4g}, Imax {ld[0, 2]:.4g}..{ld[-1, 2]:.4g}", flush=True)
+        want = {"zmax": pkg.lib.STOP_ZMAX, "istop": pkg.lib.STOP_ISTOP, "maxsteps": pkg.lib.STOP_MAXSTEPS}[stop]
+        assert rh == want and rd == want
+        assert nd == nh and zd == zh
+        assert len(set(ld[1:, 1])) > 1 and ld[1:, 1].max() < dz0, "dz did not adapt"
+        np.testing.assert_array_equal(ld, lh)
+        np.testing.assert_array_equal(b.field(), a.field())
+        assert b.analyze(zd) == a.analyze(zh)
+        # the handle keeps stepping from where the loop left it
+        a.zstep(zh + dz0, dz0)
+        b.zstep(zd + dz0, dz0)
+        np.testing.assert_array_equal(b.field(), a.field())
+    finally:
+        a.close()
+        b.close()
+
+
+def test_device_loop_chunks_and_start_at_zmax(pkg):
+    """The chunk length changes when the host looks, never the steps; a start
+    at zmax takes no step."""
+    p = workload(pkg)
+    dz0 = p["dz_initial"]
+    phimax = kerr_phimax(pkg, p)
+    ms = [pkg.RTModel(p) for _ in range(3)]
+    try:
+        out = [m.run_adaptive(0.0, 23 * dz0, dz0, phimax, np.inf, 1000, chunk=c) for m, c in zip(ms, (1, 4, 0))]
+        for (z, n, r, lg), m in zip(out[1:], ms[1:]):
+            assert (z, n, r) == out[0][:3]
+            np.testing.assert_array_equal(lg, out[0][3])
+            np.testing.assert_array_equal(m.field(), ms[0].field())
+        z, n, r, lg = ms[0].run_adaptive(out[0][0], out[0][0], dz0, phimax, np.inf, 10)
+        assert (n, r, lg.shape) == (0, pkg.lib.STOP_ZMAX, (1, 4))
+    finally:
+        for m in ms:
+            m.close()
