@@ -156,6 +156,47 @@ def train_corrector(pkg, p, form, dz, factor, draws, npairs, iters, geom):
     return st, info
 
 
+def adaptive_loop(pkg, p, steps):
+    """Filament.run's z loop (Filament.jl:129-164: adaptive dz from the last
+    analysis, zstep, analysis, Istop test) over ``steps`` plain FP32 steps,
+    driven from the host (one analysis read back per step) and kept on the
+    device (fil_run_adaptive); phimax makes the Kerr rule set dz = 0.9 dz0 at
+    the start.  Both loops take the same steps to the bit."""
+    f32 = np.float32
+    dz0 = p["dz_initial"]
+    m = pkg.RTModel(p)
+    phimax = 0.9 * dz0 * m.phik * m.analyze(0.0)["Imax"]
+    m.close()
+    zmax = 1e9
+    res = {}
+    for kind in ("host", "device"):
+        m = pkg.RTModel(p)
+        m.zstep(dz0, dz0)              # code load, then back to the start
+        m.close()
+        m = pkg.RTModel(p)
+        m.synchronize()
+        t0 = time.perf_counter()
+        if kind == "host":
+            z = f32(0.0)
+            d = m.analyze(0.0)
+            for _ in range(steps):
+                dz = f32(m.dz_adaptive(dz0, phimax, d["Imax"], d["rhomax"]))
+                z = f32(z + dz)
+                m.zstep(float(z), float(dz))
+                d = m.analyze(float(z))
+                if d["Imax"] > p["Istop"]:
+                    break
+            z = float(z)
+        else:
+            z, n, _, _ = m.run_adaptive(0.0, zmax, dz0, phimax, p["Istop"], steps)
+        m.synchronize()
+        res[kind] = (time.perf_counter() - t0, z, m.field())
+        m.close()
+    (th, zh, Eh), (td, zd, Ed) = res["host"], res["device"]
+    return dict(steps=steps, host_loop_steps_per_s=steps / th, device_loop_steps_per_s=steps / td,
+                same_z_and_field=bool(zh == zd and np.array_equal(Eh, Ed)), form="plain FP32")
+
+
 def cpu_baseline(pkg, p, sample_steps, dz):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _helpers as H
@@ -194,6 +235,8 @@ def main():
     ap.add_argument("--corr-iters", type=int, default=1500)
     ap.add_argument("--cpu-steps", type=int, default=30, help="oracle z-steps timed for the CPU baseline (~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adaptive-steps", type=int, default=200,
+                    help="steps of the host-driven and the device-kept adaptive z loop (0: skip)")
     ap.add_argument("--factor-sweep", default="4,8",
                     help="other coarse factors run (own corrector each) and reported beside the headline factor")
     ap.add_argument("--timing-every", type=int, default=8,
@@ -329,6 +372,12 @@ def main():
                                "frac": t_bf16 * 1e6 / tc,
                                "note": "measured time includes the corrector and the amortised verification"}
 
+    adapt = None
+    if args.adaptive_steps > 0:
+        log("adaptive z loop: host-driven vs kept on the device")
+        adapt = adaptive_loop(pkg, p, args.adaptive_steps)
+        log(f"adaptive z loop: {adapt}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pkg, p, args.cpu_steps, dz)
@@ -352,6 +401,7 @@ def main():
         "speedup_chosen_vs_plain": forms["plain_fp32"]["ms_per_step"] / forms["chosen"]["ms_per_fine_step_equivalent"],
         "roofline": roof,
         "step_roofline": step_roof,
+        "adaptive_loop": adapt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
