@@ -42,11 +42,11 @@ def workload(pkg):
     return pkg.workloads.rt_air(128, 512, 8.0, a0=1e-3, dz=4e-3, seed=3)
 
 
-def kerr_phimax(pkg, p, precision=4, form=0):
+def kerr_phimax(pkg, p, precision=4, form=0, model=None):
     """phimax for which the Kerr rule sets dz = 0.9 dz0 at the start, from a
     model of its own (every analysis projects the field again, so the models
     compared must see the same calls)."""
-    m = pkg.RTModel(p, precision=precision, form=form)
+    m = (model or pkg.RTModel)(p, precision=precision, form=form)
     try:
         return 0.9 * p["dz_initial"] * m.phik * m.analyze(0.0)["Imax"]
     finally:
@@ -114,3 +114,35 @@ def test_device_loop_chunks_and_start_at_zmax(pkg):
     finally:
         for m in ms:
             m.close()
+
+
+def geometry_case(pkg, geometry):
+    if geometry == "T":
+        return pkg.TModel, pkg.workloads.t_air(1024, I0=6e17, tmin=-150.0, tmax=150.0, tau0=15e-15, dz=2e-3)
+    return pkg.XYTModel, pkg.workloads.xyt_air(32, 32, 256, 3.0, xmax=4.0, ymax=4.0, tmin=-120.0, tmax=120.0,
+                                                ax=0.8e-3, ay=0.6e-3, tau0=20e-15, noise=0.05)
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+@pytest.mark.parametrize("geometry", ["T", "XYT"])
+def test_device_loop_other_geometries(pkg, geometry, precision):
+    """The T (one row, diagonal linear step) and XYT (2-D transform linear
+    step) handles run the same device loop; stop inside a chunk."""
+    model, p = geometry_case(pkg, geometry)
+    dz0 = p["dz_initial"]
+    phimax = kerr_phimax(pkg, p, precision, 0, model)
+    a = model(p, precision=precision)
+    b = model(p, precision=precision)
+    try:
+        zh, nh, rh, lh = host_loop(pkg, a, 0.0, 30 * dz0, dz0, phimax, np.inf, 1000)
+        chunk = next(c for c in (7, 5, 6) if nh % c)
+        zd, nd, rd, ld = b.run_adaptive(0.0, 30 * dz0, dz0, phimax, np.inf, 1000, chunk=chunk)
+        print(f"{geometry} fp{8 * precision}: {nd} steps (host {nh}), chunk {chunk}, "
+              f"dz {ld[1, 1]:.4g}..{ld[-1, 1]:.4g}, Imax {ld[0, 2]:.4g}..{ld[-1, 2]:.4g}", flush=True)
+        assert (nd, zd, rd) == (nh, zh, rh) == (nh, zh, pkg.lib.STOP_ZMAX)
+        assert ld[1, 1] < np.float32(dz0), "the first step did not adapt"
+        np.testing.assert_array_equal(ld, lh)
+        np.testing.assert_array_equal(b.field(), a.field())
+    finally:
+        a.close()
+        b.close()
