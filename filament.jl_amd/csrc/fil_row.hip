@@ -1039,28 +1039,34 @@ hipError_t launch_corr_apply(cx<float>* S, const cx<float>* tw, const CorrGeom& 
   return hipSuccess;
 }
 
-// Max over rows with the analyzer's fold (start at 0, a NaN row never wins).
-__device__ double adapt_rowmax(const double* v, int n, double* red) {
-  double m = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, v[i]);
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  m = red[0];
-  __syncthreads();
-  return m;
-}
-
-__global__ __launch_bounds__(256) void adapt_kernel(AdaptRule rule, AdaptState* state, const double* Imax_row,
-                                                   const double* rhomax_row, double* log) {
-  __shared__ double red[256];
+// Max over rows with the analyzer's fold (start at 0, a NaN row never wins):
+// per thread, across the wave by shuffles, then across the waves in LDS.
+constexpr int ADAPT_THREADS = 1024;
+__global__ __launch_bounds__(ADAPT_THREADS) void adapt_kernel(AdaptRule rule, AdaptState* state,
+                                                              const double* Imax_row, const double* rhomax_row,
+                                                              double* log) {
+  __shared__ double red[2][ADAPT_THREADS / 64];
   if (state->done) return;
-  const double Imax = adapt_rowmax(Imax_row, rule.nrows, red);
-  const double rhomax = rule.plasma ? adapt_rowmax(rhomax_row, rule.nrows, red) : 0.0;
+  double im = 0, rm = 0;
+  for (int i = threadIdx.x; i < rule.nrows; i += ADAPT_THREADS) {
+    im = fmax(im, Imax_row[i]);
+    if (rule.plasma) rm = fmax(rm, rhomax_row[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    im = fmax(im, __shfl_xor(im, o));
+    rm = fmax(rm, __shfl_xor(rm, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = im;
+    red[1][threadIdx.x >> 6] = rm;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  double Imax = 0, rhomax = 0;
+  for (int w = 0; w < ADAPT_THREADS / 64; w++) {
+    Imax = fmax(Imax, red[0][w]);
+    rhomax = fmax(rhomax, red[1][w]);
+  }
   const int s = state->steps;
   log[4 * s + 2] = Imax;
   log[4 * s + 3] = rhomax;
@@ -1094,7 +1100,7 @@ __global__ __launch_bounds__(256) void adapt_kernel(AdaptRule rule, AdaptState* 
 
 hipError_t launch_adapt(const AdaptRule& rule, AdaptState* state, const double* Imax_row, const double* rhomax_row,
                         double* log, hipStream_t st) {
-  hipLaunchKernelGGL(adapt_kernel, dim3(1), dim3(256), 0, st, rule, state, Imax_row, rhomax_row, log);
+  hipLaunchKernelGGL(adapt_kernel, dim3(1), dim3(ADAPT_THREADS), 0, st, rule, state, Imax_row, rhomax_row, log);
   return hipGetLastError();
 }
 
