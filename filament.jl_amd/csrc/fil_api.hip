@@ -514,14 +514,15 @@ int analyze_device(fil_handle* h, AnalyzerRows& v, int member) {
 
 // The analyzer's row pass over the whole state as analyze_device's first pass
 // runs it (the plasma solve when stale, the pending filter, the projection,
-// the Imax rows), without the column sums and host copies.
+// the Imax rows), without the intensity and spectrum rows, the column sums
+// and the host copies.
 template <typename R>
 int analysis_rows(fil_handle* h) {
   if (h->d.plasma && !h->plasma_valid)
     if (int rc = plasma_only<R>(h)) return rc;
   HIPCHK(launch_analyze_rows<R>(h->S.as<cx<R>>(), h->tw.as<cx<R>>(), h->Rg.as<R>(), h->Tg.as<R>(),
-                                h->pending_filter ? 1 : 0, (R)h->dt, h->aFr.as<double>(), h->aIt.as<R>(),
-                                h->aSw.as<R>(), h->aImax.as<double>(), h->rows(), h->Nt, 1, 0, h->B, h->st));
+                                h->pending_filter ? 1 : 0, (R)h->dt, h->aFr.as<double>(), nullptr, nullptr,
+                                h->aImax.as<double>(), h->rows(), h->Nt, 1, 0, h->B, h->st));
   h->pending_filter = false;
   h->acache.valid = false;
   return 0;

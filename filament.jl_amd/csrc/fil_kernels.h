@@ -56,7 +56,8 @@ hipError_t launch_to_spectrum(const cx<R>* Et, cx<R>* S, const cx<R>* tw, int Nr
 template <typename R>
 hipError_t launch_to_time(const cx<R>* S, cx<R>* Et, const cx<R>* tw, const R* Rg, const R* Tg, int apply,
                           int Nr, int N, int rs, int ro, hipStream_t st);
-// Analyzer rows: output row i (< Nr) reads row (i rs + ro) of S with the r guard
+// Analyzer rows (It and Sw may be null: only Fr, Imax_row and the projected S):
+// output row i (< Nr) reads row (i rs + ro) of S with the r guard
 // Rg[i / gdiv]: every row of a batch with (1, 0, batch), one member with
 // (batch, member, 1).
 template <typename R>

@@ -698,7 +698,7 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
     if (apply_filter) { v = (g * Tg[t]) * v; Z[lds_index(t)] = v; }
     double I = (double)norm2(v);
     fr += I;
-    It[(size_t)i * N + t] = (R)I;
+    if (It) It[(size_t)i * N + t] = (R)I;
   }
   // block sum of fr
   const int lane = tid & 63, wv = tid >> 6;
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
   fft_lds<R, N, NTH, +1>(Z, tw);
   const R inv = (R)1 / (R)N;
   const double c = 2.0 * (double)N * (double)dt;
-  for (int w = tid; w <= NH; w += NTH) {
+  for (int w = tid; Sw && w <= NH; w += NTH) {
     cx<R> v = inv * Z[lds_index(w)];
     double re = (double)v.x, im = (double)v.y;
     if (w >= 1 && w < NH) { re *= 0.5; im *= 0.5; }   // aspec2rspec
