@@ -537,11 +537,17 @@ int run_adaptive_impl(fil_handle* h, const fil_adapt_params* p, double* z, int m
                       int* reason) {
   if (int rc = corr_materialize(h)) return rc;
   const size_t nb = h->ns() * h->csz(), nr = (size_t)h->rows() * 8, nlog = ((size_t)max_steps + 1) * 4 * 8;
-  if (!h->ad_state.p && h->ad_state.alloc(sizeof(AdaptState)) != hipSuccess)
-    return fail(FIL_ENOMEM, "device allocation failed");
-  if (h->ad_log.n < nlog && h->ad_log.alloc(nlog) != hipSuccess) return fail(FIL_ENOMEM, "device allocation failed");
-  if (!h->ad_S.p && (h->ad_S.alloc(nb) != hipSuccess || h->ad_re.alloc(nr) != hipSuccess ||
-                     h->ad_rm.alloc(nr) != hipSuccess))
+  // each buffer on its own: a failed allocation leaves that buffer empty
+  // (DevBuf keeps the requested size), and the next call tries it again
+  auto need = [](DevBuf& b, size_t bytes) {
+    if (b.p && b.n >= bytes) return true;
+    if (b.alloc(bytes) == hipSuccess) return true;
+    b.p = nullptr;
+    b.n = 0;
+    return false;
+  };
+  if (!need(h->ad_state, sizeof(AdaptState)) || !need(h->ad_log, nlog) || !need(h->ad_S, nb) ||
+      !need(h->ad_re, nr) || !need(h->ad_rm, nr))
     return fail(FIL_ENOMEM, "device allocation failed");
   AdaptRule rule{};
   rule.dz0 = p->dz0; rule.phimax = p->phimax; rule.phik = p->phik; rule.phip = p->phip; rule.Istop = p->Istop;
