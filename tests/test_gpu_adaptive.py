@@ -146,3 +146,24 @@ def test_device_loop_other_geometries(pkg, geometry, precision):
     finally:
         a.close()
         b.close()
+
+
+def test_device_loop_rejects_batches_and_bad_arguments(pkg):
+    """Batched handles (one dz for pulses whose rules differ) and negative
+    step limits or chunks are refused with an error, the state untouched."""
+    p = workload(pkg)
+    dz0 = p["dz_initial"]
+    ens = pkg.RTEnsemble([p, pkg.workloads.rt_air(128, 512, 6.0, a0=1e-3, dz=4e-3, seed=4)])
+    m = pkg.RTModel(p)
+    try:
+        with pytest.raises(pkg.lib.FilamentError, match="single-pulse"):
+            ens.run_adaptive(0.0, 10 * dz0, dz0, np.inf, np.inf, 10)
+        E0 = m.field()
+        with pytest.raises(pkg.lib.FilamentError, match=">= 0"):
+            m.run_adaptive(0.0, 10 * dz0, dz0, np.inf, np.inf, 10, chunk=-1)
+        with pytest.raises(pkg.lib.FilamentError, match=">= 0"):
+            m.run_adaptive(0.0, 10 * dz0, dz0, np.inf, np.inf, -1)
+        np.testing.assert_array_equal(m.field(), E0)
+    finally:
+        ens.close()
+        m.close()
