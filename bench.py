@@ -23,7 +23,10 @@ Prints one JSON line.  `value` is the chosen form's fine-equivalent z-steps/s
 when its fluence profile stays within 1e-3 relative L2 of the FP64 build's at
 the end of the trajectory, otherwise the plain port's z-steps/s.  By default
 the trajectory is the workload's whole 2000 z-steps (1 m), so the error is the
-one at its far end.
+one at its far end.  Beside it: `adaptive_loop`, Filament.run's z loop with
+the adaptive step (plain FP32, 200 steps), driven from the host and kept on
+the device (fil_run_adaptive), with the check that both end on the same z
+and field.
 """
 import argparse
 import json
