@@ -7,29 +7,36 @@ nonlinear step (row kernel), then the two Hankel-transform GEMMs with the
 propagator multiply.  The field is resident in HBM when the timed region
 starts.
 
-Forms, each run over the same trajectory (W warm-up + K timed fine-step
-equivalents) from the same initial field:
+Forms, each run over the same trajectory from the same initial field: W
+untimed warm-up steps, K timed steps, then untimed steps to the trajectory's
+end (the workload's 2000 z-steps = 1 m, or W + K when that is longer), where
+the errors against the FP64 build are read:
   fp64    the FP64 build, fine steps: the error reference;
   plain   the plain FP32 port, fine steps;
   chosen  FP32 with the split-BF16 Hankel transform and the FP64 plasma scan,
           advanced by coarse steps of F*dz, each followed by the learned
           corrector, with a fine-step verification (and fallback) every M
           coarse steps.  Its rate counts fine-step equivalents (a coarse step
-          counts F; verification steps are charged to the run).  The
-          corrector is trained on the device during set-up, outside the timed
-          region, on FP64 fine-step targets of other pulses.
+          counts F; verification steps are charged to the run).  W is rounded
+          up to whole coarse steps and K to whole verification periods (M F
+          fine-step equivalents), so every timed window pays for its checks.
+          The corrector is trained on the device during set-up, outside the
+          timed region, on FP64 fine-step targets of other pulses.  The same
+          factor without the network, and the other coarse factors with and
+          without a network of their own, are reported beside it.
 
 Prints one JSON line.  `value` is the chosen form's fine-equivalent z-steps/s
 when its fluence profile stays within 1e-3 relative L2 of the FP64 build's at
-the end of the trajectory, otherwise the plain port's z-steps/s.  By default
-the trajectory is the workload's whole 2000 z-steps (1 m), so the error is the
-one at its far end.  Beside it: `adaptive_loop`, Filament.run's z loop with
-the adaptive step (plain FP32, 200 steps), driven from the host and kept on
-the device (fil_run_adaptive), with the check that both end on the same z
-and field.
+the end of the trajectory, otherwise the plain port's z-steps/s; `steps` and
+`warmup` are the counts that form actually ran.  Beside it: `adaptive_loop`,
+Filament.run's z loop with the adaptive step (plain FP32, 200 steps), driven
+from the host and kept on the device (fil_run_adaptive), with the check that
+both end on the same z and field; `cpu_baseline`, the oracle's FP32 instance
+on the host's cores and on one core.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -53,6 +60,13 @@ def load_pkg():
     sys.modules["filament_amd"] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def _plain(o):
+    """numpy scalars and arrays as plain JSON values."""
+    if hasattr(o, "tolist"):
+        return o.tolist()
+    raise TypeError(f"{type(o).__name__} is not JSON serialisable")
 
 
 def log(msg):
@@ -92,10 +106,11 @@ def kernel_ms(kt):
     return dict(row=kt["row_ms"] / n, dht1=kt["dht1_ms"] / n, dht2=kt["dht2_ms"] / n)
 
 
-def run_fine(pkg, p, precision, form, W, K, dz, every):
-    """W warm-up and K timed fine steps; per-kernel HIP-event times over every
-    ``every``-th timed step (events on the library's stream, carried by the
-    kernel dispatches)."""
+def run_fine(pkg, p, precision, form, W, K, dz, every, total):
+    """W warm-up and K timed fine steps, then untimed fine steps up to
+    ``total`` (the trajectory the errors are read at); per-kernel HIP-event
+    times over every ``every``-th timed step (events on the library's stream,
+    carried by the kernel dispatches)."""
     m = pkg.RTModel(p, precision=precision, form=form)
     W0 = m.analyze(0.0)["W"]
     z = 0.0
@@ -112,15 +127,18 @@ def run_fine(pkg, p, precision, form, W, K, dz, every):
     ms = (time.perf_counter() - t0) * 1e3 / K
     kt = m.timing_read()
     m.timing(False)
+    for _ in range(total - W - K):
+        z += dz
+        m.zstep(z, dz)
     dg = diagnostics(m, z)
     dg["energy_ratio"] = dg["W"] / W0
     m.close()
     return ms, kernel_ms(kt), dg
 
 
-def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geom, every):
-    """The chosen form: coarse steps (+ corrector) with verification, W/K in
-    fine-step equivalents."""
+def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geom, every, total):
+    """The chosen form: coarse steps (+ corrector) with verification; W, K and
+    ``total`` in fine-step equivalents (multiples of the factor)."""
     m = pkg.RTModel(p, precision=pkg.lib.FP32, form=form)
     W0 = m.analyze(0.0)["W"]
     corr = None
@@ -139,6 +157,7 @@ def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geo
     kt = m.timing_read()
     m.timing(False)
     s1 = st.stats()
+    z = st.advance(z, dz, total - W - K)
     dg = diagnostics(m, z)
     dg["energy_ratio"] = dg["W"] / W0
     m.close()
@@ -181,11 +200,13 @@ def adaptive_loop(pkg, p, steps):
         t0 = time.perf_counter()
         if kind == "host":
             z = f32(0.0)
+            n = 0
             d = m.analyze(0.0)
             for _ in range(steps):
                 dz = f32(m.dz_adaptive(dz0, phimax, d["Imax"], d["rhomax"]))
                 z = f32(z + dz)
                 m.zstep(float(z), float(dz))
+                n += 1
                 d = m.analyze(float(z))
                 if d["Imax"] > p["Istop"]:
                     break
@@ -193,39 +214,57 @@ def adaptive_loop(pkg, p, steps):
         else:
             z, n, _, _ = m.run_adaptive(0.0, zmax, dz0, phimax, p["Istop"], steps)
         m.synchronize()
-        res[kind] = (time.perf_counter() - t0, z, m.field())
+        res[kind] = (time.perf_counter() - t0, z, m.field(), n)
         m.close()
-    (th, zh, Eh), (td, zd, Ed) = res["host"], res["device"]
-    return dict(steps=steps, host_loop_steps_per_s=steps / th, device_loop_steps_per_s=steps / td,
-                same_z_and_field=bool(zh == zd and np.array_equal(Eh, Ed)), form="plain FP32")
+    (th, zh, Eh, nh), (td, zd, Ed, nd) = res["host"], res["device"]
+    return dict(requested_steps=steps, host_loop_steps=nh, device_loop_steps=nd,
+                host_loop_steps_per_s=nh / th, device_loop_steps_per_s=nd / td,
+                same_z_and_field=bool(nh == nd and zh == zd and np.array_equal(Eh, Ed)), form="plain FP32")
 
 
-def cpu_baseline(pkg, p, sample_steps, dz):
+def cpu_baseline(pkg, p, sample_steps, sample_steps_1core, dz):
+    """The oracle's FP32 instance (the plain port restated in C++, OpenMP over
+    rows and over Hankel columns) on this host: ``sample_steps`` z-steps on
+    every thread OpenMP is given, then ``sample_steps_1core`` on one thread,
+    each after one untimed warm-up step (Filament.jl:166-170)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _helpers as H
     orc = H.load_oracle()
     g = pkg.physics.GridRT(p["rmax"], p["Nr"], p["tmin"], p["tmax"], p["Nt"])
     E0 = p["initial_condition"](g.r, g.t, p["ru"], p["tu"], p["Iu"])
-    log("cpu baseline: oracle setup")
-    o = orc.Oracle(H.oracle_config(p), E0, 4)
-    o.zstep(dz, dz)            # warm-up step excluded (Filament.jl:166-170)
-    t0 = time.perf_counter()
-    for n in range(sample_steps):
-        o.zstep((n + 2) * dz, dz)
-    dt = time.perf_counter() - t0
-    log(f"cpu baseline: {sample_steps} steps in {dt:.2f} s")
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = orc.max_threads()
+
+    def timed(nthreads, nsteps):
+        orc.set_threads(nthreads)
+        log(f"cpu baseline: oracle setup, {nthreads} threads")
+        o = orc.Oracle(H.oracle_config(p), E0, 4)
+        o.zstep(dz, dz)
+        t0 = time.perf_counter()
+        for n in range(nsteps):
+            o.zstep((n + 2) * dz, dz)
+        dt = time.perf_counter() - t0
+        log(f"cpu baseline: {nsteps} steps in {dt:.2f} s on {nthreads} threads")
+        return dt
+
+    dt = timed(threads, sample_steps)
+    one = None
+    if sample_steps_1core > 0:
+        dt1 = timed(1, sample_steps_1core)
+        one = dict(value=sample_steps_1core / dt1, unit="z-steps/s", cores=1,
+                   sample=f"{sample_steps_1core} fine z-steps after 1 warm-up step, {dt1:.1f} s wall")
+    orc.set_threads(threads)
     return dict(value=sample_steps / dt, unit="z-steps/s", cores=threads, kind="port",
                 sample=f"FP32 C++ restatement of the plain port, RT 1024x4096, {sample_steps} fine z-steps after "
-                       f"1 warm-up step, {dt:.1f} s wall")
+                       f"1 warm-up step, {dt:.1f} s wall", one_core=one)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000,
-                    help="timed fine-step equivalents (2000: the workload's full 1 m at dz = 0.5 mm)")
-    ap.add_argument("--warmup", type=int, default=16, help="warm-up fine-step equivalents")
+                    help="timed fine-step equivalents; coarse forms round it up to whole verification periods")
+    ap.add_argument("--warmup", type=int, default=16,
+                    help="warm-up fine-step equivalents; coarse forms round it up to whole coarse steps")
     ap.add_argument("--factor", type=int, default=16, help="coarse step = factor * dz in the chosen form")
     ap.add_argument("--check-every", type=int, default=800,
                     help="fine-step equivalents (z distance in dz) between fine-step verifications; "
@@ -237,6 +276,8 @@ def main():
     ap.add_argument("--corr-pairs", type=int, default=24)
     ap.add_argument("--corr-iters", type=int, default=1500)
     ap.add_argument("--cpu-steps", type=int, default=30, help="oracle z-steps timed for the CPU baseline (~10 s)")
+    ap.add_argument("--cpu-steps-1core", type=int, default=2,
+                    help="oracle z-steps timed on one thread for the CPU baseline's 1-core figure (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adaptive-steps", type=int, default=200,
                     help="steps of the host-driven and the device-kept adaptive z loop (0: skip)")
@@ -258,12 +299,30 @@ def main():
     p = pkg.workloads.rt_1024x4096()
     dz = p["dz_initial"]
     F = args.factor
-    K, W = args.steps, args.warmup
-    if K % F or W % F:
-        sys.exit("--steps and --warmup must be multiples of --factor")
     Nr, Nt = p["Nr"], p["Nt"]
     Nh = Nt // 2
     chosen_form = L.FORM_DHT_BF16X3 | L.FORM_SCAN_FP64
+    sweep_factors = [int(x) for x in args.factor_sweep.split(",") if x.strip() and int(x) != F]
+
+    # Step counts actually run.  Fine forms: W and K as asked.  Coarse forms: the warm-up rounded up to
+    # whole coarse steps and the timed region rounded up to whole verification periods, so every timed
+    # window carries its share of the fine-step checks.  Every form then continues, untimed, to the same
+    # trajectory end (the workload's 2000 z-steps = 1 m, or further when W + K asks for more), where the
+    # errors against the FP64 build are read.
+    def ceil_to(n, q):
+        return -(-max(n, 0) // q) * q
+
+    def coarse_counts(f):
+        period = max(1, args.check_every // f) * f
+        return ceil_to(args.warmup, f), max(period, ceil_to(args.steps, period)), period
+
+    W, K = args.warmup, max(1, args.steps)
+    lcm = 1
+    for f in [F] + sweep_factors:
+        lcm = lcm * f // math.gcd(lcm, f)
+    total = ceil_to(max([p["nsteps"], W + K] + [sum(coarse_counts(f)[:2]) for f in [F] + sweep_factors]), lcm)
+    log(f"trajectory: {total} fine-step equivalents ({total * dz:.3f} m); fine forms W={W} K={K}; "
+        f"chosen x{F}: W, K, period = {coarse_counts(F)}")
 
     want = set(args.forms.split(","))
     full = want == {"fp64", "plain", "chosen"}
@@ -275,15 +334,15 @@ def main():
 
     if "fp64" in want:
         log("fp64: fine steps")
-        ms64, kt64, dg64 = run_fine(pkg, p, L.FP64, 0, W, K, dz, args.timing_every)
+        ms64, kt64, dg64 = run_fine(pkg, p, L.FP64, 0, W, K, dz, args.timing_every, total)
         Fr64 = dg64["Fr"]
-        forms["fp64"] = dict(ms_per_step=ms64, z_steps_per_s=1e3 / ms64, kernel_ms=kt64,
+        forms["fp64"] = dict(ms_per_step=ms64, z_steps_per_s=1e3 / ms64, kernel_ms=kt64, warmup=W, steps=K,
                              energy_ratio_end_to_start=dg64["energy_ratio"])
         log(f"fp64: {ms64:.4f} ms/step")
     if "plain" in want:
         log("plain FP32: fine steps")
-        msp, ktp, dgp = run_fine(pkg, p, L.FP32, 0, W, K, dz, args.timing_every)
-        forms["plain_fp32"] = dict(ms_per_step=msp, z_steps_per_s=1e3 / msp, kernel_ms=ktp,
+        msp, ktp, dgp = run_fine(pkg, p, L.FP32, 0, W, K, dz, args.timing_every, total)
+        forms["plain_fp32"] = dict(ms_per_step=msp, z_steps_per_s=1e3 / msp, kernel_ms=ktp, warmup=W, steps=K,
                                    fluence_err_vs_fp64=fluence_error(dgp["Fr"], Fr64) if Fr64 is not None else None,
                                    errors_vs_fp64=errs(dgp), energy_ratio_end_to_start=dgp["energy_ratio"])
         log(f"plain FP32: {msp:.4f} ms/step")
@@ -291,49 +350,53 @@ def main():
         m0 = pkg.RTModel(p, precision=L.FP32, form=chosen_form)
         geom = pkg.corrector.window_geometry(m0.grid, m0.unit, m0.w0)
         m0.close()
-        cst, cinfo = None, None
-        if not args.no_corrector:
-            draws = pkg.corrector.ensemble_draws(args.corr_draws, seed=1)
-            cst, cinfo = train_corrector(pkg, p, chosen_form, dz, F, draws, args.corr_pairs, args.corr_iters, geom)
-        log(f"chosen: coarse x{F}, corrector {'off' if cst is None else 'on'}")
-        msc, ktc, dgc, timed = run_coarse(pkg, p, chosen_form, W, K, dz, F, max(1, args.check_every // F), args.check_tol,
-                                          cst, geom, args.timing_every)
-        forms["chosen"] = dict(
-            ms_per_fine_step_equivalent=msc, z_steps_per_s=1e3 / msc, kernel_ms_per_zstep=ktc,
-            fluence_err_vs_fp64=fluence_error(dgc["Fr"], Fr64) if Fr64 is not None else None,
-            errors_vs_fp64=errs(dgc), energy_ratio_end_to_start=dgc["energy_ratio"],
-            description=f"FP32 + split-BF16 Hankel MFMA + FP64 plasma scan, coarse steps of {F} dz"
-                        f"{'' if cst is None else ' + learned corrector'}, fine-step verification every "
-                        f"{max(1, args.check_every // F)} coarse steps ({args.check_every} dz, tol {args.check_tol:g})",
-            timed_region=timed, corrector_training=cinfo)
-        log(f"chosen: {msc:.4f} ms per fine-step equivalent, {timed}")
-        if Fr64 is not None:
-            log(f"chosen: fluence err vs fp64 {forms['chosen']['fluence_err_vs_fp64']:.3e}")
-        # the same form at other coarse factors (4 dz is the learned part's design point), each with
-        # a corrector trained for that factor; reported only, the headline is --factor
-        sweep = []
-        for f in [int(x) for x in args.factor_sweep.split(",") if x.strip()]:
-            if f == F or K % f or W % f:
-                continue
-            cs = None
-            if not args.no_corrector:
-                cs, _ = train_corrector(pkg, p, chosen_form, dz, f, draws, args.corr_pairs, args.corr_iters, geom)
-            msf, _, dgf, tf = run_coarse(pkg, p, chosen_form, W, K, dz, f, max(1, args.check_every // f),
-                                         args.check_tol, cs, geom, args.timing_every)
-            ef = fluence_error(dgf["Fr"], Fr64) if Fr64 is not None else None
-            sweep.append(dict(factor=f, z_steps_per_s=1e3 / msf, fluence_err_vs_fp64=ef, errors_vs_fp64=errs(dgf),
-                              timed_region=tf))
-            log(f"chosen x{f}: {msf:.4f} ms per fine-step equivalent, fluence err {ef}")
-        forms["chosen"]["factor_sweep"] = sweep
+        draws = pkg.corrector.ensemble_draws(args.corr_draws, seed=1)
+
+        def coarse_form(f, with_corrector):
+            """One coarse factor, with or without a corrector trained for it."""
+            cst, cinfo = None, None
+            if with_corrector:
+                cst, cinfo = train_corrector(pkg, p, chosen_form, dz, f, draws, args.corr_pairs, args.corr_iters,
+                                             geom)
+            Wf, Kf, period = coarse_counts(f)
+            log(f"chosen x{f}: corrector {'on' if with_corrector else 'off'}, W={Wf} K={Kf}")
+            ms, kt, dg, timed = run_coarse(pkg, p, chosen_form, Wf, Kf, dz, f, period // f, args.check_tol, cst, geom,
+                                           args.timing_every, total)
+            ef = fluence_error(dg["Fr"], Fr64) if Fr64 is not None else None
+            log(f"chosen x{f} corrector {'on' if with_corrector else 'off'}: {ms:.4f} ms per fine-step equivalent, "
+                f"fluence err {ef}, {timed}")
+            return dict(factor=f, corrector=with_corrector, ms_per_fine_step_equivalent=ms, z_steps_per_s=1e3 / ms,
+                        kernel_ms_per_zstep=kt, warmup=Wf, steps=Kf, fluence_err_vs_fp64=ef, errors_vs_fp64=errs(dg),
+                        energy_ratio_end_to_start=dg["energy_ratio"], timed_region=timed,
+                        corrector_training=cinfo)
+
+        use_corr = not args.no_corrector
+        forms["chosen"] = coarse_form(F, use_corr)
+        forms["chosen"]["description"] = (
+            f"FP32 + split-BF16 Hankel MFMA + FP64 plasma scan, coarse steps of {F} dz"
+            f"{' + learned corrector' if use_corr else ''}, fine-step verification every "
+            f"{max(1, args.check_every // F)} coarse steps ({args.check_every} dz, tol {args.check_tol:g})")
+        # what the learned part earns: the same factors without the network, and the other coarse factors
+        # (4 dz is its design point) with and without a network trained for each; reported only
+        sweep = [coarse_form(F, False)] if use_corr else []
+        for f in sweep_factors:
+            if use_corr:
+                sweep.append(coarse_form(f, True))
+            sweep.append(coarse_form(f, False))
+        forms["chosen"]["factor_sweep"] = [{k: v for k, v in e.items() if k != "kernel_ms_per_zstep"}
+                                           for e in sweep]
     if not full:
-        print(json.dumps({"profiling_forms": forms}))
+        print(json.dumps({"profiling_forms": forms}, default=_plain))
         return
 
     err_c = forms["chosen"]["fluence_err_vs_fp64"]
     if err_c <= GATE:
-        name, ms, kt = "chosen", forms["chosen"]["ms_per_fine_step_equivalent"], forms["chosen"]["kernel_ms_per_zstep"]
+        fc = forms["chosen"]
+        name, ms, kt, Wr, Kr = "chosen", fc["ms_per_fine_step_equivalent"], fc["kernel_ms_per_zstep"], fc["warmup"], \
+            fc["steps"]
     else:
-        name, ms, kt = "plain_fp32", forms["plain_fp32"]["ms_per_step"], forms["plain_fp32"]["kernel_ms"]
+        fp = forms["plain_fp32"]
+        name, ms, kt, Wr, Kr = "plain_fp32", fp["ms_per_step"], fp["kernel_ms"], fp["warmup"], fp["steps"]
 
     # roofline of the dominant kernel of the reported form (per launch, HIP-event averages)
     gemm_flop = 2.0 * Nr * (2 * Nh) * Nr          # real GEMM M=Nr, N=2*Nh, K=Nr
@@ -370,7 +433,7 @@ def main():
                        "frac": t_plain * 1e6 / (forms["plain_fp32"]["ms_per_step"] * 1e3)},
     }
     if "chosen" in forms:
-        tc = forms["chosen"]["ms_per_fine_step_equivalent"] * args.factor * 1e3
+        tc = forms["chosen"]["ms_per_fine_step_equivalent"] * F * 1e3
         step_roof["chosen"] = {"t_roof_us_per_coarse_step": t_bf16 * 1e6, "t_measured_us_per_coarse_step": tc,
                                "frac": t_bf16 * 1e6 / tc,
                                "note": "measured time includes the corrector and the amortised verification"}
@@ -383,15 +446,15 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(pkg, p, args.cpu_steps, dz)
+        cpu = cpu_baseline(pkg, p, args.cpu_steps, args.cpu_steps_1core, dz)
 
     out = {
         "metric": "z-steps/s RT 1024x4096: plain FP32 vs chosen form at <=1e-3 fluence err",
         "value": 1e3 / ms,
         "unit": "z-steps/s",
         "n_gpus": 1,
-        "steps": K,
-        "warmup": W,
+        "steps": Kr,
+        "warmup": Wr,
         "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
@@ -399,7 +462,9 @@ def main():
         "dtype": "fp32" if name == "plain_fp32" else "fp32+bf16x3-mfma+fp64-scan",
         "data": "synthetic seeded Gaussian pulse (RT air, 5 Pcr), 1e-3 amplitude noise",
         "config": {"workload": "RT Nr=1024 x Nt=4096, air 800 nm, Kerr+plasma, RK3, dz=0.5 mm", "form": name,
-                   "Nr": Nr, "Nt": Nt, "steps_unit": "fine-step equivalents of dz", "parallelism": "single GPU"},
+                   "Nr": Nr, "Nt": Nt, "steps_unit": "fine-step equivalents of dz", "parallelism": "single GPU",
+                   "trajectory_steps": total, "errors_at_z_m": total * dz,
+                   "requested": {"steps": args.steps, "warmup": args.warmup}},
         "forms": forms,
         "speedup_chosen_vs_plain": forms["plain_fp32"]["ms_per_step"] / forms["chosen"]["ms_per_fine_step_equivalent"],
         "roofline": roof,
@@ -407,7 +472,7 @@ def main():
         "adaptive_loop": adapt,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    print(json.dumps(out, default=_plain))
 
 
 if __name__ == "__main__":

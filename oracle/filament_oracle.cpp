@@ -867,6 +867,24 @@ extern "C" {
 
 const char* orc_last_error(void) { return g_err.c_str(); }
 
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n < 1) { g_err = "thread count must be >= 1"; return 1; }
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+  return 0;
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
 void* orc_create(const orc_config* cfg, const double* E0, int precision) {
   Base* b = nullptr;
   int rc = guarded([&] {
@@ -1016,6 +1034,31 @@ int orc_linear_step(void* h, double dz) {
   return dispatch(h, [&](auto& s) {
     using R = typename std::decay_t<decltype(s)>::C::value_type;
     s.linear_step((R)dz);
+  });
+}
+
+int orc_set_plasma(void* h, const double* rho, const double* kdrho) {
+  return dispatch(h, [&](auto& s) {
+    using R = typename std::decay_t<decltype(s)>::C::value_type;
+    for (size_t q = 0; q < s.rho.size(); q++) {
+      if (rho) s.rho[q] = (R)rho[q];
+      if (kdrho) s.kdrho[q] = (R)kdrho[q];
+    }
+  });
+}
+
+int orc_nl_step(void* h, double* Es, double dz) {
+  return dispatch(h, [&](auto& s) {
+    using Cx = typename std::decay_t<decltype(s)>::C;
+    using R = typename Cx::value_type;
+    const int Nt = s.Nt;
+    std::vector<Cx> row(Nt), scratch((size_t)6 * Nt);
+    for (int i = 0; i < s.Nr; i++) {
+      double* p = Es + 2 * (size_t)i * Nt;
+      for (int j = 0; j < Nt; j++) row[j] = Cx((R)p[2 * j], (R)p[2 * j + 1]);
+      s.nl_step_row(i, row.data(), (R)dz, scratch);
+      for (int j = 0; j < Nt; j++) { p[2 * j] = row[j].real(); p[2 * j + 1] = row[j].imag(); }
+    }
   });
 }
 

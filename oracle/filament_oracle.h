@@ -71,6 +71,9 @@ typedef struct orc_diag {
 void* orc_create(const orc_config* cfg, const double* E0, int precision);
 void  orc_destroy(void* h);
 const char* orc_last_error(void);
+/* OpenMP threads of later calls (the CPU baseline's 1-core figure) */
+int orc_set_threads(int n);
+int orc_max_threads(void);
 
 /* tables (always returned in FP64, after rounding to the working precision) */
 int orc_get_grid(void* h, double* r, double* dr, double* k, double* t, double* w);
@@ -106,6 +109,11 @@ int orc_dht_apply(void* h, double* f, int inverse);      /* one column [Nr] comp
 double orc_tf_eval(void* h, int comp, double I, int power);
 int orc_linear_step(void* h, double dz);                 /* spectral.jl:16-26 on spectral E */
 int orc_nl_rhs(void* h, const double* Es, double* dE);   /* nonlinear_propagators.jl:38-67 */
+/* one RK3 step of the nonlinear propagator on spectral rows Es (in place),
+ * with the plasma arrays as they stand (nonlinear_propagators.jl:70-74) */
+int orc_nl_step(void* h, double* Es, double dz);
+/* replaces rho / kdrho [rows][Nt] (either may be NULL) */
+int orc_set_plasma(void* h, const double* rho, const double* kdrho);
 
 #ifdef __cplusplus
 }

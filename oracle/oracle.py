@@ -93,6 +93,14 @@ def lib():
         L.orc_tf_eval.restype = ctypes.c_double
         L.orc_tf_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
         L.orc_linear_step.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_nl_step.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
+        L.orc_nl_step.restype = ctypes.c_int
+        L.orc_set_plasma.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_set_plasma.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = ctypes.c_int
+        L.orc_max_threads.argtypes = []
+        L.orc_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -104,6 +112,15 @@ def _p(a):
 def _check(rc):
     if rc != 0:
         raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's later calls."""
+    _check(lib().orc_set_threads(int(n)))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
 
 
 def bessel_j0_zeros(n):
@@ -245,6 +262,18 @@ class Oracle:
         out = np.zeros_like(a)
         _check(lib().orc_nl_rhs(self.hp, _p(a.view(np.float64)), _p(out.view(np.float64))))
         return out
+
+    def nl_step(self, Es, dz):
+        """One RK3 step of the nonlinear propagator on spectral rows (plasma as it stands)."""
+        a = np.ascontiguousarray(Es, dtype=np.complex128).copy()
+        _check(lib().orc_nl_step(self.hp, _p(a.view(np.float64)), float(dz)))
+        return a
+
+    def set_plasma(self, rho=None, kdrho=None):
+        r = None if rho is None else np.ascontiguousarray(np.broadcast_to(rho, (self.Nr, self.Nt)), dtype=np.float64)
+        k = None if kdrho is None else np.ascontiguousarray(np.broadcast_to(kdrho, (self.Nr, self.Nt)),
+                                                             dtype=np.float64)
+        _check(lib().orc_set_plasma(self.hp, None if r is None else _p(r), None if k is None else _p(k)))
 
     def analyze(self, z=0.0):
         d = OrcDiag()
