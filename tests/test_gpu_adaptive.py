@@ -167,3 +167,50 @@ def test_device_loop_rejects_batches_and_bad_arguments(pkg):
     finally:
         ens.close()
         m.close()
+
+
+@pytest.mark.parametrize("precision,tol", [(4, 1e-4), (8, 1e-6)])
+def test_device_loop_matches_oracle(pkg, orc, precision, tol):
+    """fil_run_adaptive against the oracle's main loop (Filament.jl:129-164
+    with AStepKerrPlasma, AdaptiveSteps.jl:96-107) on the same input with the
+    Kerr limit active: the same number of steps and stop, and z, dz, Imax,
+    rhomax after every step and the final field within the precision's gate.
+    The reference's rule is FP64 arithmetic (dz0, phimax, phik and phip are
+    the input file's Float64 values, Input.jl:116, AdaptiveSteps.jl:30-72) on
+    the analyzer's Imax/rhomax, with dz converted once to the field's type;
+    the device loop keeps z and dz in single precision as the RT run does
+    (Input.jl:107), the FP64 oracle keeps them in FP64, so the z values agree
+    to single precision.  The FP64 handle runs on the oracle's KZ table."""
+    import _helpers as H
+    p = workload(pkg)
+    dz0 = p["dz_initial"]
+    g = pkg.physics.GridRT(p["rmax"], p["Nr"], p["tmin"], p["tmax"], p["Nt"])
+    E0 = p["initial_condition"](g.r, g.t, p["ru"], p["tu"], p["Iu"])
+    o = orc.Oracle(H.oracle_config(p), E0, 8)
+    phimax = kerr_phimax(pkg, p, 8)
+    n = 24
+    rows = o.run(n, dz0, phimax=phimax)
+    zo = np.array([r["z"] for r in rows])
+    # stop by zmax halfway between the oracle's 20th and 21st z
+    zmax = 0.5 * (zo[20] + zo[21])
+    m = pkg.RTModel(p, precision=precision, KZ=o.KZ() if precision == 8 else None)
+    try:
+        z, nd, reason, lg = m.run_adaptive(0.0, zmax, dz0, phimax, np.inf, 1000, chunk=6)
+        assert (nd, reason) == (21, pkg.lib.STOP_ZMAX)
+        ref = np.array([(r["z"], 0.0, r["Imax"], r["rhomax"]) for r in rows[:nd + 1]])
+        ref[1:, 1] = np.diff(ref[:, 0])
+        rel = lambda a, b: np.max(np.abs(a - b) / np.abs(b))
+        e = dict(z=rel(lg[1:, 0], ref[1:, 0]), dz=rel(lg[1:, 1], ref[1:, 1]), Imax=rel(lg[:, 2], ref[:, 2]),
+                 rhomax=rel(lg[:, 3], ref[:, 3]))
+        # the oracle's field after the same 21 steps
+        o2 = orc.Oracle(H.oracle_config(p), E0, 8)
+        o2.run(nd, dz0, phimax=phimax)
+        ef = float(np.linalg.norm(m.field() - o2.field()) / np.linalg.norm(o2.field()))
+        print(f"device loop vs oracle fp{8 * precision}: {nd} steps, dz {lg[1, 1]:.5g}..{lg[-1, 1]:.5g} (dz0 {dz0}), "
+              f"rel errors {e}, field {ef:.3e}", flush=True)
+        assert lg[1:, 1].max() < 0.95 * dz0, "the Kerr limit is not active"
+        assert e["z"] < max(tol, 2e-7) and e["dz"] < max(tol, 2e-7)
+        assert e["Imax"] < 10 * tol and e["rhomax"] < 100 * tol
+        assert ef < tol
+    finally:
+        m.close()

@@ -100,16 +100,41 @@ def test_full_steps_match_oracle(pkg, orc, precision, tol, oracle_kz):
     assert abs(d["tau"] - rows[-1]["tau"]) <= g.dt + 1e-12
 
 
-def test_zstep_without_analyze_tracks_oracle(pkg, orc):
-    """The bench path (zstep only, no analyzer projection in between)."""
+def analytic_part(E):
+    """Projection of time-domain rows onto their positive frequencies (bins
+    0 .. Nt/2-1 of P \\ E, the analytic spectrum's support)."""
+    S = np.fft.ifft(E, axis=1)
+    S[:, E.shape[1] // 2:] = 0
+    return np.fft.fft(S, axis=1)
+
+
+@pytest.mark.parametrize("precision,tol,oracle_kz", [(4, 1e-4, False), (8, 1e-5, True)])
+def test_zstep_without_analyze_matches_oracle_zsteps(pkg, orc, precision, tol, oracle_kz):
+    """The bench path: Models.zstep repeated with no analysis in between,
+    against the oracle's zstep loop (Models.jl:75-121, same semantics).  The
+    reference then keeps the negative frequencies that its field filter and
+    its propagator create between analyses; the device's half-spectrum state
+    drops them every step.  Their energy is ~1e-10 of the field's here, so
+    the full fields differ by ~1.1e-5 relative L2 after 30 steps (the
+    negative-frequency content itself), and the positive-frequency parts by
+    ~3.4e-6 (that content's feedback through |E|^2 and Re(E)).  Gated: the
+    positive-frequency parts at the precision's tolerance, the full field at
+    2e-5 (FP64) / 1e-4 (FP32)."""
     p = small_rt(pkg, power=6.0)
     o = oracle_for(orc, p)
     dz = 5e-3
-    o.run(30, dz)
-    m = run_gpu(pkg, p, 30, dz, 4, analyze_each=False)
-    err = rel_l2(m.field(), o.field())
-    print(f"zstep-only rel_l2={err:.3e}")
-    assert err < 1e-4
+    for n in range(30):
+        o.zstep((n + 1) * dz, dz)
+    Eo = o.field()
+    m = run_gpu(pkg, p, 30, dz, precision, analyze_each=False, KZ=o.KZ() if oracle_kz else None)
+    E = m.field()
+    e_pos = rel_l2(analytic_part(E), analytic_part(Eo))
+    e_full = rel_l2(E, Eo)
+    e_neg = rel_l2(Eo - analytic_part(Eo), analytic_part(Eo))
+    print(f"zstep-only fp{8 * precision}: positive-frequency part {e_pos:.3e}, full field {e_full:.3e}, "
+          f"oracle's negative-frequency content {e_neg:.3e}")
+    assert e_pos < tol
+    assert e_full < max(tol, 2e-5)
 
 
 @pytest.mark.parametrize("precision,tol", [(4, 1e-4), (8, 1e-10)])
@@ -178,3 +203,42 @@ def test_plasma_air_avalanche_matches_oracle(pkg, orc, precision, tol):
     print(f"air plasma precision={precision} rho {e1:.3e} kdrho {e2:.3e} max rho {rho_o.max():.3e}")
     assert rho_o.max() > 0
     assert e1 < tol and e2 < tol
+
+
+@pytest.mark.parametrize("which", ["cubic", "raman"])
+@pytest.mark.parametrize("precision,tol,oracle_kz", [(4, 1e-4, False), (8, 1e-10, True)])
+def test_thg_steps_match_oracle(pkg, orc, which, precision, tol, oracle_kz):
+    """The THG variants of the cubic and Raman responses (calc_cubic, cubic.jl:33-38;
+    calc_raman, raman.jl:46-54), which the reference's RT and XYT examples run
+    (examples/RT/air.jl:40,64), with plasma and the other responses, on a t grid
+    that resolves 3 w0 (Nt = 1024 on +-125 fs: w_max = 12.9 rad/fs > 3 w0 =
+    7.1 rad/fs).  The third-harmonic content must be present and the THG run
+    must differ from the no-THG one far beyond the gate."""
+    def params(thg):
+        q = pkg.workloads.rt_air(64, 1024, 6.0, tmin=-125.0, tmax=125.0, tau0=20e-15, raman=(which == "raman"))
+        for r in q["responses"]:
+            if pkg.responses.response_kind(r["init"]) == which:
+                r["THG"] = thg
+        return q
+
+    p = params(True)
+    o = oracle_for(orc, p)
+    dz = 5e-3
+    o.run(20, dz)
+    m = run_gpu(pkg, p, 20, dz, precision, KZ=o.KZ() if oracle_kz else None)
+    E, Eo = m.field(), o.field()
+    err = rel_l2(E, Eo)
+    m0 = run_gpu(pkg, params(False), 20, dz, precision, KZ=o.KZ() if oracle_kz else None)
+    effect = rel_l2(m0.field(), Eo)
+    # third-harmonic band of the on-axis spectrum (2.5 w0 .. 3.5 w0) against the oracle's
+    g = m.grid
+    w = np.abs(2 * np.pi * np.fft.fftfreq(g.Nt, g.dt)) / p["tu"]
+    w0 = 2 * np.pi * 299792458.0 / p["lam0"]
+    band = (w > 2.5 * w0) & (w < 3.5 * w0)
+    S, So = np.abs(np.fft.ifft(E[0])) ** 2, np.abs(np.fft.ifft(Eo[0])) ** 2
+    e_band = float(np.linalg.norm(S[band] - So[band]) / np.linalg.norm(So[band]))
+    frac = So[band].sum() / So.sum()
+    print(f"THG {which} fp{8 * precision}: rel_l2 {err:.3e} (without THG {effect:.3e}), "
+          f"3w0 band energy fraction {frac:.3e}, band spectrum error {e_band:.3e}")
+    assert err < tol and effect > 100 * tol
+    assert frac > 1e-12 and e_band < max(100 * tol, 1e-6)
