@@ -83,7 +83,8 @@ def test_rt256x2048_full_run_matches_oracle(pkg, rt256_oracle, precision, tol):
     print(f"RT 256x2048 fp{8 * precision}, {n} steps ({z:.3f} m): " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()),
           flush=True)
     assert e["field"] <= tol
-    assert e["fluence"] <= tol and e["peak_intensity"] <= 2 * tol and e["onaxis_spectrum"] <= 2 * tol
+    # |E|^2 quantities carry about twice the field's relative error
+    assert e["fluence"] <= 2 * tol and e["peak_intensity"] <= 2 * tol and e["onaxis_spectrum"] <= 2 * tol
     assert e["max_electron_density"] <= 20 * tol          # Ne ~ I^8: eight times the intensity error and more
 
 

@@ -130,7 +130,7 @@ def test_zstep_without_analyze_matches_oracle_zsteps(pkg, orc, precision, tol, o
     E = m.field()
     e_pos = rel_l2(analytic_part(E), analytic_part(Eo))
     e_full = rel_l2(E, Eo)
-    e_neg = rel_l2(Eo - analytic_part(Eo), analytic_part(Eo))
+    e_neg = float(np.linalg.norm(Eo - analytic_part(Eo)) / np.linalg.norm(analytic_part(Eo)))
     print(f"zstep-only fp{8 * precision}: positive-frequency part {e_pos:.3e}, full field {e_full:.3e}, "
           f"oracle's negative-frequency content {e_neg:.3e}")
     assert e_pos < tol
@@ -240,5 +240,5 @@ def test_thg_steps_match_oracle(pkg, orc, which, precision, tol, oracle_kz):
     frac = So[band].sum() / So.sum()
     print(f"THG {which} fp{8 * precision}: rel_l2 {err:.3e} (without THG {effect:.3e}), "
           f"3w0 band energy fraction {frac:.3e}, band spectrum error {e_band:.3e}")
-    assert err < tol and effect > 100 * tol
+    assert err < tol and effect > 100 * err and effect > 1e-3
     assert frac > 1e-12 and e_band < max(100 * tol, 1e-6)
