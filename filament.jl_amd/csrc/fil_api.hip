@@ -657,7 +657,7 @@ int analyze_impl(fil_handle* h, int member, double z, fil_diag* d, double* Fr, d
   return 0;
 }
 
-// FieldAnalyzers.analyze! XYT (FieldAnalyzers.jl:287-353)
+// FieldAnalyzers.analyze! XYT (FieldAnalyzers.jl:308-353)
 template <typename R>
 int analyze_xyt_impl(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, double* Ft, double* rho_end, double* S) {
   const int Nr = h->Nr, Nt = h->Nt, Nh = h->Nh, Nx = h->Nx, Ny = h->Ny;
@@ -1062,7 +1062,7 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
           return fail(FIL_EINVAL, "missing Rnl table");
       if (raman && (!R[3] || !t->Hraman)) return fail(FIL_EINVAL, "missing Raman tables (Rnl_raman, Hraman)");
     }
-    // Raman response spectrum (raman.jl:189-202), divided by Nt: the
+    // Raman response spectrum (raman.jl:15-29), divided by Nt: the
     // convolution (raman.jl:68-111) is FFT-(Hraman .* (1/Nt) FFT+(x))
     if (raman) {
       HostArr hv((size_t)Nt, f64, true);

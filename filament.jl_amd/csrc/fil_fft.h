@@ -1,7 +1,7 @@
 // fil_fft.h -- workgroup-wide FFT of one t-row held in LDS (gfx950).
 //
 // Replaces the cuFFT plan along t of the reference (plan_fft!(E, [2]),
-// src/modules/Fields.jl:353; used at Models.jl:162-171, 180-183).
+// src/modules/Fields.jl:113; used at Models.jl:162-171, 180-183).
 //
 // Stockham decimation-in-frequency with radix-FIL_FFT_MAXRAD passes (a last
 // smaller-radix pass when needed): at N = 4096 and radix 8 four passes, each

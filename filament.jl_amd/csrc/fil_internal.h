@@ -48,7 +48,7 @@ inline void fil_launch(F kernel, const dim3& grid, const dim3& block, uint32_t s
 }
 
 // Plasma-equation parameters passed by value to the row kernel
-// (photoionization_avalanche.jl:126-183), in the working precision.
+// (photoionization_avalanche.jl:1-116, right-hand sides :119-205), in the working precision.
 template <typename R>
 struct PlasmaParams {
   int ncomp, ereal, kdep, avalanche;

@@ -321,7 +321,7 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
 // elements it passes in ev when the register passes are not used.
 //   F_cubic = 3/4 |E|^2 Re E  or Re(E)^3          cubic.jl:36,44
 //   F_free  = rho Re E                             current_free.jl:35
-//   F_loss  = kdrho Re E / |E|^2 (0 below 1e-30)   current_losses.jl:131-133, :71-83
+//   F_loss  = kdrho Re E / |E|^2 (0 below 1e-30)   current_losses.jl:36-44, :71-83
 // each multiplied by Rguard(r) Tguard(t) (nonlinear_propagators.jl:50), taken
 // to the analytic spectrum (Models.jl:180-183) and weighted by i QZ Rnl
 // (nonlinear_propagators.jl:51,57).  Two real signals share one complex FFT:

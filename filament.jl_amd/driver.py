@@ -100,7 +100,7 @@ def run(p, precision=L.FP32, form=0, device=0, verbose=True):
         os.makedirs(prefix_dir, exist_ok=True)
     model = RTModel(p, precision=precision, form=form, device=device, basedir=p.get("basedir", "."))
     # the reference keeps z, zmax and dz of the RT geometry in FloatGPU = Float32
-    # (Input.jl:294-297, AdaptiveSteps.jl:106), which fixes the number of steps
+    # (Input.jl:107-108, AdaptiveSteps.jl:106), which fixes the number of steps
     f32 = np.float32
     z = f32(p["z"])
     d = model.analyze(float(z))

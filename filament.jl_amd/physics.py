@@ -19,7 +19,7 @@ HBAR = 1.054571817e-34
 
 
 # ---------------------------------------------------------------------------
-# Units (src/modules/Units.jl:269-286, 339-343)
+# Units (src/modules/Units.jl:67-85, 137-141)
 # ---------------------------------------------------------------------------
 class UnitRT:
     def __init__(self, ru, zu, tu, Iu, rhou):
@@ -42,7 +42,7 @@ class UnitT:
 
 
 def field_unit(unit, n0):
-    """Units.E (Units.jl:339-343)."""
+    """Units.E (Units.jl:137-141)."""
     return math.sqrt(unit.I / (n0 * EPS0 * C0 / 2))
 
 
@@ -160,7 +160,7 @@ def unit_permeability(w):
 
 
 # ---------------------------------------------------------------------------
-# Guards (src/modules/Guards.jl:289-320, 516-581)
+# Guards (src/modules/Guards.jl:78-195, windows :322-390)
 # ---------------------------------------------------------------------------
 def guard_window_left(x, width, p=6):
     if width >= x[-1] - x[0]:
@@ -264,7 +264,7 @@ def kz_table(unit, grid, medium, w0, paraxial):
 
 
 def kz_table_t(unit, grid, medium, w0):
-    """KZ T (spectral.jl:55-75): kt = 0 and paraxial (Input.jl:285-291 forces
+    """KZ T (spectral.jl:55-75): kt = 0 and paraxial (Input.jl:96-103 forces
     it for T), i.e. (beta(w) - w / vf) zu."""
     vf = medium.group_velocity(w0)
     w = grid.w * unit.w
@@ -273,7 +273,7 @@ def kz_table_t(unit, grid, medium, w0):
 
 
 # ---------------------------------------------------------------------------
-# Nonlinear prefactor (nonlinear_propagators.jl:323-343, 404-413), paraxial
+# Nonlinear prefactor (nonlinear_propagators.jl:139-160, 210-230), paraxial
 # ---------------------------------------------------------------------------
 def qz_vector(unit, grid, medium, w0, Eu):
     w = grid.w * unit.w
@@ -317,7 +317,7 @@ def mlejnek_raman_response(t):
 
 
 def rnl_current_free(unit, grid, Eu, nuc, mr):
-    """current_free.jl:58-82."""
+    """current_free.jl:4-28."""
     MR = mr * ME
     w = grid.w * unit.w
     out = np.zeros(grid.Nt, dtype=complex)
@@ -327,7 +327,7 @@ def rnl_current_free(unit, grid, Eu, nuc, mr):
 
 
 def rnl_current_losses(unit, grid, Eu, w0):
-    """current_losses.jl:95-124."""
+    """current_losses.jl:4-33."""
     w = grid.w * unit.w
     out = np.zeros(grid.Nt, dtype=complex)
     nz = w != 0
@@ -336,7 +336,7 @@ def rnl_current_losses(unit, grid, Eu, w0):
 
 
 # ---------------------------------------------------------------------------
-# Plasma equation (photoionization_avalanche.jl:126-183)
+# Plasma equation (photoionization_avalanche.jl:1-116)
 # ---------------------------------------------------------------------------
 def read_tf(fname):
     """Two-column rate table (x [W/m^2], y [1/s])."""
@@ -372,7 +372,7 @@ def _resolve(fname, basedir):
 
 
 # ---------------------------------------------------------------------------
-# QDHT plan (HankelTransforms.plan_dht, used at Fields.jl:348-351):
+# QDHT plan (HankelTransforms.plan_dht, used at Fields.jl:108-110):
 #   T_mn = 2 J0(j_m j_n / S) / (S |J1(j_m)| |J1(j_n)|),  S = j_{N+1}
 #   forward  f -> JV .* (T * (RJ .* f)),  RJ = R/|J1|, JV = |J1|/V, V = S/(2 pi R)
 #   inverse  F -> (T * (F ./ JV)) ./ RJ
@@ -406,7 +406,7 @@ def phi_plasma(unit, medium, w0, Eu, mr, nuc):
 
 
 def rsig2asig(E):
-    """AnalyticSignals.rsig2asig! along the last axis (Fields.jl:355): the
+    """AnalyticSignals.rsig2asig! along the last axis (Fields.jl:115): the
     spectrum keeps bin 0, doubles bins 1..N/2-1 and zeroes N/2..N-1."""
     N = E.shape[-1]
     S = np.fft.ifft(E, axis=-1)

@@ -5,7 +5,7 @@ geometry "RT" / "XYT" (src/Filament.jl:69-76): Unit, Grid, Field, Medium,
 Guard, Model and AStep are built on the host in FP64 and the tables are handed
 to the library once.  ``zstep`` is Models.zstep (src/modules/Models.jl:75-121)
 on the GPU and ``analyze`` is FieldAnalyzers.analyze! (RT
-FieldAnalyzers.jl:166-215, XYT :287-353).
+FieldAnalyzers.jl:166-215, XYT :308-353).
 
 ``p`` carries the reference's input-file names (src/Input.jl:106-178 and the
 medium / initial-condition files), e.g. for RT ``ru, zu, tu, Iu, rhou, rmax,
@@ -41,7 +41,7 @@ class _Model:
         self.plasma = bool(p["PLASMA"])
         if self.nonlinearity and not p.get("QPARAXIAL", True):
             raise NotImplementedError("QPARAXIAL=false is not implemented")
-        # responses (nonlinear_propagators.jl:203-210)
+        # responses (nonlinear_propagators.jl:20-27)
         resp_flags = 0
         self.Rnl = {}
         self.losses_ereal = 0
@@ -256,7 +256,7 @@ class RTModel(_Model):
         self.guard = ph.GuardRT(u, g, m, p["rguard"], p["tguard"], p["kguard"], p["wguard"])
         self.KZ = ph.kz_table(u, g, m, self.w0, bool(p["KPARAXIAL"])) if KZ is None else np.asarray(KZ)
 
-        # initial field (Fields.jl:336-360): user IC on the host in FP64, analytic signal
+        # initial field (Fields.jl:96-120): user IC on the host in FP64, analytic signal
         E0 = np.asarray(p["initial_condition"](g.r, g.t, u.r, u.t, u.I), dtype=float)
         self.E0 = ph.rsig2asig(E0)
         self.T, self.RJ, self.JV = ph.dht_plan(g)

@@ -2,7 +2,7 @@
 
 The reference advances the field by fine steps dz chosen by
 AdaptiveSteps.AStepKerrPlasma (src/modules/AdaptiveSteps.jl:96-107) inside
-Filament.main_loop (src/Filament.jl:140-176).  ``CoarseStepper`` advances the
+Filament.main_loop (src/Filament.jl:111-184).  ``CoarseStepper`` advances the
 same model by ``factor * dz`` instead, and every ``check_every`` coarse steps
 it verifies one of them: from the same saved state it runs the coarse step and
 the ``factor`` fine steps it replaces, compares the two fluence profiles

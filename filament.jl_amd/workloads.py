@@ -50,7 +50,7 @@ def gaussian_ic(a0, tau0, I0, lam0, noise=0.0, seed=0):
 
 def rt_air(Nr, Nt, power_pcr, rmax=10.0, tmin=-500.0, tmax=500.0, a0=1e-3, tau0=35e-15, noise=1e-3,
            seed=0, n2=1e-23, nonlinearity=True, plasma=True, dz=5e-4, raman=False):
-    """Parameter dict in the reference's input-file names (src/Input.jl:293-317).
+    """Parameter dict in the reference's input-file names (examples/RT/input.jl:1-49, read by src/Input.jl:20-130).
     raman: add the stimulated Raman response of air with the same n2 and no THG
     (the response list of examples/RT/air.jl:88)."""
     lam0 = 800e-9

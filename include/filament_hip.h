@@ -10,7 +10,7 @@
  * (nonlinear_propagators.jl:38-74,246-283), the spectral linear step with the
  * Hankel transform (RT) or the 2-D FFT (XYT) (linear_propagators/spectral.jl:16-26,
  * Models.jl:135-148), the guard filters (Guards.jl:199-318) and the RT and XYT
- * field analyzers (FieldAnalyzers.jl:166-215, 287-353).  The host keeps Input.prepare, Units, Grids,
+ * field analyzers (FieldAnalyzers.jl:166-215, 308-353).  The host keeps Input.prepare, Units, Grids,
  * Media and the table construction; it hands the tables over once.
  *
  * Conventions
@@ -106,26 +106,26 @@ typedef struct fil_tables {
   const double* dr;        /* [Nr] */
   const double* t;         /* [Nt] t nodes (grid units) */
   double dt;               /* t step (grid units) */
-  /* Linear propagator (spectral.jl:78-101) and spectral guard (Guards.jl:302-313) */
+  /* Linear propagator (spectral.jl:78-101) and spectral guard (Guards.jl:108-119) */
   const double* KZ;        /* [Nr x Nt] complex, column-major */
   const double* Kguard;    /* [Nr x Nt] */
   const double* Wguard;    /* [Nt] */
-  /* Field guard (Guards.jl:299-300) */
+  /* Field guard (Guards.jl:105-106) */
   const double* Rguard;    /* [Nr] */
   const double* Tguard;    /* [Nt] */
-  /* Hankel plan (HankelTransforms.plan_dht, Fields.jl:348-351):
+  /* Hankel plan (HankelTransforms.plan_dht, Fields.jl:108-110):
    * forward f -> JV .* (T * (RJ .* f)), inverse F -> (T * (F ./ JV)) ./ RJ */
   const double* dht_T;     /* [Nr x Nr] */
   const double* dht_RJ;    /* [Nr] */
   const double* dht_JV;    /* [Nr] */
-  /* Nonlinear propagator (nonlinear_propagators.jl:323-343) and responses */
+  /* Nonlinear propagator (nonlinear_propagators.jl:139-160, 210-230) and responses */
   const double* QZ;        /* [Nt] complex: QPARAXIAL prefactor (independent of r) */
   const double* Rnl_cubic; /* [Nt] complex (cubic.jl:11-12, broadcast) */
-  const double* Rnl_free;  /* [Nt] complex (current_free.jl:67-74) */
-  const double* Rnl_losses;/* [Nt] complex (current_losses.jl:102-108) */
-  const double* Rnl_raman; /* [Nt] complex (raman.jl:186-187, broadcast) */
-  const double* Hraman;    /* [Nt] complex (raman.jl:189-202) */
-  /* Plasma equation (photoionization_avalanche.jl:126-183), per component:
+  const double* Rnl_free;  /* [Nt] complex (current_free.jl:13-23) */
+  const double* Rnl_losses;/* [Nt] complex (current_losses.jl:11-20) */
+  const double* Rnl_raman; /* [Nt] complex (raman.jl:9-12, broadcast) */
+  const double* Hraman;    /* [Nt] complex (raman.jl:15-29) */
+  /* Plasma equation (photoionization_avalanche.jl:1-116), per component:
    * dimensionless frhont, Rava, K; rate table as log10 pairs of the scaled
    * table (x/Iu, y*tu), which must be uniformly spaced in log10(x). */
   double rho0;             /* initial density per component [rhou] */
@@ -155,7 +155,7 @@ typedef struct fil_tables_xyt {
   const double* KXguard;   /* [Nx] spectral guards (all ones in the reference, Guards.jl:183-184) */
   const double* KYguard;   /* [Ny] */
   const double* beta;      /* [Nt] complex beta(w) [1/m] at the grid frequencies */
-  const double* w_vf;      /* [Nt] w / vf [1/m] (moving frame, spectral.jl:141-152) */
+  const double* w_vf;      /* [Nt] w / vf [1/m] (moving frame, spectral.jl:140-150) */
   double zu;               /* [m] */
   int32_t kparaxial;       /* KPARAXIAL */
 } fil_tables_xyt;
@@ -173,7 +173,7 @@ int fil_set_tables(fil_handle* h, const fil_tables* t);
 /* XYT handles (Field/Guard/LinearPropagator XYT, Fields.jl:141-160). */
 int fil_set_tables_xyt(fil_handle* h, const fil_tables* t, const fil_tables_xyt* x);
 
-/* Field in the time domain, column-major E[i + ld*j] (Field.E, Fields.jl:342-355
+/* Field in the time domain, column-major E[i + ld*j] (Field.E, Fields.jl:102-115
  * after rsig2asig!).  Upload replaces the state; download returns what the
  * reference holds in field.E after the last zstep (guarded, time domain). */
 int fil_upload_field(fil_handle* h, const double* E, int64_t ld);
@@ -226,7 +226,7 @@ int fil_analyze(fil_handle* h, double z, fil_diag* d, double* Fr, double* Ft,
 int fil_analyze_member(fil_handle* h, int member, double z, fil_diag* d, double* Fr, double* Ft,
                        double* rho_end, double* S);
 
-/* FieldAnalyzerXYT (FieldAnalyzers.jl:262-353): Fxy [Nx x Ny] (x fastest),
+/* FieldAnalyzerXYT (FieldAnalyzers.jl:272-353): Fxy [Nx x Ny] (x fastest),
  * Ft [Nt], rho_end [Nx x Ny], S [Nt/2+1] may be NULL.  ax, ay: radii of Fxy
  * along x and y through its maximum.  Projects the field onto its analytic
  * part like the RT analyzer.  Blocks. */
@@ -262,7 +262,7 @@ typedef struct fil_adapt_params {
 int fil_run_adaptive(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log,
                      int* nsteps, int* reason);
 
-/* field.rho and field.kdrho (Fields.jl:357-358) as the last plasma solve left
+/* field.rho and field.kdrho (Fields.jl:117-118) as the last plasma solve left
  * them, column-major [i + ld*j].  Blocks. */
 int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld);
 

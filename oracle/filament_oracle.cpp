@@ -150,7 +150,7 @@ struct FFTPlan {
   }
 };
 
-// AnalyticSignals (called at src/modules/Fields.jl:355, Models.jl:181,
+// AnalyticSignals (called at src/modules/Fields.jl:115, Models.jl:181,
 // FieldAnalyzers.jl:188,196).  Assumption: half(N) = N/2 for even N; the
 // analytic spectrum keeps bin 1, doubles bins 2..N/2, zeroes N/2+1..N.
 template <typename T>
@@ -167,7 +167,7 @@ void aspec2rspec(std::complex<T>* S, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// Guard windows (src/modules/Guards.jl:516-581)
+// Guard windows (src/modules/Guards.jl:322-390)
 // ---------------------------------------------------------------------------
 std::vector<double> guard_window_left(const std::vector<double>& x, double width, int p = 6) {
   if (width >= x.back() - x.front()) throw std::runtime_error("Guard width is larger or equal than the grid size.");
@@ -321,7 +321,7 @@ void State<T>::build(const orc_config& c, const double* E0) {
   med.kind = c.medium; med.n_const = c.n_const; med.n2 = c.n2;
   plan.reset(new FFTPlan<T>(Nt));
 
-  double ku = 1.0 / c.ru, wu = 1.0 / c.tu;   // Units.jl:281-286
+  double ku = 1.0 / c.ru, wu = 1.0 / c.tu;   // Units.jl:79-85
 
   // Grids.jl:204-223
   t.resize(Nt);
@@ -347,10 +347,10 @@ void State<T>::build(const orc_config& c, const double* E0) {
     }
   }
 
-  // Fields.jl:336-360 (RT) / :310-333 (T)
+  // Fields.jl:96-120 (RT) / :310-333 (T)
   w0 = 2 * PI * C0 / c.lam0;
   n0 = med.refractive_index(w0).real();
-  Eu = std::sqrt(c.Iu / (n0 * EPS0 * C0 / 2));    // Units.jl:339-343
+  Eu = std::sqrt(c.Iu / (n0 * EPS0 * C0 / 2));    // Units.jl:137-141
   vf = med.group_velocity(w0);                      // spectral.jl:87
   k0 = med.k(w0);
 
@@ -398,7 +398,7 @@ void State<T>::build(const orc_config& c, const double* E0) {
   }
 
   // Linear propagator KZ (spectral.jl:78-101 RT, :131-156 XYT, :55-75 T; Kfunc :160-184)
-  bool kpar = (rt || xyt) ? (c.kparaxial != 0) : true;   // Input.jl:285-291 forces T paraxial
+  bool kpar = (rt || xyt) ? (c.kparaxial != 0) : true;   // Input.jl:96-103 forces T paraxial
   KZ.assign((size_t)Nr * Nt, C(0));
   for (int j = 0; j < Nt; j++) {
     double ww = w[j] * wu;
@@ -416,7 +416,7 @@ void State<T>::build(const orc_config& c, const double* E0) {
       KZ[(size_t)i * Nt + j] = C((T)kz.real(), (T)kz.imag());
     }
   }
-  // Nonlinear prefactor QZ (nonlinear_propagators.jl:323-343; Qfunc :394-426)
+  // Nonlinear prefactor QZ (nonlinear_propagators.jl:139-160; Qfunc :210-242)
   bool qpar = (rt || xyt) ? (c.qparaxial != 0) : true;
   if (!qpar) throw std::runtime_error("QPARAXIAL=false is not restated");
   QZ.assign((size_t)Nr * Nt, C(0));
@@ -430,14 +430,14 @@ void State<T>::build(const orc_config& c, const double* E0) {
 
   // Responses (medium_responses/*.jl)
   double chi3 = med.chi3(w0);
-  double Rk = EPS0 * chi3 * Eu * Eu * Eu;   // cubic.jl:11-12, raman.jl:186-187
+  double Rk = EPS0 * chi3 * Eu * Eu * Eu;   // cubic.jl:11-12, raman.jl:9-12
   for (int q = 0; q < 4; q++) Rnl[q].assign(Nt, C(0));
   if (c.nonlinearity) {
     if (c.cubic) { Ron[0] = true; Rkind[0] = c.cubic; for (int j = 0; j < Nt; j++) Rnl[0][j] = C((T)Rk, 0); }
     if (c.raman) {
       Ron[1] = true; Rkind[1] = c.raman;
       for (int j = 0; j < Nt; j++) Rnl[1][j] = C((T)Rk, 0);
-      // raman.jl:189-202 with the Mlejnek response of test/diffraction/RT/air.jl:48-58
+      // raman.jl:15-29 with the Mlejnek response of test/diffraction/RT/air.jl:48-58
       double tshift = c.tmin + 0.5 * (c.tmax - c.tmin);
       std::vector<cd> H(Nt);
       for (int j = 0; j < Nt; j++) {
@@ -507,7 +507,7 @@ void State<T>::build(const orc_config& c, const double* E0) {
     phip = QZ0 * std::fabs(Rp.real());
   }
 
-  // DHT plan (HankelTransforms.plan_dht, used at Fields.jl:348-351).
+  // DHT plan (HankelTransforms.plan_dht, used at Fields.jl:108-110).
   // Guizar-Sicairos & Gutierrez-Vega, JOSA A 21, 53 (2004):
   //   T_mn = 2 J0(j_m j_n / S) / (S |J1(j_m)| |J1(j_n)|), S = j_{N+1}
   //   forward f -> JV .* (T * (RJ .* f)), RJ = R/|J1|, JV = |J1|/V, V = S/(2 pi R)
@@ -528,7 +528,7 @@ void State<T>::build(const orc_config& c, const double* E0) {
 }
 
 // QDHT of columns 0..Nh-1 of A [Nr][Nt] in place (HankelTransforms `P * E` /
-// `P \ E` with region (Nr, Nthalf), Fields.jl:348-350; Models.jl:135-148).
+// `P \ E` with region (Nr, Nthalf), Fields.jl:108-110; Models.jl:135-148).
 template <typename T>
 void State<T>::dht_columns(C* A, int inverse) {
   std::vector<C> X((size_t)Nr * Nh);

@@ -67,7 +67,7 @@ typedef struct orc_diag {
 } orc_diag;
 
 /* precision: 8 = FP64, 4 = FP32.  E0 is the real initial field [Nr][Nt]
- * (the user initial_condition evaluated on the grid, Fields.jl:342). */
+ * (the user initial_condition evaluated on the grid, Fields.jl:102). */
 void* orc_create(const orc_config* cfg, const double* E0, int precision);
 void  orc_destroy(void* h);
 const char* orc_last_error(void);
@@ -93,7 +93,7 @@ int orc_get_plasma(void* h, double* rho, double* kdrho);
 /* the hot path */
 int orc_zstep(void* h, double z, double dz);            /* Models.jl:75-121 */
 int orc_plasma_solve(void* h);                          /* plasma_equations.jl */
-/* RT (FieldAnalyzers.jl:166-215): Fr [Nr]; XYT (:287-353): Fr = Fxy [rows],
+/* RT (FieldAnalyzers.jl:166-215): Fr [Nr]; XYT (:308-353): Fr = Fxy [rows],
  * rfil = ax, rpl = ay, tau = 0. */
 int orc_analyze(void* h, double z, orc_diag* d, double* Fr, double* Ft,
                 double* rho_end, double* S);

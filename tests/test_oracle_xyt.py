@@ -1,5 +1,5 @@
 """Pins the oracle's XYT geometry (Grids.jl:158-201, Guards.jl:162-195,318,
-spectral.jl:131-184, FieldAnalyzers.jl:287-353) before it checks the device:
+spectral.jl:131-184, FieldAnalyzers.jl:308-353) before it checks the device:
 the 2-D FFT conventions against numpy, Gaussian-beam diffraction against its
 closed form, the guard tables (with and without the reference's y-index
 quirk) against the window formulas, and the full nonlinear step against the
