@@ -1,12 +1,14 @@
-"""Filament.prepare / Filament.run for the RT geometry, host side.
+"""Filament.prepare / Filament.run for the RT, XYT and T geometries, host side.
 
 prepare(fname) follows src/Input.jl:20-200: the input file is executed (it
 is Python here, with the reference's variable names), the process changes
 directory to the input file's directory, then the initial-condition and
-medium files are executed into the same namespace.  run(input) follows
-src/Filament.jl:51-184: build the model, analyze, write plot.dat
-(WritePlots.jl:121-152 format) and loop z-steps with the adaptive step and
-the Istop check.  The z-step and the analyzer run in libfilament_hip.so.
+medium files are executed into the same namespace; the geometry's branch
+(RT :106-130, T :81-105, XYT :151-178) picks the keys the run needs.
+run(input) follows src/Filament.jl:51-184: build the model, analyze, write
+plot.dat (WritePlots.jl:92-116 T, :121-152 RT, :187-215 XYT) and loop
+z-steps with the adaptive step and the Istop check.  The z-step and the
+analyzer run in libfilament_hip.so.
 """
 import math
 import os
@@ -17,7 +19,7 @@ import numpy as np
 from . import lib as L
 from . import physics as ph
 from . import responses as R
-from .solver import RTModel
+from .solver import RTModel, TModel, XYTModel
 
 
 def _namespace():
@@ -39,8 +41,19 @@ class _FFTWShim:
     ifft = staticmethod(np.fft.ifft)
 
 
+_COMMON_KEYS = ("prefix", "zu", "tu", "Iu", "rhou", "tmin", "tmax", "Nt", "zmax", "dz_initial", "NONLINEARITY",
+                "PLASMA", "tguard", "wguard", "dzphimax", "Istop", "z", "lam0", "permittivity", "permeability", "n2",
+                "initial_condition")
+_GEOMETRY_KEYS = {
+    "RT": ("ru", "rmax", "Nr", "KPARAXIAL", "QPARAXIAL", "rguard", "kguard"),
+    "XYT": ("xu", "yu", "xmin", "xmax", "Nx", "ymin", "ymax", "Ny", "KPARAXIAL", "QPARAXIAL", "xguard", "yguard",
+            "kxguard", "kyguard"),
+    "T": (),
+}
+
+
 def prepare(fname):
-    """Input.prepare (src/Input.jl:20-200), RT branch."""
+    """Input.prepare (src/Input.jl:20-200), RT, XYT and T branches."""
     file_input = os.path.abspath(fname)
     ns = _namespace()
     exec(compile(open(file_input).read(), file_input, "exec"), ns)
@@ -49,13 +62,16 @@ def prepare(fname):
         path = os.path.abspath(ns[key])
         exec(compile(open(path).read(), path, "exec"), ns)
     geometry = ns["geometry"]
-    if geometry != "RT":
-        raise ValueError(f"geometry {geometry!r} is not implemented (RT only)")
-    p = {k: ns[k] for k in (
-        "prefix", "ru", "zu", "tu", "Iu", "rhou", "rmax", "Nr", "tmin", "tmax", "Nt", "zmax", "dz_initial",
-        "NONLINEARITY", "PLASMA", "KPARAXIAL", "QPARAXIAL", "rguard", "tguard", "kguard", "wguard",
-        "dzphimax", "Istop", "z", "lam0", "permittivity", "permeability", "n2", "initial_condition")}
+    if geometry not in _GEOMETRY_KEYS:
+        raise ValueError(f"geometry {geometry!r} is not implemented (RT, XYT, T)")
+    missing = [k for k in _COMMON_KEYS + _GEOMETRY_KEYS[geometry] if k not in ns]
+    if missing:
+        raise ValueError(f"input for geometry {geometry} lacks {', '.join(missing)}")
+    p = {k: ns[k] for k in _COMMON_KEYS + _GEOMETRY_KEYS[geometry]}
     p["geometry"] = geometry
+    if geometry == "T":
+        # Input.jl:96-103: the T model is paraxial in both terms
+        p["KPARAXIAL"] = p["QPARAXIAL"] = True
     p["nuc"] = ns.get("nuc", 0.0)
     p["mr"] = ns.get("mr", 1.0)
     p["responses"] = ns.get("responses", []) if p["NONLINEARITY"] else []
@@ -65,12 +81,23 @@ def prepare(fname):
     return p
 
 
-# plot.dat columns for RT (WritePlots.jl:121-152)
+# plot.dat columns: (name, SI unit, unit factor, analyzer key) per geometry
+# (WritePlots.jl:92-116 T, :121-152 RT, :187-215 XYT)
 def _plotvars(p):
-    ru, zu, tu, Iu, rhou = p["ru"], p["zu"], p["tu"], p["Iu"], p["rhou"]
-    return [("z", "m", zu), ("Fmax", "J/m^2", tu * Iu), ("Imax", "W/m^2", Iu), ("Nemax", "1/m^3", rhou),
-            ("De", "1/m", ru ** 2 * rhou), ("rfil", "m", ru), ("rpl", "m", ru), ("tau", "s", tu),
-            ("W", "J", ru ** 2 * tu * Iu)]
+    g = p.get("geometry", "RT")
+    zu, tu, Iu, rhou = p["zu"], p["tu"], p["Iu"], p["rhou"]
+    if g == "T":
+        return [("z", "m", zu, "z"), ("Imax", "W/m^2", Iu, "Imax"), ("Nemax", "1/m^3", rhou, "rhomax"),
+                ("duration", "s", tu, "duration"), ("F", "J/m^2", tu * Iu, "F")]
+    if g == "XYT":
+        xu, yu = p["xu"], p["yu"]
+        return [("z", "m", zu, "z"), ("Fmax", "J/m^2", tu * Iu, "Fmax"), ("Imax", "W/m^2", Iu, "Imax"),
+                ("Nemax", "1/m^3", rhou, "rhomax"), ("De", "1/m", xu * yu * rhou, "De"), ("ax", "m", xu, "ax"),
+                ("ay", "m", yu, "ay"), ("W", "J", xu * yu * tu * Iu, "W")]
+    ru = p["ru"]
+    return [("z", "m", zu, "z"), ("Fmax", "J/m^2", tu * Iu, "Fmax"), ("Imax", "W/m^2", Iu, "Imax"),
+            ("Nemax", "1/m^3", rhou, "rhomax"), ("De", "1/m", ru ** 2 * rhou, "De"), ("rfil", "m", ru, "rfil"),
+            ("rpl", "m", ru, "rpl"), ("tau", "s", tu, "tau"), ("W", "J", ru ** 2 * tu * Iu, "W")]
 
 
 def _fmt(x):
@@ -80,15 +107,28 @@ def _fmt(x):
 def write_plotdat_header(fname, p):
     pv = _plotvars(p)
     with open(fname, "w") as f:
-        f.write("#" + "".join(" %-18s" % n for n, _, _ in pv) + "\n")
-        f.write("#" + "".join(" %-18s" % s for _, s, _ in pv) + "\n")
-        f.write("#" + "".join(" %-18s" % repr(float(u)) for _, _, u in pv) + "\n")
+        f.write("#" + "".join(" %-18s" % v[0] for v in pv) + "\n")
+        f.write("#" + "".join(" %-18s" % v[1] for v in pv) + "\n")
+        f.write("#" + "".join(" %-18s" % repr(float(v[2])) for v in pv) + "\n")
 
 
-def write_plotdat_row(fname, d):
+def write_plotdat_row(fname, d, p=None):
+    keys = [v[3] for v in _plotvars(p or {"zu": 1, "tu": 1, "Iu": 1, "rhou": 1, "ru": 1})]
     with open(fname, "a") as f:
-        f.write("  " + "".join(_fmt(d[k]) + " " for k in ("z", "Fmax", "Imax", "rhomax", "De", "rfil", "rpl",
-                                                          "tau", "W")) + "\n")
+        f.write("  " + "".join(_fmt(d[k]) + " " for k in keys) + "\n")
+
+
+def make_model(p, precision=L.FP32, form=0, device=0):
+    """The geometry's model (Filament.jl:69-76)."""
+    g = p.get("geometry", "RT")
+    kw = dict(precision=precision, form=form, device=device, basedir=p.get("basedir", "."))
+    if g == "RT":
+        return RTModel(p, **kw)
+    if g == "XYT":
+        return XYTModel(p, **kw)
+    if g == "T":
+        return TModel(p, **kw)
+    raise ValueError(f"geometry {g!r} is not implemented")
 
 
 def run(p, precision=L.FP32, form=0, device=0, verbose=True):
@@ -98,27 +138,29 @@ def run(p, precision=L.FP32, form=0, device=0, verbose=True):
     prefix_dir, prefix_name = os.path.dirname(prefix), os.path.basename(prefix)
     if prefix_dir:
         os.makedirs(prefix_dir, exist_ok=True)
-    model = RTModel(p, precision=precision, form=form, device=device, basedir=p.get("basedir", "."))
-    # the reference keeps z, zmax and dz of the RT geometry in FloatGPU = Float32
-    # (Input.jl:107-108, AdaptiveSteps.jl:106), which fixes the number of steps
-    f32 = np.float32
-    z = f32(p["z"])
+    geometry = p.get("geometry", "RT")
+    model = make_model(p, precision=precision, form=form, device=device)
+    # the reference keeps z, zmax and dz of the RT and XYT geometries in FloatGPU = Float32
+    # (Input.jl:107-108, :152-153; AdaptiveSteps.jl:106), the T geometry's in Float64 (Input.jl:82-83),
+    # which fixes the number of steps
+    ft = np.float64 if geometry == "T" else np.float32
+    z = ft(p["z"])
     d = model.analyze(float(z))
     fplot = os.path.join(prefix_dir, prefix_name + "plot.dat")
     write_plotdat_header(fplot, p)
-    write_plotdat_row(fplot, d)
+    write_plotdat_row(fplot, d, p)
     rows = [d]
-    zmax = f32(p["zmax"])
+    zmax = ft(p["zmax"])
     t0 = time.time()
     try:
         while z < zmax:
             if verbose:
                 print(f"z={_fmt(z)}[zu] I={_fmt(d['Imax'])}[Iu] rho={_fmt(d['rhomax'])}[rhou]")
-            dz = f32(model.dz_adaptive(p["dz_initial"], p["dzphimax"], d["Imax"], d["rhomax"]))
-            z = f32(z + dz)
+            dz = ft(model.dz_adaptive(p["dz_initial"], p["dzphimax"], d["Imax"], d["rhomax"]))
+            z = ft(z + dz)
             model.zstep(float(z), float(dz))
             d = model.analyze(float(z))
-            write_plotdat_row(fplot, d)
+            write_plotdat_row(fplot, d, p)
             rows.append(d)
             if d["Imax"] > p["Istop"]:
                 if verbose:

@@ -84,6 +84,16 @@ def rt_air(Nr, Nt, power_pcr, rmax=10.0, tmin=-500.0, tmax=500.0, a0=1e-3, tau0=
     return p
 
 
+def gaussian_ic_t(tau0, I0, lam0):
+    """initial_condition(t, tu, Iu) of a Gaussian pulse sqrt(I0/Iu)
+    exp(-t^2/(2 tau0^2)) cos(w0 t) (the T geometry's form)."""
+    def initial_condition(t, tu, Iu):
+        w0 = 2 * math.pi * ph.C0 / lam0
+        tt = np.asarray(t) * tu
+        return math.sqrt(I0 / Iu) * np.exp(-0.5 * tt ** 2 / tau0 ** 2) * np.cos(w0 * tt)
+    return initial_condition
+
+
 def gaussian_ic_xyt(ax, ay, tau0, I0, lam0, noise=0.0, corr=1e-4, seed=0):
     """initial_condition(x, y, t, xu, yu, tu, Iu) of examples/XYT/gauss.jl form:
     an elliptical Gaussian beam and pulse, times (1 + noise N(x, y)) with N
@@ -202,12 +212,7 @@ def t_air(Nt, I0=1e16, tmin=-100.0, tmax=100.0, tau0=20e-15, nonlinearity=True, 
     responses and plasma of rt_air."""
     lam0 = 800e-9
     nuc, mr = 5e12, 1.0
-
-    def initial_condition(t, tu, Iu):
-        w0 = 2 * math.pi * ph.C0 / lam0
-        tt = np.asarray(t) * tu
-        return math.sqrt(I0 / Iu) * np.exp(-0.5 * tt ** 2 / tau0 ** 2) * np.cos(w0 * tt)
-
+    initial_condition = gaussian_ic_t(tau0, I0, lam0)
     q = rt_air(2, Nt, 1.0, tmin=tmin, tmax=tmax, n2=n2, nonlinearity=nonlinearity, plasma=plasma, dz=dz, raman=raman)
     p = {k: v for k, v in q.items() if k not in ("ru", "rmax", "Nr", "rguard", "kguard", "I0", "Pcr", "power_pcr")}
     p.update(geometry="T", tguard=tguard, initial_condition=initial_condition, I0=I0, nuc=nuc, mr=mr)

@@ -1,7 +1,10 @@
-"""The reference's analytic physics tests (test/test_diffraction_rt.jl,
-test/test_dispersion_rt.jl) restated: same input sets (tests/golden/*_rt/),
-run through prepare/run on the GPU, plot.dat read back and compared with the
-Gaussian-beam / Gaussian-pulse closed forms at the reference's tolerances."""
+"""The reference's analytic physics tests restated through prepare/run on the
+GPU: test/test_diffraction_rt.jl, test/test_dispersion_rt.jl and
+test/test_dispersion_t.jl with their input sets (written from
+tests/golden/reference_inputs.json), plot.dat read back and compared with the
+Gaussian-beam / Gaussian-pulse closed forms at the reference's tolerances;
+and an XYT Gaussian beam (the closed form of test/test_diffraction_xy.jl) run
+the same way."""
 import json
 import math
 import os
@@ -10,15 +13,17 @@ import shutil
 import numpy as np
 import pytest
 
+import _inputs
+
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def _run(pkg, case, precision):
+def _run(pkg, case, precision, tmp_path):
     here = os.getcwd()
     try:
-        p = pkg.driver.prepare(os.path.join(GOLD, case, "input.py"))
+        p = pkg.driver.prepare(_inputs.write_input_set(case, str(tmp_path / case)))
         pkg.driver.run(p, precision=precision, verbose=False)
         data = np.loadtxt(os.path.join("results", "plot.dat"), comments="#").T
         shutil.rmtree("results")
@@ -27,10 +32,14 @@ def _run(pkg, case, precision):
     return data
 
 
+def _gold():
+    return json.load(open(os.path.join(GOLD, "reference", "constants.json")))
+
+
 @pytest.mark.parametrize("precision", [4, 8])
-def test_diffraction_rt(pkg, precision):
-    gold = json.load(open(os.path.join(GOLD, "reference", "constants.json")))
-    data = _run(pkg, "diffraction_rt", precision)
+def test_diffraction_rt(pkg, precision, tmp_path):
+    gold = _gold()
+    data = _run(pkg, "diffraction_rt", precision, tmp_path)
     z, I, a = data[0], data[2], data[5] * 1e-3
     lam0, a0 = 800e-9, 1e-3
     zd = 2 * math.pi / lam0 * a0 ** 2
@@ -44,12 +53,52 @@ def test_diffraction_rt(pkg, precision):
 
 
 @pytest.mark.parametrize("precision", [4, 8])
-def test_dispersion_rt(pkg, precision):
-    gold = json.load(open(os.path.join(GOLD, "reference", "constants.json")))
-    data = _run(pkg, "dispersion_rt", precision)
+def test_dispersion_rt(pkg, precision, tmp_path):
+    gold = _gold()
+    data = _run(pkg, "dispersion_rt", precision, tmp_path)
     z, I = data[0], data[2]
     tau0 = 20e-15
     zd = tau0 ** 2 / gold["k2_s2_per_m"]
     Ith = tau0 / (tau0 * np.sqrt(1 + (z / zd) ** 2))
     print(f"dispersion_rt precision={precision}: max|I-Ith|={np.max(np.abs(I - Ith)):.3e}")
+    assert len(z) == 101
     assert np.max(np.abs(I - Ith)) <= gold["dispersion_rt_intensity_tol"]
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_dispersion_t(pkg, precision, tmp_path):
+    """test/test_dispersion_t.jl: plot.dat columns z, Imax, Nemax, duration, F."""
+    gold = _gold()
+    data = _run(pkg, "dispersion_t", precision, tmp_path)
+    z, I, tau = data[0], data[1], data[3]
+    tau0 = 20e-15
+    zd = tau0 ** 2 / gold["k2_s2_per_m"]
+    tauth = tau0 * np.sqrt(1 + (z / zd) ** 2)
+    Ith = tau0 / tauth
+    print(f"dispersion_t precision={precision}: max|tau-tauth|={np.max(np.abs(tau - tauth)):.3e} s, "
+          f"max|I-Ith|={np.max(np.abs(I - Ith)):.3e}")
+    assert len(z) == 101
+    assert np.max(np.abs(tau - tauth)) <= gold["dispersion_t_duration_tol_s"]
+    assert np.max(np.abs(I - Ith)) <= gold["dispersion_rt_intensity_tol"]
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_diffraction_xyt(pkg, precision, tmp_path):
+    """An XYT Gaussian beam through prepare/run: the 1/e radii ax, ay of the
+    fluence follow a0 sqrt(1 + (z/zd)^2) to one grid step (the level crossing
+    lands on grid nodes, test/test_diffraction_xy.jl's criterion at its grid
+    step) and the peak intensity (a0/a)^2 to 1e-3."""
+    c = _inputs.cases()["diffraction_xyt"]
+    data = _run(pkg, "diffraction_xyt", precision, tmp_path)
+    z, I, ax, ay = data[0], data[2], data[5], data[6]
+    g = c["grid"]
+    dx = (g["xmax"] - g["xmin"]) / (g["Nx"] - 1) * c["units"]["xu"]
+    lam0, a0 = c["pulse"]["lam0"], c["pulse"]["a0"]
+    zd = 2 * math.pi / lam0 * a0 ** 2
+    ath = a0 * np.sqrt(1 + (z / zd) ** 2)
+    Ith = (a0 / ath) ** 2
+    print(f"diffraction_xyt precision={precision}: max|ax-ath|={np.max(np.abs(ax - ath)):.3e} m, "
+          f"max|ay-ath|={np.max(np.abs(ay - ath)):.3e} m (dx {dx:.3e}), max|I-Ith|={np.max(np.abs(I - Ith)):.3e}")
+    assert len(z) == 101
+    assert np.max(np.abs(ax - ath)) <= dx and np.max(np.abs(ay - ath)) <= dx
+    assert np.max(np.abs(I - Ith)) <= 1e-3
