@@ -70,14 +70,15 @@ def test_dispersion_t(pkg, precision, tmp_path):
     """test/test_dispersion_t.jl: plot.dat columns z, Imax, Nemax, duration, F."""
     gold = _gold()
     data = _run(pkg, "dispersion_t", precision, tmp_path)
-    z, I, tau = data[0], data[1], data[3]
+    z, I, tau = data[0], data[1], data[3] * 1e-15     # plot.dat columns in the unit set (tu = 1 fs)
     tau0 = 20e-15
     zd = tau0 ** 2 / gold["k2_s2_per_m"]
     tauth = tau0 * np.sqrt(1 + (z / zd) ** 2)
     Ith = tau0 / tauth
     print(f"dispersion_t precision={precision}: max|tau-tauth|={np.max(np.abs(tau - tauth)):.3e} s, "
           f"max|I-Ith|={np.max(np.abs(I - Ith)):.3e}")
-    assert len(z) == 101
+    # z and dz are Float64 in the T geometry (Input.jl:82): 100 steps of 0.2 end a rounding short of zmax
+    assert len(z) in (101, 102)
     assert np.max(np.abs(tau - tauth)) <= gold["dispersion_t_duration_tol_s"]
     assert np.max(np.abs(I - Ith)) <= gold["dispersion_rt_intensity_tol"]
 
@@ -87,16 +88,23 @@ def test_diffraction_xyt(pkg, precision, tmp_path):
     """An XYT Gaussian beam through prepare/run: the 1/e radii ax, ay of the
     fluence follow a0 sqrt(1 + (z/zd)^2) to one grid step (the level crossing
     lands on grid nodes, test/test_diffraction_xy.jl's criterion at its grid
-    step) and the peak intensity (a0/a)^2 to 1e-3."""
+    step) and the peak intensity (a0/a)^2 tau0/tau(z) to 1e-3 (the 50 fs
+    pulse also disperses in air: tau(z) = tau0 sqrt(1 + (z k2/tau0^2)^2)),
+    sampled where the grid has its nodes nearest the axis, half a step off it
+    in x and in y (an even number of nodes on a symmetric window)."""
     c = _inputs.cases()["diffraction_xyt"]
     data = _run(pkg, "diffraction_xyt", precision, tmp_path)
-    z, I, ax, ay = data[0], data[2], data[5], data[6]
+    xu, yu = c["units"]["xu"], c["units"]["yu"]
+    z, I, ax, ay = data[0], data[2], data[5] * xu, data[6] * yu
     g = c["grid"]
     dx = (g["xmax"] - g["xmin"]) / (g["Nx"] - 1) * c["units"]["xu"]
     lam0, a0 = c["pulse"]["lam0"], c["pulse"]["a0"]
     zd = 2 * math.pi / lam0 * a0 ** 2
     ath = a0 * np.sqrt(1 + (z / zd) ** 2)
-    Ith = (a0 / ath) ** 2
+    tau0 = c["pulse"]["tau0"]
+    dy = (g["ymax"] - g["ymin"]) / (g["Ny"] - 1) * c["units"]["yu"]
+    Ith = ((a0 / ath) ** 2 * np.exp(-((dx / 2) ** 2 + (dy / 2) ** 2) / ath ** 2)
+           / np.sqrt(1 + (z * _gold()["k2_s2_per_m"] / tau0 ** 2) ** 2))
     print(f"diffraction_xyt precision={precision}: max|ax-ath|={np.max(np.abs(ax - ath)):.3e} m, "
           f"max|ay-ath|={np.max(np.abs(ay - ath)):.3e} m (dx {dx:.3e}), max|I-Ith|={np.max(np.abs(I - Ith)):.3e}")
     assert len(z) == 101
