@@ -13,6 +13,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -528,27 +529,27 @@ int analysis_rows(fil_handle* h) {
   return 0;
 }
 
+// Device memory of the device z loop: the loop state, the log, and the copy
+// of the state at the start of a chunk (for a stop inside it).
+struct AdaptBuffers {
+  AdaptState* state;
+  double* log;          // [max_steps + 1][4]
+  void* S;              // ns() * csz() bytes
+  double* re;           // rows() doubles each
+  double* rm;
+};
+
 // Filament.run's z loop (Filament.jl:129-164) on the device, checked by the
 // host every `chunk` steps.  Steps past the stop within a chunk are undone:
 // the state at the chunk's start is restored and the chunk's logged steps are
-// run again with their dz from the host, which gives the same bits.
+// run again with their dz from the host, which gives the same bits.  With
+// `log` given, the logged dz of the steps to run again are read into it (no
+// host buffer); without it into a vector.
 template <typename R>
-int run_adaptive_impl(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log, int* nsteps,
-                      int* reason) {
+int run_adaptive_loop(fil_handle* h, const AdaptBuffers& b, const fil_adapt_params* p, double* z, int max_steps,
+                      double* log, int* nsteps, int* reason) {
   if (int rc = corr_materialize(h)) return rc;
-  const size_t nb = h->ns() * h->csz(), nr = (size_t)h->rows() * 8, nlog = ((size_t)max_steps + 1) * 4 * 8;
-  // each buffer on its own: a failed allocation leaves that buffer empty
-  // (DevBuf keeps the requested size), and the next call tries it again
-  auto need = [](DevBuf& b, size_t bytes) {
-    if (b.p && b.n >= bytes) return true;
-    if (b.alloc(bytes) == hipSuccess) return true;
-    b.p = nullptr;
-    b.n = 0;
-    return false;
-  };
-  if (!need(h->ad_state, sizeof(AdaptState)) || !need(h->ad_log, nlog) || !need(h->ad_S, nb) ||
-      !need(h->ad_re, nr) || !need(h->ad_rm, nr))
-    return fail(FIL_ENOMEM, "device allocation failed");
+  const size_t nb = h->ns() * h->csz(), nr = (size_t)h->rows() * 8;
   AdaptRule rule{};
   rule.dz0 = p->dz0; rule.phimax = p->phimax; rule.phik = p->phik; rule.phip = p->phip; rule.Istop = p->Istop;
   rule.zmax = (float)p->zmax;
@@ -557,8 +558,8 @@ int run_adaptive_impl(fil_handle* h, const fil_adapt_params* p, double* z, int m
   rule.nrows = h->rows();
   rule.max_steps = max_steps;
   const int chunk = p->chunk > 0 ? p->chunk : 32;
-  AdaptState* dst = h->ad_state.as<AdaptState>();
-  double* dlog = h->ad_log.as<double>();
+  AdaptState* dst = b.state;
+  double* dlog = b.log;
   const double* dz_dev = &dst->dz;
   const double* rm = h->d.plasma ? h->rho_max.as<double>() : nullptr;
   AdaptState s{};
@@ -571,13 +572,14 @@ int run_adaptive_impl(fil_handle* h, const fil_adapt_params* p, double* z, int m
   HIPCHK(launch_adapt(rule, dst, h->aImax.as<double>(), rm, dlog, h->st));
   HIPCHK(hipMemcpyAsync(&s, dst, sizeof s, hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
+  std::vector<double> lgv;
   while (!s.done) {
     const int k0 = s.steps - 1;                    // steps completed before this chunk
     const int n = std::min(chunk, max_steps - k0);
     const bool plasma_valid0 = h->plasma_valid;
-    HIPCHK(hipMemcpyAsync(h->ad_S.p, h->S.p, nb, hipMemcpyDeviceToDevice, h->st));
-    HIPCHK(hipMemcpyAsync(h->ad_re.p, h->rho_end.p, nr, hipMemcpyDeviceToDevice, h->st));
-    HIPCHK(hipMemcpyAsync(h->ad_rm.p, h->rho_max.p, nr, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(b.S, h->S.p, nb, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(b.re, h->rho_end.p, nr, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(b.rm, h->rho_max.p, nr, hipMemcpyDeviceToDevice, h->st));
     for (int i = 0; i < n; i++) {
       if (int rc = zstep_impl<R>(h, 0.0, dz_dev)) return rc;
       if (int rc = analysis_rows<R>(h)) return rc;
@@ -590,12 +592,18 @@ int run_adaptive_impl(fil_handle* h, const fil_adapt_params* p, double* z, int m
     const int taken = s.done ? s.steps - k0 : s.steps - 1 - k0;
     if (taken < n) {
       // the stop came inside the chunk: back to its start, then its counted steps again
-      std::vector<double> lg((size_t)taken * 4);
+      double* lg = nullptr;
+      if (log) {
+        lg = log + 4 * ((size_t)k0 + 1);
+      } else {
+        lgv.assign((size_t)taken * 4, 0.0);
+        lg = lgv.data();
+      }
       if (taken)
-        HIPCHK(hipMemcpy(lg.data(), dlog + 4 * ((size_t)k0 + 1), lg.size() * 8, hipMemcpyDeviceToHost));
-      HIPCHK(hipMemcpyAsync(h->S.p, h->ad_S.p, nb, hipMemcpyDeviceToDevice, h->st));
-      HIPCHK(hipMemcpyAsync(h->rho_end.p, h->ad_re.p, nr, hipMemcpyDeviceToDevice, h->st));
-      HIPCHK(hipMemcpyAsync(h->rho_max.p, h->ad_rm.p, nr, hipMemcpyDeviceToDevice, h->st));
+        HIPCHK(hipMemcpy(lg, dlog + 4 * ((size_t)k0 + 1), (size_t)taken * 4 * 8, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpyAsync(h->S.p, b.S, nb, hipMemcpyDeviceToDevice, h->st));
+      HIPCHK(hipMemcpyAsync(h->rho_end.p, b.re, nr, hipMemcpyDeviceToDevice, h->st));
+      HIPCHK(hipMemcpyAsync(h->rho_max.p, b.rm, nr, hipMemcpyDeviceToDevice, h->st));
       h->pending_filter = false;
       h->plasma_valid = plasma_valid0;
       for (int i = 0; i < taken; i++) {
@@ -610,6 +618,28 @@ int run_adaptive_impl(fil_handle* h, const fil_adapt_params* p, double* z, int m
   *nsteps = s.steps;
   *reason = s.done;
   return 0;
+}
+
+// fil_run_adaptive: the loop's buffers are the handle's, allocated on the
+// first call that needs them (each on its own: a failed allocation leaves
+// that buffer empty and the next call tries it again).
+template <typename R>
+int run_adaptive_impl(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log, int* nsteps,
+                      int* reason) {
+  const size_t nb = h->ns() * h->csz(), nr = (size_t)h->rows() * 8, nlog = ((size_t)max_steps + 1) * 4 * 8;
+  auto need = [](DevBuf& b, size_t bytes) {
+    if (b.p && b.n >= bytes) return true;
+    if (b.alloc(bytes) == hipSuccess) return true;
+    b.p = nullptr;
+    b.n = 0;
+    return false;
+  };
+  if (!need(h->ad_state, sizeof(AdaptState)) || !need(h->ad_log, nlog) || !need(h->ad_S, nb) ||
+      !need(h->ad_re, nr) || !need(h->ad_rm, nr))
+    return fail(FIL_ENOMEM, "device allocation failed");
+  const AdaptBuffers b{h->ad_state.as<AdaptState>(), h->ad_log.as<double>(), h->ad_S.p, h->ad_re.as<double>(),
+                       h->ad_rm.as<double>()};
+  return run_adaptive_loop<R>(h, b, p, z, max_steps, log, nsteps, reason);
 }
 
 template <typename R>
@@ -1273,6 +1303,68 @@ int fil_run_adaptive(fil_handle* h, const fil_adapt_params* p, double* z, int ma
   HIPCHK(hipSetDevice(h->d.device));
   return h->fp64 ? run_adaptive_impl<double>(h, p, z, max_steps, log, nsteps, reason)
                  : run_adaptive_impl<float>(h, p, z, max_steps, log, nsteps, reason);
+}
+
+// ---------------------------------------------------------------------------
+// fil_run_adaptive_noalloc: the device z loop with its buffers in `scratch`, a
+// fixed array of device memory that every GPU the library runs on holds from
+// the library's load to the end of the process.  In host code `scratch` names
+// a host-side stand-in; its device address on the current device comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[FIL_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current).
+int scratch_address(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(FIL_EINVAL, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FIL_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = p;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_dev[device]);
+  return 0;
+}
+
+size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+int fil_run_adaptive_noalloc(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log,
+                             int* nsteps, int* reason) {
+  if (!valid_handle(h) || !p || !z || !log || !nsteps || !reason)
+    return fail(FIL_EINVAL, "null argument (fil_run_adaptive_noalloc needs the log array)");
+  if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_run_adaptive_noalloc");
+  if (h->B != 1) return fail(FIL_EINVAL, "fil_run_adaptive_noalloc takes single-pulse handles");
+  if (max_steps < 0 || p->chunk < 0) return fail(FIL_EINVAL, "max_steps and chunk must be >= 0");
+  if (!std::isfinite(p->dz0) || !std::isfinite(*z)) return fail(FIL_EINVAL, "dz0 and z must be finite");
+  // scratch layout: loop state | log | state copy | rho_end copy | rho_max copy
+  const size_t nb = h->ns() * h->csz(), nr = (size_t)h->rows() * 8;
+  const size_t o_log = round256(sizeof(AdaptState));
+  const size_t o_S = o_log + round256(((size_t)max_steps + 1) * 4 * 8);
+  const size_t o_re = o_S + round256(nb);
+  const size_t o_rm = o_re + round256(nr);
+  const size_t need = o_rm + round256(nr);
+  if (need > (size_t)FIL_NOALLOC_SCRATCH_BYTES)
+    return fail(FIL_EINVAL, "fil_run_adaptive_noalloc needs " + std::to_string(need) + " bytes of scratch for this " +
+                                "grid and max_steps; its maximum is " + std::to_string((size_t)FIL_NOALLOC_SCRATCH_BYTES));
+  HIPCHK(hipSetDevice(h->d.device));
+  unsigned char* base = nullptr;
+  if (int rc = scratch_address(h->d.device, &base)) return rc;
+  const AdaptBuffers b{reinterpret_cast<AdaptState*>(base), reinterpret_cast<double*>(base + o_log), base + o_S,
+                       reinterpret_cast<double*>(base + o_re), reinterpret_cast<double*>(base + o_rm)};
+  // per-phase timing creates its events on demand: off for this call
+  const bool timing = h->timing;
+  h->timing = false;
+  const int rc = h->fp64 ? run_adaptive_loop<double>(h, b, p, z, max_steps, log, nsteps, reason)
+                         : run_adaptive_loop<float>(h, b, p, z, max_steps, log, nsteps, reason);
+  h->timing = timing;
+  return rc;
 }
 
 int fil_group_create(fil_handle* const* hs, int n, fil_group** out) {

@@ -74,6 +74,7 @@ class FilCorrDesc(ctypes.Structure):
 CORR_NPARAM = 20258
 
 STOP_ZMAX, STOP_ISTOP, STOP_MAXSTEPS = 1, 2, 3
+NOALLOC_SCRATCH_BYTES = 48 << 20
 
 
 class FilAdaptParams(ctypes.Structure):
@@ -85,7 +86,7 @@ EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_set_tables_xyt
            "fil_analyze_member", "fil_analyze_xyt", "fil_download_plasma", "fil_synchronize", "fil_destroy",
            "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_state_copy",
            "fil_fluence", "fil_group_create", "fil_group_zstep", "fil_group_stats", "fil_group_destroy",
-           "fil_set_tiling", "fil_get_tiling", "fil_run_adaptive",
+           "fil_set_tiling", "fil_get_tiling", "fil_run_adaptive", "fil_run_adaptive_noalloc",
            "fil_corrector_init", "fil_corrector_params", "fil_corrector_scale", "fil_corrector_features",
            "fil_corrector_store_pair", "fil_corrector_train", "fil_corrector_eval", "fil_corrector_forward",
            "fil_corrector_apply")
@@ -125,6 +126,7 @@ def load():
     L.fil_zstep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
     L.fil_run_adaptive.argtypes = [ctypes.c_void_p, ctypes.POINTER(FilAdaptParams), _dp, ctypes.c_int, _dp,
                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.fil_run_adaptive_noalloc.argtypes = L.fil_run_adaptive.argtypes
     L.fil_analyze.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(FilDiag), _dp, _dp, _dp, _dp]
     L.fil_download_plasma.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int64]
     L.fil_synchronize.argtypes = [ctypes.c_void_p]

@@ -189,6 +189,20 @@ class _Model:
                                            ctypes.byref(n), ctypes.byref(reason)))
         return float(zz[0]), n.value, reason.value, log[:n.value + 1]
 
+    def run_adaptive_noalloc(self, z, zmax, dz0, phimax, Istop, max_steps, chunk=0):
+        """run_adaptive through fil_run_adaptive_noalloc: the same loop with its
+        device buffers in the library's fixed scratch array (no device
+        allocation; grids and step counts beyond it are refused)."""
+        p = L.FilAdaptParams(zmax=float(zmax), dz0=float(dz0), phimax=float(phimax), phik=float(self.phik),
+                             phip=float(self.phip), Istop=float(Istop), nonlinearity=int(bool(self.nonlinearity)),
+                             chunk=int(chunk))
+        zz = np.array([float(z)])
+        log = np.zeros((max(int(max_steps), 0) + 1, 4))
+        n, reason = ctypes.c_int(), ctypes.c_int()
+        L.check(self._lib.fil_run_adaptive_noalloc(self.h, ctypes.byref(p), L.ptr(zz), int(max_steps), L.ptr(log),
+                                                   ctypes.byref(n), ctypes.byref(reason)))
+        return float(zz[0]), n.value, reason.value, log[:n.value + 1]
+
     def state_save(self, slot):
         L.check(self._lib.fil_state_save(self.h, int(slot)))
 

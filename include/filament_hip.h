@@ -262,6 +262,22 @@ typedef struct fil_adapt_params {
 int fil_run_adaptive(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log,
                      int* nsteps, int* reason);
 
+/* fil_run_adaptive without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * loop's buffers (loop state, the log, and the copy of the state, rho_end and
+ * rho_max at each chunk's start) live in one fixed device array of
+ * FIL_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the
+ * process ends; a grid and max_steps whose buffers exceed it are refused
+ * with FIL_EINVAL.  `log` is required ([max_steps + 1][4]).  Per-phase timing
+ * is off during the call.  The array keeps the last call's contents and is
+ * shared by every call on that GPU: calls that overlap on one GPU (from
+ * several host threads) race.  Steps, log and state equal fil_run_adaptive's
+ * bit for bit. */
+#define FIL_NOALLOC_SCRATCH_BYTES (48u << 20)
+int fil_run_adaptive_noalloc(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log,
+                             int* nsteps, int* reason);
+
 /* field.rho and field.kdrho (Fields.jl:117-118) as the last plasma solve left
  * them, column-major [i + ld*j].  Blocks. */
 int fil_download_plasma(fil_handle* h, double* rho, double* kdrho, int64_t ld);
