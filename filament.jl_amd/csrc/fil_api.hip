@@ -168,7 +168,20 @@ struct fil_group {
   };
   std::vector<Entry> cache;            // at most 4 instantiated graphs
   int64_t captures = 0, replays = 0;
+  bool orphaned = false;               // a handle of the group was destroyed
 };
+
+// Live groups (their graphs keep the handles' device pointers).
+static std::mutex g_groups_mu;
+static std::vector<fil_group*> g_groups;
+
+static bool in_live_group(const fil_handle* h) {
+  std::lock_guard<std::mutex> lock(g_groups_mu);
+  for (auto* g : g_groups)
+    for (auto* x : g->hs)
+      if (x == h) return true;
+  return false;
+}
 
 namespace {
 
@@ -666,10 +679,12 @@ int analyze_impl(fil_handle* h, int member, double z, fil_diag* d, double* Fr, d
   }
   // scalar diagnostics (FieldAnalyzers.jl:201-213)
   double Fmax = 0, Imax = 0, rhomax = 0, De = 0, W = 0;
+  // maximum() propagates NaN in the reference (FieldAnalyzers.jl:199-203)
+  auto nanmax = [](double a, double b) { return (a != a || b != b) ? a + b : std::max(a, b); };
   for (int i = 0; i < Nr; i++) {
-    Fmax = std::max(Fmax, v.fr[i]);
-    Imax = std::max(Imax, v.im[i]);
-    rhomax = std::max(rhomax, v.rm[i]);
+    Fmax = nanmax(Fmax, v.fr[i]);
+    Imax = nanmax(Imax, v.im[i]);
+    rhomax = nanmax(rhomax, v.rm[i]);
     De += v.re[i] * h->rdr[i];
     W += v.fr[i] * h->rdr[i];
   }
@@ -697,8 +712,13 @@ int analyze_xyt_impl(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, doub
   o.z = z;
   const int imax = (int)(std::max_element(v.fr.begin(), v.fr.end()) - v.fr.begin());   // findmax, x fastest
   o.Fmax = v.fr[imax];
-  o.Imax = *std::max_element(v.im.begin(), v.im.end());
-  o.rhomax = *std::max_element(v.rm.begin(), v.rm.end());
+  auto nanmax = [](double a, double b) { return (a != a || b != b) ? a + b : std::max(a, b); };
+  o.Imax = 0;
+  o.rhomax = 0;
+  for (int i = 0; i < Nr; i++) {
+    o.Imax = nanmax(o.Imax, v.im[i]);
+    o.rhomax = nanmax(o.rhomax, v.rm[i]);
+  }
   double De = 0, W = 0;
   for (int i = 0; i < Nr; i++) De += v.re[i];
   o.De = De * h->dx * h->dy;
@@ -880,13 +900,16 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (ok && d->keep_plasma)
     ok = h->rho_full.alloc(NR * Nt * rz) == hipSuccess && h->kd_full.alloc(NR * Nt * rz) == hipSuccess;
   if (!ok) { delete h; return fail(FIL_ENOMEM, "device allocation failed"); }
-  (void)hipMemset(h->S.p, 0, NS * cz);
-  if (!xyt && !tgeo) (void)hipMemset(h->X.p, 0, NS * cz);
-  (void)hipMemset(h->rho_end.p, 0, NR * 8);
-  (void)hipMemset(h->rho_max.p, 0, NR * 8);
-  if (d->keep_plasma) {
-    (void)hipMemset(h->rho_full.p, 0, NR * Nt * rz);
-    (void)hipMemset(h->kd_full.p, 0, NR * Nt * rz);
+  // the padding rows of S and X must be zero for the Hankel products
+  hipError_t em = hipMemset(h->S.p, 0, NS * cz);
+  if (em == hipSuccess && !xyt && !tgeo) em = hipMemset(h->X.p, 0, NS * cz);
+  if (em == hipSuccess) em = hipMemset(h->rho_end.p, 0, NR * 8);
+  if (em == hipSuccess) em = hipMemset(h->rho_max.p, 0, NR * 8);
+  if (em == hipSuccess && d->keep_plasma) em = hipMemset(h->rho_full.p, 0, NR * Nt * rz);
+  if (em == hipSuccess && d->keep_plasma) em = hipMemset(h->kd_full.p, 0, NR * Nt * rz);
+  if (em != hipSuccess) {
+    delete h;
+    return fail(FIL_EDEVICE, std::string("clearing the device state failed: ") + hipGetErrorString(em));
   }
   *out = h;
   return FIL_OK;
@@ -1192,6 +1215,8 @@ extern "C" {
 int fil_set_tables(fil_handle* h, const fil_tables* t) {
   if (!valid_handle(h) || !t) return fail(FIL_EINVAL, "null argument");
   if (h->xyt) return fail(FIL_EINVAL, "XYT handles take fil_set_tables_xyt");
+  if (in_live_group(h))
+    return fail(FIL_ESTATE, "the handle belongs to a group whose graphs hold its tables: destroy the group first");
   HIPCHK(hipSetDevice(h->d.device));
   return set_tables_impl(h, t, nullptr);
 }
@@ -1199,6 +1224,8 @@ int fil_set_tables(fil_handle* h, const fil_tables* t) {
 int fil_set_tables_xyt(fil_handle* h, const fil_tables* t, const fil_tables_xyt* x) {
   if (!valid_handle(h) || !t || !x) return fail(FIL_EINVAL, "null argument");
   if (!h->xyt) return fail(FIL_EINVAL, "fil_set_tables_xyt needs an XYT handle");
+  if (in_live_group(h))
+    return fail(FIL_ESTATE, "the handle belongs to a group whose graphs hold its tables: destroy the group first");
   HIPCHK(hipSetDevice(h->d.device));
   return set_tables_impl(h, t, x);
 }
@@ -1393,6 +1420,10 @@ int fil_group_create(fil_handle* const* hs, int n, fil_group** out) {
     fil_group_destroy(g);
     return fail(FIL_EDEVICE, "stream or event creation failed");
   }
+  {
+    std::lock_guard<std::mutex> lock(g_groups_mu);
+    g_groups.push_back(g);
+  }
   *out = g;
   return FIL_OK;
 }
@@ -1447,6 +1478,7 @@ int group_capture(fil_group* g, double dz, hipGraphExec_t* exec) {
 int fil_group_zstep(fil_group* g, double z, double dz) {
   (void)z;
   if (!g) return fail(FIL_EINVAL, "null group");
+  if (g->orphaned) return fail(FIL_ESTATE, "a handle of the group was destroyed: destroy the group");
   if (!std::isfinite(dz)) return fail(FIL_EINVAL, "dz must be finite");
   const int n = (int)g->hs.size();
   for (auto* h : g->hs)
@@ -1512,6 +1544,10 @@ int fil_group_destroy(fil_group* g) {
   if (g->fork) (void)hipEventDestroy(g->fork);
   if (g->post) (void)hipEventDestroy(g->post);
   if (g->st) (void)hipStreamDestroy(g->st);
+  {
+    std::lock_guard<std::mutex> lock(g_groups_mu);
+    g_groups.erase(std::remove(g_groups.begin(), g_groups.end(), g), g_groups.end());
+  }
   delete g;
   return FIL_OK;
 }
@@ -1682,6 +1718,12 @@ int fil_synchronize(fil_handle* h) {
 
 int fil_destroy(fil_handle* h) {
   if (!h) return FIL_OK;
+  {
+    // a group that still holds the handle can no longer be stepped
+    std::lock_guard<std::mutex> lock(g_groups_mu);
+    for (auto* g : g_groups)
+      if (std::find(g->hs.begin(), g->hs.end(), h) != g->hs.end()) g->orphaned = true;
+  }
   (void)hipSetDevice(h->d.device);
   (void)hipStreamSynchronize(h->st);
   for (auto e : h->ev) (void)hipEventDestroy(e);

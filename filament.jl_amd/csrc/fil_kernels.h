@@ -196,6 +196,9 @@ hipError_t launch_dht_gemm_bf16x3_from_planes(const uint16_t* Mhi, const uint16_
 // Adaptive z-step on the device (fil_row.hip): the step rule of AStepKerrPlasma
 // (AdaptiveSteps.jl:96-107) and the loop tests of Filament.run
 // (Filament.jl:129-164), evaluated after each step's analysis.
+// max that returns NaN when either argument is NaN (Julia's maximum does)
+__device__ inline double nan_max(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
+
 struct AdaptRule {
   double dz0, phimax, phik, phip, Istop;
   float zmax;             // z and zmax are single precision, as the RT run keeps them

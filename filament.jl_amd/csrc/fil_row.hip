@@ -731,13 +731,14 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
   }
   __syncthreads();
   fft_lds<R, N, NTH, -1>(Z, tw);
+  // maximum(abs2, E) propagates NaN (FieldAnalyzers.jl:201), so does this reduction
   double mx = 0;
-  for (int t = tid; t < N; t += NTH) mx = fmax(mx, (double)norm2(Z[lds_index(t)]));
+  for (int t = tid; t < N; t += NTH) mx = nan_max(mx, (double)norm2(Z[lds_index(t)]));
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  for (int off = 32; off > 0; off >>= 1) mx = nan_max(mx, __shfl_xor(mx, off, 64));
   if (lane == 0) red[wv] = mx;
   __syncthreads();
-  if (tid == 0) { double m = 0; for (int w = 0; w < NTH / 64; w++) m = fmax(m, red[w]); Imax_row[i] = m; }
+  if (tid == 0) { double m = 0; for (int w = 0; w < NTH / 64; w++) m = nan_max(m, red[w]); Imax_row[i] = m; }
 }
 
 // Diagonal linear step of the T geometry: S[i][w] *= P[w] for every row.
@@ -1049,12 +1050,12 @@ __global__ __launch_bounds__(ADAPT_THREADS) void adapt_kernel(AdaptRule rule, Ad
   if (state->done) return;
   double im = 0, rm = 0;
   for (int i = threadIdx.x; i < rule.nrows; i += ADAPT_THREADS) {
-    im = fmax(im, Imax_row[i]);
-    if (rule.plasma) rm = fmax(rm, rhomax_row[i]);
+    im = nan_max(im, Imax_row[i]);
+    if (rule.plasma) rm = nan_max(rm, rhomax_row[i]);
   }
   for (int o = 32; o > 0; o >>= 1) {
-    im = fmax(im, __shfl_xor(im, o));
-    rm = fmax(rm, __shfl_xor(rm, o));
+    im = nan_max(im, __shfl_xor(im, o));
+    rm = nan_max(rm, __shfl_xor(rm, o));
   }
   if ((threadIdx.x & 63) == 0) {
     red[0][threadIdx.x >> 6] = im;
@@ -1064,8 +1065,8 @@ __global__ __launch_bounds__(ADAPT_THREADS) void adapt_kernel(AdaptRule rule, Ad
   if (threadIdx.x != 0) return;
   double Imax = 0, rhomax = 0;
   for (int w = 0; w < ADAPT_THREADS / 64; w++) {
-    Imax = fmax(Imax, red[0][w]);
-    rhomax = fmax(rhomax, red[1][w]);
+    Imax = nan_max(Imax, red[0][w]);
+    rhomax = nan_max(rhomax, red[1][w]);
   }
   const int s = state->steps;
   log[4 * s + 2] = Imax;
@@ -1073,7 +1074,9 @@ __global__ __launch_bounds__(ADAPT_THREADS) void adapt_kernel(AdaptRule rule, Ad
   // Filament.jl:160-164: stop after the step whose analysis exceeds Istop;
   // Filament.jl:129: loop while z < zmax
   int done = 0;
+  // a NaN analysis makes the reference's dz and z NaN, which ends `while z < zmax`
   if (s > 0 && Imax > rule.Istop) done = FIL_STOP_ISTOP;
+  else if (Imax != Imax || rhomax != rhomax) done = FIL_STOP_NONFINITE;
   else if (!(state->z < rule.zmax)) done = FIL_STOP_ZMAX;
   else if (s >= rule.max_steps) done = FIL_STOP_MAXSTEPS;
   if (done) {

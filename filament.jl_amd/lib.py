@@ -73,7 +73,7 @@ class FilCorrDesc(ctypes.Structure):
 
 CORR_NPARAM = 20258
 
-STOP_ZMAX, STOP_ISTOP, STOP_MAXSTEPS = 1, 2, 3
+STOP_ZMAX, STOP_ISTOP, STOP_MAXSTEPS, STOP_NONFINITE = 1, 2, 3, 4
 NOALLOC_SCRATCH_BYTES = 48 << 20
 
 

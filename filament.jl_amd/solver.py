@@ -165,9 +165,13 @@ class _Model:
         L.check(self._lib.fil_synchronize(self.h))
 
     def dz_adaptive(self, dz0, phimax, Imax, rhomax):
-        """AStepKerrPlasma (AdaptiveSteps.jl:96-107)."""
+        """AStepKerrPlasma (AdaptiveSteps.jl:96-107).  Julia's min propagates
+        NaN: a NaN Imax or rhomax gives a NaN dz (and z), which ends the run's
+        `while z < zmax`."""
         if not self.nonlinearity:
             return dz0
+        if math.isnan(Imax) or math.isnan(rhomax):
+            return math.nan
         with np.errstate(divide="ignore"):
             dzk = phimax / (self.phik * Imax) if self.phik * Imax != 0 else math.inf
             dzp = phimax / (self.phip * rhomax) if self.phip * rhomax != 0 else math.inf

@@ -192,7 +192,10 @@ int fil_zstep(fil_handle* h, double z, double dz);
  * as a HIP graph with one branch per handle; later calls replay it.  Replaces
  * the loop over independent runs' Models.zstep calls (Models.jl:75-121,
  * Filament.jl:136).  Ordered after each handle's earlier work and before its
- * later work; asynchronous.  Destroy the group before its handles. */
+ * later work; asynchronous.  Destroy the group before its handles: while a
+ * group lives, fil_set_tables on its handles is refused (its graphs hold their
+ * device pointers), and once one of its handles is destroyed the group refuses
+ * to step. */
 #define FIL_GROUP_MAX 64
 typedef struct fil_group fil_group;
 int fil_group_create(fil_handle* const* hs, int n, fil_group** g);
@@ -242,7 +245,8 @@ int fil_analyze_xyt(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, doubl
  * phimax/(phip rhomax)) from the previous analysis (FP64, then rounded to
  * single precision), z += dz in single precision, zstep, then the analysis
  * (its projection included); the loop ends when z >= zmax, or after a step
- * whose Imax > Istop, or after max_steps steps.  The host checks the loop
+ * whose Imax > Istop, or when an analysis gives a NaN Imax or rhomax, or
+ * after max_steps steps.  The host checks the loop
  * every `chunk` steps (0: 32) instead of after every step.  Steps and their
  * analyses are the kernels fil_zstep and fil_analyze run, so the state after
  * fil_run_adaptive equals the host loop's (fil_analyze, then zstep(z + dz, dz)
@@ -254,6 +258,7 @@ int fil_analyze_xyt(fil_handle* h, double z, fil_diag_xyt* d, double* Fxy, doubl
 #define FIL_STOP_ZMAX 1
 #define FIL_STOP_ISTOP 2
 #define FIL_STOP_MAXSTEPS 3
+#define FIL_STOP_NONFINITE 4   /* an analysis gave a NaN Imax or rhomax (the reference's z becomes NaN) */
 typedef struct fil_adapt_params {
   double zmax, dz0, phimax, phik, phip, Istop;
   int nonlinearity;            /* 0: dz = dz0 (AStepKerrPlasma.NONLINEARITY) */

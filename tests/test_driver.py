@@ -88,3 +88,20 @@ def test_plotdat_columns_of_t_and_xyt(pkg, tmp_path):
     assert lines[1].split()[1:] == ["m", "J/m^2", "W/m^2", "1/m^3", "1/m", "m", "m", "J"]
     assert float(lines[2].split()[5]) == 1e-3 * 2e-3 * 2.5e25 and float(lines[2].split()[7]) == 2e-3
     assert np.allclose(np.loadtxt(f, comments="#"), np.arange(1, 9))
+
+
+def test_adaptive_rule_propagates_nan(pkg):
+    """AStepKerrPlasma with a NaN analysis (AdaptiveSteps.jl:100-106): Julia's
+    min returns NaN, so dz and z become NaN and `while z < zmax` ends the run
+    (Filament.jl:129); a finite analysis gives the usual minimum."""
+    import math
+
+    class Rule:
+        nonlinearity, phik, phip = True, 2.0, 3.0
+    rule = pkg.solver._Model.dz_adaptive
+    assert math.isnan(rule(Rule(), 0.1, 0.5, math.nan, 0.0))
+    assert math.isnan(rule(Rule(), 0.1, 0.5, 1.0, math.nan))
+    assert rule(Rule(), 0.1, 0.5, 1.0, 0.0) == 0.1
+    assert rule(Rule(), 0.1, 0.5, 10.0, 0.0) == 0.5 / 20.0
+    z = np.float32(0.0) + np.float32(math.nan)
+    assert not (z < np.float32(1.0))

@@ -214,3 +214,26 @@ def test_device_loop_matches_oracle(pkg, orc, precision, tol):
         assert ef < tol
     finally:
         m.close()
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_device_loop_stops_on_nan_analysis(pkg, precision):
+    """A field with a NaN sample: the analysis's Imax is NaN (maximum(abs2, E)
+    propagates it, FieldAnalyzers.jl:201), which makes the reference's dz and z
+    NaN and ends its loop before a step.  The device loop stops with
+    FIL_STOP_NONFINITE and takes no step; the host's rule returns NaN.  (No
+    plasma, so no rate-table lookup sees the NaN intensity.)"""
+    p = pkg.workloads.rt_air(64, 256, 3.0, tmin=-125.0, tmax=125.0, tau0=20e-15, plasma=False)
+    m = pkg.RTModel(p, precision=precision)
+    try:
+        E = m.field()
+        E[3, 100] = np.nan
+        m.upload(E)
+        d = m.analyze(0.0)
+        assert np.isnan(d["Imax"])
+        assert np.isnan(m.dz_adaptive(1e-3, 0.1, d["Imax"], d["rhomax"]))
+        z, n, reason, lg = m.run_adaptive(0.0, 1.0, 1e-3, 0.1, np.inf, 50, chunk=4)
+        print(f"fp{8 * precision}: {n} steps, reason {reason}, log {lg}")
+        assert (n, reason) == (0, pkg.lib.STOP_NONFINITE) and z == 0.0 and np.isnan(lg[0, 2])
+    finally:
+        m.close()

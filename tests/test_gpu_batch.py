@@ -213,3 +213,29 @@ def test_group_graph_replays_equal_eager_steps(pkg):
     grp.close()
     for m in eager + graphed:
         m.close()
+
+
+def test_group_guards_its_handles_tables(pkg):
+    """A group's graphs hold its handles' device pointers: fil_set_tables on a
+    grouped handle is refused until the group is destroyed, and a group one of
+    whose handles was destroyed refuses to step (no replay of a graph that
+    points at freed memory)."""
+    L = pkg.lib
+    p = pkg.workloads.rt_air(64, 256, 3.0, tmin=-125.0, tmax=125.0, tau0=20e-15)
+    a, b = pkg.RTModel(p), pkg.RTModel(p)
+    grp = pkg.StepGroup([a, b])
+    try:
+        grp.zstep(1e-3, 1e-3)
+        grp.synchronize()
+        with pytest.raises(L.FilamentError, match="destroy the group first"):
+            a._set_tables()
+        grp.close()
+        a._set_tables()                                  # allowed once the group is gone
+        grp = pkg.StepGroup([a, b])
+        b.close()
+        with pytest.raises(L.FilamentError, match="was destroyed"):
+            grp.zstep(2e-3, 1e-3)
+    finally:
+        grp.close()
+        a.close()
+        b.close()
