@@ -7,7 +7,7 @@ construction, the run driver and plot.dat output.  Device side
 """
 from . import lib, physics, responses, workloads  # noqa: F401
 from .solver import RTEnsemble, RTModel, StepGroup, TModel, XYTModel  # noqa: F401
-from . import corrector, driver, stepping, tuning  # noqa: F401
+from . import corrector, driver, plotfield, stepping, tuning  # noqa: F401
 from .stepping import CoarseStepper  # noqa: F401
 from .driver import prepare, run  # noqa: F401
 from .responses import (RK3, init_cubic, init_current_free, init_current_losses,  # noqa: F401

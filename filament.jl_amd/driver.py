@@ -19,6 +19,7 @@ import numpy as np
 from . import lib as L
 from . import physics as ph
 from . import responses as R
+from .plotfield import PlotField
 from .solver import RTModel, TModel, XYTModel
 
 
@@ -72,6 +73,8 @@ def prepare(fname):
     if geometry == "T":
         # Input.jl:96-103: the T model is paraxial in both terms
         p["KPARAXIAL"] = p["QPARAXIAL"] = True
+    if "dz_plothdf" in ns:
+        p["dz_plothdf"] = ns["dz_plothdf"] if geometry == "T" else float(np.float32(ns["dz_plothdf"]))
     p["nuc"] = ns.get("nuc", 0.0)
     p["mr"] = ns.get("mr", 1.0)
     p["responses"] = ns.get("responses", []) if p["NONLINEARITY"] else []
@@ -131,9 +134,11 @@ def make_model(p, precision=L.FP32, form=0, device=0):
     raise ValueError(f"geometry {g!r} is not implemented")
 
 
-def run(p, precision=L.FP32, form=0, device=0, verbose=True):
-    """Filament.run (src/Filament.jl:51-184) without the HDF5 and info.txt
-    outputs.  Returns the list of diagnostics rows."""
+def run(p, precision=L.FP32, form=0, device=0, verbose=True, plotfield=True):
+    """Filament.run (src/Filament.jl:51-184) without info.txt: plot.dat, and
+    with ``plotfield`` the field snapshots every dz_plothdf and the per-step
+    analyzer profiles of plot.h5 in the ``plot.fdata`` container
+    (filament_amd.plotfield).  Returns the list of diagnostics rows."""
     prefix = p["prefix"]
     prefix_dir, prefix_name = os.path.dirname(prefix), os.path.basename(prefix)
     if prefix_dir:
@@ -145,10 +150,17 @@ def run(p, precision=L.FP32, form=0, device=0, verbose=True):
     # which fixes the number of steps
     ft = np.float64 if geometry == "T" else np.float32
     z = ft(p["z"])
-    d = model.analyze(float(z))
+    arrays = plotfield and geometry in ("RT", "XYT")
+    d = model.analyze(float(z), arrays=arrays)
     fplot = os.path.join(prefix_dir, prefix_name + "plot.dat")
     write_plotdat_header(fplot, p)
     write_plotdat_row(fplot, d, p)
+    pf = None
+    if plotfield:
+        # plot.h5's content and schedule (Filament.jl:93-98; dz_zdata = lam0 / 2 / zu)
+        pf = PlotField(os.path.join(prefix_dir, prefix_name + "plot.fdata"), p, model.grid, z,
+                       p.get("dz_plothdf", math.inf), p["lam0"] / 2 / p["zu"], ftype=ft)
+        pf.write(model, d, z)
     rows = [d]
     zmax = ft(p["zmax"])
     t0 = time.time()
@@ -159,8 +171,10 @@ def run(p, precision=L.FP32, form=0, device=0, verbose=True):
             dz = ft(model.dz_adaptive(p["dz_initial"], p["dzphimax"], d["Imax"], d["rhomax"]))
             z = ft(z + dz)
             model.zstep(float(z), float(dz))
-            d = model.analyze(float(z))
+            d = model.analyze(float(z), arrays=arrays)
             write_plotdat_row(fplot, d, p)
+            if pf is not None:
+                pf.write(model, d, z)
             rows.append(d)
             if d["Imax"] > p["Istop"]:
                 if verbose:

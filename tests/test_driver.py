@@ -105,3 +105,44 @@ def test_adaptive_rule_propagates_nan(pkg):
     assert rule(Rule(), 0.1, 0.5, 10.0, 0.0) == 0.5 / 20.0
     z = np.float32(0.0) + np.float32(math.nan)
     assert not (z < np.float32(1.0))
+
+
+def test_plotfield_container_and_schedule(pkg, tmp_path):
+    """plot.h5's write schedule (WritePlots.jl:263-283) and content in the
+    plot.fdata container: a field dataset when z >= the next field distance
+    (then + dz_plothdf) and z differs from the last zdata z; a zdata row when
+    z >= the next zdata distance (then z + dz_zdata); Float32 distances."""
+    class Stub:
+        precision = 4
+
+        def __init__(self):
+            self.E = np.zeros((3, 8), dtype=complex)
+
+        def field(self):
+            return self.E
+
+    class Grid:
+        r = np.arange(3.0)
+        t = np.arange(8.0)
+
+    p = dict(geometry="RT", ru=1e-3, zu=1.0, tu=1e-15, Iu=1e16, rhou=2.5e25)
+    path = str(tmp_path / "plot.fdata")
+    pf = pkg.plotfield.PlotField(path, p, Grid(), 0.0, 0.25, 0.1, ftype=np.float32)
+    m = Stub()
+    zs = [0.0, 0.05, 0.1, 0.2, 0.26, 0.3, 0.55]
+    for k, z in enumerate(zs):
+        m.E = (k + 1) * (1.0 + 1j) * np.ones((3, 8))
+        d = dict(Fr=np.full(3, k), rho_end=np.zeros(3), Ft=np.full(8, 2 * k), S=np.full(5, 3 * k))
+        pf.write(m, d, z)
+    meta = pkg.plotfield.meta(path)
+    assert meta["version"] == "3.0" and meta["units"]["zu"] == 1.0 and meta["grid"] == {"r": 3, "t": 8}
+    assert [(e["dset"], e["z"]) for e in meta["field"]] == [("000", 0.0), ("001", float(np.float32(0.26))),
+                                                           ("002", float(np.float32(0.55)))]
+    E, z = pkg.plotfield.read_field(path, "001")
+    assert E.dtype == np.float32 and np.all(E == 5.0) and z == float(np.float32(0.26))
+    # zdata at 0 (next 0.1), 0.1 (next 0.2), 0.2 (0.3), 0.3 (0.4), 0.55
+    assert np.allclose(pkg.plotfield.read_zdata(path, "z")[:, 0], np.float32([0.0, 0.1, 0.2, 0.3, 0.55]))
+    assert pkg.plotfield.read_zdata(path, "Fzt").shape == (5, 8)
+    assert np.allclose(pkg.plotfield.read_zdata(path, "iSzf")[:, 0], [0, 6, 9, 15, 18])
+    Er, zm = pkg.plotfield.restart(path, "002")
+    assert np.all(Er == 7.0) and zm == float(np.float32(0.55))
