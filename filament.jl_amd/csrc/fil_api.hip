@@ -135,6 +135,11 @@ struct fil_handle {
   } corr;
   PlasmaParams<float> pp32{};
   PlasmaParams<double> pp64{};
+  // non-paraxial nonlinear term (fil_desc.nonparaxial): Q(k, w) = QZ(k, w) / QZ(w)
+  // Kguard Wguard [Nrp][Nh], the stage right-hand side, the stage derivatives
+  // K1, K2, the next stage's input, and the first stage's plasma rows
+  bool nonparax = false;
+  DevBuf npQ, npR, npK1, npK2, npU, np_rho, np_kd;
   // device z loop (fil_run_adaptive): loop state, log, the state at the last host check
   DevBuf ad_state, ad_log, ad_S, ad_re, ad_rm;
   // timing
@@ -341,6 +346,8 @@ int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr) {
   a.rho_max = h->rho_max.as<double>();
   a.rho_full = h->d.keep_plasma ? h->rho_full.as<R>() : nullptr;
   a.kd_full = h->d.keep_plasma ? h->kd_full.as<R>() : nullptr;
+  if (h->nonparax)
+    if (int rc = corr_materialize(h)) return rc;
   if constexpr (sizeof(R) == 4) {
     if (h->corr_pending) {
       // the row kernel adds the corrector output before the field filter
@@ -362,7 +369,31 @@ int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr) {
       return fail(FIL_EDEVICE, std::string(#launch_) + ": " + hipGetErrorString(e_));               \
     }                                                                                               \
   } while (0)
-  FIL_TIMED(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
+  if (h->nonparax) {
+    // three stages: right-hand side rows, then M2 (Q .* (M1 R)) -> K_s, then Kutta's combination
+    const size_t n = h->ns();
+    cx<R>* S = h->S.as<cx<R>>();
+    cx<R>* K1 = h->npK1.as<cx<R>>();
+    cx<R>* K2 = h->npK2.as<cx<R>>();
+    cx<R>* U = h->npU.as<cx<R>>();
+    cx<R>* Rb = h->npR.as<cx<R>>();
+    cx<R>* Ks[3] = {K1, K2, U};      // the third stage's input is consumed before K3 is written
+    for (int s = 0; s < 3; s++) {
+      if (s == 0)
+        FIL_TIMED(launch_np_stage<R>(a, nullptr, Rb, h->np_rho.as<R>(), h->np_kd.as<R>(), 1, h->rows(), h->Nt,
+                                     h->scan64, h->st));
+      else
+        HIPCHK(launch_np_stage<R>(a, U, Rb, h->np_rho.as<R>(), h->np_kd.as<R>(), 0, h->rows(), h->Nt, h->scan64,
+                                  h->st));
+      HIPCHK(launch_dht_gemm<R>(h->M1.as<R>(), Rb, h->X.as<cx<R>>(), h->npQ.as<cx<R>>(), h->Nrp, h->Nh * h->B, h->Nh,
+                                h->st, h->tiling));
+      HIPCHK(launch_dht_gemm<R>(h->M2.as<R>(), h->X.as<cx<R>>(), Ks[s], nullptr, h->Nrp, h->Nh * h->B, h->Nh, h->st,
+                                h->tiling));
+      HIPCHK(launch_rk3_combine<R>(s == 2 ? S : U, S, K1, K2, U, (R)dz, dz_dev, s + 1, n, h->st));
+    }
+  } else {
+    FIL_TIMED(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
+  }
   h->corr_pending = false;
   if (h->tgeo) {
     // propagate!(E, LP, dz) for T (spectral.jl:16-26): E(w) *= exp(i KZ dz) Wguard
@@ -863,6 +894,8 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (d->plasma && (d->ncomp < 1 || d->ncomp > FIL_MAXCOMP)) return fail(FIL_EINVAL, "ncomp must be in [1, 4]");
   if ((d->form & FIL_FORM_DHT_BF16X3) && d->precision != FIL_FP32)
     return fail(FIL_EINVAL, "the split-BF16 Hankel form needs FIL_FP32");
+  if (d->nonparaxial && (d->geometry != FIL_GEOM_RT || (d->form & FIL_FORM_DHT_BF16X3)))
+    return fail(FIL_EINVAL, "the non-paraxial nonlinear term is built for the RT geometry with the FP32/FP64 Hankel products");
   if ((d->responses & FIL_RESP_RAMAN_THG) && (d->responses & FIL_RESP_RAMAN_NOTHG))
     return fail(FIL_EINVAL, "choose one Raman response");
   if ((d->responses & FIL_RESP_CUBIC_THG) && (d->responses & FIL_RESP_CUBIC_NOTHG))
@@ -899,6 +932,13 @@ int fil_create(const fil_desc* d, fil_handle** out) {
             h->acol.alloc(std::min<size_t>(COLSUM_MAXCHUNK, (h->Nr + 63) / 64) * batch * Nt * 8) == hipSuccess;
   if (ok && d->keep_plasma)
     ok = h->rho_full.alloc(NR * Nt * rz) == hipSuccess && h->kd_full.alloc(NR * Nt * rz) == hipSuccess;
+  h->nonparax = d->nonparaxial && d->nonlinearity;
+  if (ok && h->nonparax)
+    ok = h->npQ.alloc(NP * cz) == hipSuccess && h->npR.alloc(NS * cz) == hipSuccess &&
+         h->npK1.alloc(NS * cz) == hipSuccess && h->npK2.alloc(NS * cz) == hipSuccess &&
+         h->npU.alloc(NS * cz) == hipSuccess &&
+         h->np_rho.alloc(d->plasma ? NR * Nt * rz : 0) == hipSuccess &&
+         h->np_kd.alloc(d->plasma ? NR * Nt * rz : 0) == hipSuccess;
   if (!ok) { delete h; return fail(FIL_ENOMEM, "device allocation failed"); }
   // the padding rows of S and X must be zero for the Hankel products
   hipError_t em = hipMemset(h->S.p, 0, NS * cz);
@@ -907,6 +947,9 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (em == hipSuccess) em = hipMemset(h->rho_max.p, 0, NR * 8);
   if (em == hipSuccess && d->keep_plasma) em = hipMemset(h->rho_full.p, 0, NR * Nt * rz);
   if (em == hipSuccess && d->keep_plasma) em = hipMemset(h->kd_full.p, 0, NR * Nt * rz);
+  // the stage buffers' padding rows enter the Hankel products (times zero)
+  for (DevBuf* b : {&h->npQ, &h->npR, &h->npK1, &h->npK2, &h->npU})
+    if (em == hipSuccess && b->p) em = hipMemset(b->p, 0, b->n);
   if (em != hipSuccess) {
     delete h;
     return fail(FIL_EDEVICE, std::string("clearing the device state failed: ") + hipGetErrorString(em));
@@ -1075,6 +1118,21 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
     HIPCHK(upload(h->KZ64, kz.data(), kz.size() * 8));
     HIPCHK(upload(h->KG64, kg.data(), kg.size() * 8));
     h->P_dz = NAN;
+  }
+  if (h->nonparax) {
+    // Q(k, w) = QZ(k, w) / QZ(w) Kguard Wguard: the non-paraxial prefactor
+    // (nonlinear_propagators.jl:139-160, 232-242) over the paraxial one the
+    // stage right-hand side carries, with func!'s spectral filter (:61)
+    if (!t->QZ_kw || !t->QZ) return fail(FIL_EINVAL, "missing QZ_kw or QZ table");
+    HostArr q((size_t)Nrp * Nh, f64, true);
+    for (int m = 0; m < Nr; m++)
+      for (int w = 0; w < Nh; w++) {
+        const size_t src = m + (size_t)Nr * w;
+        const std::complex<double> qk(t->QZ_kw[2 * src], t->QZ_kw[2 * src + 1]), q0(t->QZ[2 * w], t->QZ[2 * w + 1]);
+        const std::complex<double> v = (q0 == 0.0) ? 0.0 : qk / q0 * (t->Kguard[src] * t->Wguard[w]);
+        q.setc((size_t)m * Nh + w, v.real(), v.imag());
+      }
+    HIPCHK(upload(h->npQ, q.b.data(), q.b.size()));
   }
   }  // RT linear-step tables
   // twiddles, field guards: Rg per row, Tg per t sample (Guards.jl:265-318).  The

@@ -44,6 +44,17 @@ struct RowArgs {
 
 template <typename R>
 hipError_t launch_row(const RowArgs<R>& a, int Nr, int N, bool scan64, hipStream_t st);
+// Non-paraxial nonlinear step (fil_nonpar.hip): one RK stage's paraxial
+// right-hand side of every row into Rout (first: from S, with the pending
+// filter and the plasma solve, whose rows go to rho_g / kd_g; else from U),
+// and Kutta's combination for stage 1, 2 (out = next stage's input) or 3
+// (out = the new state) over n elements.
+template <typename R>
+hipError_t launch_np_stage(const RowArgs<R>& a, const cx<R>* U, cx<R>* Rout, R* rho_g, R* kd_g, int first, int rows,
+                           int N, bool scan64, hipStream_t st);
+template <typename R>
+hipError_t launch_rk3_combine(cx<R>* out, const cx<R>* S, const cx<R>* K1, const cx<R>* K2, const cx<R>* K3, R dz,
+                              const double* dz_dev, int stage, size_t n, hipStream_t st);
 // T geometry linear step: S[i][w] *= P[w], i < rows, w < Nh
 template <typename R>
 hipError_t launch_scale_rows(cx<R>* S, const cx<R>* P, int rows, int Nh, hipStream_t st);
@@ -193,12 +204,12 @@ hipError_t launch_dht_gemm_bf16x3_to_planes(const uint16_t* Mhi, const uint16_t*
 hipError_t launch_dht_gemm_bf16x3_from_planes(const uint16_t* Mhi, const uint16_t* Mlo, const uint16_t* Xp,
                                               cx<float>* Out, int Nr, int Nh, int Pw, hipStream_t st, int tiling = 0);
 
-// Adaptive z-step on the device (fil_row.hip): the step rule of AStepKerrPlasma
-// (AdaptiveSteps.jl:96-107) and the loop tests of Filament.run
-// (Filament.jl:129-164), evaluated after each step's analysis.
 // max that returns NaN when either argument is NaN (Julia's maximum does)
 __device__ inline double nan_max(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
 
+// Adaptive z-step on the device (fil_row.hip): the step rule of AStepKerrPlasma
+// (AdaptiveSteps.jl:96-107) and the loop tests of Filament.run
+// (Filament.jl:129-164), evaluated after each step's analysis.
 struct AdaptRule {
   double dz0, phimax, phik, phip, Istop;
   float zmax;             // z and zmax are single precision, as the RT run keeps them

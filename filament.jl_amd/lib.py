@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libfilament_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAXCOMP = 4
 GEOM_RT, GEOM_XYT, GEOM_T = 0, 1, 2
 FP32, FP64 = 4, 8
@@ -28,7 +28,7 @@ class FilDesc(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "abi_version", "geometry", "Nr", "Nt", "precision", "form", "nonlinearity", "plasma",
         "responses", "losses_ereal", "plasma_ereal", "plasma_kdep", "ncomp", "keep_plasma", "device",
-        "Nx", "Ny", "guard3d_bug", "batch")]
+        "Nx", "Ny", "guard3d_bug", "batch", "nonparaxial")]
 
 
 class FilTables(ctypes.Structure):
@@ -42,6 +42,7 @@ class FilTables(ctypes.Structure):
         ("frhont", ctypes.c_double * MAXCOMP), ("Rava", ctypes.c_double * MAXCOMP),
         ("K", ctypes.c_double * MAXCOMP), ("ntab", ctypes.c_int32 * MAXCOMP),
         ("tab_log10x", _dp * MAXCOMP), ("tab_log10y", _dp * MAXCOMP),
+        ("QZ_kw", _dp),
     ]
 
 

@@ -284,6 +284,20 @@ def qz_vector(unit, grid, medium, w0, Eu):
     return Q * unit.z / Eu
 
 
+def qz_table_rt(unit, grid, medium, w0, Eu):
+    """Non-paraxial QZ(k, w) of QZfunc RT with Qfunc_nonparaxial
+    (nonlinear_propagators.jl:139-160, 232-242): mu0 mu w^2 / (2 kz) zu / Eu,
+    kz = sqrt(beta^2 - kt^2 + 0im), 0 where kz = 0; [Nr, Nt] complex."""
+    w = grid.w * unit.w
+    beta = np.asarray(medium.beta(w), dtype=complex)
+    mu = np.vectorize(medium.permeability, otypes=[float])(w)
+    kt = np.asarray(grid.k) * unit.k
+    kz = np.sqrt(beta[None, :] ** 2 - kt[:, None] ** 2 + 0j)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        Q = np.where(kz == 0, 0.0, MU0 * mu[None, :] * w[None, :] ** 2 / (2 * kz))
+    return Q * unit.z / Eu
+
+
 # ---------------------------------------------------------------------------
 # Responses (medium_responses/*.jl)
 # ---------------------------------------------------------------------------

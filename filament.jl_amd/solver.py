@@ -39,8 +39,9 @@ class _Model:
         self.Eu = ph.field_unit(u, self.n0)
         self.nonlinearity = bool(p["NONLINEARITY"])
         self.plasma = bool(p["PLASMA"])
-        if self.nonlinearity and not p.get("QPARAXIAL", True):
-            raise NotImplementedError("QPARAXIAL=false is not implemented")
+        self.nonparaxial = bool(self.nonlinearity and not p.get("QPARAXIAL", True))
+        if self.nonparaxial and not isinstance(self, RTModel):
+            raise NotImplementedError("QPARAXIAL=false is built for the RT geometry")
         # responses (nonlinear_propagators.jl:20-27)
         resp_flags = 0
         self.Rnl = {}
@@ -95,6 +96,7 @@ class _Model:
             d.ncomp = len(self.pt["components"])
         d.keep_plasma = int(keep_plasma)
         d.device = device
+        d.nonparaxial = int(getattr(self, "nonparaxial", False))
         return d
 
     def _create(self, d):
@@ -296,6 +298,9 @@ class RTModel(_Model):
         t.Kguard = arr(self.guard.Kguard.T)
         t.Rguard = arr(self.guard.Rguard)
         t.dht_T, t.dht_RJ, t.dht_JV = arr(self.T.T), arr(self.RJ), arr(self.JV)
+        if self.nonparaxial:
+            QZk = ph.qz_table_rt(self.unit, g, self.medium, self.w0, self.Eu)
+            t.QZ_kw = arr(QZk.T, True)         # column-major [Nr x Nt]
         L.check(self._lib.fil_set_tables(self.h, ctypes.byref(t)))
 
     def analyze(self, z=0.0, arrays=False):

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define FIL_ABI_VERSION 3
+#define FIL_ABI_VERSION 4
 #define FIL_MAXCOMP 4
 
 typedef struct fil_handle fil_handle;
@@ -96,6 +96,10 @@ typedef struct fil_desc {
    * field and plasma, the tables are shared.  Batched handles take the
    * *_member entry points and no corrector. */
   int32_t batch;
+  /* RT only: QPARAXIAL = false, the non-paraxial prefactor of the nonlinear
+   * term (nonlinear_propagators.jl:58-63, 232-242); needs fil_tables.QZ_kw
+   * and the FP32/FP64 Hankel products (not FIL_FORM_DHT_BF16X3).  0: paraxial. */
+  int32_t nonparaxial;
 } fil_desc;
 
 /* Host-precomputed tables, FP64 (the library rounds them to the working
@@ -135,6 +139,10 @@ typedef struct fil_tables {
   int32_t ntab[FIL_MAXCOMP];
   const double* tab_log10x[FIL_MAXCOMP];
   const double* tab_log10y[FIL_MAXCOMP];
+  /* RT with fil_desc.nonparaxial: QZ(k, w) = mu0 mu w^2 / (2 kz) zu / Eu,
+   * kz = sqrt(beta^2 - k^2 + 0im) (nonlinear_propagators.jl:139-160, 232-242),
+   * [Nr x Nt] complex, column-major; NULL otherwise. */
+  const double* QZ_kw;
 } fil_tables;
 
 /* XYT tables (Grids.jl:158-201, Guards.jl:162-195, spectral.jl:131-184), FP64.

@@ -254,6 +254,7 @@ struct Base {
   double dt = 0, w0 = 0, n0 = 0, Eu = 0, vf = 0, k0 = 0, phik = 0, phip = 0;
   std::vector<TabFunc> tabs;
   std::vector<double> frhont, Rava, Ks;
+  bool qpar = true;
   virtual int precision() const = 0;
 };
 
@@ -282,6 +283,8 @@ struct State : Base {
   void plasma_solve();
   void rhs_row(int i, const C* Es, C* dE, C* Et, C* F) const;
   void nl_step_row(int i, C* Es, T dz, std::vector<C>& scratch) const;
+  void qz_space(C* dE);
+  void nl_step_all(C* u, T dz);
   void linear_step(T dz);
   void zstep(T z, T dz);
   void analyze(double z, orc_diag* d, double* Fr, double* Ft, double* rho_end, double* S);
@@ -363,7 +366,7 @@ void State<T>::build(const orc_config& c, const double* E0) {
   rho.assign((size_t)Nr * Nt, 0); kdrho.assign((size_t)Nr * Nt, 0);
   Ftmp.assign((size_t)Nr * Nt, C(0));
 
-  // Guards.jl:289-320 (RT), :272-283 (T), :162-195 (XYT)
+  // Guards.jl:95-126 (RT), :78-89 (T), :162-195 (XYT)
   auto Tgd = guard_window_both(t, c.tguard);
   Tg.assign(Tgd.begin(), Tgd.end());
   Rg.assign(Nr, 1);
@@ -416,16 +419,30 @@ void State<T>::build(const orc_config& c, const double* E0) {
       KZ[(size_t)i * Nt + j] = C((T)kz.real(), (T)kz.imag());
     }
   }
-  // Nonlinear prefactor QZ (nonlinear_propagators.jl:139-160; Qfunc :210-242)
-  bool qpar = (rt || xyt) ? (c.qparaxial != 0) : true;
-  if (!qpar) throw std::runtime_error("QPARAXIAL=false is not restated");
+  // Nonlinear prefactor QZ (nonlinear_propagators.jl:139-208; Qfunc :210-242):
+  // paraxial mu0 mu w^2 / (2 beta), or with kz = sqrt(beta^2 - kt^2 + 0im)
+  // mu0 mu w^2 / (2 kz) (0 where beta, resp. kz, is 0)
+  qpar = (rt || xyt) ? (c.qparaxial != 0) : true;
   QZ.assign((size_t)Nr * Nt, C(0));
   for (int j = 0; j < Nt; j++) {
     double ww = w[j] * wu;
     cd beta = med.beta(ww);
-    cd Q = (beta == cd(0)) ? cd(0) : MU0 * med.permeability(ww) * ww * ww / (2.0 * beta);
-    Q = Q * c.zu / Eu;
-    for (int i = 0; i < Nr; i++) QZ[(size_t)i * Nt + j] = C((T)Q.real(), (T)Q.imag());
+    for (int i = 0; i < Nr; i++) {
+      cd Q;
+      if (qpar) {
+        Q = (beta == cd(0)) ? cd(0) : MU0 * med.permeability(ww) * ww * ww / (2.0 * beta);
+      } else {
+        double kt = k[i] * ku;
+        if (xyt) {
+          const double kxs = kx[i % Nx] / c.xu, kys = ky[i / Nx] / c.yu;
+          kt = std::sqrt(kxs * kxs + kys * kys);
+        }
+        cd kz = std::sqrt(beta * beta - kt * kt + cd(0));
+        Q = (kz == cd(0)) ? cd(0) : MU0 * med.permeability(ww) * ww * ww / (2.0 * kz);
+      }
+      Q = Q * c.zu / Eu;
+      QZ[(size_t)i * Nt + j] = C((T)Q.real(), (T)Q.imag());
+    }
   }
 
   // Responses (medium_responses/*.jl)
@@ -675,7 +692,47 @@ void State<T>::rhs_row(int i, const C* Es, C* dE, C* Et, C* F) const {
     rspec2aspec(F, Nt);
     for (int j = 0; j < Nt; j++) dE[j] += Rnl[q][j] * F[j];   // update_dE!
   }
-  for (int j = 0; j < Nt; j++) dE[j] = C(0, 1) * QZ[o + j] * dE[j];
+  if (qpar) for (int j = 0; j < Nt; j++) dE[j] = C(0, 1) * QZ[o + j] * dE[j];
+}
+
+// func!'s non-paraxial end (nonlinear_propagators.jl:58-63) on all rows of dE:
+// forward space transform, times i QZ(k, w), spectral filter, inverse.
+template <typename T>
+void State<T>::qz_space(C* dE) {
+  if (rt) dht_columns(dE, 0);
+  if (xyt) fft2_columns(dE, 0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < Nr; i++)
+    for (int j = 0; j < Nt; j++) {
+      const size_t q = (size_t)i * Nt + j;
+      dE[q] = C(0, 1) * QZ[q] * dE[q];
+      dE[q] = dE[q] * (Kg[q] * Wg[j]);      // apply_spectral_filter! (Guards.jl:232-261)
+    }
+  if (rt) dht_columns(dE, 1);
+  if (xyt) fft2_columns(dE, 1);
+}
+
+// ODEIntegrators.step! with RK3 over all rows at once, for the non-paraxial
+// right-hand side, which couples the rows through the space transform.
+template <typename T>
+void State<T>::nl_step_all(C* u, T dz) {
+  const size_t n = (size_t)Nr * Nt;
+  std::vector<C> k1(n), k2(n), k3(n), ut(n);
+  auto rhs = [&](const C* in, C* out) {
+#pragma omp parallel
+    {
+      std::vector<C> Et(Nt), F(Nt);
+#pragma omp for schedule(dynamic, 2)
+      for (int i = 0; i < Nr; i++) rhs_row(i, in + (size_t)i * Nt, out + (size_t)i * Nt, Et.data(), F.data());
+    }
+    qz_space(out);
+  };
+  rhs(u, k1.data());
+  for (size_t q = 0; q < n; q++) ut[q] = u[q] + dz * ((T)0.5 * k1[q]);
+  rhs(ut.data(), k2.data());
+  for (size_t q = 0; q < n; q++) ut[q] = u[q] + dz * (-k1[q] + (T)2 * k2[q]);
+  rhs(ut.data(), k3.data());
+  for (size_t q = 0; q < n; q++) u[q] = u[q] + dz * (k1[q] / (T)6 + (T)2 * k2[q] / (T)3 + k3[q] / (T)6);
 }
 
 // ODEIntegrators.step! with RK3 on one row (nonlinear_propagators.jl:70-74)
@@ -740,7 +797,9 @@ void State<T>::zstep(T z, T dz) {
   if (cfg.plasma) plasma_solve();
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < Nr; i++) time_fwd(&E[(size_t)i * Nt]);
-  if (cfg.nonlinearity) {
+  if (cfg.nonlinearity && !qpar) {
+    nl_step_all(E.data(), dz);
+  } else if (cfg.nonlinearity) {
 #pragma omp parallel
     {
       std::vector<C> scratch((size_t)6 * Nt);
@@ -1052,6 +1111,13 @@ int orc_nl_step(void* h, double* Es, double dz) {
     using Cx = typename std::decay_t<decltype(s)>::C;
     using R = typename Cx::value_type;
     const int Nt = s.Nt;
+    if (!s.qpar) {
+      std::vector<Cx> all((size_t)s.Nr * Nt);
+      for (size_t q = 0; q < all.size(); q++) all[q] = Cx((R)Es[2 * q], (R)Es[2 * q + 1]);
+      s.nl_step_all(all.data(), (R)dz);
+      for (size_t q = 0; q < all.size(); q++) { Es[2 * q] = all[q].real(); Es[2 * q + 1] = all[q].imag(); }
+      return;
+    }
     std::vector<Cx> row(Nt), scratch((size_t)6 * Nt);
     for (int i = 0; i < s.Nr; i++) {
       double* p = Es + 2 * (size_t)i * Nt;
@@ -1067,12 +1133,13 @@ int orc_nl_rhs(void* h, const double* Es, double* dE) {
     using Cx = typename std::decay_t<decltype(s)>::C;
     using R = typename Cx::value_type;
     int Nt = s.Nt;
-    std::vector<Cx> in(Nt), out(Nt), Et(Nt), F(Nt);
+    std::vector<Cx> in(Nt), Et(Nt), F(Nt), out((size_t)s.Nr * Nt);
     for (int i = 0; i < s.Nr; i++) {
       for (int j = 0; j < Nt; j++) in[j] = Cx((R)Es[2 * ((size_t)i * Nt + j)], (R)Es[2 * ((size_t)i * Nt + j) + 1]);
-      s.rhs_row(i, in.data(), out.data(), Et.data(), F.data());
-      for (int j = 0; j < Nt; j++) { dE[2 * ((size_t)i * Nt + j)] = out[j].real(); dE[2 * ((size_t)i * Nt + j) + 1] = out[j].imag(); }
+      s.rhs_row(i, in.data(), &out[(size_t)i * Nt], Et.data(), F.data());
     }
+    if (!s.qpar) s.qz_space(out.data());
+    for (size_t q = 0; q < out.size(); q++) { dE[2 * q] = out[q].real(); dE[2 * q + 1] = out[q].imag(); }
   });
 }
 

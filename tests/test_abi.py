@@ -32,8 +32,8 @@ def test_library_exports_every_declared_symbol(pkg):
 
 
 def test_struct_layouts_match_header(pkg):
-    # fil_desc: 19 int32 fields; fil_diag: 9 doubles; fil_diag_xyt: 8 doubles
-    assert ctypes.sizeof(pkg.lib.FilDesc) == 19 * 4
+    # fil_desc: 20 int32 fields; fil_diag: 9 doubles; fil_diag_xyt: 8 doubles
+    assert ctypes.sizeof(pkg.lib.FilDesc) == 20 * 4
     assert ctypes.sizeof(pkg.lib.FilDiag) == 9 * 8
     assert ctypes.sizeof(pkg.lib.FilDiagXYT) == 8 * 8
     text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)

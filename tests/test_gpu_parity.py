@@ -242,3 +242,47 @@ def test_thg_steps_match_oracle(pkg, orc, which, precision, tol, oracle_kz):
           f"3w0 band energy fraction {frac:.3e}, band spectrum error {e_band:.3e}")
     assert err < tol and effect > 100 * err and effect > 1e-3
     assert frac > 1e-12 and e_band < max(100 * tol, 1e-6)
+
+
+def narrow_rt(pkg, qparaxial, case):
+    """Beams narrow enough for the non-paraxial prefactor of the nonlinear term
+    to depart from the paraxial one: "kerr", a 2 um beam at 6000 Iu without
+    plasma (kt/beta up to ~0.2, a 3e-4 effect over 20 steps of 1 um); "plasma",
+    a 50 um beam at 60 Iu with the plasma equation (its rows carried from the
+    first stage to the other two)."""
+    if case == "kerr":
+        p = pkg.workloads.rt_air(64, 512, 0.08, rmax=0.02, tmin=-125.0, tmax=125.0, a0=2e-6, tau0=20e-15, dz=1e-6,
+                                 plasma=False)
+        p["rguard"] = 0.002
+    else:
+        p = pkg.workloads.rt_air(64, 512, 0.5, rmax=0.5, tmin=-125.0, tmax=125.0, a0=50e-6, tau0=20e-15, dz=1e-4)
+        p["rguard"] = 0.05
+    p["QPARAXIAL"] = qparaxial
+    return p
+
+
+@pytest.mark.parametrize("case", ["kerr", "plasma"])
+@pytest.mark.parametrize("precision,tol,oracle_kz", [(4, 1e-4, False), (8, 1e-10, True)])
+def test_nonparaxial_steps_match_oracle(pkg, orc, case, precision, tol, oracle_kz):
+    """QPARAXIAL = false (nonlinear_propagators.jl:58-63, 139-160, 232-242): the
+    device's three stage launches with the Hankel products of the stage
+    right-hand sides against the oracle's all-rows RK3; in the "kerr" case the
+    non-paraxial term matters (the paraxial run differs by 3e-4, far beyond the
+    error)."""
+    p = narrow_rt(pkg, False, case)
+    o = oracle_for(orc, p)
+    dz = p["dz_initial"]
+    rows = o.run(20, dz)
+    m = run_gpu(pkg, p, 20, dz, precision, KZ=o.KZ() if oracle_kz else None)
+    err = rel_l2(m.field(), o.field())
+    d = m.analyze(20 * dz)
+    m0 = run_gpu(pkg, narrow_rt(pkg, True, case), 20, dz, precision, KZ=o.KZ() if oracle_kz else None)
+    effect = rel_l2(m0.field(), o.field())
+    print(f"nonparaxial {case} fp{8 * precision}: rel_l2 {err:.3e} (paraxial {effect:.3e}), Imax {d['Imax']:.6e} "
+          f"oracle {rows[-1]['Imax']:.6e}, rhomax {d['rhomax']:.4e} oracle {rows[-1]['rhomax']:.4e}")
+    assert err < tol
+    assert abs(d["Imax"] / rows[-1]["Imax"] - 1) < 10 * tol
+    if case == "kerr":
+        assert effect > 1e-4 and effect > 10 * err
+    else:
+        assert rows[-1]["rhomax"] > 0 and abs(d["rhomax"] / rows[-1]["rhomax"] - 1) < 100 * tol

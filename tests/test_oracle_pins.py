@@ -230,3 +230,52 @@ def test_analyzer_gaussian_closed_forms_and_parseval(orc):
     assert abs(d["tau"] - tau0) <= dt
     assert abs(d["rfil"] - (r[0] + a)) <= np.max(g["dr"])
     assert abs(d["Imax"] - 1) < 2e-3 and d["rhomax"] == 0 and d["De"] == 0
+
+
+def _narrow_beam(orc, qparaxial, Nr=128, Nt=256, rmax=0.05, a=0.004, k=30, A=0.5):
+    """A monochromatic Gaussian beam exp(-r^2/(2a^2)) at t-bin k (grid units:
+    mm, fs) in a constant medium, narrow enough that kt^2/beta^2 ~ 1e-3."""
+    c = base_cfg(Nr=Nr, Nt=Nt, rmax=rmax, tmin=-100.0, tmax=100.0, nonlinearity=1, cubic=2, medium=1, n_const=1.0,
+                 rguard=0.0, tguard=0.0, kguard=90.0, qparaxial=qparaxial)
+    o = orc.Oracle(c, np.zeros((Nr, Nt)))
+    g = o.grid()
+    Es = np.zeros((Nr, Nt), dtype=complex)
+    Es[:, k] = A * np.exp(-g["r"] ** 2 / (2 * a * a))
+    return o, g, Es, k
+
+
+def test_nonparaxial_rhs_is_qz_ratio_in_hankel_space(orc):
+    """func! with QPARAXIAL = false (nonlinear_propagators.jl:58-63, Qfunc_nonparaxial
+    :232-242): in Hankel space the right-hand side is the paraxial one times
+    beta / kz(k), kz = sqrt(beta^2 - k^2), times the spectral filter Kguard
+    (kguard = 90 deg: exp(-((k/beta)^2)^20)) and Wguard.  A narrow beam makes
+    beta/kz - 1 reach ~1e-3 over its spectrum, and a transform taken the wrong
+    way round or the paraxial factor kept fails by far more than the 1e-9 gate."""
+    op, g, Es, k = _narrow_beam(orc, 1)
+    onp, _, _, _ = _narrow_beam(orc, 0)
+    rp, rn = op.nl_rhs(Es)[:, k], onp.nl_rhs(Es)[:, k]
+    Hp, Hn = op.dht_apply(rp), op.dht_apply(rn)
+    w = g["w"][k] * 1e15
+    beta = w / C0
+    kt = g["k"] * 1e3
+    kz = np.sqrt(beta ** 2 - kt ** 2 + 0j)
+    Kg = np.exp(-((kt / beta) ** 2) ** 20)
+    Wg = np.exp(-((w ** 2 / 1e16 ** 2) ** 20))
+    expect = Hp * beta / kz * Kg * Wg
+    sup = np.abs(Hp) > 1e-3 * np.abs(Hp).max()
+    assert np.max(np.abs(beta / kz[sup] - 1)) > 5e-4           # the non-paraxial factor matters here
+    assert np.max(np.abs(Hn - expect)) < 1e-9 * np.max(np.abs(Hp))
+    assert np.max(np.abs(onp.nl_rhs(Es)[:, np.arange(256) != k])) < 1e-12 * np.max(np.abs(rn))
+
+
+def test_nonparaxial_step_tends_to_paraxial_step_for_wide_beams(orc):
+    """The all-rows RK3 of the non-paraxial term (the same Kutta combination)
+    on a wide beam, where beta / kz - 1 < 1e-8: it reproduces the paraxial row-wise
+    step to that order."""
+    op, g, Es, k = _narrow_beam(orc, 1, rmax=10.0, a=1.5)
+    onp, _, _, _ = _narrow_beam(orc, 0, rmax=10.0, a=1.5)
+    w = g["w"][k] * 1e15
+    dz = 0.3 / ((w / C0) * 1e-23 * 0.25 * 1e16)
+    sp, sn = op.nl_step(Es, dz), onp.nl_step(Es, dz)
+    d = np.max(np.abs(sn - sp)) / np.max(np.abs(sp - Es))
+    assert d < 1e-6 and np.max(np.abs(sp - Es)) > 1e-3 * np.max(np.abs(Es))
