@@ -60,8 +60,10 @@ __global__ __launch_bounds__(NTH) void np_stage_kernel(RowArgs<R> a, const cx<R>
     }
     store_padded<R, N, NTH>(Z, S0);
     fft_lds<R, N, NTH, -1>(Z, a.tw);
+#pragma unroll
+    for (int u = 0; u < TP; u++) e[u] = Z[lds_index(tid + u * NTH)];
     if (a.plasma) {
-      plasma_row<R, A, N, NTH>(Z, rho, kd, a.pp, a.dt, red);
+      plasma_row<R, A, N, NTH, true>(Z, rho, kd, a.pp, a.dt, red);
       R mx = 0;
       for (int t = tid; t < N; t += NTH) mx = max(mx, rho[rk_index<R>(t)]);
       mx = block_max<R>(mx, reinterpret_cast<R*>(red));
@@ -78,9 +80,6 @@ __global__ __launch_bounds__(NTH) void np_stage_kernel(RowArgs<R> a, const cx<R>
       for (int t = tid; t < N; t += NTH) { rho[rk_index<R>(t)] = 0; kd[rk_index<R>(t)] = 0; }
       __syncthreads();
     }
-#pragma unroll
-    for (int u = 0; u < TP; u++) e[u] = Z[lds_index(tid + u * NTH)];
-    __syncthreads();
   } else {
     cx<R> Us[SP];
 #pragma unroll

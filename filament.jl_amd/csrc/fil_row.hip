@@ -109,8 +109,12 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
     fft_lds<R, N, NTH, -1>(Z, a.tw);
   }
 
+  // stage 1 of the RK3 starts from this field; the plasma scan then takes Z
+  // as its cache (its barriers order these reads before its writes)
+#pragma unroll
+  for (int u = 0; u < TP; u++) e[u] = Z[lds_index(tid + u * NTH)];
   if (a.plasma) {
-    plasma_row<R, A, N, NTH>(Z, rho, kd, a.pp, a.dt, red);
+    plasma_row<R, A, N, NTH, true>(Z, rho, kd, a.pp, a.dt, red);
     R mx = 0;
     for (int t = tid; t < N; t += NTH) mx = max(mx, rho[rk_index<R>(t)]);
     mx = block_max<R>(mx, reinterpret_cast<R*>(red));
@@ -137,10 +141,6 @@ __global__ __launch_bounds__(NTH) void row_kernel(RowArgs<R> a) {
   // RK3 (Kutta) in z: k1 = f(S0); k2 = f(S0 + dz/2 k1); k3 = f(S0 - dz k1 + 2 dz k2)
   const R dz = a.dz_dev ? (R)*a.dz_dev : a.dz;
   cx<R> k1[SP], k2[SP], k[SP], U[SP];
-  // stage 1 starts from the field the plasma scan read (Z)
-#pragma unroll
-  for (int u = 0; u < TP; u++) e[u] = Z[lds_index(tid + u * NTH)];
-  if constexpr (REGS) __syncthreads();
   stage_rhs<R, N, NTH, RAMAN>(Z, e, rho, kd, a.tw, a.coef, a.hr, a.sc, g_r, a.Tg, a.responses, a.losses_ereal, k1);
 #pragma unroll
   for (int m = 0; m < SP; m++) U[m] = S0[m] + ((R)0.5 * dz) * k1[m];

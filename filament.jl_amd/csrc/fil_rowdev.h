@@ -128,7 +128,9 @@ template <typename R> __host__ __device__ constexpr int rk_size(int n) { return 
 
 // ---------------------------------------------------------------------------
 // Plasma density of one row from the time-domain field Z (LDS).
-// Writes rho[rk_index(t)], kd[rk_index(t)] (LDS).  Without avalanche every component obeys the
+// Writes rho[rk_index(t)], kd[rk_index(t)] (LDS).  CLOBBER_Z: Z may serve as
+// the scan's cache and holds no field on return (the caller has read what it
+// needs of it beforehand).  Without avalanche every component obeys the
 // linear ODE d rho/dt = W(t) (f - rho); one RK3 step of it (the reference's
 // ODEIntegrators.RK3, stages at t_j, t_j + dt/2, t_j + dt with the field
 // linearly interpolated) is exactly the affine map
@@ -138,8 +140,8 @@ template <typename R> __host__ __device__ constexpr int rk_size(int n) { return 
 // carried in A (FP64 for the fp64-scan form).  With avalanche the RHS is
 // quadratic and one lane walks the row sequentially.
 // ---------------------------------------------------------------------------
-template <typename R, typename A, int N, int NTH>
-__device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>& pp, R dt, A* red) {
+template <typename R, typename A, int N, int NTH, bool CLOBBER_Z = false>
+__device__ void plasma_row(cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>& pp, R dt, A* red) {
   const int tid = threadIdx.x;
   const int nc = pp.ncomp;
   if (pp.avalanche) {
@@ -205,45 +207,61 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
   constexpr int C = N / NTH;   // contiguous chunk of t per thread
   const int j0 = tid * C;
   const A dtA = (A)dt;
-  // One component: pass 1 leaves the gain h_j of every sample (rounded to R,
-  // and used rounded by both passes, so the scan stays consistent) in rho[j+1]
-  // and K W1(t_j) in kd[j], the slots pass 2 overwrites with rho and kdrho
-  // (each thread owns the slots of its chunk in both passes); pass 2 then
-  // evaluates no rates.
-  const bool cache = (nc == 1);
+  // Pass 1 evaluates the rates of every sample and leaves what pass 2 needs,
+  // so that pass 2 evaluates none:
+  //   one component:  the gain h_j (rounded to R, and used rounded by both
+  //                   passes, so the scan stays consistent) in rho[j+1] and
+  //                   K W1(t_j) in kd[j];
+  //   two components (CLOBBER_Z): the gains of both (rounded to R) in Z[j],
+  //                   K W1(t_j) of component 0 in rho[j+1], of component 1 in
+  //                   kd[j];
+  // the slots pass 2 overwrites with rho and kdrho.  Each thread owns the
+  // slots of its chunk in both passes; Z[j0], which the previous thread reads
+  // as its last E1, is written only after the scan's barriers.  Otherwise
+  // pass 2 evaluates the rates again.
+  const bool cache = (nc == 1), cache2 = CLOBBER_Z && (nc == 2);
   // pass 1: this thread's composed affine map for each component
   Aff<A> m[FIL_MAXCOMP];
 #pragma unroll
   for (int q = 0; q < FIL_MAXCOMP; q++) { m[q].a = (A)1; m[q].b = (A)0; }
-  R Wprev[FIL_MAXCOMP];
-  R Pprev = 0;
+  R Wprev[FIL_MAXCOMP], Pprev[FIL_MAXCOMP];
   bool have_prev = false;
+  const bool slope = (cache || cache2) && pp.kdep;
+  cx<R> E1c = mk((R)0, (R)0), hz = mk((R)0, (R)0);
   for (int c = 0; c < C; c++) {
     const int j = j0 + c;
     if (j >= N - 1) break;
-    cx<R> E0 = Z[lds_index(j)], E1 = Z[lds_index(j + 1)];
+    cx<R> E0 = have_prev ? E1c : Z[lds_index(j)], E1 = Z[lds_index(j + 1)];
+    E1c = E1;
     cx<R> Em = (R)0.5 * (E0 + E1);
     R I0 = fiarg(E0, pp.ereal), Im = fiarg(Em, pp.ereal), I1 = fiarg(E1, pp.ereal);
-    const bool slope = cache && pp.kdep;
+    R hq[2] = {0, 0};
 #pragma unroll
     for (int q = 0; q < FIL_MAXCOMP; q++) {
       if (q >= nc) break;
-      R pw1 = Pprev, pw3 = 0;
+      R pw1 = have_prev ? Pprev[q] : (R)0, pw3 = 0;
       R W1 = have_prev ? Wprev[q] : tf_eval<R>(pp, q, I0, slope ? &pw1 : nullptr);
       R W2 = tf_eval<R>(pp, q, Im, nullptr);
       R W3 = tf_eval<R>(pp, q, I1, slope ? &pw3 : nullptr);
       Wprev[q] = W3;
-      Pprev = pw3;
+      Pprev[q] = pw3;
       A w1 = W1, w2 = W2, w3 = W3;
       A h = rk3_gain<A>(dtA, w1, w2, w3);
-      if (cache) {
+      if (cache || cache2) {
         const R hr = (R)h;
         h = (A)hr;
-        rho[rk_index<R>(j + 1)] = hr;
-        kd[rk_index<R>(j)] = (pp.kdep ? pw1 : (R)pp.K[0]) * W1;
+        const R kw1 = (pp.kdep ? pw1 : (R)pp.K[q]) * W1;
+        if (q == 0) rho[rk_index<R>(j + 1)] = cache ? hr : kw1;
+        if (q == 0 && cache) kd[rk_index<R>(j)] = kw1;
+        if (q == 1) kd[rk_index<R>(j)] = kw1;
+        if (q < 2) hq[q] = hr;
       }
       Aff<A> st; st.a = (A)1 - h; st.b = h * (A)pp.frhont[q];
       m[q] = compose(st, m[q]);
+    }
+    if (cache2) {
+      if (c == 0) hz = mk(hq[0], hq[1]);
+      else Z[lds_index(j)] = mk(hq[0], hq[1]);
     }
     have_prev = true;
   }
@@ -271,6 +289,25 @@ __device__ void plasma_row(const cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>&
       r0 = ((A)1 - h) * r0 + h * f;
       rho[rk_index<R>(j + 1)] = (R)r0;
       kd[rk_index<R>(j)] = kw1 * (R)(f - r0);
+    }
+    if (tid == NTH - 1) kd[rk_index<R>(N - 1)] = 0;
+    __syncthreads();
+    return;
+  }
+  if (cache2) {
+    // the scan's barriers have ordered every pass-1 read of Z[j0]
+    if (j0 < N - 1) Z[lds_index(j0)] = hz;
+    const A f0 = (A)pp.frhont[0], f1 = (A)pp.frhont[1];
+    A r0 = r[0], r1 = r[1];
+    for (int c = 0; c < C; c++) {
+      const int j = j0 + c;
+      if (j >= N - 1) break;
+      const cx<R> h = Z[lds_index(j)];
+      const R kw0 = rho[rk_index<R>(j + 1)], kw1 = kd[rk_index<R>(j)];
+      r0 = ((A)1 - (A)h.x) * r0 + (A)h.x * f0;
+      r1 = ((A)1 - (A)h.y) * r1 + (A)h.y * f1;
+      rho[rk_index<R>(j + 1)] = (R)(r0 + r1);
+      kd[rk_index<R>(j)] = kw0 * (R)(f0 - r0) + kw1 * (R)(f1 - r1);
     }
     if (tid == NTH - 1) kd[rk_index<R>(N - 1)] = 0;
     __syncthreads();
