@@ -20,50 +20,60 @@
 #include "fil_fft.h"
 #include "fil_internal.h"
 #include "fil_kernels.h"
+#include "fil_host.h"
 
 using namespace fil;
 
-namespace {
+namespace fil {
 
-thread_local std::string g_err;
+static thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) {
-      if (p) (void)hipFree(p);
-      p = o.p; n = o.n;
-      o.p = nullptr; o.n = 0;
+// Per-pass twiddles of an N-point transform (fil_fft.h layout) appended to
+// tw starting at entry o; returns the next entry.
+size_t append_twiddles(HostArr& tw, size_t o, int N) {
+  for (int s = 1; N / s > 1; s *= fft_radix(N, s)) {
+    const int rad = fft_radix(N, s);
+    if (s < N / rad) {
+      const int P = N / (s * rad);
+      for (int p = 0; p < P; p++)
+        for (int k = 0; k < FIL_TW_STRIDE; k++) {
+          const double a = -2.0 * kPi * (double)(((long)p * s * k) % N) / N;
+          if (k < rad) tw.setc(o, std::cos(a), std::sin(a));
+          o++;
+        }
     }
-    return *this;
   }
-  ~DevBuf() { if (p) (void)hipFree(p); }
-  hipError_t alloc(size_t bytes) {
-    if (p) { (void)hipFree(p); p = nullptr; }
-    n = bytes;
-    if (!bytes) return hipSuccess;
-    return hipMalloc(&p, bytes);
+  return o;
+}
+
+// ext: also the half-length table and the real-signal post-processing factors
+// the row kernel uses (fft_tw_ext_size layout).
+int build_twiddles(DevBuf& buf, int N, bool f64, bool ext) {
+  const int n = ext ? fft_tw_ext_size(N) : fft_tw_size(N);
+  HostArr tw(std::max(1, n), f64, true);
+  size_t o = append_twiddles(tw, 0, N);
+  if ((int)o != fft_tw_size(N)) return fail(FIL_EINVAL, "internal: twiddle table size");
+  if (ext) {
+    o = append_twiddles(tw, o, N / 2);
+    if ((int)o != fft_tw_post_offset(N)) return fail(FIL_EINVAL, "internal: twiddle table size");
+    for (int w = 0; w < N / 2; w++) {
+      const double a = 2.0 * kPi * (double)w / N;
+      tw.setc(o++, std::cos(a), std::sin(a));
+    }
   }
-  template <typename T> T* as() const { return static_cast<T*>(p); }
-};
+  HIPCHK(upload(buf, tw.b.data(), tw.b.size()));
+  return 0;
+}
 
-#define HIPCHK(expr)                                                                    \
-  do {                                                                                  \
-    hipError_t e_ = (expr);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail(FIL_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
-  } while (0)
+}  // namespace fil
 
-constexpr double kPi = 3.14159265358979323846;
+namespace {
+
 
 double radius(const double* x, const double* y, int n) {
   double ymax = -INFINITY;
@@ -200,20 +210,6 @@ __global__ void convert_state_kernel(const cx<S>* __restrict__ src, cx<D>* __res
 template <typename R>
 void put_real(std::vector<unsigned char>& buf, size_t idx, double v) { reinterpret_cast<R*>(buf.data())[idx] = (R)v; }
 
-// host array of the working type
-struct HostArr {
-  std::vector<unsigned char> b;
-  bool fp64;
-  HostArr(size_t n, bool f64, bool complex) : b(n * (f64 ? 8 : 4) * (complex ? 2 : 1), 0), fp64(f64) {}
-  void set(size_t i, double v) { if (fp64) reinterpret_cast<double*>(b.data())[i] = v; else reinterpret_cast<float*>(b.data())[i] = (float)v; }
-  void setc(size_t i, double re, double im) { set(2 * i, re); set(2 * i + 1, im); }
-};
-
-hipError_t upload(DevBuf& d, const void* src, size_t bytes) {
-  hipError_t e = d.alloc(bytes);
-  if (e != hipSuccess) return e;
-  return hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice);
-}
 
 // round to bf16 (nearest even), returning the bits and the rounded value
 uint16_t to_bf16(float f, float* back) {
@@ -962,43 +958,6 @@ int fil_create(const fil_desc* d, fil_handle** out) {
 
 namespace {
 
-// per-pass twiddle table of an N-point LDS FFT in the layout fft_lds reads (fil_fft.h)
-// Per-pass twiddles of an N-point transform (fil_fft.h layout) appended to
-// tw starting at entry o; returns the next entry.
-size_t append_twiddles(HostArr& tw, size_t o, int N) {
-  for (int s = 1; N / s > 1; s *= fft_radix(N, s)) {
-    const int rad = fft_radix(N, s);
-    if (s < N / rad) {
-      const int P = N / (s * rad);
-      for (int p = 0; p < P; p++)
-        for (int k = 0; k < FIL_TW_STRIDE; k++) {
-          const double a = -2.0 * kPi * (double)(((long)p * s * k) % N) / N;
-          if (k < rad) tw.setc(o, std::cos(a), std::sin(a));
-          o++;
-        }
-    }
-  }
-  return o;
-}
-
-// ext: also the half-length table and the real-signal post-processing factors
-// the row kernel uses (fft_tw_ext_size layout).
-int build_twiddles(DevBuf& buf, int N, bool f64, bool ext) {
-  const int n = ext ? fft_tw_ext_size(N) : fft_tw_size(N);
-  HostArr tw(std::max(1, n), f64, true);
-  size_t o = append_twiddles(tw, 0, N);
-  if ((int)o != fft_tw_size(N)) return fail(FIL_EINVAL, "internal: twiddle table size");
-  if (ext) {
-    o = append_twiddles(tw, o, N / 2);
-    if ((int)o != fft_tw_post_offset(N)) return fail(FIL_EINVAL, "internal: twiddle table size");
-    for (int w = 0; w < N / 2; w++) {
-      const double a = 2.0 * kPi * (double)w / N;
-      tw.setc(o++, std::cos(a), std::sin(a));
-    }
-  }
-  HIPCHK(upload(buf, tw.b.data(), tw.b.size()));
-  return 0;
-}
 
 int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt) {
   const int Nr = h->Nr, Nrp = h->Nrp, Nt = h->Nt, Nh = h->Nh;

@@ -288,7 +288,11 @@ __global__ __launch_bounds__(64 * NWM * NWW * KS) void dht_gemm_bf16x3_kernel(co
                                                               const uint16_t* __restrict__ Mlo,
                                                               const void* __restrict__ In_, void* __restrict__ Out_,
                                                               const cx<float>* __restrict__ P, int Nr, int Nh, int Pw) {
-  constexpr int SK = BK + 8;              // padded row: (BK + 8) bf16
+  // padded row of (BK + 16) bf16: a 160-byte row stride keeps both the
+  // staging stores (8 lanes per 128-byte row) and the operand reads (16 rows x
+  // 4 k-chunks per instruction) free of LDS bank conflicts on gfx950; a
+  // 144-byte stride doubles the operand reads
+  constexpr int SK = BK + 16;
   constexpr int NT = 64 * NWM * NWW * KS;
   constexpr int WM = BM / NWM, WW = BW / NWW;
   constexpr int MT = WM / 16, WT = WW / 16;

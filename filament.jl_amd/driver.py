@@ -1,12 +1,14 @@
-"""Filament.prepare / Filament.run for the RT, XYT and T geometries, host side.
+"""Filament.prepare / Filament.run for the RT, XYT, T, R and XY geometries, host side.
 
 prepare(fname) follows src/Input.jl:20-200: the input file is executed (it
 is Python here, with the reference's variable names), the process changes
 directory to the input file's directory, then the initial-condition and
 medium files are executed into the same namespace; the geometry's branch
-(RT :106-130, T :81-105, XYT :151-178) picks the keys the run needs.
+(R :37-56, T :81-105, RT :106-130, XY :131-150, XYT :151-178) picks the keys
+the run needs.
 run(input) follows src/Filament.jl:51-184: build the model, analyze, write
-plot.dat (WritePlots.jl:92-116 T, :121-152 RT, :187-215 XYT) and loop
+plot.dat (WritePlots.jl:65-87 R, :92-116 T, :121-152 RT, :158-182 XY,
+:187-215 XYT) and loop
 z-steps with the adaptive step and the Istop check.  The z-step and the
 analyzer run in libfilament_hip.so.
 """
@@ -19,13 +21,14 @@ import numpy as np
 from . import lib as L
 from . import physics as ph
 from . import responses as R
+from .responses import alg_order
 from .plotfield import PlotField
-from .solver import RTModel, TModel, XYTModel
+from .solver import RModel, RTModel, TModel, XYModel, XYTModel
 
 
 def _namespace():
     ns = dict(C0=ph.C0, EPS0=ph.EPS0, MU0=ph.MU0, QE=ph.QE, ME=ph.ME, HBAR=ph.HBAR, pi=math.pi,
-              math=math, np=np, RK3=R.RK3, Inf=math.inf,
+              math=math, np=np, RK3=R.RK3, RK4=R.RK4, Inf=math.inf,
               init_cubic=R.init_cubic, init_current_free=R.init_current_free,
               init_current_losses=R.init_current_losses, init_raman=R.init_raman,
               init_photoionization_avalanche=R.init_photoionization_avalanche,
@@ -45,7 +48,14 @@ class _FFTWShim:
 _COMMON_KEYS = ("prefix", "zu", "tu", "Iu", "rhou", "tmin", "tmax", "Nt", "zmax", "dz_initial", "NONLINEARITY",
                 "PLASMA", "tguard", "wguard", "dzphimax", "Istop", "z", "lam0", "permittivity", "permeability", "n2",
                 "initial_condition")
+# R and XY (Input.jl:37-56, 131-150): no t grid, no plasma
+_SPATIAL_COMMON_KEYS = ("prefix", "zu", "Iu", "zmax", "dz_initial", "NONLINEARITY", "dzphimax", "Istop", "z", "lam0",
+                        "permittivity", "permeability", "n2", "initial_condition")
+_SPATIAL = ("R", "XY")
 _GEOMETRY_KEYS = {
+    "R": ("ru", "rmax", "Nr", "KPARAXIAL", "QPARAXIAL", "rguard", "kguard"),
+    "XY": ("xu", "yu", "xmin", "xmax", "Nx", "ymin", "ymax", "Ny", "KPARAXIAL", "QPARAXIAL", "xguard", "yguard",
+           "kxguard", "kyguard"),
     "RT": ("ru", "rmax", "Nr", "KPARAXIAL", "QPARAXIAL", "rguard", "kguard"),
     "XYT": ("xu", "yu", "xmin", "xmax", "Nx", "ymin", "ymax", "Ny", "KPARAXIAL", "QPARAXIAL", "xguard", "yguard",
             "kxguard", "kyguard"),
@@ -64,12 +74,18 @@ def prepare(fname):
         exec(compile(open(path).read(), path, "exec"), ns)
     geometry = ns["geometry"]
     if geometry not in _GEOMETRY_KEYS:
-        raise ValueError(f"geometry {geometry!r} is not implemented (RT, XYT, T)")
-    missing = [k for k in _COMMON_KEYS + _GEOMETRY_KEYS[geometry] if k not in ns]
+        raise ValueError(f"geometry {geometry!r} is not implemented (R, T, RT, XY, XYT)")
+    keys = (_SPATIAL_COMMON_KEYS if geometry in _SPATIAL else _COMMON_KEYS) + _GEOMETRY_KEYS[geometry]
+    missing = [k for k in keys if k not in ns]
     if missing:
         raise ValueError(f"input for geometry {geometry} lacks {', '.join(missing)}")
-    p = {k: ns[k] for k in _COMMON_KEYS + _GEOMETRY_KEYS[geometry]}
+    p = {k: ns[k] for k in keys}
     p["geometry"] = geometry
+    p["ALG"] = ns.get("ALG")
+    if geometry in _SPATIAL:
+        p["PLASMA"] = False          # Input.jl:51, :145
+        if ns.get("HTLOAD", False):
+            raise NotImplementedError("HTLOAD: the Hankel plan is computed, not loaded from a JLD2 file")
     if geometry == "T":
         # Input.jl:96-103: the T model is paraxial in both terms
         p["KPARAXIAL"] = p["QPARAXIAL"] = True
@@ -88,7 +104,16 @@ def prepare(fname):
 # (WritePlots.jl:92-116 T, :121-152 RT, :187-215 XYT)
 def _plotvars(p):
     g = p.get("geometry", "RT")
-    zu, tu, Iu, rhou = p["zu"], p["tu"], p["Iu"], p["rhou"]
+    zu, Iu = p["zu"], p["Iu"]
+    if g == "R":
+        ru = p["ru"]
+        return [("z", "m", zu, "z"), ("Imax", "W/m^2", Iu, "Imax"), ("rfil", "m", ru, "rfil"),
+                ("P", "W", ru ** 2 * Iu, "P")]
+    if g == "XY":
+        xu, yu = p["xu"], p["yu"]
+        return [("z", "m", zu, "z"), ("Imax", "W/m^2", Iu, "Imax"), ("ax", "m", xu, "ax"), ("ay", "m", yu, "ay"),
+                ("P", "W", xu * yu * Iu, "P")]
+    tu, rhou = p["tu"], p["rhou"]
     if g == "T":
         return [("z", "m", zu, "z"), ("Imax", "W/m^2", Iu, "Imax"), ("Nemax", "1/m^3", rhou, "rhomax"),
                 ("duration", "s", tu, "duration"), ("F", "J/m^2", tu * Iu, "F")]
@@ -124,6 +149,13 @@ def write_plotdat_row(fname, d, p=None):
 def make_model(p, precision=L.FP32, form=0, device=0):
     """The geometry's model (Filament.jl:69-76)."""
     g = p.get("geometry", "RT")
+    if g in _SPATIAL:
+        if form:
+            raise ValueError("the chosen forms apply to the RT geometry")
+        cls = RModel if g == "R" else XYModel
+        return cls(p, precision=precision, device=device, basedir=p.get("basedir", "."))
+    if alg_order(p.get("ALG")) != 3:
+        raise NotImplementedError(f"geometry {g} integrates RK3 only")
     kw = dict(precision=precision, form=form, device=device, basedir=p.get("basedir", "."))
     if g == "RT":
         return RTModel(p, **kw)
@@ -150,7 +182,7 @@ def run(p, precision=L.FP32, form=0, device=0, verbose=True, plotfield=True):
     # which fixes the number of steps
     ft = np.float64 if geometry == "T" else np.float32
     z = ft(p["z"])
-    arrays = plotfield and geometry in ("RT", "XYT")
+    arrays = plotfield and geometry in ("RT", "XYT", "XY")
     d = model.analyze(float(z), arrays=arrays)
     fplot = os.path.join(prefix_dir, prefix_name + "plot.dat")
     write_plotdat_header(fplot, p)
@@ -166,9 +198,11 @@ def run(p, precision=L.FP32, form=0, device=0, verbose=True, plotfield=True):
     t0 = time.time()
     try:
         while z < zmax:
-            if verbose:
+            if verbose and geometry in _SPATIAL:
+                print(f"z={_fmt(z)}[zu] I={_fmt(d['Imax'])}[Iu]")
+            elif verbose:
                 print(f"z={_fmt(z)}[zu] I={_fmt(d['Imax'])}[Iu] rho={_fmt(d['rhomax'])}[rhou]")
-            dz = ft(model.dz_adaptive(p["dz_initial"], p["dzphimax"], d["Imax"], d["rhomax"]))
+            dz = ft(model.dz_adaptive(p["dz_initial"], p["dzphimax"], d["Imax"], d.get("rhomax", 0.0)))
             z = ft(z + dz)
             model.zstep(float(z), float(dz))
             d = model.analyze(float(z), arrays=arrays)

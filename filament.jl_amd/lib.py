@@ -12,9 +12,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libfilament_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAXCOMP = 4
-GEOM_RT, GEOM_XYT, GEOM_T = 0, 1, 2
+GEOM_RT, GEOM_XYT, GEOM_T, GEOM_R, GEOM_XY = 0, 1, 2, 3, 4
+ALG_RK3, ALG_RK4 = 3, 4
 FP32, FP64 = 4, 8
 FORM_DHT_BF16X3, FORM_SCAN_FP64 = 1, 2
 RESP_CUBIC_THG, RESP_CUBIC_NOTHG, RESP_FREE, RESP_LOSSES = 1, 2, 4, 8
@@ -72,6 +73,27 @@ class FilCorrDesc(ctypes.Structure):
                [(n, ctypes.c_double) for n in ("w0", "tmin", "dt", "a_scale", "rho_scale")]
 
 
+class FilSpatialDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "abi_version", "geometry", "Nr", "Nx", "Ny", "precision", "nonlinearity", "nonparaxial", "alg", "device")]
+
+
+class FilSpatialTables(ctypes.Structure):
+    _fields_ = [("KZ", _dp), ("Kguard", _dp), ("KXguard", _dp), ("KYguard", _dp),
+                ("Rguard", _dp), ("Xguard", _dp), ("Yguard", _dp),
+                ("dht_T", _dp), ("dht_RJ", _dp), ("dht_JV", _dp), ("r", _dp), ("dr", _dp),
+                ("x", _dp), ("y", _dp), ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("Rnl", ctypes.c_double), ("QZ", _dp)]
+
+
+class FilDiagSpatial(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("z", "Imax", "rfil", "ax", "ay", "P")] + \
+               [("imax", ctypes.c_int64), ("jmax", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 CORR_NPARAM = 20258
 
 STOP_ZMAX, STOP_ISTOP, STOP_MAXSTEPS, STOP_NONFINITE = 1, 2, 3, 4
@@ -90,7 +112,9 @@ EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_set_tables_xyt
            "fil_set_tiling", "fil_get_tiling", "fil_run_adaptive", "fil_run_adaptive_noalloc",
            "fil_corrector_init", "fil_corrector_params", "fil_corrector_scale", "fil_corrector_features",
            "fil_corrector_store_pair", "fil_corrector_train", "fil_corrector_eval", "fil_corrector_forward",
-           "fil_corrector_apply")
+           "fil_corrector_apply", "fil_spatial_create", "fil_spatial_set_tables", "fil_spatial_upload_field",
+           "fil_spatial_download_field", "fil_spatial_zstep", "fil_spatial_analyze", "fil_spatial_synchronize",
+           "fil_spatial_destroy")
 
 _lib = None
 
@@ -149,6 +173,14 @@ def load():
     L.fil_corrector_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _fp]
     L.fil_corrector_forward.argtypes = [ctypes.c_void_p, _fp, _fp]
     L.fil_corrector_apply.argtypes = [ctypes.c_void_p]
+    L.fil_spatial_create.argtypes = [ctypes.POINTER(FilSpatialDesc), ctypes.POINTER(ctypes.c_void_p)]
+    L.fil_spatial_set_tables.argtypes = [ctypes.c_void_p, ctypes.POINTER(FilSpatialTables)]
+    L.fil_spatial_upload_field.argtypes = [ctypes.c_void_p, _dp]
+    L.fil_spatial_download_field.argtypes = [ctypes.c_void_p, _dp]
+    L.fil_spatial_zstep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+    L.fil_spatial_analyze.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(FilDiagSpatial), _dp, _dp, _dp]
+    L.fil_spatial_synchronize.argtypes = [ctypes.c_void_p]
+    L.fil_spatial_destroy.argtypes = [ctypes.c_void_p]
     for name in EXPORTS:
         if name != "fil_last_error":
             getattr(L, name).restype = ctypes.c_int

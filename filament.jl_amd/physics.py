@@ -41,6 +41,18 @@ class UnitT:
         self.z, self.t, self.w, self.lam, self.I, self.rho = zu, tu, 1.0 / tu, tu, Iu, rhou
 
 
+class UnitR:
+    """Units.jl:28-41 (R): r, k, z and intensity units."""
+    def __init__(self, ru, zu, Iu):
+        self.r, self.k, self.z, self.I = ru, 1.0 / ru, zu, Iu
+
+
+class UnitXY:
+    """Units.jl:90-104 (XY): x, y, kx, ky, z and intensity units."""
+    def __init__(self, xu, yu, zu, Iu):
+        self.x, self.y, self.kx, self.ky, self.z, self.I = xu, yu, 1.0 / xu, 1.0 / yu, zu, Iu
+
+
 def field_unit(unit, n0):
     """Units.E (Units.jl:137-141)."""
     return math.sqrt(unit.I / (n0 * EPS0 * C0 / 2))
@@ -71,6 +83,31 @@ class GridRT:
         self.t = np.linspace(tmin, tmax, Nt)
         self.dt = (tmax - tmin) / (Nt - 1)
         self.w = 2 * np.pi * np.fft.fftfreq(Nt, self.dt)
+
+
+class GridR:
+    """Grids.jl:33-45 (R): the axial grid of _grid_spatial_axial (:204-216),
+    the same r, dr and k as the RT grid's."""
+    def __init__(self, rmax, Nr):
+        j = sps.jn_zeros(0, Nr + 1)
+        self.jzeros = j
+        self.rmax, self.Nr = rmax, Nr
+        self.r = j[:Nr] * rmax / j[Nr]
+        self.k = j[:Nr] / rmax
+        self.dr = _step(self.r)
+
+
+class GridXY:
+    """Grids.jl:127-155 (XY): uniform x and y (_grid_spatial_rectangular),
+    kx = 2 pi fftfreq(Nx, 1/dx)."""
+    def __init__(self, xmin, xmax, Nx, ymin, ymax, Ny):
+        self.Nx, self.Ny = Nx, Ny
+        self.x = np.linspace(xmin, xmax, Nx)
+        self.dx = (xmax - xmin) / (Nx - 1)
+        self.kx = 2 * np.pi * np.fft.fftfreq(Nx, self.dx)
+        self.y = np.linspace(ymin, ymax, Ny)
+        self.dy = (ymax - ymin) / (Ny - 1)
+        self.ky = 2 * np.pi * np.fft.fftfreq(Ny, self.dy)
 
 
 class GridT:
@@ -221,6 +258,66 @@ class GuardXYT:
         self.KXguard = np.ones(grid.Nx)
         self.KYguard = np.ones(grid.Ny)
         self.Wguard = np.exp(-(((grid.w * unit.w) ** 2 / wguard ** 2) ** 20))
+
+
+class GuardR:
+    """Guards.jl:41-57 (R): Rguard = guard_window_right(r, rguard) and
+    Kguard = exp(-((k ku)^2 / kmax^2)^20), kmax = k0 sind(kguard)."""
+    def __init__(self, unit, grid, medium, w0, rguard, kguard):
+        self.Rguard = guard_window_right(grid.r, rguard)
+        kmax = float(medium.k(w0)) * math.sin(math.radians(kguard))
+        self.Kguard = np.exp(-(((grid.k * unit.k) ** 2 / kmax ** 2) ** 20))
+
+
+class GuardXY:
+    """Guards.jl:132-157 (XY): field windows X(x), Y(y) (guard_window_both)
+    and the spectral KX(kx), KY(ky) with kmax = k0 sind(kxguard), sind(kyguard)."""
+    def __init__(self, unit, grid, medium, w0, xguard, yguard, kxguard, kyguard):
+        self.Xguard = guard_window_both(grid.x, xguard)
+        self.Yguard = guard_window_both(grid.y, yguard)
+        k0 = float(medium.k(w0))
+        kxmax, kymax = k0 * math.sin(math.radians(kxguard)), k0 * math.sin(math.radians(kyguard))
+        self.KXguard = np.exp(-(((grid.kx * unit.kx) ** 2 / kxmax ** 2) ** 20))
+        self.KYguard = np.exp(-(((grid.ky * unit.ky) ** 2 / kymax ** 2) ** 20))
+
+
+def kt_spatial(unit, grid):
+    """Transverse wavenumber [1/m] of every spectral point: R k ku [Nr];
+    XY sqrt((kx kxu)^2 + (ky kyu)^2) [Nx, Ny]."""
+    if hasattr(grid, "k"):
+        return np.asarray(grid.k) * unit.k
+    return np.sqrt((grid.kx[:, None] * unit.kx) ** 2 + (grid.ky[None, :] * unit.ky) ** 2)
+
+
+def kz_spatial(unit, grid, medium, w0, paraxial):
+    """KZ R (spectral.jl:30-50) / XY (:104-128): (Kfunc(w0, kt) - w0 / vf) zu,
+    Kfunc = beta - kt^2 / (2 beta) (paraxial) or sqrt(beta^2 - kt^2 + 0im)
+    (:160-184)."""
+    vf = medium.group_velocity(w0)
+    beta = complex(medium.beta(w0))
+    kt = kt_spatial(unit, grid)
+    if paraxial:
+        K = (beta - kt ** 2 / (2 * beta)) if beta != 0 else np.zeros(kt.shape, dtype=complex)
+    else:
+        K = np.sqrt(beta ** 2 - kt ** 2 + 0j)
+    return K * unit.z - w0 / vf * unit.z
+
+
+def qz_spatial(unit, grid, medium, w0, Eu, paraxial):
+    """QZ R (nonlinear_propagators.jl:78-96) / XY (:162-180): Qfunc(w0, kt)
+    zu / Eu; paraxial mu0 mu w0^2 / (2 beta) (the same everywhere), else
+    mu0 mu w0^2 / (2 kz), kz = sqrt(beta^2 - kt^2 + 0im), 0 where kz = 0
+    (:210-242)."""
+    beta = complex(medium.beta(w0))
+    mu = medium.permeability(w0)
+    kt = kt_spatial(unit, grid)
+    if paraxial:
+        Q = np.full(kt.shape, 0.0 if beta == 0 else MU0 * mu * w0 ** 2 / (2 * beta), dtype=complex)
+    else:
+        kz = np.sqrt(beta ** 2 - kt ** 2 + 0j)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            Q = np.where(kz == 0, 0.0, MU0 * mu * w0 ** 2 / (2 * kz))
+    return Q * unit.z / Eu
 
 
 def kz_factors_xyt(unit, grid, medium, w0, vf=None):

@@ -9,12 +9,14 @@ Container: a directory ``<prefix>plot.fdata/`` with
                      the attribute z in z units), the zdata row count and
                      column shapes;
   grid/<name>.npy    the grid axes (r, t, x, y as the geometry has them);
-  field/<dset>.npy   Array(real.(field.E)) for RT and XYT, Array(field.E) for T
-                     (WritePlots.jl:310-331), in the reference's array shape;
+  field/<dset>.npy   Array(real.(field.E)) for RT and XYT, Array(field.E) for
+                     T, R and XY (WritePlots.jl:310-331), in the reference's
+                     array shape;
   zdata/<name>.bin   rows appended as raw little-endian float32 (FloatGPU):
                      RT  z, Fzr [Nr], rhozr [Nr], Fzt [Nt], iSzf [Nt/2+1]
+                     XY  z, Izx [Nx] = I[:, Ny/2], Izy [Ny] = I[Nx/2, :]
                      XYT z, Fzxy [Nx, Ny], rhozxy [Nx, Ny], iSzf [Nt/2+1]
-                     (WritePlots.jl:367-416, 463-501).
+                     (WritePlots.jl:367-416, 422-459, 463-501).
 Schedule (WritePlots.jl:263-283): a field dataset when z differs from the
 last zdata z and z >= the next field distance (then + dz_field); a zdata row
 when z >= the next zdata distance (then z + dz_zdata, dz_zdata = lam0/2/zu,
@@ -42,7 +44,7 @@ class PlotField:
         RT and XYT, Input.jl:107-110, :152-155)."""
         self.path = path
         self.geometry = p.get("geometry", "RT")
-        self.zdata = self.geometry in ("RT", "XYT")
+        self.zdata = self.geometry in ("RT", "XY", "XYT")
         os.makedirs(os.path.join(path, "field"), exist_ok=True)
         os.makedirs(os.path.join(path, "grid"), exist_ok=True)
         units = {k: float(p[k]) for k in ("ru", "xu", "yu", "zu", "tu", "Iu", "rhou") if k in p}
@@ -79,9 +81,10 @@ class PlotField:
         z = self.ft(z)
         if z != self.zprev_field and z >= self.znext_field:
             E = model.field()
-            data = E if self.geometry == "T" else np.ascontiguousarray(E.real)
+            cplx = self.geometry in ("T", "R", "XY")
+            data = E if cplx else np.ascontiguousarray(E.real)
             if model.precision == 4:
-                data = data.astype(np.complex64 if self.geometry == "T" else np.float32)
+                data = data.astype(np.complex64 if cplx else np.float32)
             dset = "%03d" % self.ifield
             np.save(os.path.join(self.path, "field", dset + ".npy"), data)
             self.meta["field"].append(dict(dset=dset, z=float(z)))
@@ -91,6 +94,8 @@ class PlotField:
         if self.zdata and z >= self.znext_zdata:
             if self.geometry == "RT":
                 cols = dict(z=np.array([float(z)]), Fzr=d["Fr"], rhozr=d["rho_end"], Fzt=d["Ft"], iSzf=d["S"])
+            elif self.geometry == "XY":
+                cols = dict(z=np.array([float(z)]), Izx=d["Izx"], Izy=d["Izy"])
             else:
                 cols = dict(z=np.array([float(z)]), Fzxy=d["Fxy"], rhozxy=d["rho_end"], iSzf=d["S"])
             for name, v in cols.items():

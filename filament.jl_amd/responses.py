@@ -31,5 +31,20 @@ def response_kind(init):
 
 
 class RK3:
-    """Placeholder for ODEIntegrators.RK3() in input files (the only
-    tableau the library integrates with)."""
+    """Placeholder for ODEIntegrators.RK3() in input files: Kutta's
+    third-order method, the tableau of every geometry."""
+
+
+class RK4:
+    """Placeholder for ODEIntegrators.RK4() in input files: the classic
+    fourth-order method, integrated for the spatial geometries R and XY (the
+    reference's R and XY examples select it, examples/R/input.jl:38)."""
+
+
+def alg_order(alg):
+    """3 for RK3() (or no ALG), 4 for RK4(); other integrators are not built."""
+    if alg is None or isinstance(alg, RK3) or alg is RK3:
+        return 3
+    if isinstance(alg, RK4) or alg is RK4:
+        return 4
+    raise NotImplementedError(f"ALG {alg!r}: the library integrates RK3 and (R, XY) RK4")

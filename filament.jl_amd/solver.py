@@ -21,7 +21,7 @@ import numpy as np
 
 from . import lib as L
 from . import physics as ph
-from .responses import response_kind
+from .responses import alg_order, response_kind
 
 
 class _Model:
@@ -501,6 +501,184 @@ class XYTModel(_Model):
                        S=S)
             return out
         L.check(self._lib.fil_analyze_xyt(self.h, float(z), ctypes.byref(d), None, None, None, None))
+        return d.as_dict()
+
+
+class _SpatialModel:
+    """The spatial geometries R and XY (Input.jl:37-56, 131-150): a
+    monochromatic beam E(r) or E(x, y), no time axis and no plasma.  Field,
+    Guard, LinearPropagator and NonlinearPropagator are built on the host in
+    FP64 (Fields.jl:31-51, 123-138; Guards.jl:41-57, 132-157; spectral.jl:30-50,
+    104-128; nonlinear_propagators.jl:78-96, 162-180) and handed to a
+    fil_spatial handle; ``zstep`` is Models.zstep for these geometries and
+    ``analyze`` FieldAnalyzerR / FieldAnalyzerXY (FieldAnalyzers.jl:17-70,
+    218-266)."""
+
+    def _setup(self, p, geometry, precision, device, KZ=None):
+        u, g = self.unit, self.grid
+        self.p = p
+        self.precision = precision
+        self.medium = m = ph.Medium(p["permittivity"], p.get("permeability", ph.unit_permeability), p["n2"])
+        self.w0 = 2 * math.pi * ph.C0 / p["lam0"]
+        self.n0 = float(np.real(m.refractive_index(self.w0)))
+        self.Eu = ph.field_unit(u, self.n0)
+        self.nonlinearity = bool(p["NONLINEARITY"])
+        self.nonparaxial = bool(self.nonlinearity and not p.get("QPARAXIAL", True))
+        self.alg = alg_order(p.get("ALG"))
+        # the responses with a spatial form: the cubic one (cubic.jl:18-19,
+        # calc_cubic_nothg_spatial); several add their Rnl F terms
+        self.Rnl = 0.0
+        if self.nonlinearity:
+            for item in p["responses"]:
+                kind = response_kind(item["init"])
+                if kind != "cubic":
+                    raise NotImplementedError(f"response {kind} has no spatial form (geometry {geometry})")
+                self.Rnl += float(np.real(ph.rnl_cubic(u, m, self.w0, self.Eu, 1, item.get("n2"))[0]))
+        self.QZ = ph.qz_spatial(u, g, m, self.w0, self.Eu, paraxial=not self.nonparaxial)
+        self.KZ = ph.kz_spatial(u, g, m, self.w0, bool(p["KPARAXIAL"])) if KZ is None else np.asarray(KZ)
+        # AStepKerr (AdaptiveSteps.jl:29-34, 46-57)
+        self.phik = ph.phi_kerr(u, m, self.w0, self.Eu) if p["n2"] != 0 else 0.0
+        d = L.FilSpatialDesc()
+        d.abi_version = L.ABI_VERSION
+        d.geometry = geometry
+        d.precision = precision
+        d.nonlinearity = int(self.nonlinearity)
+        d.nonparaxial = int(self.nonparaxial)
+        d.alg = self.alg
+        d.device = device
+        return d
+
+    def _create(self, d, t):
+        self._lib = L.load()
+        h = ctypes.c_void_p()
+        L.check(self._lib.fil_spatial_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        L.check(self._lib.fil_spatial_set_tables(self.h, ctypes.byref(t)))
+
+    @staticmethod
+    def _arr(keep, a, cplx=False):
+        a = np.ascontiguousarray(a, dtype=np.complex128 if cplx else np.float64)
+        keep.append(a)
+        return L.ptr(a.view(np.float64) if cplx else a)
+
+    def upload(self, E):
+        E = np.ascontiguousarray(np.asarray(E, dtype=np.complex128).ravel(order="F"))
+        if E.size != self.n:
+            raise ValueError(f"field has {E.size} points, the grid {self.n}")
+        L.check(self._lib.fil_spatial_upload_field(self.h, L.ptr(E.view(np.float64))))
+
+    def field(self):
+        out = np.zeros(self.n, dtype=np.complex128)
+        L.check(self._lib.fil_spatial_download_field(self.h, L.ptr(out.view(np.float64))))
+        return out.reshape(self.shape, order="F")
+
+    def zstep(self, z, dz):
+        L.check(self._lib.fil_spatial_zstep(self.h, float(z), float(dz)))
+
+    def synchronize(self):
+        L.check(self._lib.fil_spatial_synchronize(self.h))
+
+    def dz_adaptive(self, dz0, phimax, Imax, rhomax=0.0):
+        """AStepKerr (AdaptiveSteps.jl:75-88): min(dz0, phimax / (phik Imax))
+        with the nonlinearity, dz0 without; a NaN Imax gives a NaN dz."""
+        if not self.nonlinearity:
+            return dz0
+        if math.isnan(Imax):
+            return math.nan
+        dzk = phimax / (self.phik * Imax) if self.phik * Imax != 0 else math.inf
+        return min(dz0, dzk)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.fil_spatial_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RModel(_SpatialModel):
+    """Geometry R (Input.jl:37-56): ``p`` with ru, zu, Iu, rmax, Nr, lam0,
+    permittivity, permeability, n2, NONLINEARITY, KPARAXIAL, QPARAXIAL,
+    rguard, kguard, responses, ALG and initial_condition(r, ru, Iu)."""
+
+    def __init__(self, p, precision=L.FP32, device=0, basedir=".", KZ=None):
+        """``KZ`` overrides the host-computed linear-propagator table ([Nr]
+        complex), e.g. to drive the device with another implementation's."""
+        self.unit = u = ph.UnitR(p["ru"], p["zu"], p["Iu"])
+        self.grid = g = ph.GridR(p["rmax"], p["Nr"])
+        self.n, self.shape = g.Nr, (g.Nr,)
+        d = self._setup(p, L.GEOM_R, precision, device, KZ)
+        d.Nr = g.Nr
+        self.guard = ph.GuardR(u, g, self.medium, self.w0, p["rguard"], p["kguard"])
+        self.T, self.RJ, self.JV = ph.dht_plan(g)
+        self.E0 = np.asarray(p["initial_condition"](g.r, u.r, u.I), dtype=complex)
+        keep = []
+        a = lambda v, c=False: self._arr(keep, v, c)
+        t = L.FilSpatialTables()
+        t.KZ, t.QZ, t.Rnl = a(self.KZ, True), a(self.QZ, True), self.Rnl
+        t.Kguard, t.Rguard = a(self.guard.Kguard), a(self.guard.Rguard)
+        t.dht_T, t.dht_RJ, t.dht_JV = a(self.T.T), a(self.RJ), a(self.JV)
+        t.r, t.dr = a(g.r), a(g.dr)
+        self._create(d, t)
+        self.upload(self.E0)
+
+    def analyze(self, z=0.0, arrays=False, full=False):
+        """arrays or full: also I = |E|^2 [Nr]."""
+        d = L.FilDiagSpatial()
+        arrays = arrays or full
+        I = np.zeros(self.n) if arrays else None
+        L.check(self._lib.fil_spatial_analyze(self.h, float(z), ctypes.byref(d), L.ptr(I), None, None))
+        out = d.as_dict()
+        if arrays:
+            out["I"] = I
+        return out
+
+
+class XYModel(_SpatialModel):
+    """Geometry XY (Input.jl:131-150): ``p`` with xu, yu, zu, Iu, xmin, xmax,
+    Nx, ymin, ymax, Ny, lam0, medium entries, NONLINEARITY, KPARAXIAL,
+    QPARAXIAL, xguard, yguard, kxguard, kyguard, responses, ALG and
+    initial_condition(x, y, xu, yu, Iu) -> [Nx, Ny]."""
+
+    def __init__(self, p, precision=L.FP32, device=0, basedir=".", KZ=None):
+        """``KZ``: an [Nx, Ny] complex table in place of the host's."""
+        self.unit = u = ph.UnitXY(p["xu"], p["yu"], p["zu"], p["Iu"])
+        self.grid = g = ph.GridXY(p["xmin"], p["xmax"], p["Nx"], p["ymin"], p["ymax"], p["Ny"])
+        self.n, self.shape = g.Nx * g.Ny, (g.Nx, g.Ny)
+        d = self._setup(p, L.GEOM_XY, precision, device, KZ)
+        d.Nx, d.Ny = g.Nx, g.Ny
+        self.guard = ph.GuardXY(u, g, self.medium, self.w0, p["xguard"], p["yguard"], p["kxguard"], p["kyguard"])
+        self.E0 = np.asarray(p["initial_condition"](g.x, g.y, u.x, u.y, u.I), dtype=complex)
+        keep = []
+        a = lambda v, c=False: self._arr(keep, v, c)
+        t = L.FilSpatialTables()
+        t.KZ, t.QZ, t.Rnl = a(self.KZ.ravel(order="F"), True), a(self.QZ.ravel(order="F"), True), self.Rnl
+        t.KXguard, t.KYguard = a(self.guard.KXguard), a(self.guard.KYguard)
+        t.Xguard, t.Yguard = a(self.guard.Xguard), a(self.guard.Yguard)
+        t.x, t.y, t.dx, t.dy = a(g.x), a(g.y), g.dx, g.dy
+        self._create(d, t)
+        self.upload(self.E0)
+
+    def analyze(self, z=0.0, arrays=False, full=False):
+        """arrays: also the zdata lines Izx = I[:, Ny/2], Izy = I[Nx/2, :]
+        (1-based, WritePlots.jl:448-456); full: also I [Nx, Ny]."""
+        g = self.grid
+        d = L.FilDiagSpatial()
+        if arrays or full:
+            I = np.zeros(self.n) if full else None
+            Izx, Izy = np.zeros(g.Nx), np.zeros(g.Ny)
+            L.check(self._lib.fil_spatial_analyze(self.h, float(z), ctypes.byref(d), L.ptr(I), L.ptr(Izx),
+                                                  L.ptr(Izy)))
+            out = d.as_dict()
+            out.update(Izx=Izx, Izy=Izy)
+            if full:
+                out["I"] = I.reshape(g.Nx, g.Ny, order="F")
+            return out
+        L.check(self._lib.fil_spatial_analyze(self.h, float(z), ctypes.byref(d), None, None, None))
         return d.as_dict()
 
 

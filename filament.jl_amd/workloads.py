@@ -21,7 +21,7 @@ import math
 import numpy as np
 
 from . import physics as ph
-from .responses import (RK3, init_cubic, init_current_free, init_current_losses, init_photoionization_avalanche,
+from .responses import (RK3, RK4, init_cubic, init_current_free, init_current_losses, init_photoionization_avalanche,
                         init_raman)
 
 
@@ -81,6 +81,61 @@ def rt_air(Nr, Nt, power_pcr, rmax=10.0, tmin=-500.0, tmax=500.0, a0=1e-3, tau0=
         initial_condition=gaussian_ic(a0, tau0, I0, lam0, noise, seed),
         I0=I0, Pcr=Pcr, power_pcr=power_pcr,
     )
+    return p
+
+
+def gaussian_ic_r(a0, I0, focus=None, lam0=800e-9, n0=1.0):
+    """initial_condition(r, ru, Iu) of test/diffraction/R/gauss.jl form,
+    sqrt(I0/Iu) exp(-r^2/(2 a0^2)), with the focusing phase
+    exp(-i k0 r^2 / (2 f)) of examples/R/gauss.jl when ``focus`` is given."""
+    def initial_condition(r, ru, Iu):
+        rr = np.asarray(r) * ru
+        E = math.sqrt(I0 / Iu) * np.exp(-0.5 * rr ** 2 / a0 ** 2) + 0j
+        if focus:
+            k0 = n0 * 2 * math.pi / lam0
+            E = E * np.exp(-1j * k0 * rr ** 2 / (2 * focus))
+        return E
+    return initial_condition
+
+
+def gaussian_ic_xy(ax, ay, I0, noise=0.0, seed=0):
+    """initial_condition(x, y, xu, yu, Iu) of test/diffraction/XY/gauss.jl
+    form, sqrt(I0/Iu) exp(-x^2/(2 ax^2) - y^2/(2 ay^2)), times (1 + noise u),
+    u ~ U[-1, 1) seeded, when ``noise`` is given."""
+    def initial_condition(x, y, xu, yu, Iu):
+        xx = np.asarray(x)[:, None] * xu
+        yy = np.asarray(y)[None, :] * yu
+        E = math.sqrt(I0 / Iu) * np.exp(-0.5 * xx ** 2 / ax ** 2 - 0.5 * yy ** 2 / ay ** 2) + 0j
+        if noise:
+            E = E * (1 + noise * (2 * np.random.default_rng(seed).random(E.shape) - 1))
+        return E
+    return initial_condition
+
+
+def spatial_air(geometry, N, power_pcr=1.0, extent=10.0, a0=1e-3, n2=1e-23, nonlinearity=True, qparaxial=True,
+                alg=4, dz=1e-2, guard=1.0, kguard=45.0, noise=0.0):
+    """Parameter dict of the R or XY geometry (examples/R/input.jl,
+    examples/XY/input.jl): a Gaussian beam of power_pcr critical powers in
+    air at 800 nm, Nr = N points on [0, extent] (R) or N x N on
+    [-extent, extent]^2 (XY), guards of width ``guard`` and cut-off angle
+    ``kguard``, the cubic response."""
+    lam0 = 800e-9
+    medium = ph.Medium(ph.peck_reeder_air, ph.unit_permeability, n2)
+    w0 = 2 * math.pi * ph.C0 / lam0
+    Pcr = medium.critical_power(w0)
+    I0 = power_pcr * Pcr / (math.pi * a0 ** 2)
+    p = dict(prefix="results/", geometry=geometry, zu=1.0, Iu=1e16, lam0=lam0,
+             permittivity=ph.peck_reeder_air, permeability=ph.unit_permeability, n2=n2,
+             NONLINEARITY=nonlinearity, PLASMA=False, KPARAXIAL=False, QPARAXIAL=qparaxial,
+             dz_initial=dz, dzphimax=math.inf, Istop=1e3, ALG=RK4() if alg == 4 else RK3(), z=0.0,
+             responses=[{"init": init_cubic, "THG": False, "n2": n2}], I0=I0, Pcr=Pcr, power_pcr=power_pcr)
+    if geometry == "R":
+        p.update(ru=1e-3, rmax=extent, Nr=N, rguard=guard, kguard=kguard,
+                 initial_condition=gaussian_ic_r(a0, I0))
+    else:
+        p.update(xu=1e-3, yu=1e-3, xmin=-extent, xmax=extent, Nx=N, ymin=-extent, ymax=extent, Ny=N,
+                 xguard=guard, yguard=guard, kxguard=kguard, kyguard=kguard,
+                 initial_condition=gaussian_ic_xy(a0, a0, I0, noise))
     return p
 
 

@@ -1,7 +1,8 @@
 /*
  * filament_hip.h -- C ABI of libfilament_hip.so, the MI355X (gfx950)
  * implementation of Filament.jl's per-z-step propagation path for the
- * axisymmetric RT and the three-dimensional XYT geometries.
+ * axisymmetric RT and the three-dimensional XYT geometries, the T geometry,
+ * and the spatial R and XY geometries (the last section).
  *
  * The library replaces the device side of one reference call,
  *   Models.zstep(z, dz, grid, field, guard, model)   (src/modules/Models.jl:75-121)
@@ -33,7 +34,7 @@
 extern "C" {
 #endif
 
-#define FIL_ABI_VERSION 4
+#define FIL_ABI_VERSION 5
 #define FIL_MAXCOMP 4
 
 typedef struct fil_handle fil_handle;
@@ -385,6 +386,94 @@ int fil_corrector_apply(fil_handle* h);
  * z-steps. */
 int fil_timing_enable(fil_handle* h, int enable);
 int fil_timing_read(fil_handle* h, double* ms_row, double* ms_dht1, double* ms_dht2, int64_t* count);
+
+/* ---------------------------------------------------------------------------
+ * The spatial geometries R and XY: a monochromatic beam without a time axis
+ * and without plasma (Input.jl:37-56 R, :131-150 XY force PLASMA = false).
+ * A spatial handle replaces Field/Guard/LinearPropagator/NonlinearPropagator R
+ * (Fields.jl:31-51, Guards.jl:41-57, spectral.jl:30-50,
+ * nonlinear_propagators.jl:78-96) and XY (Fields.jl:123-138, Guards.jl:132-157,
+ * spectral.jl:104-128, nonlinear_propagators.jl:162-180) and runs
+ * Models.zstep for them (Models.jl:75-121: no plasma, no t transforms):
+ *   nonlinear RK step of func! (nonlinear_propagators.jl:38-74) with the
+ *     spatial cubic response 3/4 |E|^2 E (cubic.jl:49-54), the field filter
+ *     on the response and i QZ (paraxial) or the space transform, i QZ,
+ *     spectral filter and inverse transform (non-paraxial);
+ *   linear step: forward transform (R: Hankel, XY: 2-D FFT), exp(i KZ dz),
+ *     spectral filter, inverse transform (spectral.jl:16-26);
+ *   field filter (Guards.jl:199-215, kernel! :263-278).
+ * Arrays are complex interleaved (re, im) doubles in the reference's layout:
+ * R [Nr]; XY [Nx x Ny], x fastest (E[ix + Nx iy]).
+ * ------------------------------------------------------------------------- */
+enum { FIL_GEOM_R = 3, FIL_GEOM_XY = 4 };
+/* ODEIntegrators algorithms of the nonlinear step (ALG in the input file) */
+enum { FIL_ALG_RK3 = 3, FIL_ALG_RK4 = 4 };
+
+typedef struct fil_spatial fil_spatial;
+
+typedef struct fil_spatial_desc {
+  int32_t abi_version;     /* FIL_ABI_VERSION */
+  int32_t geometry;        /* FIL_GEOM_R or FIL_GEOM_XY */
+  int32_t Nr;              /* R: points, 2..16384 */
+  int32_t Nx, Ny;          /* XY: powers of two in [16, 4096] */
+  int32_t precision;       /* FIL_FP32 (FloatGPU) or FIL_FP64 */
+  int32_t nonlinearity;    /* NONLINEARITY: the cubic response */
+  int32_t nonparaxial;     /* QPARAXIAL = false */
+  int32_t alg;             /* FIL_ALG_RK3 (Kutta) or FIL_ALG_RK4 (classic) */
+  int32_t device;          /* HIP device ordinal */
+} fil_spatial_desc;
+
+typedef struct fil_spatial_tables {
+  /* KZ (spectral.jl:30-50 R, :104-128 XY; Kfunc :160-184): complex [n] */
+  const double* KZ;
+  /* spectral guards (Guards.jl:41-57, 132-157): R Kguard [Nr]; XY KXguard [Nx], KYguard [Ny] */
+  const double* Kguard;
+  const double* KXguard;
+  const double* KYguard;
+  /* field guards: R Rguard [Nr]; XY Xguard [Nx], Yguard [Ny] */
+  const double* Rguard;
+  const double* Xguard;
+  const double* Yguard;
+  /* R: Hankel plan (as fil_tables.dht_*) and r, dr [Nr] (grid units) */
+  const double* dht_T;
+  const double* dht_RJ;
+  const double* dht_JV;
+  const double* r;
+  const double* dr;
+  /* XY: grid x [Nx], y [Ny] and steps (grid units) */
+  const double* x;
+  const double* y;
+  double dx, dy;
+  /* nonlinear term: Rnl of the cubic response (cubic.jl:4-30, a real scalar)
+   * and QZ (QZfunc R nonlinear_propagators.jl:78-96, XY :162-180), complex [n]
+   * indexed like the field (paraxial: applied in real space, as the
+   * reference's 1im * QZ .* dE; non-paraxial: in transform space) */
+  double Rnl;
+  const double* QZ;
+} fil_spatial_tables;
+
+/* FieldAnalyzerR (FieldAnalyzers.jl:17-70) and FieldAnalyzerXY (:218-266):
+ * Imax, R: rfil = 2 radius(r, I), P = 2 pi sum(I r dr); XY: ax = radius(x,
+ * I[:, jmax]), ay = radius(y, I[imax, :]), P = sum(I) dx dy.  imax, jmax: the
+ * (first, x-fastest) position of the maximum (R: jmax = 0). */
+typedef struct fil_diag_spatial {
+  double z, Imax, rfil, ax, ay, P;
+  int64_t imax, jmax;
+} fil_diag_spatial;
+
+int fil_spatial_create(const fil_spatial_desc* d, fil_spatial** h);
+int fil_spatial_set_tables(fil_spatial* h, const fil_spatial_tables* t);
+/* field.E (Fields.jl:31-51, 123-138): complex [n]; upload replaces the state. */
+int fil_spatial_upload_field(fil_spatial* h, const double* E);
+int fil_spatial_download_field(fil_spatial* h, double* E);
+/* Models.zstep (Models.jl:75-121) for R and XY.  Asynchronous. */
+int fil_spatial_zstep(fil_spatial* h, double z, double dz);
+/* analyze! R / XY.  I [n] (|E|^2), and for the XY zdata of plot.h5
+ * (WritePlots.jl:422-459) Izx = I[:, Ny/2] and Izy = I[Nx/2, :] (1-based
+ * indices of the reference), may be NULL.  Blocks. */
+int fil_spatial_analyze(fil_spatial* h, double z, fil_diag_spatial* d, double* I, double* Izx, double* Izy);
+int fil_spatial_synchronize(fil_spatial* h);
+int fil_spatial_destroy(fil_spatial* h);
 
 #ifdef __cplusplus
 }

@@ -898,6 +898,257 @@ void State<T>::analyze(double z, orc_diag* d, double* Fr, double* Ft, double* rh
   if (S) std::copy(sp.begin(), sp.end(), S);
 }
 
+// ---------------------------------------------------------------------------
+// Spatial geometries R and XY
+// ---------------------------------------------------------------------------
+struct SpatialBase {
+  virtual ~SpatialBase() {}
+  virtual int precision() const = 0;
+};
+
+template <typename T>
+struct Spatial : SpatialBase {
+  using C = std::complex<T>;
+  int precision() const override { return sizeof(T); }
+  orc_spatial_config cfg{};
+  bool xy = false;
+  int n = 0, Nr = 0, Nx = 0, Ny = 0;
+  Medium med{};
+  std::vector<double> r, dr, k, x, y, kx, ky, rdr;
+  double dx = 0, dy = 0, w0 = 0, n0 = 0, Eu = 0, vf = 0, k0 = 0, phik = 0;
+  T Rnl = 0;
+  std::vector<C> E, KZ, QZ;
+  std::vector<T> Kg, Fg, Xg, Yg, KXg, KYg;
+  std::vector<T> Tm, RJ, JV;
+  std::unique_ptr<FFTPlan<T>> planx, plany;
+
+  void build(const orc_spatial_config& c, const double* E0) {
+    cfg = c;
+    xy = c.geometry == ORC_GEOM_XY;
+    if (!xy && c.geometry != ORC_GEOM_R) throw std::runtime_error("geometry must be R or XY");
+    if (c.alg != 3 && c.alg != 4) throw std::runtime_error("alg must be 3 or 4");
+    med.kind = c.medium; med.n_const = c.n_const; med.n2 = c.n2;
+    std::vector<double> jz;
+    if (!xy) {
+      // Grids.jl:42-45 with _grid_spatial_axial (:204-216)
+      Nr = n = c.Nr;
+      if (Nr < 2) throw std::runtime_error("Nr must be >= 2");
+      jz.resize(Nr + 1);
+      bessel_j0_zeros(Nr + 1, jz.data());
+      const double S = jz[Nr];
+      r.resize(Nr); k.resize(Nr); dr.resize(Nr); rdr.resize(Nr);
+      for (int i = 0; i < Nr; i++) { r[i] = jz[i] * c.rmax / S; k[i] = jz[i] / c.rmax; }
+      for (int i = 0; i < Nr; i++) {
+        dr[i] = (i == 0) ? r[1] - r[0] : (i == Nr - 1 ? r[Nr - 1] - r[Nr - 2] : (r[i + 1] - r[i - 1]) / 2);
+        rdr[i] = r[i] * dr[i];
+      }
+    } else {
+      // Grids.jl:144-155 with _grid_spatial_rectangular: range(min, max, N), 2 pi fftfreq(N, 1/dx)
+      Nx = c.Nx; Ny = c.Ny; n = Nx * Ny;
+      if (Nx < 2 || Ny < 2 || (Nx & (Nx - 1)) || (Ny & (Ny - 1))) throw std::runtime_error("Nx, Ny must be powers of two");
+      planx.reset(new FFTPlan<T>(Nx));
+      plany.reset(new FFTPlan<T>(Ny));
+      dx = (c.xmax - c.xmin) / (Nx - 1);
+      dy = (c.ymax - c.ymin) / (Ny - 1);
+      x.resize(Nx); y.resize(Ny); kx.resize(Nx); ky.resize(Ny);
+      for (int i = 0; i < Nx; i++) { x[i] = c.xmin + i * dx; kx[i] = 2 * PI * ((i < (Nx + 1) / 2) ? i : i - Nx) / (Nx * dx); }
+      for (int i = 0; i < Ny; i++) { y[i] = c.ymin + i * dy; ky[i] = 2 * PI * ((i < (Ny + 1) / 2) ? i : i - Ny) / (Ny * dy); }
+    }
+    // Fields.jl:31-51, 123-138; Units.jl:137-141
+    w0 = 2 * PI * C0 / c.lam0;
+    n0 = med.refractive_index(w0).real();
+    Eu = std::sqrt(c.Iu / (n0 * EPS0 * C0 / 2));
+    vf = med.group_velocity(w0);
+    k0 = med.k(w0);
+    E.resize(n);
+    for (int q = 0; q < n; q++) E[q] = C((T)E0[2 * q], (T)E0[2 * q + 1]);
+    // transverse wavenumber of every spectral point [1/m]
+    auto kt_of = [&](int q) {
+      if (!xy) return k[q] / c.ru;
+      const double a = kx[q % Nx] / c.xu, b = ky[q / Nx] / c.yu;
+      return std::sqrt(a * a + b * b);
+    };
+    // Guards.jl:41-57 (R), :132-157 (XY)
+    Kg.assign(n, 1); Fg.assign(n, 1);
+    if (!xy) {
+      auto Rgd = guard_window_right(r, c.rguard);
+      const double kmax = k0 * std::sin(c.kguard * PI / 180.0);
+      for (int i = 0; i < n; i++) {
+        Fg[i] = (T)Rgd[i];
+        Kg[i] = (T)std::exp(-std::pow(std::pow(k[i] / c.ru, 2) / (kmax * kmax), 20));
+      }
+    } else {
+      auto Xd = guard_window_both(x, c.xguard), Yd = guard_window_both(y, c.yguard);
+      const double kxmax = k0 * std::sin(c.kxguard * PI / 180.0), kymax = k0 * std::sin(c.kyguard * PI / 180.0);
+      Xg.resize(Nx); Yg.resize(Ny); KXg.resize(Nx); KYg.resize(Ny);
+      for (int i = 0; i < Nx; i++) {
+        Xg[i] = (T)Xd[i];
+        KXg[i] = (T)std::exp(-std::pow(std::pow(kx[i] / c.xu, 2) / (kxmax * kxmax), 20));
+      }
+      for (int i = 0; i < Ny; i++) {
+        Yg[i] = (T)Yd[i];
+        KYg[i] = (T)std::exp(-std::pow(std::pow(ky[i] / c.yu, 2) / (kymax * kymax), 20));
+      }
+      for (int q = 0; q < n; q++) { Fg[q] = Xg[q % Nx] * Yg[q / Nx]; Kg[q] = KXg[q % Nx] * KYg[q / Nx]; }
+    }
+    // KZ (spectral.jl:30-50 R, :104-128 XY; Kfunc :160-184)
+    const cd beta = med.beta(w0);
+    KZ.resize(n);
+    for (int q = 0; q < n; q++) {
+      const double kt = kt_of(q);
+      cd K;
+      if (c.kparaxial) K = (beta != cd(0)) ? beta - kt * kt / (2.0 * beta) : cd(0);
+      else K = std::sqrt(beta * beta - kt * kt + cd(0));
+      const cd kz = K * c.zu - w0 / vf * c.zu;
+      KZ[q] = C((T)kz.real(), (T)kz.imag());
+    }
+    // QZ (nonlinear_propagators.jl:78-96 R, :162-180 XY; Qfunc :210-242)
+    QZ.resize(n);
+    const double mu = med.permeability(w0);
+    for (int q = 0; q < n; q++) {
+      cd Q;
+      if (c.qparaxial) {
+        Q = (beta == cd(0)) ? cd(0) : MU0 * mu * w0 * w0 / (2.0 * beta);
+      } else {
+        const double kt = kt_of(q);
+        const cd kz = std::sqrt(beta * beta - kt * kt + cd(0));
+        Q = (kz == cd(0)) ? cd(0) : MU0 * mu * w0 * w0 / (2.0 * kz);
+      }
+      Q = Q * c.zu / Eu;
+      QZ[q] = C((T)Q.real(), (T)Q.imag());
+    }
+    // cubic.jl:4-30: Rnl = EPS0 chi3 Eu^3; AdaptiveSteps.jl:46-57: phik
+    const double chi3 = med.chi3(w0);
+    Rnl = (T)(EPS0 * chi3 * Eu * Eu * Eu);
+    const double QZ0 = MU0 * mu * w0 * w0 / (2 * k0) * c.zu / Eu;
+    phik = (c.n2 != 0) ? QZ0 * std::fabs(EPS0 * chi3 * 3.0 / 4.0 * Eu * Eu * Eu) : 0.0;
+    // Hankel plan (Fields.jl:38-42): the same quasi-discrete transform as RT
+    if (!xy) {
+      const double S = jz[Nr], R = c.rmax, V = S / (2 * PI * R);
+      std::vector<double> J1(Nr);
+      for (int i = 0; i < Nr; i++) J1[i] = std::fabs(std::cyl_bessel_j(1.0, jz[i]));
+      Tm.resize((size_t)Nr * Nr); RJ.resize(Nr); JV.resize(Nr);
+      for (int i = 0; i < Nr; i++) { RJ[i] = (T)(R / J1[i]); JV[i] = (T)(J1[i] / V); }
+#pragma omp parallel for schedule(static)
+      for (int m = 0; m < Nr; m++)
+        for (int j = 0; j < Nr; j++)
+          Tm[(size_t)m * Nr + j] = (T)(2.0 * std::cyl_bessel_j(0.0, jz[m] * jz[j] / S) / (S * J1[m] * J1[j]));
+    }
+  }
+
+  // forward_transform_space! / inverse_transform_space! (Models.jl:135-148)
+  void transform(std::vector<C>& A, bool inverse) {
+    if (!xy) {
+      std::vector<C> X(n, C(0)), B(n);
+      for (int j = 0; j < n; j++) B[j] = inverse ? A[j] / JV[j] : RJ[j] * A[j];
+#pragma omp parallel for schedule(static)
+      for (int m = 0; m < n; m++) {
+        C acc(0);
+        const T* row = &Tm[(size_t)m * n];
+        for (int j = 0; j < n; j++) acc += row[j] * B[j];
+        X[m] = acc;
+      }
+      for (int m = 0; m < n; m++) A[m] = inverse ? X[m] / RJ[m] : JV[m] * X[m];
+      return;
+    }
+    const int sign = inverse ? +1 : -1;   // P * E forward (exp(-i)), P \ E inverse normalised
+    for (int iy = 0; iy < Ny; iy++) planx->run(&A[(size_t)iy * Nx], sign);
+    std::vector<C> col(Ny);
+    for (int ix = 0; ix < Nx; ix++) {
+      for (int iy = 0; iy < Ny; iy++) col[iy] = A[(size_t)iy * Nx + ix];
+      plany->run(col.data(), sign);
+      for (int iy = 0; iy < Ny; iy++) A[(size_t)iy * Nx + ix] = col[iy];
+    }
+  }
+
+  // func! (nonlinear_propagators.jl:38-67) for the spatial cubic response
+  void rhs(const std::vector<C>& U, std::vector<C>& dE) {
+    dE.resize(n);
+    for (int q = 0; q < n; q++) {
+      C F = (T)0.75 * std::norm(U[q]) * U[q];   // calc_cubic_nothg_spatial
+      F = F * Fg[q];                             // apply_field_filter!
+      dE[q] = Rnl * F;                           // update_dE!
+    }
+    if (cfg.qparaxial) {
+      for (int q = 0; q < n; q++) dE[q] = C(0, 1) * QZ[q] * dE[q];
+    } else {
+      transform(dE, false);
+      for (int q = 0; q < n; q++) dE[q] = C(0, 1) * QZ[q] * dE[q];
+      for (int q = 0; q < n; q++) dE[q] = dE[q] * Kg[q];   // apply_spectral_filter!
+      transform(dE, true);
+    }
+  }
+
+  // ODEIntegrators step! (nonlinear_propagators.jl:70-74): RK3 (Kutta) or RK4
+  void nl_step(T h) {
+    std::vector<C> k1, k2, k3, k4, U(n);
+    rhs(E, k1);
+    for (int q = 0; q < n; q++) U[q] = E[q] + (T)0.5 * h * k1[q];
+    rhs(U, k2);
+    if (cfg.alg == 3) {
+      for (int q = 0; q < n; q++) U[q] = E[q] + h * ((T)2 * k2[q] - k1[q]);
+      rhs(U, k3);
+      for (int q = 0; q < n; q++) E[q] += h * (k1[q] / (T)6 + (T)2 * k2[q] / (T)3 + k3[q] / (T)6);
+    } else {
+      for (int q = 0; q < n; q++) U[q] = E[q] + (T)0.5 * h * k2[q];
+      rhs(U, k3);
+      for (int q = 0; q < n; q++) U[q] = E[q] + h * k3[q];
+      rhs(U, k4);
+      for (int q = 0; q < n; q++) E[q] += h / (T)6 * (k1[q] + (T)2 * k2[q] + (T)2 * k3[q] + k4[q]);
+    }
+  }
+
+  // propagate!(E, LP, dz) (spectral.jl:16-26)
+  void linear_step(double dz) {
+    transform(E, false);
+    for (int q = 0; q < n; q++) {
+      const std::complex<double> kz((double)KZ[q].real(), (double)KZ[q].imag());
+      const std::complex<double> p = std::exp(std::complex<double>(0, 1) * kz * dz);
+      E[q] = E[q] * C((T)p.real(), (T)p.imag());
+      E[q] = E[q] * Kg[q];                      // apply_spectral_filter!
+    }
+    transform(E, true);
+  }
+
+  // Models.zstep (Models.jl:75-121) without plasma and t transforms
+  void zstep(double dz) {
+    if (cfg.nonlinearity) nl_step((T)dz);
+    linear_step(dz);
+    for (int q = 0; q < n; q++) E[q] = E[q] * Fg[q];   // apply_field_filter!
+  }
+
+  // FieldAnalyzerR (FieldAnalyzers.jl:52-70) / XY (:251-266)
+  void analyze(double* o) {
+    std::vector<double> I(n);
+    double Imax = -INFINITY, P = 0;
+    long idx = 0;
+    for (int q = 0; q < n; q++) {
+      I[q] = (double)std::norm(E[q]);
+      if (I[q] > Imax) { Imax = I[q]; idx = q; }
+    }
+    for (int q = 0; q < n; q++) P += xy ? I[q] : I[q] * rdr[q];
+    o[0] = Imax; o[1] = o[2] = o[3] = 0;
+    if (!xy) {
+      o[1] = 2 * radius(r.data(), I.data(), n);
+      o[4] = 2 * PI * P;
+    } else {
+      const int im = (int)(idx % Nx), jm = (int)(idx / Nx);
+      std::vector<double> line(Nx), colv(Ny);
+      for (int i = 0; i < Nx; i++) line[i] = I[(size_t)jm * Nx + i];
+      for (int j = 0; j < Ny; j++) colv[j] = I[(size_t)j * Nx + im];
+      o[2] = radius(x.data(), line.data(), Nx);
+      o[3] = radius(y.data(), colv.data(), Ny);
+      o[4] = P * dx * dy;
+    }
+    o[5] = (double)idx;
+  }
+};
+
+SpatialBase* SB(void* h) { return static_cast<SpatialBase*>(h); }
+
+template <typename Fn>
+int sdispatch(void* h, Fn&& fn);
+
 template <typename F>
 int guarded(F&& f) {
   try { f(); return 0; }
@@ -918,6 +1169,15 @@ int dispatch(void* h, Fn&& fn) {
 template <typename C>
 void put_complex(const std::vector<C>& v, double* out) {
   for (size_t q = 0; q < v.size(); q++) { out[2 * q] = v[q].real(); out[2 * q + 1] = v[q].imag(); }
+}
+
+template <typename Fn>
+int sdispatch(void* h, Fn&& fn) {
+  if (!h) { g_err = "null handle"; return 1; }
+  return guarded([&] {
+    if (SB(h)->precision() == 8) fn(*static_cast<Spatial<double>*>(SB(h)));
+    else fn(*static_cast<Spatial<float>*>(SB(h)));
+  });
 }
 
 }  // namespace
@@ -1142,5 +1402,64 @@ int orc_nl_rhs(void* h, const double* Es, double* dE) {
     for (size_t q = 0; q < out.size(); q++) { dE[2 * q] = out[q].real(); dE[2 * q + 1] = out[q].imag(); }
   });
 }
+
+
+void* orc_spatial_create(const orc_spatial_config* cfg, const double* E0, int precision) {
+  SpatialBase* b = nullptr;
+  int rc = guarded([&] {
+    if (precision == 8) { auto* s = new Spatial<double>(); b = s; s->build(*cfg, E0); }
+    else if (precision == 4) { auto* s = new Spatial<float>(); b = s; s->build(*cfg, E0); }
+    else throw std::runtime_error("precision must be 4 or 8");
+  });
+  if (rc) { delete b; return nullptr; }
+  return b;
+}
+
+void orc_spatial_destroy(void* h) { delete SB(h); }
+
+int orc_spatial_set_field(void* h, const double* E) {
+  return sdispatch(h, [&](auto& s) {
+    using C = typename std::decay_t<decltype(s)>::C;
+    using T = typename C::value_type;
+    for (int q = 0; q < s.n; q++) s.E[q] = C((T)E[2 * q], (T)E[2 * q + 1]);
+  });
+}
+
+int orc_spatial_get_field(void* h, double* E) { return sdispatch(h, [&](auto& s) { put_complex(s.E, E); }); }
+
+int orc_spatial_get_grid(void* h, double* a, double* b, double* c, double* d) {
+  return sdispatch(h, [&](auto& s) {
+    auto put = [](const std::vector<double>& v, double* o) { if (o) std::copy(v.begin(), v.end(), o); };
+    if (!s.xy) { put(s.r, a); put(s.dr, b); put(s.k, c); }
+    else { put(s.x, a); put(s.y, b); put(s.kx, c); put(s.ky, d); }
+  });
+}
+
+int orc_spatial_get_tables(void* h, double* KZ, double* QZ, double* Kg, double* Fg, double* s7) {
+  return sdispatch(h, [&](auto& s) {
+    if (KZ) put_complex(s.KZ, KZ);
+    if (QZ) put_complex(s.QZ, QZ);
+    for (int q = 0; q < s.n; q++) {
+      if (Kg) Kg[q] = s.Kg[q];
+      if (Fg) Fg[q] = s.Fg[q];
+    }
+    if (s7) {
+      s7[0] = s.w0; s7[1] = s.n0; s7[2] = s.Eu; s7[3] = s.vf; s7[4] = s.k0; s7[5] = s.phik; s7[6] = s.Rnl;
+    }
+  });
+}
+
+int orc_spatial_zstep(void* h, double dz) { return sdispatch(h, [&](auto& s) { s.zstep(dz); }); }
+
+int orc_spatial_nl_step(void* h, double dz) {
+  return sdispatch(h, [&](auto& s) {
+    using T = typename std::decay_t<decltype(s)>::C::value_type;
+    s.nl_step((T)dz);
+  });
+}
+
+int orc_spatial_linear_step(void* h, double dz) { return sdispatch(h, [&](auto& s) { s.linear_step(dz); }); }
+
+int orc_spatial_analyze(void* h, double* out6) { return sdispatch(h, [&](auto& s) { s.analyze(out6); }); }
 
 }  // extern "C"

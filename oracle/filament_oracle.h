@@ -115,6 +115,47 @@ int orc_nl_step(void* h, double* Es, double dz);
 /* replaces rho / kdrho [rows][Nt] (either may be NULL) */
 int orc_set_plasma(void* h, const double* rho, const double* kdrho);
 
+/* ---------------------------------------------------------------------------
+ * The spatial geometries R and XY (Input.jl:37-56, 131-150): a monochromatic
+ * field E(r) [Nr] or E(x, y) [Nx x Ny] (x fastest), complex, no plasma; one
+ * cubic response (calc_cubic_nothg_spatial, cubic.jl:49-54) with the medium's
+ * n2.  The transforms follow the reference's plans literally: the Hankel
+ * pair JV .* (T * (RJ .* f)) and (T * (F ./ JV)) ./ RJ, the 2-D FFT as 1-D
+ * transforms along x, then y.
+ * ------------------------------------------------------------------------- */
+enum { ORC_GEOM_R = 3, ORC_GEOM_XY = 4 };
+
+typedef struct orc_spatial_config {
+  int32_t geometry;                 /* ORC_GEOM_R or ORC_GEOM_XY */
+  int32_t Nr;                       /* R */
+  double rmax, ru;                  /* R: grid units */
+  int32_t Nx, Ny;                   /* XY: powers of two */
+  double xmin, xmax, ymin, ymax, xu, yu;
+  double zu, Iu, lam0;
+  int32_t medium;                   /* ORC_MEDIUM_* */
+  double n_const, n2;
+  int32_t nonlinearity, kparaxial, qparaxial;
+  int32_t alg;                      /* 3: RK3 (Kutta), 4: RK4 (classic) */
+  double rguard, kguard;            /* R */
+  double xguard, yguard, kxguard, kyguard;   /* XY */
+} orc_spatial_config;
+
+/* E0: complex [n], the initial_condition on the grid (Fields.jl:31-51, 123-138) */
+void* orc_spatial_create(const orc_spatial_config* cfg, const double* E0, int precision);
+void  orc_spatial_destroy(void* h);
+int orc_spatial_set_field(void* h, const double* E);
+int orc_spatial_get_field(void* h, double* E);
+/* R: a = r, b = dr, c = k (grid units); XY: a = x, b = y, c = kx, d = ky */
+int orc_spatial_get_grid(void* h, double* a, double* b, double* c, double* d);
+/* KZ, QZ complex [n]; Kg, Fg [n] (spectral and field guards, XY products of
+ * the x and y factors); s7 = w0, n0, Eu, vf, k0, phik, Rnl */
+int orc_spatial_get_tables(void* h, double* KZ, double* QZ, double* Kg, double* Fg, double* s7);
+int orc_spatial_zstep(void* h, double dz);          /* Models.jl:75-121 */
+int orc_spatial_nl_step(void* h, double dz);        /* nonlinear_propagators.jl:70-74 alone */
+int orc_spatial_linear_step(void* h, double dz);    /* spectral.jl:16-26 alone */
+/* FieldAnalyzerR / XY: out6 = Imax, rfil, ax, ay, P, index of the maximum */
+int orc_spatial_analyze(void* h, double* out6);
+
 #ifdef __cplusplus
 }
 #endif

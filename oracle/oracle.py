@@ -51,6 +51,25 @@ class OrcConfig(ctypes.Structure):
     ]
 
 
+GEOM_R, GEOM_XY = 3, 4
+
+
+class OrcSpatialConfig(ctypes.Structure):
+    _fields_ = [
+        ("geometry", ctypes.c_int32), ("Nr", ctypes.c_int32), ("rmax", ctypes.c_double), ("ru", ctypes.c_double),
+        ("Nx", ctypes.c_int32), ("Ny", ctypes.c_int32),
+        ("xmin", ctypes.c_double), ("xmax", ctypes.c_double), ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+        ("xu", ctypes.c_double), ("yu", ctypes.c_double),
+        ("zu", ctypes.c_double), ("Iu", ctypes.c_double), ("lam0", ctypes.c_double),
+        ("medium", ctypes.c_int32), ("n_const", ctypes.c_double), ("n2", ctypes.c_double),
+        ("nonlinearity", ctypes.c_int32), ("kparaxial", ctypes.c_int32), ("qparaxial", ctypes.c_int32),
+        ("alg", ctypes.c_int32),
+        ("rguard", ctypes.c_double), ("kguard", ctypes.c_double),
+        ("xguard", ctypes.c_double), ("yguard", ctypes.c_double), ("kxguard", ctypes.c_double),
+        ("kyguard", ctypes.c_double),
+    ]
+
+
 class OrcDiag(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in
                 ("z", "Fmax", "Imax", "rhomax", "De", "rfil", "rpl", "tau", "W")]
@@ -101,6 +120,16 @@ def lib():
         L.orc_set_threads.restype = ctypes.c_int
         L.orc_max_threads.argtypes = []
         L.orc_max_threads.restype = ctypes.c_int
+        L.orc_spatial_create.restype = ctypes.c_void_p
+        L.orc_spatial_create.argtypes = [ctypes.POINTER(OrcSpatialConfig), _dp, ctypes.c_int]
+        L.orc_spatial_destroy.argtypes = [ctypes.c_void_p]
+        for name in ("orc_spatial_set_field", "orc_spatial_get_field", "orc_spatial_get_grid",
+                     "orc_spatial_get_tables", "orc_spatial_analyze"):
+            getattr(L, name).argtypes = None
+            getattr(L, name).restype = ctypes.c_int
+        for name in ("orc_spatial_zstep", "orc_spatial_nl_step", "orc_spatial_linear_step"):
+            getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_double]
+            getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -298,3 +327,73 @@ class Oracle:
                                               ctypes.byref(d), ctypes.byref(dz)))
             rows.append(d.as_dict())
         return rows
+
+
+class SpatialOracle:
+    """The oracle of the spatial geometries R and XY (filament_oracle.h,
+    orc_spatial_*).  ``cfg`` is an OrcSpatialConfig, ``E0`` the complex
+    initial field ([Nr], or [Nx, Ny])."""
+
+    def __init__(self, cfg, E0, precision=8):
+        self.cfg = cfg
+        self.xy = cfg.geometry == GEOM_XY
+        self.shape = (cfg.Nx, cfg.Ny) if self.xy else (cfg.Nr,)
+        self.n = int(np.prod(self.shape))
+        e = np.ascontiguousarray(np.asarray(E0, dtype=np.complex128).ravel(order="F"))
+        self.h = lib().orc_spatial_create(ctypes.byref(cfg), _p(e.view(np.float64)), int(precision))
+        if not self.h:
+            raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_spatial_destroy(ctypes.c_void_p(self.h))
+            self.h = None
+
+    def _h(self):
+        return ctypes.c_void_p(self.h)
+
+    def field(self):
+        out = np.zeros(self.n, dtype=np.complex128)
+        _check(lib().orc_spatial_get_field(self._h(), _p(out.view(np.float64))))
+        return out.reshape(self.shape, order="F")
+
+    def set_field(self, E):
+        e = np.ascontiguousarray(np.asarray(E, dtype=np.complex128).ravel(order="F"))
+        _check(lib().orc_spatial_set_field(self._h(), _p(e.view(np.float64))))
+
+    def grid(self):
+        if self.xy:
+            a, b, c, d = np.zeros(self.cfg.Nx), np.zeros(self.cfg.Ny), np.zeros(self.cfg.Nx), np.zeros(self.cfg.Ny)
+            _check(lib().orc_spatial_get_grid(self._h(), _p(a), _p(b), _p(c), _p(d)))
+            return dict(x=a, y=b, kx=c, ky=d)
+        a, b, c = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        _check(lib().orc_spatial_get_grid(self._h(), _p(a), _p(b), _p(c), None))
+        return dict(r=a, dr=b, k=c)
+
+    def tables(self):
+        KZ, QZ = np.zeros(self.n, dtype=np.complex128), np.zeros(self.n, dtype=np.complex128)
+        Kg, Fg, s7 = np.zeros(self.n), np.zeros(self.n), np.zeros(7)
+        _check(lib().orc_spatial_get_tables(self._h(), _p(KZ.view(np.float64)), _p(QZ.view(np.float64)), _p(Kg),
+                                            _p(Fg), _p(s7)))
+        sh = lambda a: a.reshape(self.shape, order="F")
+        return dict(KZ=sh(KZ), QZ=sh(QZ), Kg=sh(Kg), Fg=sh(Fg), w0=s7[0], n0=s7[1], Eu=s7[2], vf=s7[3], k0=s7[4],
+                    phik=s7[5], Rnl=s7[6])
+
+    def zstep(self, dz):
+        _check(lib().orc_spatial_zstep(self._h(), float(dz)))
+
+    def nl_step(self, dz):
+        _check(lib().orc_spatial_nl_step(self._h(), float(dz)))
+
+    def linear_step(self, dz):
+        _check(lib().orc_spatial_linear_step(self._h(), float(dz)))
+
+    def analyze(self):
+        o = np.zeros(6)
+        _check(lib().orc_spatial_analyze(self._h(), _p(o)))
+        return dict(Imax=o[0], rfil=o[1], ax=o[2], ay=o[3], P=o[4], index=int(o[5]))
+
+    def run(self, nsteps, dz):
+        for _ in range(nsteps):
+            self.zstep(dz)
+        return self.field()

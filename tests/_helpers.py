@@ -88,3 +88,37 @@ def xyt_rows(E):
 
 def response_name(init):
     return load_pkg().responses.response_kind(init)
+
+
+def spatial_oracle(p, precision=8, E0=None):
+    """The oracle of an R or XY parameter dict (filament_amd.workloads.spatial_air
+    or driver.prepare output), with E0 the initial field on the grid (default:
+    p's initial_condition on the oracle's own grid)."""
+    o = load_oracle()
+    pk = load_pkg()
+    c = o.OrcSpatialConfig()
+    xy = p["geometry"] == "XY"
+    c.geometry = o.GEOM_XY if xy else o.GEOM_R
+    if xy:
+        c.Nx, c.Ny = p["Nx"], p["Ny"]
+        c.xmin, c.xmax, c.ymin, c.ymax, c.xu, c.yu = p["xmin"], p["xmax"], p["ymin"], p["ymax"], p["xu"], p["yu"]
+        c.xguard, c.yguard, c.kxguard, c.kyguard = p["xguard"], p["yguard"], p["kxguard"], p["kyguard"]
+    else:
+        c.Nr, c.rmax, c.ru = p["Nr"], p["rmax"], p["ru"]
+        c.rguard, c.kguard = p["rguard"], p["kguard"]
+    c.zu, c.Iu, c.lam0 = p["zu"], p["Iu"], p["lam0"]
+    c.medium = o.MEDIUM_AIR
+    c.n2 = p["n2"]
+    c.nonlinearity = int(bool(p["NONLINEARITY"]))
+    c.kparaxial, c.qparaxial = int(bool(p["KPARAXIAL"])), int(bool(p["QPARAXIAL"]))
+    c.alg = pk.responses.alg_order(p.get("ALG"))
+    if E0 is None:
+        if xy:
+            x = np.linspace(p["xmin"], p["xmax"], p["Nx"])
+            y = np.linspace(p["ymin"], p["ymax"], p["Ny"])
+            E0 = p["initial_condition"](x, y, p["xu"], p["yu"], p["Iu"])
+        else:
+            r = o.bessel_j0_zeros(p["Nr"] + 1)
+            r = r[:-1] * p["rmax"] / r[-1]
+            E0 = p["initial_condition"](r, p["ru"], p["Iu"])
+    return o.SpatialOracle(c, E0, precision)

@@ -30,11 +30,11 @@ def write_input_set(case, directory):
              _assign("file_medium", "medium.py"), _assign("geometry", geometry)]
     for group in ("units", "grid", "guards"):
         lines += [_assign(k, v) for k, v in c[group].items()]
-    lines += [_assign(k, m[k]) for k in ("zmax", "NONLINEARITY", "PLASMA", "Istop", "z")]
+    lines += [_assign(k, m[k]) for k in ("zmax", "NONLINEARITY", "PLASMA", "Istop", "z") if k in m]
     lines += [_assign(k, m[k]) for k in ("KPARAXIAL", "QPARAXIAL") if k in m]
     lines.append(_assign("dz_initial", m["zmax"] / m["steps"]))
     lines.append(_assign("dzphimax", math.pi * m["dzphimax_over_pi"]))
-    lines.append("ALG = RK3()\n")
+    lines.append(f"ALG = {m.get('ALG', 'RK3')}()\n")
     with open(os.path.join(directory, "input.py"), "w") as f:
         f.writelines(lines)
 
@@ -43,10 +43,15 @@ def write_input_set(case, directory):
         ic = f"gaussian_ic({q['a0']!r}, {q['tau0']!r}, {q['I0']!r}, {q['lam0']!r})"
     elif geometry == "T":
         ic = f"gaussian_ic_t({q['tau0']!r}, {q['I0']!r}, {q['lam0']!r})"
+    elif geometry == "R":
+        ic = f"gaussian_ic_r({q['a0']!r}, {q['I0']!r})"
+    elif geometry == "XY":
+        ic = f"gaussian_ic_xy({q['a0']!r}, {q['a0']!r}, {q['I0']!r})"
     else:
         ic = f"gaussian_ic_xyt({q['a0']!r}, {q['a0']!r}, {q['tau0']!r}, {q['I0']!r}, {q['lam0']!r})"
     with open(os.path.join(directory, "initial.py"), "w") as f:
-        f.write("from filament_amd.workloads import gaussian_ic, gaussian_ic_t, gaussian_ic_xyt\n")
+        f.write("from filament_amd.workloads import (gaussian_ic, gaussian_ic_r, gaussian_ic_t, gaussian_ic_xy,\n"
+                "                                    gaussian_ic_xyt)\n")
         f.write(_assign("lam0", q["lam0"]))
         f.write(f"initial_condition = {ic}\n")
 

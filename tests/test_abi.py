@@ -94,3 +94,46 @@ def test_errors_are_reported_without_a_device(pkg):
     d = pkg.lib.FilDesc()
     d.abi_version, d.geometry, d.Nx, d.Ny, d.Nt, d.precision, d.ncomp, d.batch = pkg.lib.ABI_VERSION, 1, 16, 16, 256, 4, 1, 2
     assert L.fil_create(ctypes.byref(d), ctypes.byref(h)) == 1 and b"batch" in L.fil_last_error()
+
+
+def _struct_fields(text, name):
+    body = text[text.index(f"typedef struct {name} {{"):text.index(f"}} {name};")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    out = []
+    for decl in re.findall(r"(?:int32_t|int64_t|double|const double\*)\s+([a-zA-Z0-9_, ]+);", body):
+        out += [f.strip() for f in decl.split(",")]
+    return out
+
+
+def test_spatial_structs_match_header(pkg):
+    """fil_spatial_desc / fil_spatial_tables / fil_diag_spatial field by field."""
+    text = open(HEADER).read()
+    L = pkg.lib
+    for cname, pyname in (("fil_spatial_desc", "FilSpatialDesc"), ("fil_spatial_tables", "FilSpatialTables"),
+                          ("fil_diag_spatial", "FilDiagSpatial")):
+        assert _struct_fields(text, cname) == [n for n, _ in getattr(L, pyname)._fields_], cname
+    assert ctypes.sizeof(L.FilSpatialDesc) == 10 * 4
+    assert ctypes.sizeof(L.FilDiagSpatial) == 6 * 8 + 2 * 8
+    assert ctypes.sizeof(L.FilSpatialTables) == 14 * 8 + 2 * 8 + 8 + 8
+    for name, val in (("FIL_GEOM_R", L.GEOM_R), ("FIL_GEOM_XY", L.GEOM_XY), ("FIL_ALG_RK3", L.ALG_RK3),
+                      ("FIL_ALG_RK4", L.ALG_RK4)):
+        assert int(re.search(rf"\b{name} = (\d+)", text).group(1)) == val
+
+
+def test_spatial_errors_are_reported_without_a_device(pkg):
+    L = pkg.lib.load()
+    h = ctypes.c_void_p()
+    assert L.fil_spatial_create(None, ctypes.byref(h)) == 1 and b"null" in L.fil_last_error()
+    d = pkg.lib.FilSpatialDesc()
+    d.abi_version, d.geometry, d.precision, d.alg, d.Nx, d.Ny = 99, pkg.lib.GEOM_XY, 4, 4, 64, 64
+    assert L.fil_spatial_create(ctypes.byref(d), ctypes.byref(h)) == 1 and b"ABI" in L.fil_last_error()
+    d.abi_version = pkg.lib.ABI_VERSION
+    d.geometry = pkg.lib.GEOM_RT
+    assert L.fil_spatial_create(ctypes.byref(d), ctypes.byref(h)) == 1 and b"geometry" in L.fil_last_error()
+    d.geometry, d.Nx = pkg.lib.GEOM_XY, 100
+    assert L.fil_spatial_create(ctypes.byref(d), ctypes.byref(h)) == 1 and b"power" in L.fil_last_error()
+    d.Nx, d.alg = 64, 2
+    assert L.fil_spatial_create(ctypes.byref(d), ctypes.byref(h)) == 1 and b"alg" in L.fil_last_error()
+    assert L.fil_spatial_zstep(None, 0.0, 1e-3) == 1
+    assert L.fil_spatial_upload_field(None, None) == 1
+    assert L.fil_spatial_destroy(None) == 0

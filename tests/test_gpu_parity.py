@@ -176,14 +176,15 @@ def test_raman_steps_match_oracle(pkg, orc, precision, tol, oracle_kz):
     assert err < tol and effect > 100 * tol
 
 
-def air_plasma(p):
+def air_plasma(p, kdep=True, nuc=5e12):
     """The plasma equation of examples/RT/air.jl:95-120: N2 and O2 with the
-    reference's multiphoton tables, K depending on intensity, avalanche on."""
+    reference's multiphoton tables, K depending on intensity, avalanche on
+    (nuc = 0 turns the avalanche term off: sigmaB = 0)."""
     import os
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference")
     tab = {g: tuple(np.loadtxt(os.path.join(gold, f"multiphoton_{g}.tf")).T) for g in ("N2", "O2")}
     q = dict(p)
-    q["plasma_equation"] = dict(p["plasma_equation"], KDEP=True, nuc=5e12, mr=1.0, rho_nt=2.5e25, components=[
+    q["plasma_equation"] = dict(p["plasma_equation"], KDEP=kdep, nuc=nuc, mr=1.0, rho_nt=2.5e25, components=[
         {"fraction": 0.79, "ionization_energy": 15.576, "ionization_rate": tab["N2"]},
         {"fraction": 0.21, "ionization_energy": 12.063, "ionization_rate": tab["O2"]}])
     return q
@@ -201,6 +202,25 @@ def test_plasma_air_avalanche_matches_oracle(pkg, orc, precision, tol):
     rho, kd = m.plasma_arrays()
     e1, e2 = rel_l2(rho, rho_o), rel_l2(kd, kd_o)
     print(f"air plasma precision={precision} rho {e1:.3e} kdrho {e2:.3e} max rho {rho_o.max():.3e}")
+    assert rho_o.max() > 0
+    assert e1 < tol and e2 < tol
+
+
+@pytest.mark.parametrize("kdep", [True, False])
+@pytest.mark.parametrize("precision,form,tol", [(4, 0, 2e-5), (4, 2, 2e-5), (8, 0, 1e-10)])
+def test_plasma_air_two_components_scan_matches_oracle(pkg, orc, kdep, precision, form, tol):
+    """N2 and O2 without avalanche: the block scan of affine maps, whose first
+    pass leaves both components' gains and rates for the second
+    (photoionization_avalanche.jl:119-205, plasma_equations.jl:27-72); with
+    the FP32 and the FP64 scan."""
+    p = air_plasma(small_rt(pkg, power=8.0, nonlinearity=False, plasma=True), kdep=kdep, nuc=0.0)
+    o = oracle_for(orc, p)
+    rho_o, kd_o = o.plasma()
+    m = pkg.RTModel(p, precision=precision, form=form, keep_plasma=True)
+    rho, kd = m.plasma_arrays()
+    e1, e2 = rel_l2(rho, rho_o), rel_l2(kd, kd_o)
+    print(f"air plasma, no avalanche, kdep={kdep} precision={precision} form={form} rho {e1:.3e} kdrho {e2:.3e} "
+          f"max rho {rho_o.max():.3e}")
     assert rho_o.max() > 0
     assert e1 < tol and e2 < tol
 

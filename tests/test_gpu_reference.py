@@ -1,6 +1,7 @@
 """The reference's analytic physics tests restated through prepare/run on the
 GPU: test/test_diffraction_rt.jl, test/test_dispersion_rt.jl and
-test/test_dispersion_t.jl with their input sets (written from
+test/test_dispersion_t.jl, test/test_diffraction_r.jl (its R part) and
+test/test_diffraction_xy.jl with their input sets (written from
 tests/golden/reference_inputs.json), plot.dat read back and compared with the
 Gaussian-beam / Gaussian-pulse closed forms at the reference's tolerances;
 and an XYT Gaussian beam (the closed form of test/test_diffraction_xy.jl) run
@@ -109,4 +110,41 @@ def test_diffraction_xyt(pkg, precision, tmp_path):
           f"max|ay-ath|={np.max(np.abs(ay - ath)):.3e} m (dx {dx:.3e}), max|I-Ith|={np.max(np.abs(I - Ith)):.3e}")
     assert len(z) == 101
     assert np.max(np.abs(ax - ath)) <= dx and np.max(np.abs(ay - ath)) <= dx
+    assert np.max(np.abs(I - Ith)) <= 1e-3
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_diffraction_r(pkg, precision, tmp_path):
+    """test/test_diffraction_r.jl, geometry R (Nr = 2000, 100 steps to 10 m):
+    plot.dat columns z, Imax, rfil, P; the 1/e radius within 0.02 mm of
+    a0 sqrt(1 + (z/zd)^2) and isapprox(I, (a0/a)^2, rtol=1e-3)."""
+    data = _run(pkg, "diffraction_r", precision, tmp_path)
+    z, I, a = data[0], data[1], data[2] * 1e-3
+    lam0, a0 = 800e-9, 1e-3
+    zd = 2 * math.pi / lam0 * a0 ** 2
+    ath = a0 * np.sqrt(1 + (z / zd) ** 2)
+    Ith = (a0 / ath) ** 2
+    dI = np.linalg.norm(I - Ith) / max(np.linalg.norm(I), np.linalg.norm(Ith))
+    print(f"diffraction_r precision={precision}: max|a-ath|={np.max(np.abs(a - ath)):.3e} m, "
+          f"|I-Ith|/|I|={dI:.3e}")
+    assert len(z) == 101
+    assert np.max(np.abs(a - ath)) <= 0.02e-3
+    assert dI <= 1e-3
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_diffraction_xy(pkg, precision, tmp_path):
+    """test/test_diffraction_xy.jl (Nx = Ny = 2048, 100 steps to 10 m):
+    plot.dat columns z, Imax, ax, ay, P; both radii within 0.01 mm of
+    a0 sqrt(1 + (z/zd)^2), the intensity within 1e-3 of (a0/a)^2."""
+    data = _run(pkg, "diffraction_xy", precision, tmp_path)
+    z, I, ax, ay = data[0], data[1], data[2] * 1e-3, data[3] * 1e-3
+    lam0, a0 = 800e-9, 1e-3
+    zd = 2 * math.pi / lam0 * a0 ** 2
+    ath = a0 * np.sqrt(1 + (z / zd) ** 2)
+    Ith = (a0 / ath) ** 2
+    print(f"diffraction_xy precision={precision}: max|ax-ath|={np.max(np.abs(ax - ath)):.3e} m, "
+          f"max|ay-ath|={np.max(np.abs(ay - ath)):.3e} m, max|I-Ith|={np.max(np.abs(I - Ith)):.3e}")
+    assert len(z) == 101
+    assert np.max(np.abs(ax - ath)) <= 0.01e-3 and np.max(np.abs(ay - ath)) <= 0.01e-3
     assert np.max(np.abs(I - Ith)) <= 1e-3
