@@ -173,7 +173,9 @@ def train_corrector(pkg, p, form, dz, factor, draws, npairs, iters, geom):
     def make(d):
         return pkg.workloads.rt_air(p["Nr"], p["Nt"], d["power_pcr"], a0=d["a0"], tau0=d["tau0"], seed=d["seed"])
 
-    st, info = C.fit(pkg.RTModel, make, draws, geom, dz, factor, form, npairs=npairs, iters=iters, log=log)
+    # pre-collapse pairs only (a trajectory ends once its window peak has grown threefold)
+    st, info = C.fit(pkg.RTModel, make, draws, geom, dz, factor, form, npairs=npairs, iters=iters, log=log,
+                     stop_growth=3.0)
     log(f"corrector: {info}")
     return st, info
 

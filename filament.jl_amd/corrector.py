@@ -214,31 +214,67 @@ def rms(a):
 
 
 def fit(RTModel, make_params, draws, geom, dz, factor, form, npairs=32, spacing=None, iters=2000, batch=16,
-        lr=1e-3, seed=0, log=None):
+        lr=1e-3, seed=0, log=None, max_pairs=None, stop_growth=None):
     """Gathers FP64-target pairs along the trajectory of every draw and trains
     a corrector on the device.  Returns (state, info); ``state`` loads into any
-    Corrector with the same window geometry."""
+    Corrector with the same window geometry.
+
+    The draws share the grid and medium of the first one (a draw changes the
+    pulse: power, radius, duration, noise), so one set of models serves all of
+    them: each draw uploads its initial field (``RTModel.reset_field``).  Pairs
+    go to the device as they are made, every ``spacing`` fine steps, up to
+    ``npairs`` per draw and ``max_pairs`` in all (default: npairs per draw);
+    ``stop_growth``: a trajectory ends once the window's peak intensity
+    feature exceeds that multiple of its first value (past the collapse a
+    coarse step is no longer a small correction away from the fine steps);
+    it also ends at a pair with a non-finite value."""
     import time
     from . import lib as _L
     spacing = spacing or 2 * factor
+    cap = int(max_pairs or npairs * len(draws))
+    batch = min(batch, cap)
     t0 = time.perf_counter()
-    pairs = []
-    for d in draws:
-        ps, _ = gather_pairs_fp64(RTModel, make_params(d), geom, dz, factor, npairs, spacing, form)
-        pairs += ps
-        if log:
-            log(f"corrector data: draw {d}: {len(ps)} pairs, target rms {rms(np.array([T for _, T in ps])):.3e}")
-    t1 = time.perf_counter()
-    batch = min(batch, len(pairs))
-    tm = RTModel(make_params(draws[0]), precision=_L.FP32, form=form)
+    p0 = make_params(draws[0])
+    models = [RTModel(p0, precision=_L.FP32, form=form), RTModel(p0, precision=_L.FP64),
+              RTModel(p0, precision=_L.FP32, form=form), RTModel(p0, precision=_L.FP32, form=form)]
+    m32, m64, mp, tm = models
+    stored, sq, ndraws = 0, 0.0, 0
     try:
-        tc = Corrector(tm, geom=geom, capacity=len(pairs), batch_max=batch, seed=seed)
-        for X, T in pairs:
-            tc.store_pair(X, T)
-        loss = tc.train(iters, batch=batch, lr=lr, seed=seed)
+        c32 = Corrector(m32, geom=geom, capacity=1, batch_max=1)
+        probe = Corrector(mp, geom=geom, capacity=1, batch_max=1)
+        tc = Corrector(tm, geom=geom, capacity=cap, batch_max=batch, seed=seed)
+        for d in draws:
+            if stored >= cap:
+                break
+            ndraws += 1
+            m32.reset_field(make_params(d)["initial_condition"])
+            z, n, first, dsq = 0.0, 0, None, 0.0
+            while n < npairs and stored < cap:
+                Xc, T = capture_vs_fp64(m32, c32, m64, probe, z, dz, factor)
+                if not (np.isfinite(Xc).all() and np.isfinite(T).all()):
+                    break                      # a step blew up: the trajectory ends here
+                peak = float(Xc[2].max())
+                first = peak if first is None else first
+                if stop_growth and peak > stop_growth * first:
+                    break
+                tc.store_pair(Xc, T)
+                stored += 1
+                n += 1
+                dsq += float(np.sum(np.square(T))) / T.size
+                z += factor * dz
+                for _k in range(spacing - factor):
+                    m32.zstep(z + dz, dz)
+                    z += dz
+            sq += dsq
+            if log:
+                log(f"corrector data: draw {d}: {n} pairs, target rms {math.sqrt(dsq / max(n, 1)):.3e}")
+        t1 = time.perf_counter()
+        loss = tc.train(iters, batch=min(batch, stored), lr=lr, seed=seed)
         st = tc.state()
     finally:
-        tm.close()
+        for m in models:
+            m.close()
     t2 = time.perf_counter()
-    info = dict(pairs=len(pairs), gather_s=t1 - t0, train_s=t2 - t1, last_batch_loss=loss, iters=iters)
+    info = dict(pairs=stored, draws=ndraws, target_rms=math.sqrt(sq / max(stored, 1)), gather_s=t1 - t0,
+                train_s=t2 - t1, last_batch_loss=loss, iters=iters)
     return st, info

@@ -1935,7 +1935,8 @@ int fil_corrector_train(fil_handle* h, int iters, int batch, double lr, uint64_t
     double s = 0;
     for (double v : part) s += v;
     const double rms = std::sqrt(s / ((double)c.count * 2 * HW));
-    if (!(rms > 0) || !std::isfinite(rms)) return fail(FIL_ESTATE, "stored targets are all zero");
+    if (!std::isfinite(rms)) return fail(FIL_ESTATE, "stored targets hold non-finite values");
+    if (!(rms > 0)) return fail(FIL_ESTATE, "stored targets are all zero");
     c.y_scale = rms;
   }
   // batch indices of every iteration, drawn without replacement within a batch

@@ -289,6 +289,15 @@ class RTModel(_Model):
         self._set_tables()
         self.upload(self.E0)
 
+    def reset_field(self, initial_condition):
+        """A new pulse on the same grid, medium and tables: the field of
+        ``initial_condition(r, t, ru, tu, Iu)`` (Fields.jl:96-120) replaces the
+        state (the pending filter and correction are dropped, the plasma is
+        solved again from it)."""
+        g, u = self.grid, self.unit
+        self.E0 = ph.rsig2asig(np.asarray(initial_condition(g.r, g.t, u.r, u.t, u.I), dtype=float))
+        self.upload(self.E0)
+
     def _set_tables(self):
         g = self.grid
         keep = []
