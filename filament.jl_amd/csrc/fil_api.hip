@@ -137,7 +137,7 @@ struct fil_handle {
     double y_scale = 0;
     int64_t adam_t = 0;
     DevBuf w, wbf, m, v, grad;        // flat parameters, BF16 copy, Adam moments, gradient
-    DevBuf wfm;                       // FP32 forward layout of the BF16 weights
+    DevBuf wfr;                       // the BF16 weights in MFMA operand order
     DevBuf X, Y;                      // stored pairs [cap][4][HW], [cap][2][HW]
     DevBuf Frow, feat, carrier;       // feature scratch, features [4][HW], exp(i w0 t_j) [Nt]
     DevBuf act[3], out, gA, gB, gout; // training activations and gradients [bmax][..][HW]
@@ -787,25 +787,16 @@ double normal01(uint64_t& s) {
 
 int corr_bias_off(int l) { return corr_woff(l) + corr_cout(l) * corr_cin(l) * 9; }
 
-// network forward over B samples of `in` (gathered by idx when given) into corr.out
-// in: pooled features [B][4][H][W] (idx selects stored samples), or with
-// from_rows the per-row features Frow of one sample (layer 0 pools them)
-int corr_forward(fil_handle* h, const float* in, const int* idx, int B, bool from_rows = false) {
+// network forward over B samples of `in` (gathered by idx when given) into
+// corr.out, keeping the activations for a backward pass (train); in: pooled
+// features [B][4][H][W] (idx selects stored samples), or with from_rows the
+// per-row features Frow of one sample (pooled as the kernel reads them)
+int corr_forward(fil_handle* h, const float* in, const int* idx, int B, bool from_rows = false, bool train = false) {
   auto& c = h->corr;
-  const float* w = c.w.as<float>();
-  const float* wf = c.wfm.as<float>();
-  const float* src = in;
-  const int* sidx = idx;
-  for (int l = 0; l < CORR_LAYERS; l++) {
-    float* dst = (l == CORR_LAYERS - 1) ? c.out.as<float>() : c.act[l].as<float>();
-    if (l == 0 && from_rows)
-      HIPCHK(launch_conv_fwd_pooled(src, c.g.pr, wf, w + corr_bias_off(0), dst, c.g.H, c.g.W, h->st));
-    else
-      HIPCHK(launch_conv_fwd(l, src, sidx, wf + corr_wfm_off(l), w + corr_bias_off(l), dst, B, c.g.H, c.g.W,
-                             l < CORR_LAYERS - 1 ? 1 : 0, h->st));
-    src = dst;
-    sidx = nullptr;
-  }
+  float* act[CORR_LAYERS - 1];
+  for (int l = 0; l < CORR_LAYERS - 1; l++) act[l] = c.act[l].as<float>();
+  HIPCHK(launch_conv_net(in, idx, from_rows ? c.g.pr : 0, c.wfr.as<uint16_t>(), c.w.as<float>(),
+                         train ? act : nullptr, c.out.as<float>(), B, c.g.H, c.g.W, h->st));
   return 0;
 }
 
@@ -1818,7 +1809,7 @@ int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
   c.bmax = cd->batch_max;
   const size_t HW = (size_t)H * W, NP = FIL_CORR_NPARAM, B = c.bmax;
   bool ok = c.w.alloc(NP * 4) == hipSuccess && c.wbf.alloc(NP * 2) == hipSuccess && c.m.alloc(NP * 4) == hipSuccess &&
-            c.wfm.alloc((size_t)corr_wfm_size() * 4) == hipSuccess &&
+            c.wfr.alloc((size_t)corr_frag_size() * 2) == hipSuccess &&
             c.v.alloc(NP * 4) == hipSuccess && c.grad.alloc(NP * 4) == hipSuccess &&
             c.X.alloc(std::max<size_t>(1, (size_t)c.cap) * 4 * HW * 4) == hipSuccess &&
             c.Y.alloc(std::max<size_t>(1, (size_t)c.cap) * 2 * HW * 4) == hipSuccess &&
@@ -1851,7 +1842,7 @@ int fil_corrector_init(fil_handle* h, const fil_corr_desc* cd, uint64_t seed) {
   HIPCHK(hipMemsetAsync(c.m.p, 0, NP * 4, h->st));
   HIPCHK(hipMemsetAsync(c.v.p, 0, NP * 4, h->st));
   HIPCHK(launch_round_bf16(c.w.as<float>(), c.wbf.as<uint16_t>(), (int)NP, h->st));
-  HIPCHK(launch_conv_weight_layout(c.wbf.as<uint16_t>(), c.wfm.as<float>(), h->st));
+  HIPCHK(launch_conv_weight_layout(c.wbf.as<uint16_t>(), c.wfr.as<uint16_t>(), h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   c.on = true;
   return FIL_OK;
@@ -1870,7 +1861,7 @@ int fil_corrector_params(fil_handle* h, float* w, int set) {
     HIPCHK(hipMemsetAsync(c.m.p, 0, NP * 4, h->st));
     HIPCHK(hipMemsetAsync(c.v.p, 0, NP * 4, h->st));
     HIPCHK(launch_round_bf16(c.w.as<float>(), c.wbf.as<uint16_t>(), (int)NP, h->st));
-    HIPCHK(launch_conv_weight_layout(c.wbf.as<uint16_t>(), c.wfm.as<float>(), h->st));
+    HIPCHK(launch_conv_weight_layout(c.wbf.as<uint16_t>(), c.wfr.as<uint16_t>(), h->st));
     c.adam_t = 0;
   } else {
     HIPCHK(hipMemcpyAsync(w, c.w.p, NP * 4, hipMemcpyDeviceToHost, h->st));
@@ -1958,7 +1949,7 @@ int fil_corrector_train(fil_handle* h, int iters, int batch, double lr, uint64_t
   int nblk = 0;
   for (int it = 0; it < iters; it++) {
     const int* idx = dids.as<int>() + (size_t)it * batch;
-    if (int rc = corr_forward(h, c.X.as<float>(), idx, batch)) return rc;
+    if (int rc = corr_forward(h, c.X.as<float>(), idx, batch, false, true)) return rc;
     if (int rc = corr_loss(h, idx, batch, &nblk)) return rc;
     if (int rc = corr_backward(h, c.X.as<float>(), idx, batch)) return rc;
     c.adam_t++;
@@ -1966,7 +1957,7 @@ int fil_corrector_train(fil_handle* h, int iters, int batch, double lr, uint64_t
     const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)c.adam_t)));
     HIPCHK(launch_adam(c.w.as<float>(), c.grad.as<float>(), c.m.as<float>(), c.v.as<float>(), c.wbf.as<uint16_t>(),
                        FIL_CORR_NPARAM, (float)lr, b1, b2, eps, c1, c2, h->st));
-    HIPCHK(launch_conv_weight_layout(c.wbf.as<uint16_t>(), c.wfm.as<float>(), h->st));
+    HIPCHK(launch_conv_weight_layout(c.wbf.as<uint16_t>(), c.wfr.as<uint16_t>(), h->st));
   }
   if (int rc = corr_read_loss(h, nblk, batch, loss)) return rc;
   return FIL_OK;
@@ -1983,7 +1974,7 @@ int fil_corrector_eval(fil_handle* h, int first, int count, double* loss, float*
   for (int q = 0; q < count; q++) ids[q] = first + q;
   HIPCHK(hipMemcpyAsync(c.idx.p, ids.data(), count * 4, hipMemcpyHostToDevice, h->st));
   int nblk = 0;
-  if (int rc = corr_forward(h, c.X.as<float>(), c.idx.as<int>(), count)) return rc;
+  if (int rc = corr_forward(h, c.X.as<float>(), c.idx.as<int>(), count, false, grad != nullptr)) return rc;
   if (int rc = corr_loss(h, c.idx.as<int>(), count, &nblk)) return rc;
   if (grad) {
     if (int rc = corr_backward(h, c.X.as<float>(), c.idx.as<int>(), count)) return rc;

@@ -5,9 +5,10 @@
 // Layer l maps [B][cin][H][W] -> [B][cout][H][W] with zero padding:
 //   out[co][y][x] = b[co] + sum_ci sum_{dy,dx in -1..1} w[co][ci][(dy+1)*3+(dx+1)] in[ci][y+dy][x+dx]
 // Weights are read as BF16 (the rounded copy of the FP32 master parameters)
-// and every product is accumulated in FP32.  The forward pass runs on the
-// matrix cores (FP32 MFMA); the backward passes own one pixel (data gradient)
-// or one weight (weight gradient) per thread on the vector ALUs.
+// and every product is accumulated in FP32.  The forward pass runs the four
+// layers in one launch on the BF16 matrix cores (exact products, below); the
+// backward passes own one pixel (data gradient) or one weight (weight
+// gradient) per thread on the vector ALUs.
 #include <algorithm>
 
 #include "fil_internal.h"
@@ -22,125 +23,292 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(u >> 16);
 }
 
-constexpr int CT = 128;   // threads per block of the convolution kernels
-constexpr int CG = 8;     // channels per thread in the forward / data-gradient kernels
+constexpr int CT = 128;   // threads per block of the backward convolution kernels
+constexpr int CG = 8;     // channels per thread in the data-gradient kernel
 
-// Forward on the matrix cores: the layer is an implicit GEMM
-//   out[co][p] = b[co] + sum_k A[co][k] X[k][p],   k = tap * CIN + ci,
-//   A[co][k] = w[co][ci][tap] (BF16 values, exact in FP32),
-//   X[k][p]  = in[ci][p + d(tap)]  (zero outside the image),
-// on v_mfma_f32_16x16x4f32: FP32 operands, FP32 accumulation.  A block owns an
-// MY x MX pixel tile of one sample and 16 output channels (blockIdx.z): their
-// weight rows (copied as they are from the forward layout, whose row stride
-// puts the 16 rows of an MFMA operand in distinct LDS banks) and the input
-// tile with its one-pixel halo sit in LDS; wave w computes the 16 pixels of
-// row w / 2, columns (w % 2) * 16 .. + 15.  k runs tap-major so a 4-wide k
-// step is four consecutive input channels at one tap.
+// ---------------------------------------------------------------------------
+// Forward of the whole network in one launch, on v_mfma_f32_16x16x32_bf16.
+//
+// A workgroup owns a TY x TX tile of output pixels of one sample and carries
+// it through the four layers in LDS: layer l (1..4) is computed on the tile
+// grown by 4 - l pixels on every side (the region the later layers read), so
+// no layer waits for another workgroup; pixels of a region outside the image
+// are zero, which is the zero padding every layer sees.  Regions alternate
+// between two LDS buffers (region 0 = input, 2 in buffer 0; 1, 3 in buffer 1).
+//
+// Exact products on BF16 matrix cores: every FP32 activation x is stored as
+// three BF16 values x = hi + mid + lo (hi = bf16(x), mid = bf16(x - hi),
+// lo = x - hi - mid, exact: the three take the 24 significand bits eight at a
+// time, and both differences are exact by Sterbenz's lemma), and the weights
+// are BF16 values, so each of the three MFMA terms is an exact product summed
+// in FP32: the layer is an FP32 convolution up to the order of the sums.
+//
+// Layer l as a GEMM: out[co][p] = b[co] + sum_k A[co][k] X[k][p], k = tap cin + ci;
+// one MFMA covers 16 output channels x 16 pixels x 32 k: for cin = 32 one
+// k chunk is one tap (all 32 channels); for the 4-channel input two chunks
+// cover the 36 k (taps 2 kg, 2 kg + 1 of k group kg).  Each wave takes 16-pixel
+// tiles of the region in turn and keeps the layer's weights in registers.
+// ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-constexpr int MY = 2, MX = 32;             // pixel tile of the forward kernel (one-row tiles measured slower)
-constexpr int MT = 64 * (MY * MX / 16);    // threads: one wave per 16 pixels
-// channel stride of the staged input tile: = 16 (mod 32), so the two k rows a
-// half-wave reads (adjacent channels) fall in different LDS banks
-constexpr int conv_x_stride() { return ((MY + 2) * (MX + 2) + 15) / 32 * 32 + 16; }
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// POOL: the input is the per-row feature array Frow [H pr][CIN][W] of one
-// sample, averaged over the pr rows of each pooled row while it is staged
-// (the same sums, in the same order, as corr_pool_rows_kernel).
-template <int CIN, int COUT, bool POOL = false>
-__global__ __launch_bounds__(MT) void conv_fwd_kernel(const float* __restrict__ in, const int* __restrict__ idx,
-                                                      const float* __restrict__ wfm, const float* __restrict__ bias,
-                                                      float* __restrict__ out, int H, int W, int relu, int pr = 1) {
-  static_assert(CIN % 4 == 0, "a k step covers four input channels");
-  constexpr int K = 9 * CIN, AS = (K + 31) / 32 * 32 + 2;
-  constexpr int HY = MY + 2, HX = MX + 2, XS = conv_x_stride();
-  static_assert((16 * AS) % 4 == 0, "weight rows are copied as float4");
-  __shared__ __align__(16) float wl[16 * AS];
-  __shared__ float tin[CIN * XS];
-  const int tid = threadIdx.x, mb = blockIdx.z;
-  const int HW = H * W;
-  const int b = blockIdx.y;
-  const int ntx = (W + MX - 1) / MX;
-  const int y0 = (blockIdx.x / ntx) * MY, x0 = (blockIdx.x % ntx) * MX;
-  const float* inb = in + (size_t)(idx ? idx[b] : b) * CIN * HW;
-  // staging: every global load of the thread is issued before the first LDS
-  // store, so their latencies overlap
-  constexpr int NW4 = 16 * AS / 4, PW = (NW4 + MT - 1) / MT;
-  constexpr int NIN = CIN * HY * HX, PI = (NIN + MT - 1) / MT;
-  const float4* src = reinterpret_cast<const float4*>(wfm + (size_t)mb * 16 * AS);
-  float4 wreg[PW];
-  float xv_[PI];
+constexpr int NET_T = 512;    // threads per workgroup (8 waves)
+// bytes per pixel of a 32-channel region: planes hi, mid, lo of 64 bytes and
+// 32 bytes of padding; 224 = 14 x 16 keeps the operand reads (16 pixels x 4
+// 16-byte channel groups per instruction) free of bank conflicts
+constexpr int NET_PS = 224;
+constexpr int NET_PS0 = 32;   // input region: planes of 4 channels (8 bytes each) and 8 bytes of padding
+
+__host__ __device__ constexpr int net_ry(int TY, int r) { return TY + 2 * (CORR_LAYERS - r); }
+__host__ __device__ constexpr int net_buf0(int TY, int TX) {
+  const int r0 = net_ry(TY, 0) * net_ry(TX, 0) * NET_PS0, r2 = net_ry(TY, 2) * net_ry(TX, 2) * NET_PS;
+  return r0 > r2 ? r0 : r2;
+}
+__host__ __device__ constexpr int net_buf1(int TY, int TX) {
+  const int r1 = net_ry(TY, 1) * net_ry(TX, 1) * NET_PS, r3 = net_ry(TY, 3) * net_ry(TX, 3) * NET_PS;
+  return r1 > r3 ? r1 : r3;
+}
+
+// the network's weights (fragment layout) and biases, staged once per workgroup
+constexpr int NET_WBYTES = corr_frag_size() * 2;
+__host__ __device__ constexpr int net_bias_off(int l) {
+  int o = 0;
+  for (int k = 0; k < l; k++) o += corr_cout(k);
+  return o;
+}
+constexpr int NET_BBYTES = (net_bias_off(CORR_LAYERS) * 4 + 15) / 16 * 16;
+__host__ __device__ constexpr int net_smem(int TY, int TX) {
+  return NET_WBYTES + NET_BBYTES + net_buf0(TY, TX) + net_buf1(TY, TX);
+}
+
+__device__ __forceinline__ void split3(float x, uint16_t* s) {
+  s[0] = f2bf(x);
+  const float r1 = x - bf2f(s[0]);
+  s[1] = f2bf(r1);
+  s[2] = f2bf(r1 - bf2f(s[1]));
+}
+__device__ __forceinline__ uint2 pack4(const uint16_t* v) {
+  return make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+}
+
+struct NetArgs {
+  const float* in;
+  const int* idx;
+  int pr;
+  const uint16_t* wfr;
+  const float* params;
+  float* act[CORR_LAYERS - 1];
+  float* out;
+  int H, W;
+};
+
+// layer L (1..4): region L-1 in `src` -> region L in `dst` (L < 4) or the
+// output (L = 4); (y0, x0): the image position of the output tile
+template <int L, int TY, int TX>
+__device__ __forceinline__ void net_layer(const unsigned char* src, unsigned char* dst, const bf16x8* wl,
+                                          const float* bl, const NetArgs& a, int b, int y0, int x0) {
+  constexpr int l = L - 1, CIN = corr_cin(l), COUT = corr_cout(l);
+  constexpr int NG = corr_frag_ng(l), NC = corr_frag_nc(l);
+  constexpr int RY = net_ry(TY, L), RX = net_ry(TX, L), RXI = RX + 2, G = CORR_LAYERS - L;
+  constexpr int NPX = RY * RX, NTILE = (NPX + 15) / 16;
+  constexpr int PSI = (L == 1) ? NET_PS0 : NET_PS;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, n = lane & 15, kg = lane >> 4;
+  const int H = a.H, W = a.W;
+  bf16x8 A[NG][NC];
+  {
+    const bf16x8* wf = wl + corr_frag_off(l) * 64 + lane;
 #pragma unroll
-  for (int j = 0; j < PW; j++) {
-    const int q = tid + j * MT;
-    wreg[j] = src[q < NW4 ? q : NW4 - 1];   // clamped: the extra copies are not stored
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) A[g][c] = wf[(g * NC + c) * 64];
+    // held in registers for the whole layer (the compiler would otherwise
+    // re-read them from LDS next to every MFMA and wait on each read)
+#pragma unroll
+    for (int g = 0; g < NG; g++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) asm volatile("" : "+v"(A[g][c]));
   }
+  float bias[NG][4];
 #pragma unroll
-  for (int j = 0; j < PI; j++) {
-    const int e = tid + j * MT;
-    const int ci = e / (HY * HX), r = e % (HY * HX), yy = y0 - 1 + r / HX, xx = x0 - 1 + r % HX;
-    const bool inside = e < NIN && yy >= 0 && yy < H && xx >= 0 && xx < W;
-    if constexpr (POOL) {
-      float s = 0.0f;
-      if (inside) {
-        const float* src = in + ((size_t)yy * pr * CIN + ci) * W + xx;
-#pragma unroll 8
-        for (int k = 0; k < pr; k++) s += src[(size_t)k * CIN * W];
-      }
-      xv_[j] = inside ? s / (float)pr : 0.0f;
-    } else {
-      xv_[j] = inside ? inb[(size_t)ci * HW + yy * W + xx] : 0.0f;
+  for (int g = 0; g < NG; g++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int co = 16 * g + 4 * kg + i;
+      bias[g][i] = co < COUT ? bl[net_bias_off(l) + co] : 0.0f;
     }
-  }
+  for (int t = wv; t < NTILE; t += NET_T / 64) {
+    const int p = 16 * t + n;
+    const int pc = p < NPX ? p : NPX - 1;       // lanes past the region read a valid pixel, store nothing
+    const int pin = (pc / RX) * RXI + pc % RX;   // input pixel of tap 0 (dy = dx = -1)
+    floatx4 acc[NG];
 #pragma unroll
-  for (int j = 0; j < PW; j++) {
-    const int q = tid + j * MT;
-    if (q < NW4) reinterpret_cast<float4*>(wl)[q] = wreg[j];
-  }
+    for (int g = 0; g < NG; g++) acc[g] = {0, 0, 0, 0};
 #pragma unroll
-  for (int j = 0; j < PI; j++) {
-    const int e = tid + j * MT;
-    if (e < NIN) tin[(e / (HY * HX)) * XS + e % (HY * HX)] = xv_[j];
-  }
-  __syncthreads();
-  const int wv = tid >> 6, lane = tid & 63, l16 = lane & 15, l4 = lane >> 4;
-  const int py = wv >> 1, px = (wv & 1) * 16 + l16;
-  // two interleaved accumulation chains (even and odd k steps), fully
-  // unrolled, so the LDS operand reads run ahead of the MFMA chain
-  floatx4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-  const float* ar = wl + l16 * AS + l4;
-  const float* xr = tin + l4 * XS + py * HX + px;
+    for (int c = 0; c < NC; c++) {
+      bf16x8 B[3];
+      if constexpr (L == 1) {
+        // k = 32 c + 8 kg + j: taps ta = 8 c + 2 kg (j < 4) and ta + 1 (j >= 4), channel j % 4
+        const int ta = 8 * c + 2 * kg, tb = ta + 1;
+        const unsigned char* pa = src + (pin + (ta / 3) * RXI + ta % 3) * PSI;
+        const unsigned char* pb = src + (pin + (tb / 3) * RXI + tb % 3) * PSI;
 #pragma unroll
-  for (int s = 0; s < K / 4; s++) {
-    const int tap = s / (CIN / 4), c4 = (4 * s) % CIN;
-    const float xv = xr[c4 * XS + (tap / 3) * HX + tap % 3];
-    if (s & 1) acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[4 * s], xv, acc1, 0, 0, 0);
-    else acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[4 * s], xv, acc0, 0, 0, 0);
-  }
-  const floatx4 acc = acc0 + acc1;
-  const int y = y0 + py, x = x0 + px;
-  if (y >= H || x >= W) return;
-  float* ob = out + (size_t)b * COUT * HW + (size_t)y * W + x;
+        for (int q = 0; q < 3; q++) {
+          const uint2 va = ta < 9 ? *reinterpret_cast<const uint2*>(pa + 8 * q) : make_uint2(0, 0);
+          const uint2 vb = tb < 9 ? *reinterpret_cast<const uint2*>(pb + 8 * q) : make_uint2(0, 0);
+          const u32x4 v = {va.x, va.y, vb.x, vb.y};
+          B[q] = __builtin_bit_cast(bf16x8, v);
+        }
+      } else {
+        const unsigned char* pp = src + (pin + (c / 3) * RXI + c % 3) * PSI + 16 * kg;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int co = mb * 16 + 4 * l4 + i;
-    if (co < COUT) {
-      const float v = acc[i] + bias[co];
-      ob[(size_t)co * HW] = relu ? fmaxf(v, 0.0f) : v;
+        for (int q = 0; q < 3; q++) B[q] = *reinterpret_cast<const bf16x8*>(pp + 64 * q);
+      }
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int g = 0; g < NG; g++) acc[g] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[g][c], B[q], acc[g], 0, 0, 0);
+    }
+    if (p >= NPX) continue;
+    const int ry = p / RX, rx = p % RX, yy = y0 - G + ry, xx = x0 - G + rx;
+    const bool inside = yy >= 0 && yy < H && xx >= 0 && xx < W;
+    if constexpr (L == CORR_LAYERS) {
+      if (inside && kg == 0) {
+        float* o = a.out + (size_t)b * COUT * H * W + (size_t)yy * W + xx;
+#pragma unroll
+        for (int i = 0; i < COUT; i++) o[(size_t)i * H * W] = acc[0][i] + bias[0][i];
+      }
+    } else {
+      // the tile's own pixels of this layer go to act[l] (one workgroup owns each)
+      const bool own = a.act[l] && inside && ry >= G && ry < G + TY && rx >= G && rx < G + TX;
+      float* ap = own ? a.act[l] + ((size_t)b * COUT + 4 * kg) * H * W + (size_t)yy * W + xx : nullptr;
+#pragma unroll
+      for (int g = 0; g < NG; g++) {
+        uint16_t s[4][3];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const float v = inside ? fmaxf(acc[g][i] + bias[g][i], 0.0f) : 0.0f;
+          split3(v, s[i]);
+          if (own) ap[(size_t)(16 * g + i) * H * W] = v;
+        }
+        unsigned char* dp = dst + p * NET_PS + 2 * (16 * g + 4 * kg);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          const uint16_t v4[4] = {s[0][q], s[1][q], s[2][q], s[3][q]};
+          *reinterpret_cast<uint2*>(dp + 64 * q) = pack4(v4);
+        }
+      }
     }
   }
 }
 
-// wfm of every layer from the flat BF16 parameters (one thread per entry)
-__global__ void conv_weight_layout_kernel(const uint16_t* __restrict__ wbf, float* __restrict__ wfm) {
+// LDS admits one workgroup per CU (two waves per SIMD), so the register
+// budget is that of two waves
+template <int TY, int TX>
+__global__ __launch_bounds__(NET_T) __attribute__((amdgpu_waves_per_eu(1, 2))) void conv_net_kernel(NetArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  bf16x8* wl = reinterpret_cast<bf16x8*>(smem);
+  float* bl = reinterpret_cast<float*>(smem + NET_WBYTES);
+  unsigned char* buf0 = smem + NET_WBYTES + NET_BBYTES;
+  unsigned char* buf1 = buf0 + net_buf0(TY, TX);
+  const int tid = threadIdx.x, b = blockIdx.y;
+  const int H = a.H, W = a.W, HW = H * W;
+  const int ntx = (W + TX - 1) / TX;
+  const int y0 = (blockIdx.x / ntx) * TY, x0 = (blockIdx.x % ntx) * TX;
+  constexpr int RY0 = net_ry(TY, 0), RX0 = net_ry(TX, 0), NE = (RY0 * RX0 + NET_T - 1) / NET_T;
+  constexpr int NW16 = NET_WBYTES / 16, NWL = (NW16 + NET_T - 1) / NET_T;
+  // every global load of the staging is issued before the first LDS store:
+  // the weights and biases of all four layers, and region 0 (the four input
+  // channels of the tile and its 4-pixel halo; Frow rows pooled on the way)
+  uint4 wreg[NWL];
+  const uint4* wsrc = reinterpret_cast<const uint4*>(a.wfr);
+#pragma unroll
+  for (int j = 0; j < NWL; j++) {
+    const int q = tid + j * NET_T;
+    wreg[j] = wsrc[q < NW16 ? q : NW16 - 1];   // clamped: the extra copies are not stored
+  }
+  float bv = 0.0f;
+  if (tid < net_bias_off(CORR_LAYERS)) {
+    int l = 0;
+    while (l + 1 < CORR_LAYERS && tid >= net_bias_off(l + 1)) l++;
+    bv = a.params[corr_woff(l) + corr_cout(l) * corr_cin(l) * 9 + tid - net_bias_off(l)];
+  }
+  const float* inb = a.in + (size_t)(a.idx ? a.idx[b] : b) * CORR_IN * HW;
+  float v[NE][CORR_IN];
+#pragma unroll
+  for (int u = 0; u < NE; u++) {
+    const int e = tid + u * NET_T;
+    const int yy = y0 - CORR_LAYERS + e / RX0, xx = x0 - CORR_LAYERS + e % RX0;
+#pragma unroll
+    for (int ci = 0; ci < CORR_IN; ci++) v[u][ci] = 0.0f;
+    if (e < RY0 * RX0 && yy >= 0 && yy < H && xx >= 0 && xx < W) {
+      if (a.pr > 0) {
+        // mean over the pr feature rows of pooled row yy (Frow [H pr][4][W]),
+        // eight rows' loads in flight at a time
+        const int pr = a.pr;
+        const float* s = a.in + (size_t)yy * pr * CORR_IN * W + xx;
+        for (int k0 = 0; k0 < pr; k0 += 8) {
+          float t[8][CORR_IN];
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int ci = 0; ci < CORR_IN; ci++)
+              t[k][ci] = (k0 + k < pr) ? s[((size_t)(k0 + k) * CORR_IN + ci) * W] : 0.0f;
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int ci = 0; ci < CORR_IN; ci++) v[u][ci] += t[k][ci];
+        }
+#pragma unroll
+        for (int ci = 0; ci < CORR_IN; ci++) v[u][ci] /= (float)pr;
+      } else {
+#pragma unroll
+        for (int ci = 0; ci < CORR_IN; ci++) v[u][ci] = inb[(size_t)ci * HW + yy * W + xx];
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NWL; j++) {
+    const int q = tid + j * NET_T;
+    if (q < NW16) reinterpret_cast<uint4*>(wl)[q] = wreg[j];
+  }
+  if (tid < net_bias_off(CORR_LAYERS)) bl[tid] = bv;
+#pragma unroll
+  for (int u = 0; u < NE; u++) {
+    const int e = tid + u * NET_T;
+    if (e < RY0 * RX0) {
+      uint16_t s[CORR_IN][3];
+#pragma unroll
+      for (int ci = 0; ci < CORR_IN; ci++) split3(v[u][ci], s[ci]);
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const uint16_t v4[4] = {s[0][q], s[1][q], s[2][q], s[3][q]};
+        *reinterpret_cast<uint2*>(buf0 + e * NET_PS0 + 8 * q) = pack4(v4);
+      }
+    }
+  }
+  __syncthreads();
+  net_layer<1, TY, TX>(buf0, buf1, wl, bl, a, b, y0, x0);
+  __syncthreads();
+  net_layer<2, TY, TX>(buf1, buf0, wl, bl, a, b, y0, x0);
+  __syncthreads();
+  net_layer<3, TY, TX>(buf0, buf1, wl, bl, a, b, y0, x0);
+  __syncthreads();
+  net_layer<4, TY, TX>(buf1, nullptr, wl, bl, a, b, y0, x0);
+}
+
+// wfr of every layer from the flat BF16 parameters (one thread per value)
+__global__ void conv_weight_layout_kernel(const uint16_t* __restrict__ wbf, uint16_t* __restrict__ wfr) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= corr_wfm_size()) return;
+  if (q >= corr_frag_size()) return;
+  const int f = q / 512, lane = (q / 8) % 64, j = q % 8;
   int l = 0;
-  while (l + 1 < CORR_LAYERS && q >= corr_wfm_off(l + 1)) l++;
-  const int r = q - corr_wfm_off(l), AS = corr_a_stride(l), cin = corr_cin(l);
-  const int co = r / AS, k = r % AS, tap = k / cin, ci = k % cin;
-  float v = 0.0f;
-  if (co < corr_cout(l) && k < 9 * cin) v = bf2f(wbf[corr_woff(l) + (co * cin + ci) * 9 + tap]);
-  wfm[q] = v;
+  while (l + 1 < CORR_LAYERS && f >= corr_frag_off(l + 1)) l++;
+  const int fl = f - corr_frag_off(l), nc = corr_frag_nc(l), cin = corr_cin(l);
+  const int g = fl / nc, c = fl % nc;
+  const int co = 16 * g + lane % 16, k = 32 * c + 8 * (lane / 16) + j;
+  const int tap = k / cin, ci = k % cin;
+  wfr[q] = (co < corr_cout(l) && tap < 9) ? wbf[corr_woff(l) + (co * cin + ci) * 9 + tap] : (uint16_t)0;
 }
 
 // din[ci][q] = sum_co sum_tap w[co][ci][tap] dout[co][q - d(tap)], times [act > 0];
@@ -310,29 +478,36 @@ __global__ void round_bf16_kernel(const float* __restrict__ w, uint16_t* wbf, in
   }
 static_assert(CORR_LAYERS == 4, "layer dispatch covers four layers");
 
-hipError_t launch_conv_weight_layout(const uint16_t* wbf, float* wfm, hipStream_t st) {
-  constexpr int n = corr_wfm_size();
-  hipLaunchKernelGGL(conv_weight_layout_kernel, dim3((n + 255) / 256), dim3(256), 0, st, wbf, wfm);
+hipError_t launch_conv_weight_layout(const uint16_t* wbf, uint16_t* wfr, hipStream_t st) {
+  constexpr int n = corr_frag_size();
+  hipLaunchKernelGGL(conv_weight_layout_kernel, dim3((n + 255) / 256), dim3(256), 0, st, wbf, wfr);
   return hipGetLastError();
 }
 
-hipError_t launch_conv_fwd_pooled(const float* Frow, int pr, const float* wfm, const float* bias, float* out, int H,
-                                  int W, hipStream_t st) {
-  static_assert(corr_cin(0) == CORR_IN && corr_cout(0) == CORR_CH, "layer 0");
-  const int tiles = ((H + MY - 1) / MY) * ((W + MX - 1) / MX);
-  hipLaunchKernelGGL((conv_fwd_kernel<CORR_IN, CORR_CH, true>), dim3(tiles, 1, (CORR_CH + 15) / 16), dim3(MT), 0, st,
-                     Frow, nullptr, wfm, bias, out, H, W, 1, pr);
+// Output tile of the fused forward: 2 x 16 pixels for one sample (256
+// workgroups at the 64 x 128 pooled grid: the per-workgroup latency of the
+// four layers, not the halo's extra work, bounds a single sample), 4 x 16 for
+// training batches (half the halo work per pixel).
+template <int TY>
+static hipError_t launch_conv_net_t(const NetArgs& a, int B, hipStream_t st) {
+  constexpr int TX = 16;
+  constexpr int sm = net_smem(TY, TX);
+  static_assert(sm <= 160 * 1024, "LDS of one workgroup");
+  auto k = conv_net_kernel<TY, TX>;
+  hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, sm);
+  if (e != hipSuccess) return e;
+  const int tiles = ((a.H + TY - 1) / TY) * ((a.W + TX - 1) / TX);
+  hipLaunchKernelGGL(k, dim3(tiles, B), dim3(NET_T), sm, st, a);
   return hipGetLastError();
 }
 
-hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const float* wfm, const float* bias, float* out,
-                           int B, int H, int W, int relu, hipStream_t st) {
-  FIL_CORR_LAYER(l, {
-    const int tiles = ((H + MY - 1) / MY) * ((W + MX - 1) / MX);
-    hipLaunchKernelGGL((conv_fwd_kernel<CI, CO>), dim3(tiles, B, (CO + 15) / 16), dim3(MT), 0, st, in, idx, wfm, bias,
-                       out, H, W, relu);
-  });
-  return hipGetLastError();
+hipError_t launch_conv_net(const float* in, const int* idx, int pr, const uint16_t* wfr, const float* params,
+                           float* const* act, float* out, int B, int H, int W, hipStream_t st) {
+  if (B < 1 || H < 1 || W < 1 || (pr > 0 && (B != 1 || idx))) return hipErrorInvalidValue;
+  NetArgs a{};
+  a.in = in; a.idx = idx; a.pr = pr; a.wfr = wfr; a.params = params; a.out = out; a.H = H; a.W = W;
+  for (int l = 0; l < CORR_LAYERS - 1; l++) a.act[l] = act ? act[l] : nullptr;
+  return B == 1 ? launch_conv_net_t<2>(a, B, st) : launch_conv_net_t<4>(a, B, st);
 }
 
 hipError_t launch_conv_bwd_data(int l, const float* dout, const uint16_t* wbf, const float* act, float* din, int B,

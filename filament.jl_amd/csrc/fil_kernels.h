@@ -112,29 +112,32 @@ __host__ __device__ constexpr int corr_woff(int l) {
 }
 __host__ __device__ constexpr int corr_nparam() { return corr_woff(CORR_LAYERS); }
 
-// FP32 copy of the BF16 weights in the layout the forward kernel stages into
-// LDS: per layer [16 ceil(cout / 16)][corr_a_stride(l)] with k = tap * cin + ci
-// along a row (zero padding), so a block copies its 16 channel rows as they are.
-__host__ __device__ constexpr int corr_a_stride(int l) { return (9 * corr_cin(l) + 31) / 32 * 32 + 2; }
-__host__ __device__ constexpr int corr_a_rows(int l) { return 16 * ((corr_cout(l) + 15) / 16); }
-__host__ __device__ constexpr int corr_wfm_off(int l) {
+// BF16 weights in the operand order of v_mfma_f32_16x16x32_bf16 (A[m][k]
+// with m = output channel, k = tap * cin + ci, zero-padded to whole 16-row
+// groups and 32-k chunks): fragment (g, c) of layer l holds, for lane L, the
+// eight values A[16 g + L % 16][32 c + 8 (L / 16) + j], j = 0..7, so every
+// lane loads its operand with one 16-byte read.
+__host__ __device__ constexpr int corr_frag_ng(int l) { return (corr_cout(l) + 15) / 16; }
+__host__ __device__ constexpr int corr_frag_nc(int l) { return (9 * corr_cin(l) + 31) / 32; }
+__host__ __device__ constexpr int corr_frag_off(int l) {   // in fragments of 64 x 8 values
   int o = 0;
-  for (int k = 0; k < l; k++) o += corr_a_rows(k) * corr_a_stride(k);
+  for (int k = 0; k < l; k++) o += corr_frag_ng(k) * corr_frag_nc(k);
   return o;
 }
-__host__ __device__ constexpr int corr_wfm_size() { return corr_wfm_off(CORR_LAYERS); }
-// wfm <- the FP32 forward layout of every layer's BF16 weights (wbf: the flat
-// BF16 parameter copy; biases are not part of wfm)
-hipError_t launch_conv_weight_layout(const uint16_t* wbf, float* wfm, hipStream_t st);
+__host__ __device__ constexpr int corr_frag_size() { return corr_frag_off(CORR_LAYERS) * 64 * 8; }   // bf16 values
+// wfr <- the fragment layout of every layer's BF16 weights (wbf: the flat
+// BF16 parameter copy; biases are not part of wfr)
+hipError_t launch_conv_weight_layout(const uint16_t* wbf, uint16_t* wfr, hipStream_t st);
 
-// forward of layer l: out = act(conv(in) + b); in batch element b is taken from
-// in + (idx ? idx[b] : b) * cin*H*W; wfm: the layer's block of the forward layout.
-hipError_t launch_conv_fwd(int l, const float* in, const int* idx, const float* wfm, const float* bias, float* out,
-                           int B, int H, int W, int relu, hipStream_t st);
-// layer 0 of one sample straight from the per-row features Frow [H pr][4][W]
-// (pooled over pr rows as it is staged; ReLU)
-hipError_t launch_conv_fwd_pooled(const float* Frow, int pr, const float* wfm, const float* bias, float* out, int H,
-                                  int W, hipStream_t st);
+// Forward of the whole network for B samples in one launch:
+//   act[l] = relu(conv_l(...)) for l = 0, 1, 2 (when act is not null, the
+//   activations training needs), out = conv_3(act[2]) [B][2][H][W].
+// Input: with pr > 0 the per-row features Frow [H pr][4][W] of one sample
+// (B = 1), averaged over the pr rows of each pooled row as they are read;
+// otherwise X [.][4][H][W], sample b taken from X + (idx ? idx[b] : b) 4HW.
+// params: the flat FP32 parameter vector (the biases are read from it).
+hipError_t launch_conv_net(const float* in, const int* idx, int pr, const uint16_t* wfr, const float* params,
+                           float* const* act, float* out, int B, int H, int W, hipStream_t st);
 // din = conv^T(dout) (* [act > 0] when act != null)
 hipError_t launch_conv_bwd_data(int l, const float* dout, const uint16_t* wbf, const float* act, float* din, int B,
                                 int H, int W, hipStream_t st);
