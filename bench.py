@@ -224,6 +224,35 @@ def adaptive_loop(pkg, p, steps):
                 same_z_and_field=bool(nh == nd and zh == zd and np.array_equal(Eh, Ed)), form="plain FP32")
 
 
+def library_gemm(Nr, Nh, kt):
+    """The vendor GEMM (hipBLASLt, through torch.matmul) on the Hankel products' shape, for comparison with the
+    library's own products: the split-BF16 form is three BF16 products of M = Nr, N = 2 Nh real columns, K = Nr,
+    i.e. one BF16 GEMM with K = 3 Nr; the plain form one FP32 GEMM with K = Nr."""
+    import torch
+    dev = torch.device("cuda", 0)
+    out = {}
+    for name, dt, K in (("bf16_k3nr", torch.bfloat16, 3 * Nr), ("fp32_knr", torch.float32, Nr)):
+        A = torch.randn(Nr, K, device=dev).to(dt)
+        B = torch.randn(K, 2 * Nh, device=dev).to(dt)
+        for _ in range(5):
+            C = A @ B
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 50
+        e0.record()
+        for _ in range(reps):
+            C = A @ B
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        out[name] = {"us": us, "tflops": 2.0 * Nr * 2 * Nh * K / (us * 1e-6) / 1e12, "M": Nr, "N": 2 * Nh, "K": K}
+        del A, B, C
+    torch.cuda.empty_cache()
+    if kt:
+        out["this_build_split_bf16_products_us"] = {"dht1": kt["dht1"] * 1e3, "dht2": kt["dht2"] * 1e3}
+    return out
+
+
 def cpu_baseline(pkg, p, sample_steps, sample_steps_1core, dz):
     """The oracle's FP32 instance (the plain port restated in C++, OpenMP over
     rows and over Hankel columns) on this host: ``sample_steps`` z-steps on
@@ -281,6 +310,8 @@ def main():
     ap.add_argument("--cpu-steps-1core", type=int, default=2,
                     help="oracle z-steps timed on one thread for the CPU baseline's 1-core figure (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-library-gemm", action="store_true",
+                    help="skip timing the vendor GEMM (hipBLASLt via torch) on the Hankel products' shape")
     ap.add_argument("--adaptive-steps", type=int, default=200,
                     help="steps of the host-driven and the device-kept adaptive z loop (0: skip)")
     ap.add_argument("--factor-sweep", default="4,8",
@@ -446,6 +477,12 @@ def main():
         adapt = adaptive_loop(pkg, p, args.adaptive_steps)
         log(f"adaptive z loop: {adapt}")
 
+    libgemm = None
+    if not args.no_library_gemm:
+        log("vendor GEMM on the Hankel products' shape")
+        libgemm = library_gemm(Nr, Nh, forms["chosen"]["kernel_ms_per_zstep"] if "chosen" in forms else None)
+        log(f"vendor GEMM: {libgemm}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pkg, p, args.cpu_steps, args.cpu_steps_1core, dz)
@@ -472,6 +509,7 @@ def main():
         "roofline": roof,
         "step_roofline": step_roof,
         "adaptive_loop": adapt,
+        "vendor_gemm_same_shape": libgemm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out, default=_plain))
