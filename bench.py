@@ -224,30 +224,41 @@ def adaptive_loop(pkg, p, steps):
                 same_z_and_field=bool(nh == nd and zh == zd and np.array_equal(Eh, Ed)), form="plain FP32")
 
 
+_VENDOR_GEMM = r"""
+import json, sys, torch
+Nr, Nh = int(sys.argv[1]), int(sys.argv[2])
+dev = torch.device("cuda", 0)
+out = {}
+for name, dt, K in (("bf16_k3nr", torch.bfloat16, 3 * Nr), ("fp32_knr", torch.float32, Nr)):
+    A = torch.randn(Nr, K, device=dev).to(dt)
+    B = torch.randn(K, 2 * Nh, device=dev).to(dt)
+    for _ in range(5):
+        C = A @ B
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+    e0.record()
+    for _ in range(reps):
+        C = A @ B
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    out[name] = {"us": us, "tflops": 2.0 * Nr * 2 * Nh * K / (us * 1e-6) / 1e12, "M": Nr, "N": 2 * Nh, "K": K}
+print(json.dumps(out))
+"""
+
+
 def library_gemm(Nr, Nh, kt):
     """The vendor GEMM (hipBLASLt, through torch.matmul) on the Hankel products' shape, for comparison with the
     library's own products: the split-BF16 form is three BF16 products of M = Nr, N = 2 Nh real columns, K = Nr,
-    i.e. one BF16 GEMM with K = 3 Nr; the plain form one FP32 GEMM with K = Nr."""
-    import torch
-    dev = torch.device("cuda", 0)
-    out = {}
-    for name, dt, K in (("bf16_k3nr", torch.bfloat16, 3 * Nr), ("fp32_knr", torch.float32, Nr)):
-        A = torch.randn(Nr, K, device=dev).to(dt)
-        B = torch.randn(K, 2 * Nh, device=dev).to(dt)
-        for _ in range(5):
-            C = A @ B
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        reps = 50
-        e0.record()
-        for _ in range(reps):
-            C = A @ B
-        e1.record()
-        torch.cuda.synchronize()
-        us = e0.elapsed_time(e1) * 1e3 / reps
-        out[name] = {"us": us, "tflops": 2.0 * Nr * 2 * Nh * K / (us * 1e-6) / 1e12, "M": Nr, "N": 2 * Nh, "K": K}
-        del A, B, C
-    torch.cuda.empty_cache()
+    i.e. one BF16 GEMM with K = 3 Nr; the plain form one FP32 GEMM with K = Nr.  Runs in a child process: torch
+    brings its own HIP runtime, which cannot share this process with the one libfilament_hip.so uses."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", _VENDOR_GEMM, str(Nr), str(Nh)], capture_output=True, text=True,
+                       timeout=300)
+    if r.returncode != 0:
+        return {"error": (r.stderr or "")[-500:]}
+    out = json.loads(r.stdout.strip().splitlines()[-1])
     if kt:
         out["this_build_split_bf16_products_us"] = {"dht1": kt["dht1"] * 1e3, "dht2": kt["dht2"] * 1e3}
     return out
@@ -292,8 +303,9 @@ def cpu_baseline(pkg, p, sample_steps, sample_steps_1core, dz):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000,
-                    help="timed fine-step equivalents; coarse forms round it up to whole verification periods")
+    ap.add_argument("--steps", type=int, default=1600,
+                    help="timed fine-step equivalents; coarse forms round it up to whole verification periods "
+                         "(the default, two periods, keeps warm-up + timed region inside the workload's 1 m)")
     ap.add_argument("--warmup", type=int, default=16,
                     help="warm-up fine-step equivalents; coarse forms round it up to whole coarse steps")
     ap.add_argument("--factor", type=int, default=16, help="coarse step = factor * dz in the chosen form")
