@@ -26,10 +26,15 @@ def small(pkg, power=6.0):
     return pkg.workloads.rt_air(64, 512, power, tmin=-125.0, tmax=125.0, tau0=20e-15, noise=1e-3)
 
 
-def make(pkg, p, rho_scale=1.0, capacity=16, batch_max=8, seed=0):
+# pooled maps: one whose sizes are whole network tiles (2 or 4 x 16 pixels) and
+# one that leaves partial tiles at both edges
+POOLED = [(8, 32), (7, 21)]
+
+
+def make(pkg, p, rho_scale=1.0, capacity=16, batch_max=8, seed=0, pooled=(8, 32)):
     m = pkg.RTModel(p, precision=pkg.lib.FP32)
     geom = pkg.corrector.window_geometry(m.grid, m.unit, m.w0, r_window=5e-3, t_window=(-62e-15, 62e-15),
-                                         pooled=(8, 32))
+                                         pooled=pooled)
     c = pkg.corrector.Corrector(m, geom=geom, capacity=capacity, batch_max=batch_max, rho_scale=rho_scale, seed=seed)
     return m, c
 
@@ -64,8 +69,9 @@ def _grid_args(pkg, p):
     return g.r, g.t, p["ru"], p["tu"], p["Iu"]
 
 
-def test_forward_matches_oracle(pkg, oc):
-    m, c = make(pkg, small(pkg))
+@pytest.mark.parametrize("pooled", POOLED)
+def test_forward_matches_oracle(pkg, oc, pooled):
+    m, c = make(pkg, small(pkg), pooled=pooled)
     rng = np.random.default_rng(3)
     w = (rng.standard_normal(pkg.lib.CORR_NPARAM) * 0.1).astype(np.float32)
     c.set_params(w)
@@ -78,8 +84,9 @@ def test_forward_matches_oracle(pkg, oc):
     assert e < 1e-5
 
 
-def test_gradient_matches_oracle(pkg, oc):
-    m, c = make(pkg, small(pkg))
+@pytest.mark.parametrize("pooled", POOLED)
+def test_gradient_matches_oracle(pkg, oc, pooled):
+    m, c = make(pkg, small(pkg), pooled=pooled)
     rng = np.random.default_rng(4)
     w = (rng.standard_normal(pkg.lib.CORR_NPARAM) * 0.1).astype(np.float32)
     c.set_params(w)
@@ -102,9 +109,10 @@ def test_gradient_matches_oracle(pkg, oc):
         o += n
 
 
-def test_apply_matches_oracle(pkg, oc):
+@pytest.mark.parametrize("pooled", POOLED)
+def test_apply_matches_oracle(pkg, oc, pooled):
     p = small(pkg)
-    m, c = make(pkg, p, rho_scale=1e-17)
+    m, c = make(pkg, p, rho_scale=1e-17, pooled=pooled)
     rng = np.random.default_rng(5)
     w = (rng.standard_normal(pkg.lib.CORR_NPARAM) * 0.1).astype(np.float32)
     c.set_params(w)
