@@ -15,6 +15,8 @@ import math
 
 import numpy as np
 
+from .stepping import coarse_step
+
 from . import lib as L
 
 # device state slot used while capturing a training pair
@@ -106,13 +108,14 @@ class Corrector:
         L.check(self._lib.fil_corrector_store_pair(self.m.h, L.fptr(X), L.fptr(T)))
         self.stored = min(self.stored + 1, self.capacity)
 
-    def capture(self, z, dz, factor):
-        """One training pair at the model's current state: the coarse step and
-        its fine steps from the same state.  Leaves the model at the fine
-        result, z + factor * dz.  Returns (X_coarse, T)."""
+    def capture(self, z, dz, factor, scheme="lie"):
+        """One training pair at the model's current state: the coarse step (in
+        the given order of its parts, stepping.coarse_step) and its fine steps
+        from the same state.  Leaves the model at the fine result,
+        z + factor * dz.  Returns (X_coarse, T)."""
         m = self.m
         m.state_save(CAPTURE_SLOT)
-        m.zstep(z + factor * dz, factor * dz)
+        coarse_step(m, z, dz, factor, scheme)
         Xc = self.features()
         m.state_restore(CAPTURE_SLOT)
         for k in range(factor):
@@ -151,7 +154,7 @@ def ensemble_draws(n, seed=0):
                  tau0=float(rng.uniform(25e-15, 100e-15)), seed=k) for k in range(n)]
 
 
-def capture_vs_fp64(m32, c32, m64, probe, z, dz, factor):
+def capture_vs_fp64(m32, c32, m64, probe, z, dz, factor, scheme="lie"):
     """One training pair whose target is the FP64 build's fine steps: the FP32
     model takes the coarse step, the FP64 model the ``factor`` fine steps from
     the same field, and ``probe`` (a Corrector on a second FP32 model with the
@@ -159,7 +162,7 @@ def capture_vs_fp64(m32, c32, m64, probe, z, dz, factor):
     at the FP64 result.  States move between the models on the device
     (fil_state_copy), pending field filter included."""
     m64.copy_state_from(m32)
-    m32.zstep(z + factor * dz, factor * dz)
+    coarse_step(m32, z, dz, factor, scheme)
     Xc = c32.features()
     for k in range(factor):
         m64.zstep(z + (k + 1) * dz, dz)
@@ -169,7 +172,7 @@ def capture_vs_fp64(m32, c32, m64, probe, z, dz, factor):
     return Xc, (Xf[:2] - Xc[:2]).astype(np.float32)
 
 
-def gather_pairs_fp64(RTModel, p, geom, dz, factor, npairs, spacing, form):
+def gather_pairs_fp64(RTModel, p, geom, dz, factor, npairs, spacing, form, scheme="lie"):
     """Training pairs with FP64 fine-step targets along one trajectory of the
     problem ``p`` (FP32 model of the given form between captures)."""
     from . import lib as _L
@@ -181,7 +184,7 @@ def gather_pairs_fp64(RTModel, p, geom, dz, factor, npairs, spacing, form):
     z, out = 0.0, []
     try:
         for _ in range(npairs):
-            out.append(capture_vs_fp64(m32, c32, m64, probe, z, dz, factor))
+            out.append(capture_vs_fp64(m32, c32, m64, probe, z, dz, factor, scheme))
             z += factor * dz
             for _k in range(spacing - factor):
                 m32.zstep(z + dz, dz)
@@ -192,7 +195,7 @@ def gather_pairs_fp64(RTModel, p, geom, dz, factor, npairs, spacing, form):
     return out, z
 
 
-def gather_pairs(model, corr, dz, factor, npairs, spacing, z0=0.0):
+def gather_pairs(model, corr, dz, factor, npairs, spacing, z0=0.0, scheme="lie"):
     """Training pairs along one fine-step trajectory: every ``spacing`` fine
     steps a coarse-vs-fine capture (which also advances the model)."""
     if spacing < factor:
@@ -200,7 +203,7 @@ def gather_pairs(model, corr, dz, factor, npairs, spacing, z0=0.0):
     z = z0
     out = []
     for _ in range(npairs):
-        X, T = corr.capture(z, dz, factor)
+        X, T = corr.capture(z, dz, factor, scheme)
         z += factor * dz
         out.append((X, T))
         for _k in range(spacing - factor):
@@ -214,7 +217,7 @@ def rms(a):
 
 
 def fit(RTModel, make_params, draws, geom, dz, factor, form, npairs=32, spacing=None, iters=2000, batch=16,
-        lr=1e-3, seed=0, log=None, max_pairs=None, stop_growth=None):
+        lr=1e-3, seed=0, log=None, max_pairs=None, stop_growth=None, scheme="lie"):
     """Gathers FP64-target pairs along the trajectory of every draw and trains
     a corrector on the device.  Returns (state, info); ``state`` loads into any
     Corrector with the same window geometry.
@@ -250,7 +253,7 @@ def fit(RTModel, make_params, draws, geom, dz, factor, form, npairs=32, spacing=
             m32.reset_field(make_params(d)["initial_condition"])
             z, n, first, dsq = 0.0, 0, None, 0.0
             while n < npairs and stored < cap:
-                Xc, T = capture_vs_fp64(m32, c32, m64, probe, z, dz, factor)
+                Xc, T = capture_vs_fp64(m32, c32, m64, probe, z, dz, factor, scheme)
                 if not (np.isfinite(Xc).all() and np.isfinite(T).all()):
                     break                      # a step blew up: the trajectory ends here
                 peak = float(Xc[2].max())

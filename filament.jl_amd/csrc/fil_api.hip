@@ -306,10 +306,16 @@ int update_propagator(fil_handle* h, double dz) {
 }
 
 // dz_dev: dz is read from device memory by the propagator and the row kernel
-// (the device z loop); dz is then unused.
+// (the device z loop); dz is then unused.  dz_lin (when not NaN): the linear
+// step advances by dz_lin instead of dz; with dz = 0 the nonlinear step and
+// the plasma solve are skipped (only a pending field filter or corrector
+// output is applied before the linear step).
 template <typename R>
-int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr) {
+int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr, double dz_lin = NAN) {
   h->acache.valid = false;
+  const bool parts = !dz_dev && !std::isnan(dz_lin);
+  const bool linear_only = parts && dz == 0.0;
+  if (!parts) dz_lin = dz;
   if (dz_dev) {
     if (h->xyt)
       HIPCHK(launch_xyt_propagator<R>(h->xprop, 0.0, h->P.as<cx<R>>(), h->st, dz_dev));
@@ -317,7 +323,7 @@ int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr) {
       HIPCHK(launch_propagator<R>(h->KZ64.as<double>(), h->KG64.as<double>(), 0.0, h->P.as<cx<R>>(), h->Nrp, h->Nh,
                                   h->st, dz_dev));
     h->P_dz = NAN;
-  } else if (int rc = update_propagator<R>(h, dz)) {
+  } else if (int rc = update_propagator<R>(h, dz_lin)) {
     return rc;
   }
   RowArgs<R> a{};
@@ -332,8 +338,8 @@ int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr) {
   a.dz_dev = dz_dev;
   a.dt = (R)h->dt;
   a.apply_filter = h->pending_filter ? 1 : 0;
-  a.nonlinearity = h->d.nonlinearity;
-  a.plasma = h->d.plasma;
+  a.nonlinearity = linear_only ? 0 : h->d.nonlinearity;
+  a.plasma = linear_only ? 0 : h->d.plasma;
   a.batch = h->B;
   a.responses = h->d.responses;
   a.losses_ereal = h->d.losses_ereal;
@@ -387,7 +393,7 @@ int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr) {
                                 h->tiling));
       HIPCHK(launch_rk3_combine<R>(s == 2 ? S : U, S, K1, K2, U, (R)dz, dz_dev, s + 1, n, h->st));
     }
-  } else {
+  } else if (!linear_only || a.apply_filter || a.corr_D) {
     FIL_TIMED(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
   }
   h->corr_pending = false;
@@ -421,7 +427,7 @@ int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr) {
   }
 #undef FIL_TIMED
   h->pending_filter = true;
-  h->plasma_valid = h->d.plasma != 0;
+  h->plasma_valid = h->d.plasma != 0 && !linear_only;
   return 0;
 }
 
@@ -1326,6 +1332,19 @@ int fil_zstep(fil_handle* h, double z, double dz) {
   if (!std::isfinite(dz)) return fail(FIL_EINVAL, "dz must be finite");
   HIPCHK(hipSetDevice(h->d.device));
   return h->fp64 ? zstep_impl<double>(h, dz) : zstep_impl<float>(h, dz);
+}
+
+int fil_zstep_parts(fil_handle* h, double z, double dz_nonlinear, double dz_linear) {
+  (void)z;
+  if (!valid_handle(h)) return fail(FIL_EINVAL, "null handle");
+  if (!h->tables || !h->field) return fail(FIL_ESTATE, "tables and field must be set before fil_zstep_parts");
+  if (!std::isfinite(dz_nonlinear) || !std::isfinite(dz_linear)) return fail(FIL_EINVAL, "dz must be finite");
+  if (dz_nonlinear < 0) return fail(FIL_EINVAL, "dz_nonlinear must be >= 0");
+  if (h->nonparax && dz_nonlinear != dz_linear)
+    return fail(FIL_EINVAL, "the non-paraxial nonlinear term takes equal distances (fil_zstep)");
+  HIPCHK(hipSetDevice(h->d.device));
+  return h->fp64 ? zstep_impl<double>(h, dz_nonlinear, nullptr, dz_linear)
+                 : zstep_impl<float>(h, dz_nonlinear, nullptr, dz_linear);
 }
 
 int fil_run_adaptive(fil_handle* h, const fil_adapt_params* p, double* z, int max_steps, double* log, int* nsteps,

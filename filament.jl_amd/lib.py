@@ -105,7 +105,8 @@ class FilAdaptParams(ctypes.Structure):
                [("nonlinearity", ctypes.c_int32), ("chunk", ctypes.c_int32)]
 
 EXPORTS = ("fil_last_error", "fil_create", "fil_set_tables", "fil_set_tables_xyt", "fil_upload_field",
-           "fil_download_field", "fil_upload_member", "fil_download_member", "fil_zstep", "fil_analyze",
+           "fil_download_field", "fil_upload_member", "fil_download_member", "fil_zstep", "fil_zstep_parts",
+           "fil_analyze",
            "fil_analyze_member", "fil_analyze_xyt", "fil_download_plasma", "fil_synchronize", "fil_destroy",
            "fil_timing_enable", "fil_timing_read", "fil_state_save", "fil_state_restore", "fil_state_copy",
            "fil_fluence", "fil_group_create", "fil_group_zstep", "fil_group_stats", "fil_group_destroy",
@@ -149,6 +150,7 @@ def load():
     L.fil_set_tiling.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.fil_get_tiling.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
     L.fil_zstep.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+    L.fil_zstep_parts.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     L.fil_run_adaptive.argtypes = [ctypes.c_void_p, ctypes.POINTER(FilAdaptParams), _dp, ctypes.c_int, _dp,
                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.fil_run_adaptive_noalloc.argtypes = L.fil_run_adaptive.argtypes

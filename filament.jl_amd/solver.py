@@ -163,6 +163,12 @@ class _Model:
     def zstep(self, z, dz):
         L.check(self._lib.fil_zstep(self.h, float(z), float(dz)))
 
+    def zstep_parts(self, z, dz_nonlinear, dz_linear):
+        """The z-step with the nonlinear step over dz_nonlinear (0: none, and no
+        plasma solve) and the linear step over dz_linear (fil_zstep_parts); the
+        building block of symmetric (Strang) sequences of the two parts."""
+        L.check(self._lib.fil_zstep_parts(self.h, float(z), float(dz_nonlinear), float(dz_linear)))
+
     def synchronize(self):
         L.check(self._lib.fil_synchronize(self.h))
 

@@ -195,6 +195,18 @@ int fil_download_member(fil_handle* h, int member, double* E, int64_t ld);
 /* Models.zstep (Models.jl:75-121).  Asynchronous. */
 int fil_zstep(fil_handle* h, double z, double dz);
 
+/* No counterpart in the reference, whose zstep advances the nonlinear and the
+ * linear part by the same dz (Models.jl:97-107): the same z-step with the
+ * nonlinear step over dz_nonlinear (>= 0) and the linear step (propagator,
+ * spectral filter) over dz_linear; the field filter stays pending as after
+ * fil_zstep.  dz_nonlinear = 0 skips the nonlinear step and the plasma solve
+ * (a pending field filter or corrector output is still applied first).  For
+ * callers that compose symmetric (Strang) sequences of the two parts, e.g.
+ * L(h/2) [N(h) L(h)]^(n-1) N(h) L(h/2).  fil_zstep_parts(h, z, dz, dz) is
+ * fil_zstep(h, z, dz).  Handles with the non-paraxial term take equal
+ * distances only.  Asynchronous. */
+int fil_zstep_parts(fil_handle* h, double z, double dz_nonlinear, double dz_linear);
+
 /* Several handles (e.g. the per-grid batches of an ensemble of pulses of
  * uneven size) advanced by one z-step each with ONE graph launch: the first
  * call for a (dz, pending-filter pattern) records the handles' zstep kernels
