@@ -14,16 +14,19 @@ the errors against the FP64 build are read:
   fp64    the FP64 build, fine steps: the error reference;
   plain   the plain FP32 port, fine steps;
   chosen  FP32 with the split-BF16 Hankel transform and the FP64 plasma scan,
-          advanced by coarse steps of F*dz, each followed by the learned
-          corrector, with a fine-step verification (and fallback) every M
-          coarse steps.  Its rate counts fine-step equivalents (a coarse step
-          counts F; verification steps are charged to the run).  W is rounded
-          up to whole coarse steps and K to whole verification periods (M F
-          fine-step equivalents), so every timed window pays for its checks.
-          The corrector is trained on the device during set-up, outside the
-          timed region, on FP64 fine-step targets of other pulses.  The same
-          factor without the network, and the other coarse factors with and
-          without a network of their own, are reported beside it.
+          advanced by coarse steps of F*dz whose nonlinear and linear parts
+          run in the symmetric (Strang) order L(h/2) N(h) L(h/2) (consecutive
+          steps share their half linear steps: stepping.CoarseStepper), each
+          followed by the learned corrector, with a fine-step verification
+          (and fallback) every M coarse steps.  Its rate counts fine-step
+          equivalents (a coarse step counts F; verification steps are charged
+          to the run).  W is rounded up to whole coarse steps and K to whole
+          verification periods (M F fine-step equivalents), so every timed
+          window pays for its checks.  The corrector is trained on the device
+          during set-up, outside the timed region, on FP64 fine-step targets of
+          other pulses.  The same step without the network, and coarse steps in
+          the reference's zstep order (Lie) with and without a network of their
+          own, are reported beside it.
 
 Prints one JSON line.  `value` is the chosen form's fine-equivalent z-steps/s
 when its fluence profile stays within 1e-3 relative L2 of the FP64 build's at
@@ -136,7 +139,7 @@ def run_fine(pkg, p, precision, form, W, K, dz, every, total):
     return ms, kernel_ms(kt), dg
 
 
-def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geom, every, total):
+def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geom, every, total, scheme):
     """The chosen form: coarse steps (+ corrector) with verification; W, K and
     ``total`` in fine-step equivalents (multiples of the factor)."""
     m = pkg.RTModel(p, precision=pkg.lib.FP32, form=form)
@@ -145,7 +148,7 @@ def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geo
     if corr_state is not None:
         corr = pkg.corrector.Corrector(m, geom=geom, capacity=1, batch_max=1)
         corr.load_state(corr_state)
-    st = pkg.CoarseStepper(m, factor=factor, check_every=check_every, tol=tol, corrector=corr)
+    st = pkg.CoarseStepper(m, factor=factor, check_every=check_every, tol=tol, corrector=corr, scheme=scheme)
     z = st.advance(0.0, dz, W)
     m.synchronize()
     s0 = st.stats()
@@ -167,7 +170,7 @@ def run_coarse(pkg, p, form, W, K, dz, factor, check_every, tol, corr_state, geo
     return ms, kernel_ms(kt), dg, timed
 
 
-def train_corrector(pkg, p, form, dz, factor, draws, npairs, iters, geom):
+def train_corrector(pkg, p, form, dz, factor, draws, npairs, iters, geom, scheme):
     C = pkg.corrector
 
     def make(d):
@@ -175,7 +178,7 @@ def train_corrector(pkg, p, form, dz, factor, draws, npairs, iters, geom):
 
     # pre-collapse pairs only (a trajectory ends once its window peak has grown threefold)
     st, info = C.fit(pkg.RTModel, make, draws, geom, dz, factor, form, npairs=npairs, iters=iters, log=log,
-                     stop_growth=3.0)
+                     stop_growth=3.0, scheme=scheme)
     log(f"corrector: {info}")
     return st, info
 
@@ -308,7 +311,11 @@ def main():
                          "(the default, two periods, keeps warm-up + timed region inside the workload's 1 m)")
     ap.add_argument("--warmup", type=int, default=16,
                     help="warm-up fine-step equivalents; coarse forms round it up to whole coarse steps")
-    ap.add_argument("--factor", type=int, default=16, help="coarse step = factor * dz in the chosen form")
+    ap.add_argument("--factor", type=int, default=25, help="coarse step = factor * dz in the chosen form")
+    ap.add_argument("--scheme", default="strang", choices=("strang", "lie"),
+                    help="order of the coarse step's nonlinear and linear parts in the chosen form: symmetric "
+                         "(Strang, L(h/2) N(h) L(h/2), consecutive steps sharing their half steps) or the "
+                         "reference's zstep order (Lie, N(h) L(h))")
     ap.add_argument("--check-every", type=int, default=800,
                     help="fine-step equivalents (z distance in dz) between fine-step verifications; "
                          "a factor F checks every max(1, N/F) coarse steps")
@@ -326,8 +333,9 @@ def main():
                     help="skip timing the vendor GEMM (hipBLASLt via torch) on the Hankel products' shape")
     ap.add_argument("--adaptive-steps", type=int, default=200,
                     help="steps of the host-driven and the device-kept adaptive z loop (0: skip)")
-    ap.add_argument("--factor-sweep", default="4,8",
-                    help="other coarse factors run (own corrector each) and reported beside the headline factor")
+    ap.add_argument("--factor-sweep", default="lie:16,lie:4",
+                    help="other coarse steps run (own corrector each) and reported beside the headline: "
+                         "comma list of [scheme:]factor (scheme defaults to --scheme)")
     ap.add_argument("--timing-every", type=int, default=8,
                     help="per-kernel HIP-event timing on every n-th z-step of the timed region (1: every step)")
     ap.add_argument("--forms", default="fp64,plain,chosen",
@@ -347,7 +355,14 @@ def main():
     Nr, Nt = p["Nr"], p["Nt"]
     Nh = Nt // 2
     chosen_form = L.FORM_DHT_BF16X3 | L.FORM_SCAN_FP64
-    sweep_factors = [int(x) for x in args.factor_sweep.split(",") if x.strip() and int(x) != F]
+    sweep = []
+    for x in args.factor_sweep.split(","):
+        if x.strip():
+            sch, _, f = x.strip().rpartition(":")
+            e = (sch or args.scheme, int(f))
+            if e != (args.scheme, F) and e not in sweep:
+                sweep.append(e)
+    sweep_factors = [f for _, f in sweep]
 
     # Step counts actually run.  Fine forms: W and K as asked.  Coarse forms: the warm-up rounded up to
     # whole coarse steps and the timed region rounded up to whole verification periods, so every timed
@@ -397,39 +412,42 @@ def main():
         m0.close()
         draws = pkg.corrector.ensemble_draws(args.corr_draws, seed=1)
 
-        def coarse_form(f, with_corrector):
-            """One coarse factor, with or without a corrector trained for it."""
+        def coarse_form(f, scheme, with_corrector):
+            """One coarse step (factor and order of its parts), with or without a corrector trained for it."""
             cst, cinfo = None, None
             if with_corrector:
                 cst, cinfo = train_corrector(pkg, p, chosen_form, dz, f, draws, args.corr_pairs, args.corr_iters,
-                                             geom)
+                                             geom, scheme)
             Wf, Kf, period = coarse_counts(f)
-            log(f"chosen x{f}: corrector {'on' if with_corrector else 'off'}, W={Wf} K={Kf}")
+            tag = f"chosen {scheme} x{f}: corrector {'on' if with_corrector else 'off'}"
+            log(f"{tag}, W={Wf} K={Kf}")
             ms, kt, dg, timed = run_coarse(pkg, p, chosen_form, Wf, Kf, dz, f, period // f, args.check_tol, cst, geom,
-                                           args.timing_every, total)
+                                           args.timing_every, total, scheme)
             ef = fluence_error(dg["Fr"], Fr64) if Fr64 is not None else None
-            log(f"chosen x{f} corrector {'on' if with_corrector else 'off'}: {ms:.4f} ms per fine-step equivalent, "
-                f"fluence err {ef}, {timed}")
-            return dict(factor=f, corrector=with_corrector, ms_per_fine_step_equivalent=ms, z_steps_per_s=1e3 / ms,
+            log(f"{tag}: {ms:.4f} ms per fine-step equivalent, fluence err {ef}, {timed}")
+            return dict(factor=f, scheme=scheme, corrector=with_corrector, ms_per_fine_step_equivalent=ms,
+                        z_steps_per_s=1e3 / ms,
                         kernel_ms_per_zstep=kt, warmup=Wf, steps=Kf, fluence_err_vs_fp64=ef, errors_vs_fp64=errs(dg),
                         energy_ratio_end_to_start=dg["energy_ratio"], timed_region=timed,
                         corrector_training=cinfo)
 
         use_corr = not args.no_corrector
-        forms["chosen"] = coarse_form(F, use_corr)
+        forms["chosen"] = coarse_form(F, args.scheme, use_corr)
         forms["chosen"]["description"] = (
-            f"FP32 + split-BF16 Hankel MFMA + FP64 plasma scan, coarse steps of {F} dz"
+            f"FP32 + split-BF16 Hankel MFMA + FP64 plasma scan, coarse steps of {F} dz in "
+            f"{'the symmetric (Strang) order' if args.scheme == 'strang' else 'the zstep (Lie) order'}"
             f"{' + learned corrector' if use_corr else ''}, fine-step verification every "
             f"{max(1, args.check_every // F)} coarse steps ({args.check_every} dz, tol {args.check_tol:g})")
-        # what the learned part earns: the same factors without the network, and the other coarse factors
-        # (4 dz is its design point) with and without a network trained for each; reported only
-        sweep = [coarse_form(F, False)] if use_corr else []
-        for f in sweep_factors:
+        # what the learned part and the order earn: the same step without the network, and the other coarse
+        # steps (the zstep order, 4 dz the network's design point) with and without a network trained for each;
+        # reported only
+        rows = [coarse_form(F, args.scheme, False)] if use_corr else []
+        for sch, f in sweep:
             if use_corr:
-                sweep.append(coarse_form(f, True))
-            sweep.append(coarse_form(f, False))
+                rows.append(coarse_form(f, sch, True))
+            rows.append(coarse_form(f, sch, False))
         forms["chosen"]["factor_sweep"] = [{k: v for k, v in e.items() if k != "kernel_ms_per_zstep"}
-                                           for e in sweep]
+                                           for e in rows]
     if not full:
         print(json.dumps({"profiling_forms": forms}, default=_plain))
         return
