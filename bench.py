@@ -47,7 +47,7 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+PEAK = {"fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0, "fp32_valu_tflops": 78.6}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (fetch x 2.00, write x 1.00,
 # calibrated on a 16 MiB device copy), RT 1024x4096: profiles/rt1024x4096_hbm_traffic_pmc.txt
 TRAFFIC_BYTES = {"row_kernel": 39.9e6, "dht_gemm_bf16x3": 83.3e6}
@@ -479,6 +479,15 @@ def main():
                     avg_launch_ms=kt["row"])
     roof["timing"] = (f"HIP events carried by the kernel dispatches on the library's stream, every "
                       f"{args.timing_every}th z-step of the timed region")
+    # The row kernel is bound by its FP32 vector work, not by HBM: the same launch against the vector peak,
+    # with the algorithmic t-transform work of a step (14 transforms of 5 Nt log2 Nt flop per row, the fewest
+    # three responses with RK3 need) and the FP32 vector peak without packed instructions (78.6 TFLOP/s; the
+    # packed ones are not used on this machine)
+    f_fft_step = 14 * 5.0 * Nt * np.log2(Nt) * Nr
+    vach = f_fft_step / (kt["row"] * 1e-3) / 1e12
+    vector_roof = dict(kernel="row_kernel", bound="valu_fp32", achieved=vach, peak=PEAK["fp32_valu_tflops"],
+                       unit="TFLOP/s", frac=vach / PEAK["fp32_valu_tflops"],
+                       per_launch=f"{f_fft_step:.4g} flop (14*5*Nt*log2(Nt)*Nr)", avg_launch_ms=kt["row"])
 
     # whole-step roofline: T_roof = F_DHT / P_MFMA + max(B_step / BW_HBM, F_FFT / P_FP32) with
     # F_DHT = 2 transforms x 4 Nr^2 Nh flop, B_step = 4 x 8 Nr Nh bytes (the c64 half-spectrum read
@@ -537,6 +546,7 @@ def main():
         "forms": forms,
         "speedup_chosen_vs_plain": forms["plain_fp32"]["ms_per_step"] / forms["chosen"]["ms_per_fine_step_equivalent"],
         "roofline": roof,
+        "vector_roofline": vector_roof,
         "step_roofline": step_roof,
         "adaptive_loop": adapt,
         "vendor_gemm_same_shape": libgemm,
