@@ -108,7 +108,7 @@ struct NetArgs {
 template <int L, int TY, int TX>
 __device__ __forceinline__ void net_layer(const unsigned char* src, unsigned char* dst, const bf16x8* wl,
                                           const float* bl, const NetArgs& a, int b, int y0, int x0) {
-  constexpr int l = L - 1, CIN = corr_cin(l), COUT = corr_cout(l);
+  constexpr int l = L - 1, COUT = corr_cout(l);
   constexpr int NG = corr_frag_ng(l), NC = corr_frag_nc(l);
   constexpr int RY = net_ry(TY, L), RX = net_ry(TX, L), RXI = RX + 2, G = CORR_LAYERS - L;
   constexpr int NPX = RY * RX, NTILE = (NPX + 15) / 16;

@@ -8,8 +8,12 @@
 // one LDS read and one LDS write of the row.  Every pass reads its operands into
 // registers, barriers, then writes the outputs back into the same array, so
 // one buffer suffices.  The row is stored with one pad element per 16
-// (lds_index(i) = i + i/16): the strided outputs of the first pass and the
-// contiguous chunks of the plasma scan then fall in distinct LDS banks.
+// (lds_index(i) = i + i/16), which spreads the strided outputs of the first
+// passes and the contiguous chunks of the plasma scan over the LDS banks.
+// probe_fft_padding.hip times the N = 4096 transform alone for pad periods 8
+// to 128: 32 is the fastest there (fewer bank conflicts), but in the row
+// kernel with the FP64 plasma scan it costs two VGPRs over the 128 that keep
+// two workgroups per CU, and the plain row kernel gains nothing from it.
 // No normalisation is applied.
 #pragma once
 
@@ -21,8 +25,13 @@ namespace fil {
 // thread owns one radix-8 butterfly per pass.
 #define FIL_FFT_MAXRAD 8
 
-__host__ __device__ constexpr int lds_index(int i) { return i + (i >> 4); }
-__host__ __device__ constexpr int lds_size(int n) { return n + (n >> 4); }
+// one pad element per 2^FIL_LDS_PAD_SHIFT elements (a build-time constant so
+// that probe_fft_padding can build the other periods)
+#ifndef FIL_LDS_PAD_SHIFT
+#define FIL_LDS_PAD_SHIFT 4
+#endif
+__host__ __device__ constexpr int lds_index(int i) { return i + (i >> FIL_LDS_PAD_SHIFT); }
+__host__ __device__ constexpr int lds_size(int n) { return n + (n >> FIL_LDS_PAD_SHIFT); }
 
 // cos, sin of 2 pi k / 16, k = 0..7
 __device__ __forceinline__ double c16(int k) {
@@ -134,18 +143,19 @@ __host__ __device__ constexpr int fft_tw_post_offset(int N) { return fft_tw_size
 __host__ __device__ constexpr int fft_tw_ext_size(int N) { return fft_tw_post_offset(N) + N / 2; }
 
 // LDS offset of element i0 + c for a compile-time c that is a multiple of a
-// power of two C with i0 mod C + c's carry staying below 16 (every access of
-// the passes below): lds_index(i0 + c) = lds_index(i0) + c + (c >> 4).
-__host__ __device__ constexpr int lds_step(int c) { return c + (c >> 4); }
+// power of two C with i0 mod C + c's carry staying below the pad period
+// (every access of the passes below): lds_index(i0 + c) = lds_index(i0) + lds_step(c).
+__host__ __device__ constexpr int lds_step(int c) { return c + (c >> FIL_LDS_PAD_SHIFT); }
 
 // One Stockham pass of radix RAD at stride S over N points.
 // Inputs  x[q + S (p + k N/(S RAD))] = x[b + k N/RAD],  b = p S + q;
 // outputs y[q + S (RAD p + k)] = DFT_RAD(...)[k] * w^(p S k),  w = exp(SIGN 2 pi i / N).
 // LDS addresses are one padded base per thread plus compile-time offsets:
 // for reads lds_index(b + k NB) = lds_index(b) + lds_step(k NB) when NB is a
-// multiple of 16 (N >= 128); for writes lds_index(w0 + S k) = lds_index(w0) + lds_step(S k) with
-// w0 = q + S RAD p, because S, RAD and 16 are powers of two and q < S keeps
-// the low four bits of w0 + S k from carrying.
+// multiple of the pad period; for writes lds_index(w0 + S k) = lds_index(w0) + lds_step(S k) with
+// w0 = q + S RAD p, because S, RAD and the period are powers of two and q < S
+// keeps the low bits of w0 + S k from carrying (checked for every pass of
+// N = 16..8192 and periods 8..256).
 // NAT: the outputs go to z in the natural (unpadded) order, z[i] = y[i]; only
 // for the last pass of a transform whose reader wants that layout.
 template <typename T, int N, int NTH, int SIGN, int RAD, int S, bool NAT = false>
@@ -176,7 +186,7 @@ __device__ __forceinline__ void stockham_pass(cx<T>* z, const cx<T>* __restrict_
   for (int u = 0; u < BPT; u++) {
     const int b = tid + u * NTH;
     if (NB % NTH == 0 || b < NB) {
-      if constexpr (NB % 16 == 0) {
+      if constexpr (NB % (1 << FIL_LDS_PAD_SHIFT) == 0) {
         const cx<T>* zr = z + lds_index(b);
 #pragma unroll
         for (int k = 0; k < RAD; k++) a[u][k] = zr[lds_step(k * NB)];
