@@ -115,3 +115,41 @@ def test_strang_coarse_steps_track_fine_steps_closer_than_lie(pkg):
     print(f"fluence vs fine steps after {n * dz:.2f} m: Lie x{F} {eL:.2e}, Strang x{F} {eS:.2e}")
     assert eL > 1e-6
     assert eS < 0.5 * eL
+
+
+@pytest.mark.parametrize("power,a0,tau0", [(3.0, 1.5e-3, 35e-15), (8.0, 0.8e-3, 50e-15)])
+def test_symmetric_coarse_steps_hold_the_gate_on_other_pulses(pkg, power, a0, tau0):
+    """The chosen form's coarse stepping (split-BF16 Hankel, FP64 scan, 25 dz in
+    the symmetric order, a fine-step check every 800 dz with fallback at 1e-4)
+    on pulses other than the bench's, RT 512 x 2048 over 0.5 m, against the
+    FP64 fine steps: within the 1e-3 fluence gate, and closer than the same
+    coarse steps in the zstep order."""
+    L = pkg.lib
+    p = pkg.workloads.rt_air(512, 2048, power, a0=a0, tau0=tau0, seed=7)
+    dz, n, F = p["dz_initial"], 1000, 25
+    form = L.FORM_DHT_BF16X3 | L.FORM_SCAN_FP64
+
+    def fluence(m):
+        return m.analyze(n * dz, arrays=True)["Fr"]
+
+    m64 = pkg.RTModel(p, precision=L.FP64)
+    try:
+        for k in range(n):
+            m64.zstep((k + 1) * dz, dz)
+        F64 = fluence(m64)
+    finally:
+        m64.close()
+    err, stats = {}, {}
+    for scheme in ("strang", "lie"):
+        m = pkg.RTModel(p, precision=L.FP32, form=form)
+        try:
+            st = pkg.CoarseStepper(m, factor=F, check_every=800 // F, tol=1e-4, scheme=scheme)
+            st.advance(0.0, dz, n)
+            err[scheme], stats[scheme] = rel(fluence(m), F64), st.stats()
+        finally:
+            m.close()
+    print(f"{power} Pcr, a0 {a0 * 1e3:.1f} mm, tau0 {tau0 * 1e15:.0f} fs, 0.5 m, 25 dz: fluence error symmetric "
+          f"{err['strang']:.2e}, zstep order {err['lie']:.2e}; checks {stats['strang']['checks']}, failed "
+          f"{stats['strang']['failed_checks']}")
+    assert err["strang"] <= 1e-3
+    assert err["strang"] < err["lie"]
