@@ -1,5 +1,6 @@
-"""Coarse stepping with fine-step verification (filament_amd.CoarseStepper):
-the fallback reproduces the fine trajectory, the state slots restore
+"""Coarse stepping with fine-step verification (filament_amd.CoarseStepper),
+in the zstep and in the symmetric order of the coarse step's parts: the
+fallback reproduces the fine trajectory, the state slots restore
 bit-for-bit, and the device fluence (Parseval) matches the analyzer's."""
 import numpy as np
 import pytest
@@ -48,7 +49,8 @@ def test_device_fluence_matches_analyzer(pkg):
         assert rel(m2.fluence(), Fe) < tol
 
 
-def test_fallback_reproduces_fine_steps(pkg):
+@pytest.mark.parametrize("scheme", ["lie", "strang"])
+def test_fallback_reproduces_fine_steps(pkg, scheme):
     p = small(pkg)
     dz = p["dz_initial"]
     n = 32
@@ -57,14 +59,15 @@ def test_fallback_reproduces_fine_steps(pkg):
         ref.zstep((k + 1) * dz, dz)
     # tol = 0: every check fails, so every interval runs the fine steps
     m = pkg.RTModel(p, precision=pkg.lib.FP32)
-    st = pkg.CoarseStepper(m, factor=4, check_every=1, tol=0.0)
+    st = pkg.CoarseStepper(m, factor=4, check_every=1, tol=0.0, scheme=scheme)
     st.advance(0.0, dz, n)
     s = st.stats()
     assert s["fine_steps"] == n and s["failed_checks"] == n // 4
     assert np.array_equal(m.field(), ref.field())
 
 
-def test_coarse_steps_stay_close_to_fine(pkg):
+@pytest.mark.parametrize("scheme", ["lie", "strang"])
+def test_coarse_steps_stay_close_to_fine(pkg, scheme):
     p = small(pkg)
     dz = p["dz_initial"]
     n = 64
@@ -72,11 +75,11 @@ def test_coarse_steps_stay_close_to_fine(pkg):
     for k in range(n):
         ref.zstep((k + 1) * dz, dz)
     m = pkg.RTModel(p, precision=pkg.lib.FP32)
-    st = pkg.CoarseStepper(m, factor=4, check_every=4, tol=1.0)
+    st = pkg.CoarseStepper(m, factor=4, check_every=4, tol=1.0, scheme=scheme)
     st.advance(0.0, dz, n)
     s = st.stats()
     assert s["checks"] == n // 16 and s["failed_checks"] == 0
     assert s["coarse_steps"] == n // 4          # verified coarse steps included
     err = rel(m.fluence(), ref.fluence())
-    print(f"coarse x4 fluence distance {err:.3e}, last check {s['last_distance']:.3e}")
+    print(f"coarse x4 ({scheme}) fluence distance {err:.3e}, last check {s['last_distance']:.3e}")
     assert err < 1e-3
