@@ -82,6 +82,7 @@ def test_errors_are_reported_without_a_device(pkg):
     assert L.fil_corrector_init(None, ctypes.byref(c), 0) == 1
     assert L.fil_corrector_apply(None) == 1 and b"null" in L.fil_last_error()
     assert L.fil_state_save(None, 0) == 1
+    assert L.fil_zstep_parts(None, 0.0, 1e-3, 5e-4) == 1 and b"null" in L.fil_last_error()
     n, r = ctypes.c_int(), ctypes.c_int()
     assert L.fil_run_adaptive(None, None, None, 0, None, ctypes.byref(n), ctypes.byref(r)) == 1
     assert L.fil_upload_member(None, 0, None, 0) == 1
