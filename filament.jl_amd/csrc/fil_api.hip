@@ -407,8 +407,8 @@ int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr, double 
     // the three transform passes are timed together as the first linear phase
     hipEvent_t e1, e2;
     if (int rc = phase_begin(h, &e1)) return rc;
-    HIPCHK(launch_xyt_linear<R>(h->S.as<cx<R>>(), h->P.as<cx<R>>(), h->twx.as<cx<R>>(), h->twy.as<cx<R>>(), h->Nx,
-                                h->Ny, h->Nh, h->st));
+    HIPCHK(launch_xyt_linear<R>(h->S.as<cx<R>>(), h->P.as<cx<R>>(), h->xprop, h->twx.as<cx<R>>(),
+                                h->twy.as<cx<R>>(), h->st));
     if (int rc = phase_end(h, e1)) return rc;
     if (int rc = phase_begin(h, &e2)) return rc;
     if (int rc = phase_end(h, e2)) return rc;
@@ -914,9 +914,10 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   hipError_t e = hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking);
   if (e != hipSuccess) { delete h; return fail(FIL_EDEVICE, hipGetErrorString(e)); }
   const size_t cz = h->csz(), rz = h->rsz();
+  // P: the propagator [Nrp][Nh]; XYT sizes its own when the tables are set
   const size_t NS = h->ns(), NP = (size_t)h->Nrp * h->Nh, NR = (size_t)h->rows();
   bool ok = h->S.alloc(NS * cz) == hipSuccess && h->X.alloc(xyt || tgeo ? 0 : NS * cz) == hipSuccess &&
-            h->P.alloc(NP * cz) == hipSuccess && h->Et.alloc((size_t)h->Nrp * Nt * cz) == hipSuccess &&
+            h->P.alloc(xyt ? 0 : NP * cz) == hipSuccess && h->Et.alloc((size_t)h->Nrp * Nt * cz) == hipSuccess &&
             h->rho_end.alloc(NR * 8) == hipSuccess && h->rho_max.alloc(NR * 8) == hipSuccess &&
             h->aFr.alloc(NR * 8) == hipSuccess && h->aImax.alloc(NR * 8) == hipSuccess &&
             h->aIt.alloc(NR * Nt * rz) == hipSuccess && h->aSw.alloc(NR * (h->Nh + 1) * rz) == hipSuccess &&
@@ -994,7 +995,17 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
     HIPCHK(upload(h->xkt2x, xt->kt2x, h->Nx * 8));
     HIPCHK(upload(h->xkt2y, xt->kt2y, h->Ny * 8));
     HIPCHK(upload(h->xKX, xt->KXguard, h->Nx * 8));
-    HIPCHK(upload(h->xKY, xt->KYguard, h->Ny * 8));
+    // KY(iy) Wguard(iy) with the reference's 3-D filter quirk (Guards.jl:314);
+    // the stored quarter of P needs kt2 and the guards even in kx and in ky
+    std::vector<double> gy(xt->KYguard, xt->KYguard + h->Ny);
+    if (bug)
+      for (int j = 0; j < h->Ny; j++) gy[j] *= t->Wguard[j];
+    int xsym = 1, ysym = 1;
+    for (int i = 1; i < h->Nx / 2; i++)
+      if (xt->kt2x[i] != xt->kt2x[h->Nx - i] || xt->KXguard[i] != xt->KXguard[h->Nx - i]) xsym = 0;
+    for (int j = 1; j < h->Ny / 2; j++)
+      if (xt->kt2y[j] != xt->kt2y[h->Ny - j] || gy[j] != gy[h->Ny - j]) ysym = 0;
+    HIPCHK(upload(h->xKY, gy.data(), h->Ny * 8));
     HIPCHK(upload(h->xbeta, beta.data(), beta.size() * 8));
     HIPCHK(upload(h->xwvf, wvf.data(), wvf.size() * 8));
     HIPCHK(upload(h->xWg, t->Wguard, (size_t)Nt * 8));
@@ -1002,6 +1013,11 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
     xp.Nx = h->Nx; xp.Ny = h->Ny; xp.Nh = Nh;
     xp.kparaxial = xt->kparaxial != 0;
     xp.guard3d_bug = bug ? 1 : 0;
+    xp.xsym = xsym;
+    xp.ysym = ysym;
+    xp.Px = xsym ? h->Nx / 2 + 1 : h->Nx;
+    xp.Py = ysym ? h->Ny / 2 + 1 : h->Ny;
+    HIPCHK(h->P.alloc((size_t)xp.Px * xp.Py * Nh * h->csz()));
     xp.kt2x = h->xkt2x.as<double>(); xp.kt2y = h->xkt2y.as<double>();
     xp.KX = h->xKX.as<double>(); xp.KY = h->xKY.as<double>();
     xp.beta = h->xbeta.as<double>(); xp.wvf = h->xwvf.as<double>(); xp.Wg = h->xWg.as<double>();

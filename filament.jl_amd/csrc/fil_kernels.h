@@ -161,22 +161,27 @@ hipError_t launch_sum_squares(const float* x, int n, double* part, int* nblk, hi
 struct XytProp {
   int Nx, Ny, Nh;
   int kparaxial, guard3d_bug;
+  // the x (y) factors of P are the same at ix and Nx - ix (iy and Ny - iy),
+  // bit for bit, as the FFT frequency grid makes them: P is stored for
+  // ix <= Nx/2 (iy <= Ny/2) only.  Px = Nx/2 + 1 or Nx, Py = Ny/2 + 1 or Ny.
+  int xsym, ysym, Px, Py;
   const double* kt2x;   // [Nx] (kx kxu)^2 [1/m^2]
   const double* kt2y;   // [Ny]
   const double* KX;     // [Nx] KXguard
-  const double* KY;     // [Ny] KYguard
+  const double* KY;     // [Ny] KYguard, times Wguard[iy] with guard3d_bug
   const double* beta;   // [Nh] complex beta(w) [1/m]
   const double* wvf;    // [Nh] w / vf [1/m]
   const double* Wg;     // [Nt] Wguard
   double zu;            // [m]
   double norm;          // 1 / (Nx Ny), the inverse transform's normalisation
 };
+// P[(ix + Px iy) Nh + w], ix < Px, iy < Py, for the step's dz (dz_dev: dz read from device memory)
 template <typename R>
 hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStream_t st,
                                  const double* dz_dev = nullptr);
 // S <- IFFT_xy[P .* FFT_xy[S]] for every w column of S [(ix + Nx iy)][Nh]
 template <typename R>
-hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const cx<R>* twx, const cx<R>* twy, int Nx, int Ny, int Nh,
+hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const XytProp& pp, const cx<R>* twx, const cx<R>* twy,
                              hipStream_t st);
 
 // Hankel-transform phase (fil_dht.hip)

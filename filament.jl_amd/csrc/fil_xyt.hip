@@ -15,6 +15,14 @@
 // reads WB contiguous complex values per grid point (full cache lines), runs
 // WB transforms in LDS, one group of N/8 threads per transform, and writes
 // them back.  1/(Nx Ny) of the inverse transform is folded into P.
+//
+// P depends on kx^2 and ky^2 (and on |kx|, |ky| through the guards), which the
+// FFT frequency grid makes equal at ix and Nx - ix, iy and Ny - iy: P is stored
+// for ix <= Nx/2, iy <= Ny/2 only (a quarter: 135 MB in FP32 at 512^3 instead
+// of 537 MB).  In the y pass one load serves iy and Ny - iy, and the lines ix
+// and Nx - ix are given to blocks nwb apart in launch order (the same XCD
+// when nwb is a multiple of 8), so the second reader of a table row finds it
+// in L2.
 #include "fil_internal.h"
 #include "fil_fft.h"
 #include "fil_kernels.h"
@@ -41,18 +49,18 @@ __device__ __forceinline__ cd csqrt(cd z) {
 
 }  // namespace
 
-// P[(ix + Nx iy) Nh + w] for w < Nh (Kfunc spectral.jl:160-184, KZ XYT :131-156,
-// spectral filter Guards.jl:250-261 with kernel! :301-318).
+// P[(ix + Px iy) Nh + w] for w < Nh, ix < Px, iy < Py (Kfunc spectral.jl:160-184,
+// KZ XYT :131-156, spectral filter Guards.jl:250-261 with kernel! :301-318).
 template <typename R>
 __global__ void xyt_propagator_kernel(XytProp pp, double dz_arg, const double* __restrict__ dz_dev,
                                       cx<R>* __restrict__ P) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t n = (size_t)pp.Nx * pp.Ny * pp.Nh;
+  const size_t n = (size_t)pp.Px * pp.Py * pp.Nh;
   if (q >= n) return;
   const double dz = dz_dev ? *dz_dev : dz_arg;
   const int w = (int)(q % pp.Nh);
   const size_t row = q / pp.Nh;
-  const int ix = (int)(row % pp.Nx), iy = (int)(row / pp.Nx);
+  const int ix = (int)(row % pp.Px), iy = (int)(row / pp.Px);
   const double kt2 = pp.kt2x[ix] + pp.kt2y[iy];
   const cd beta = {pp.beta[2 * w], pp.beta[2 * w + 1]};
   cd K;
@@ -71,13 +79,13 @@ __global__ void xyt_propagator_kernel(XytProp pp, double dz_arg, const double* _
   const double mag = exp(-kzi * dz);
   double sn, cs;
   sincos(kzr * dz, &sn, &cs);
-  const double g = pp.KX[ix] * pp.KY[iy] * (pp.guard3d_bug ? pp.Wg[iy] : pp.Wg[w]) * pp.norm;
+  const double g = pp.KX[ix] * pp.KY[iy] * (pp.guard3d_bug ? 1.0 : pp.Wg[w]) * pp.norm;
   P[q] = mk((R)(g * mag * cs), (R)(g * mag * sn));
 }
 
 template <typename R>
 hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStream_t st, const double* dz_dev) {
-  const size_t n = (size_t)pp.Nx * pp.Ny * pp.Nh;
+  const size_t n = (size_t)pp.Px * pp.Py * pp.Nh;
   hipLaunchKernelGGL(xyt_propagator_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pp, dz, dz_dev,
                      P);
   return hipGetLastError();
@@ -85,16 +93,25 @@ hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStre
 
 // Transforms of length N along a strided dimension: WB columns per workgroup.
 // MODE 0: forward (exp(-i)), 1: inverse (exp(+i), unnormalised),
-// 2: forward, multiply by P (same layout as S), inverse.
+// 2: forward, multiply by P (the stored part, xyt_propagator_kernel), inverse.
 template <int N> constexpr int xyt_col_threads() { return N / FIL_FFT_MAXRAD >= 2 ? N / FIL_FFT_MAXRAD : 2; }
 template <typename R, int N> constexpr int xyt_cols() {
   return (sizeof(R) == 4) ? (N <= 512 ? 16 : 8) : (N <= 512 ? 8 : 4);
 }
 template <int N> constexpr int xyt_col_stride() { return lds_size(N) + 1; }   // +1: columns start in different banks
 
+// Line of the j-th group of nwb blocks in the y pass with xsym: 0, Nx/2, 1,
+// Nx - 1, 2, Nx - 2, ... (ix and Nx - ix read the same rows of P).
+__device__ __forceinline__ long xyt_paired_line(long j, int Nx) {
+  if (j < 2) return j == 0 ? 0 : Nx / 2;
+  const long k = j / 2;
+  return (j & 1) ? Nx - k : k;
+}
+
 template <typename R, int N, int MODE>
 __global__ __launch_bounds__((xyt_cols<R, N>() * xyt_col_threads<N>())) void xyt_fft_kernel(
-    cx<R>* S, const cx<R>* __restrict__ tw, const cx<R>* __restrict__ P, long line_stride, long elem_stride, int Nh) {
+    cx<R>* S, const cx<R>* __restrict__ tw, const cx<R>* __restrict__ P, XytProp pp, long line_stride,
+    long elem_stride, int Nh) {
   constexpr int WB = xyt_cols<R, N>();
   constexpr int NTHC = xyt_col_threads<N>();
   constexpr int NT = WB * NTHC;
@@ -103,7 +120,7 @@ __global__ __launch_bounds__((xyt_cols<R, N>() * xyt_col_threads<N>())) void xyt
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   const int tid = threadIdx.x, col = tid / NTHC, ctid = tid % NTHC;
   const int nwb = Nh / WB;
-  const long line = blockIdx.x / nwb;
+  const long line = (MODE == 2 && pp.xsym) ? xyt_paired_line(blockIdx.x / nwb, pp.Nx) : (long)(blockIdx.x / nwb);
   const int w0 = (blockIdx.x % nwb) * WB;
   const size_t base = (size_t)(line * line_stride + w0);
   for (int q = tid; q < N * WB; q += NT) {
@@ -118,9 +135,22 @@ __global__ __launch_bounds__((xyt_cols<R, N>() * xyt_col_threads<N>())) void xyt
     fft_lds<R, N, NTHC, -1>(Zc, tw, ctid);
   }
   if constexpr (MODE == 2) {
-    for (int q = tid; q < N * WB; q += NT) {
-      const int e = q / WB, c = q % WB;
-      Z[c * CS + lds_index(e)] = Z[c * CS + lds_index(e)] * P[base + (size_t)e * elem_stride + c];
+    // the line is ix, the elements iy: P row (min(ix, Nx - ix) or ix) + Px iy
+    const long ixp = (pp.xsym && line > pp.Nx / 2) ? pp.Nx - line : line;
+    const cx<R>* Pl = P + (size_t)ixp * Nh + w0;
+    const size_t pstride = (size_t)pp.Px * Nh;
+    if (pp.ysym) {
+      for (int q = tid; q < (N / 2 + 1) * WB; q += NT) {
+        const int e = q / WB, c = q % WB;
+        const cx<R> p = Pl[(size_t)e * pstride + c];
+        Z[c * CS + lds_index(e)] = Z[c * CS + lds_index(e)] * p;
+        if (e != 0 && e != N / 2) Z[c * CS + lds_index(N - e)] = Z[c * CS + lds_index(N - e)] * p;
+      }
+    } else {
+      for (int q = tid; q < N * WB; q += NT) {
+        const int e = q / WB, c = q % WB;
+        Z[c * CS + lds_index(e)] = Z[c * CS + lds_index(e)] * Pl[(size_t)e * pstride + c];
+      }
     }
     __syncthreads();
     fft_lds<R, N, NTHC, +1>(Zc, tw, ctid);
@@ -144,8 +174,8 @@ __global__ __launch_bounds__((xyt_cols<R, N>() * xyt_col_threads<N>())) void xyt
   }
 
 template <typename R, int N, int MODE>
-static hipError_t launch_xyt_fft_n(cx<R>* S, const cx<R>* tw, const cx<R>* P, long nlines, long line_stride,
-                                   long elem_stride, int Nh, hipStream_t st) {
+static hipError_t launch_xyt_fft_n(cx<R>* S, const cx<R>* tw, const cx<R>* P, const XytProp& pp, long nlines,
+                                   long line_stride, long elem_stride, int Nh, hipStream_t st) {
   constexpr int WB = xyt_cols<R, N>();
   constexpr int NT = WB * xyt_col_threads<N>();
   if (Nh % WB) return hipErrorInvalidValue;
@@ -153,31 +183,35 @@ static hipError_t launch_xyt_fft_n(cx<R>* S, const cx<R>* tw, const cx<R>* P, lo
   auto k = xyt_fft_kernel<R, N, MODE>;
   hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k, dim3((unsigned)(nlines * (Nh / WB))), dim3(NT), sm, st, S, tw, P, line_stride, elem_stride, Nh);
+  hipLaunchKernelGGL(k, dim3((unsigned)(nlines * (Nh / WB))), dim3(NT), sm, st, S, tw, P, pp, line_stride,
+                     elem_stride, Nh);
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const cx<R>* twx, const cx<R>* twy, int Nx, int Ny, int Nh,
+hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const XytProp& pp, const cx<R>* twx, const cx<R>* twy,
                              hipStream_t st) {
+  const int Nx = pp.Nx, Ny = pp.Ny, Nh = pp.Nh;
   const long nh = Nh;
+  // the stored part of P must match the symmetry flags
+  if (pp.Px != (pp.xsym ? Nx / 2 + 1 : Nx) || pp.Py != (pp.ysym ? Ny / 2 + 1 : Ny)) return hipErrorInvalidValue;
   hipError_t e;
   // forward along x: lines iy, elements ix
-  FIL_DISPATCH_XY(Nx, e = (launch_xyt_fft_n<R, NN, 0>(S, twx, nullptr, Ny, (long)Nx * nh, nh, Nh, st)));
+  FIL_DISPATCH_XY(Nx, e = (launch_xyt_fft_n<R, NN, 0>(S, twx, nullptr, pp, Ny, (long)Nx * nh, nh, Nh, st)));
   if (e != hipSuccess) return e;
   // along y: forward, propagator, inverse; lines ix, elements iy
-  FIL_DISPATCH_XY(Ny, e = (launch_xyt_fft_n<R, NN, 2>(S, twy, P, Nx, nh, (long)Nx * nh, Nh, st)));
+  FIL_DISPATCH_XY(Ny, e = (launch_xyt_fft_n<R, NN, 2>(S, twy, P, pp, Nx, nh, (long)Nx * nh, Nh, st)));
   if (e != hipSuccess) return e;
   // inverse along x
-  FIL_DISPATCH_XY(Nx, e = (launch_xyt_fft_n<R, NN, 1>(S, twx, nullptr, Ny, (long)Nx * nh, nh, Nh, st)));
+  FIL_DISPATCH_XY(Nx, e = (launch_xyt_fft_n<R, NN, 1>(S, twx, nullptr, pp, Ny, (long)Nx * nh, nh, Nh, st)));
   return e;
 }
 
 template hipError_t launch_xyt_propagator<float>(const XytProp&, double, cx<float>*, hipStream_t, const double*);
 template hipError_t launch_xyt_propagator<double>(const XytProp&, double, cx<double>*, hipStream_t, const double*);
-template hipError_t launch_xyt_linear<float>(cx<float>*, const cx<float>*, const cx<float>*, const cx<float>*, int, int,
-                                             int, hipStream_t);
-template hipError_t launch_xyt_linear<double>(cx<double>*, const cx<double>*, const cx<double>*, const cx<double>*, int,
-                                              int, int, hipStream_t);
+template hipError_t launch_xyt_linear<float>(cx<float>*, const cx<float>*, const XytProp&, const cx<float>*,
+                                             const cx<float>*, hipStream_t);
+template hipError_t launch_xyt_linear<double>(cx<double>*, const cx<double>*, const XytProp&, const cx<double>*,
+                                              const cx<double>*, hipStream_t);
 
 }  // namespace fil

@@ -152,7 +152,10 @@ typedef struct fil_tables {
  * (r, dr, KZ, Kguard, Rguard and dht_* are not used).  KZ is evaluated on the
  * device from its separable factors:
  *   KZ = (Kfunc(beta(w), kt) - w/vf) zu,  kt^2 = kt2x[ix] + kt2y[iy],
- *   Kfunc = beta - kt^2/(2 beta) (kparaxial) or sqrt(beta^2 - kt^2 + 0im). */
+ *   Kfunc = beta - kt^2/(2 beta) (kparaxial) or sqrt(beta^2 - kt^2 + 0im).
+ * When kt2x and KXguard (kt2y and KYguard) are even, i.e. equal at ix and
+ * Nx - ix bit for bit as on the FFT frequency grid, the device stores the
+ * propagator for ix <= Nx/2 (iy <= Ny/2) only; any tables are accepted. */
 typedef struct fil_tables_xyt {
   const double* x;         /* [Nx] grid units */
   const double* y;         /* [Ny] */

@@ -137,3 +137,62 @@ def test_xyt_raman_steps_match_oracle(pkg, orc, precision, tol, oracle_vf):
     err = rel(H.xyt_rows(m.field()), o.field())
     print(f"xyt raman precision={precision}: rel {err:.3e}")
     assert err < tol
+
+
+def white_noise(pkg, half_width_mm, kparaxial=False, seed=5):
+    """Linear-only XYT case on x, y in +-half_width_mm (32 points) whose initial
+    field is real white noise: every (kx, ky, w) of the propagator carries
+    weight.  At 0.5 mm kt reaches past beta(w) at the lowest frequencies, so
+    evanescent and near-cutoff components are part of the field."""
+    p = small(pkg, nonlinearity=False, plasma=False)
+    E = np.random.default_rng(seed).standard_normal((p["Nx"], p["Ny"], p["Nt"]))
+    return dict(p, xmin=-half_width_mm, xmax=half_width_mm, ymin=-half_width_mm, ymax=half_width_mm,
+                xguard=0.1 * half_width_mm, yguard=0.1 * half_width_mm, KPARAXIAL=kparaxial,
+                initial_condition=lambda *a: E)
+
+
+@pytest.mark.parametrize("kparaxial", [False, True])
+@pytest.mark.parametrize("precision,tol", [(4, 1e-5), (8, 1e-10)])
+def test_xyt_propagator_on_white_noise_matches_oracle(pkg, orc, kparaxial, precision, tol):
+    """P = exp(i KZ dz) KX KY W / (Nx Ny) at every (kx, ky, w), paraxial and
+    not (Kfunc spectral.jl:160-184), from the quarter of it the device stores."""
+    p = white_noise(pkg, 0.5, kparaxial)
+    o = oracle_for(pkg, orc, p)
+    dz = 2e-4
+    o.run(3, dz)
+    m = run(pkg, p, 3, dz, precision, vf=o.scalars()["vf"])
+    kz = m.kz
+    kt2 = kz["kt2x"][:, None, None] + kz["kt2y"][None, :, None]
+    evanescent = int(np.count_nonzero(kt2 > np.abs(kz["beta"][None, None, : p["Nt"] // 2]) ** 2))
+    err = rel(H.xyt_rows(m.field()), o.field())
+    print(f"white noise kparaxial={kparaxial} fp{8 * precision}: rel {err:.3e}, {evanescent} components with kt > |beta|")
+    assert evanescent > 0
+    assert err < tol
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+@pytest.mark.parametrize("precision,tol", [(4, 2e-6), (8, 1e-13)])
+def test_xyt_propagator_without_symmetric_tables_agrees(pkg, axis, precision, tol):
+    """With kt^2 and the guards even in kx and ky (the FFT grid's) the device
+    stores P for ix <= Nx/2, iy <= Ny/2 only and reads it for ix, Nx - ix and
+    iy, Ny - iy; moving kt2x[1] (kt2y[1]) by one ulp breaks that symmetry and
+    the whole x (y) extent is stored and read instead: the fields agree to the
+    size of that change (FP64) or to the roundings of the multiply, which the
+    compiler may contract differently in the two loops (FP32; a wrong row or
+    column of P would differ at O(1))."""
+    p = white_noise(pkg, 0.5)
+    dz = 2e-4
+    ms = run(pkg, p, 3, dz, precision)
+    ma = pkg.XYTModel(p, precision=precision, keep_plasma=True)
+    key = "kt2x" if axis == "x" else "kt2y"
+    ma.kz[key][1] = np.nextafter(ma.kz[key][1], np.inf)
+    ma._set_tables()
+    ma.upload(ma.E0)
+    z = 0.0
+    for _ in range(3):
+        z += dz
+        ma.zstep(z, dz)
+        ma.analyze(z)
+    err = rel(ma.field(), ms.field())
+    print(f"{axis}-symmetric vs whole stored propagator fp{8 * precision}: rel {err:.3e}")
+    assert err < tol
