@@ -4,8 +4,8 @@
 // Models.zstep for R and XY (Models.jl:75-121 without the plasma and the t
 // transforms):
 //   nonlinear RK step  dE/dz = func!(E)   (nonlinear_propagators.jl:38-74)
-//     paraxial:      i QZ Rnl g 3/4 |E|^2 E           (cubic.jl:49-54, :58)
-//     non-paraxial:  T^-1[ i QZ Kg T[ Rnl g 3/4 |E|^2 E ] ]  (:58-63)
+//     paraxial:      i QZ Rnl g 3/4 |E|^2 E           (cubic.jl:49-54; nonlinear_propagators.jl:56-57)
+//     non-paraxial:  T^-1[ i QZ Kg T[ Rnl g 3/4 |E|^2 E ] ]  (nonlinear_propagators.jl:58-63)
 //   linear step        E <- T^-1[ exp(i KZ dz) Kg T[E] ]  (spectral.jl:16-26)
 //   field filter       E <- g E                          (Guards.jl:199-215)
 // with g the field guard (R: Rguard(r); XY: Xguard(x) Yguard(y)), Kg the
