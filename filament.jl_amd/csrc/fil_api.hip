@@ -372,26 +372,36 @@ int zstep_impl(fil_handle* h, double dz, const double* dz_dev = nullptr, double 
     }                                                                                               \
   } while (0)
   if (h->nonparax) {
-    // three stages: right-hand side rows, then M2 (Q .* (M1 R)) -> K_s, then Kutta's combination
+    // three stages: right-hand side rows, then the space operator -> K_s, then
+    // Kutta's combination.  RT: K_s = M2 (Q .* (M1 R)); XYT: the rows go
+    // straight to K_s and IFFT_xy[Q .* FFT_xy[K_s]] runs in place.
     const size_t n = h->ns();
     cx<R>* S = h->S.as<cx<R>>();
     cx<R>* K1 = h->npK1.as<cx<R>>();
     cx<R>* K2 = h->npK2.as<cx<R>>();
     cx<R>* U = h->npU.as<cx<R>>();
     cx<R>* Rb = h->npR.as<cx<R>>();
-    cx<R>* Ks[3] = {K1, K2, U};      // the third stage's input is consumed before K3 is written
+    // RT: the third stage's input U is consumed by M1 before M2 writes K3 into it;
+    // XYT: K3 goes to Rb, since the stage kernel reads U while it writes
+    cx<R>* Ks[3] = {K1, K2, h->xyt ? Rb : U};
     for (int s = 0; s < 3; s++) {
+      cx<R>* rows_out = h->xyt ? Ks[s] : Rb;
       if (s == 0)
-        FIL_TIMED(launch_np_stage<R>(a, nullptr, Rb, h->np_rho.as<R>(), h->np_kd.as<R>(), 1, h->rows(), h->Nt,
-                                     h->scan64, h->st));
+        FIL_TIMED(launch_np_stage<R>(a, nullptr, rows_out, h->np_rho.as<R>(), h->np_kd.as<R>(), 1, h->rows(),
+                                     h->Nt, h->scan64, h->st));
       else
-        HIPCHK(launch_np_stage<R>(a, U, Rb, h->np_rho.as<R>(), h->np_kd.as<R>(), 0, h->rows(), h->Nt, h->scan64,
-                                  h->st));
-      HIPCHK(launch_dht_gemm<R>(h->M1.as<R>(), Rb, h->X.as<cx<R>>(), h->npQ.as<cx<R>>(), h->Nrp, h->Nh * h->B, h->Nh,
-                                h->st, h->tiling));
-      HIPCHK(launch_dht_gemm<R>(h->M2.as<R>(), h->X.as<cx<R>>(), Ks[s], nullptr, h->Nrp, h->Nh * h->B, h->Nh, h->st,
-                                h->tiling));
-      HIPCHK(launch_rk3_combine<R>(s == 2 ? S : U, S, K1, K2, U, (R)dz, dz_dev, s + 1, n, h->st));
+        HIPCHK(launch_np_stage<R>(a, U, rows_out, h->np_rho.as<R>(), h->np_kd.as<R>(), 0, h->rows(), h->Nt,
+                                  h->scan64, h->st));
+      if (h->xyt) {
+        HIPCHK(launch_xyt_linear<R>(Ks[s], h->npQ.as<cx<R>>(), h->xprop, h->twx.as<cx<R>>(), h->twy.as<cx<R>>(),
+                                    h->st));
+      } else {
+        HIPCHK(launch_dht_gemm<R>(h->M1.as<R>(), Rb, h->X.as<cx<R>>(), h->npQ.as<cx<R>>(), h->Nrp, h->Nh * h->B,
+                                  h->Nh, h->st, h->tiling));
+        HIPCHK(launch_dht_gemm<R>(h->M2.as<R>(), h->X.as<cx<R>>(), Ks[s], nullptr, h->Nrp, h->Nh * h->B, h->Nh,
+                                  h->st, h->tiling));
+      }
+      HIPCHK(launch_rk3_combine<R>(s == 2 ? S : U, S, K1, K2, Ks[2], (R)dz, dz_dev, s + 1, n, h->st));
     }
   } else if (!linear_only || a.apply_filter || a.corr_D) {
     FIL_TIMED(launch_row<R>(a, h->rows(), h->Nt, h->scan64, h->st));
@@ -887,8 +897,10 @@ int fil_create(const fil_desc* d, fil_handle** out) {
   if (d->plasma && (d->ncomp < 1 || d->ncomp > FIL_MAXCOMP)) return fail(FIL_EINVAL, "ncomp must be in [1, 4]");
   if ((d->form & FIL_FORM_DHT_BF16X3) && d->precision != FIL_FP32)
     return fail(FIL_EINVAL, "the split-BF16 Hankel form needs FIL_FP32");
-  if (d->nonparaxial && (d->geometry != FIL_GEOM_RT || (d->form & FIL_FORM_DHT_BF16X3)))
-    return fail(FIL_EINVAL, "the non-paraxial nonlinear term is built for the RT geometry with the FP32/FP64 Hankel products");
+  if (d->nonparaxial && ((d->geometry != FIL_GEOM_RT && d->geometry != FIL_GEOM_XYT) ||
+                         (d->form & FIL_FORM_DHT_BF16X3)))
+    return fail(FIL_EINVAL, "the non-paraxial nonlinear term is built for the RT geometry with the FP32/FP64 Hankel "
+                            "products and for XYT");
   if ((d->responses & FIL_RESP_RAMAN_THG) && (d->responses & FIL_RESP_RAMAN_NOTHG))
     return fail(FIL_EINVAL, "choose one Raman response");
   if ((d->responses & FIL_RESP_CUBIC_THG) && (d->responses & FIL_RESP_CUBIC_NOTHG))
@@ -928,7 +940,7 @@ int fil_create(const fil_desc* d, fil_handle** out) {
     ok = h->rho_full.alloc(NR * Nt * rz) == hipSuccess && h->kd_full.alloc(NR * Nt * rz) == hipSuccess;
   h->nonparax = d->nonparaxial && d->nonlinearity;
   if (ok && h->nonparax)
-    ok = h->npQ.alloc(NP * cz) == hipSuccess && h->npR.alloc(NS * cz) == hipSuccess &&
+    ok = h->npQ.alloc(xyt ? 0 : NP * cz) == hipSuccess && h->npR.alloc(NS * cz) == hipSuccess &&
          h->npK1.alloc(NS * cz) == hipSuccess && h->npK2.alloc(NS * cz) == hipSuccess &&
          h->npU.alloc(NS * cz) == hipSuccess &&
          h->np_rho.alloc(d->plasma ? NR * Nt * rz : 0) == hipSuccess &&
@@ -1024,6 +1036,12 @@ int set_tables_impl(fil_handle* h, const fil_tables* t, const fil_tables_xyt* xt
     xp.zu = xt->zu;
     xp.norm = 1.0 / ((double)h->Nx * h->Ny);
     h->P_dz = NAN;
+    if (h->nonparax) {
+      // the non-paraxial prefactor table, in the propagator's layout (dz-independent)
+      HIPCHK(h->npQ.alloc((size_t)xp.Px * xp.Py * Nh * h->csz()));
+      HIPCHK(h->fp64 ? launch_xyt_qz<double>(xp, h->npQ.as<cx<double>>(), h->st)
+                     : launch_xyt_qz<float>(xp, h->npQ.as<cx<float>>(), h->st));
+    }
     if (int rc = build_twiddles(h->twx, h->Nx, f64, false)) return rc;
     if (int rc = build_twiddles(h->twy, h->Ny, f64, false)) return rc;
   }

@@ -179,6 +179,10 @@ struct XytProp {
 template <typename R>
 hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStream_t st,
                                  const double* dz_dev = nullptr);
+// Q[(ix + Px iy) Nh + w]: the non-paraxial nonlinear prefactor over the paraxial
+// one, with the spectral filter and 1/(Nx Ny) (launch_xyt_linear applies it as P)
+template <typename R>
+hipError_t launch_xyt_qz(const XytProp& pp, cx<R>* Q, hipStream_t st);
 // S <- IFFT_xy[P .* FFT_xy[S]] for every w column of S [(ix + Nx iy)][Nh]
 template <typename R>
 hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const XytProp& pp, const cx<R>* twx, const cx<R>* twy,

@@ -91,6 +91,40 @@ hipError_t launch_xyt_propagator(const XytProp& pp, double dz, cx<R>* P, hipStre
   return hipGetLastError();
 }
 
+// Q[(ix + Px iy) Nh + w] for w < Nh, ix < Px, iy < Py: the non-paraxial
+// prefactor of func! (nonlinear_propagators.jl:58-63; QZfunc XYT :184-204,
+// Qfunc_nonparaxial :232-242) over the paraxial QZ(w) (Qfunc_paraxial
+// :221-230) that the stage right-hand side already carries, with func!'s
+// spectral filter (:61) and the inverse transform's 1/(Nx Ny).  mu0 mu w^2
+// zu / Eu cancels in the ratio: Q = beta / kz, kz = sqrt(beta^2 - kt^2 + 0im);
+// 0 where beta = 0 (QZ(w) = 0) or kz = 0 (the reference's zero there).
+template <typename R>
+__global__ void xyt_qz_kernel(XytProp pp, cx<R>* __restrict__ Q) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)pp.Px * pp.Py * pp.Nh;
+  if (q >= n) return;
+  const int w = (int)(q % pp.Nh);
+  const size_t row = q / pp.Nh;
+  const int ix = (int)(row % pp.Px), iy = (int)(row / pp.Px);
+  const double kt2 = pp.kt2x[ix] + pp.kt2y[iy];
+  const cd beta = {pp.beta[2 * w], pp.beta[2 * w + 1]};
+  cd v = {0.0, 0.0};
+  if (beta.x != 0.0 || beta.y != 0.0) {
+    const cd b2 = cmul(beta, beta);
+    const cd kz = csqrt({b2.x - kt2, b2.y});
+    if (kz.x != 0.0 || kz.y != 0.0) v = cdiv(beta, kz);
+  }
+  const double g = pp.KX[ix] * pp.KY[iy] * (pp.guard3d_bug ? 1.0 : pp.Wg[w]) * pp.norm;
+  Q[q] = mk((R)(g * v.x), (R)(g * v.y));
+}
+
+template <typename R>
+hipError_t launch_xyt_qz(const XytProp& pp, cx<R>* Q, hipStream_t st) {
+  const size_t n = (size_t)pp.Px * pp.Py * pp.Nh;
+  hipLaunchKernelGGL(xyt_qz_kernel<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, pp, Q);
+  return hipGetLastError();
+}
+
 // Transforms of length N along a strided dimension: WB columns per workgroup.
 // MODE 0: forward (exp(-i)), 1: inverse (exp(+i), unnormalised),
 // 2: forward, multiply by P (the stored part, xyt_propagator_kernel), inverse.
@@ -207,6 +241,8 @@ hipError_t launch_xyt_linear(cx<R>* S, const cx<R>* P, const XytProp& pp, const 
   return e;
 }
 
+template hipError_t launch_xyt_qz<float>(const XytProp&, cx<float>*, hipStream_t);
+template hipError_t launch_xyt_qz<double>(const XytProp&, cx<double>*, hipStream_t);
 template hipError_t launch_xyt_propagator<float>(const XytProp&, double, cx<float>*, hipStream_t, const double*);
 template hipError_t launch_xyt_propagator<double>(const XytProp&, double, cx<double>*, hipStream_t, const double*);
 template hipError_t launch_xyt_linear<float>(cx<float>*, const cx<float>*, const XytProp&, const cx<float>*,

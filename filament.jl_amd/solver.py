@@ -40,8 +40,8 @@ class _Model:
         self.nonlinearity = bool(p["NONLINEARITY"])
         self.plasma = bool(p["PLASMA"])
         self.nonparaxial = bool(self.nonlinearity and not p.get("QPARAXIAL", True))
-        if self.nonparaxial and not isinstance(self, RTModel):
-            raise NotImplementedError("QPARAXIAL=false is built for the RT geometry")
+        if self.nonparaxial and not isinstance(self, (RTModel, XYTModel)):
+            raise NotImplementedError("QPARAXIAL=false is built for the RT and XYT geometries")
         # responses (nonlinear_propagators.jl:20-27)
         resp_flags = 0
         self.Rnl = {}

@@ -97,9 +97,11 @@ typedef struct fil_desc {
    * field and plasma, the tables are shared.  Batched handles take the
    * *_member entry points and no corrector. */
   int32_t batch;
-  /* RT only: QPARAXIAL = false, the non-paraxial prefactor of the nonlinear
-   * term (nonlinear_propagators.jl:58-63, 232-242); needs fil_tables.QZ_kw
-   * and the FP32/FP64 Hankel products (not FIL_FORM_DHT_BF16X3).  0: paraxial. */
+  /* RT and XYT: QPARAXIAL = false, the non-paraxial prefactor of the
+   * nonlinear term (nonlinear_propagators.jl:58-63, 232-242).  RT needs
+   * fil_tables.QZ_kw and the FP32/FP64 Hankel products (not
+   * FIL_FORM_DHT_BF16X3); XYT evaluates QZ(kx, ky, w) / QZ(w) on the device
+   * from fil_tables_xyt.  0: paraxial. */
   int32_t nonparaxial;
 } fil_desc;
 

@@ -196,3 +196,32 @@ def test_xyt_propagator_without_symmetric_tables_agrees(pkg, axis, precision, to
     err = rel(ma.field(), ms.field())
     print(f"{axis}-symmetric vs whole stored propagator fp{8 * precision}: rel {err:.3e}")
     assert err < tol
+
+
+def narrow_beam(pkg, qparaxial):
+    """An elliptical 6 x 4 um beam (x, y in +-0.1 mm, 64 points; Kerr, free
+    current and losses, no plasma), narrow enough that QZ(kx, ky, w) differs
+    from its paraxial value by ~1e-3 over the beam's spectrum."""
+    p = pkg.workloads.xyt_air(64, 64, 256, 0.5, xmax=0.1, ymax=0.1, tmin=-120.0, tmax=120.0, ax=6e-6, ay=4e-6,
+                              tau0=20e-15, noise=0.0, plasma=False)
+    return dict(p, xguard=0.01, yguard=0.01, QPARAXIAL=qparaxial)
+
+
+@pytest.mark.parametrize("precision,tol", [(4, 1e-4), (8, 1e-10)])
+def test_xyt_nonparaxial_steps_match_oracle(pkg, orc, precision, tol):
+    """QPARAXIAL = false for XYT: func!'s space branch (nonlinear_propagators.jl:58-63,
+    QZfunc XYT :184-204) as the three RK stages with IFFT_xy[Q .* FFT_xy[.]]
+    between the rows and the combination.  Against the oracle, and distinct
+    from the paraxial step by far more than the gate."""
+    dz, n = 5e-5, 6
+    p = narrow_beam(pkg, False)
+    o = oracle_for(pkg, orc, p)
+    o.run(n, dz)
+    vf = o.scalars()["vf"]
+    m = run(pkg, p, n, dz, precision, vf=vf)
+    err = rel(H.xyt_rows(m.field()), o.field())
+    mp = run(pkg, narrow_beam(pkg, True), n, dz, precision, vf=vf)
+    dpar = rel(mp.field(), m.field())
+    print(f"xyt non-paraxial fp{8 * precision}: rel {err:.3e} vs oracle; paraxial step differs by {dpar:.3e}")
+    assert err < tol
+    assert dpar > 1e-4 and dpar > 100 * err

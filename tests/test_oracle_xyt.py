@@ -11,6 +11,9 @@ import os
 import numpy as np
 
 import _helpers as H
+from test_oracle import base_cfg
+
+C0 = 299792458.0
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference")
 
@@ -164,3 +167,47 @@ def test_t_mode_reproduces_reference_dispersion_test(pkg, orc):
     kz = ph.kz_table_t(ph.UnitT(p["zu"], p["tu"], p["Iu"], p["rhou"]), g,
                        ph.Medium(p["permittivity"], p["permeability"], p["n2"]), 2 * np.pi * ph.C0 / p["lam0"])
     assert np.max(np.abs(kz - o.KZ().ravel())) <= 1e-8 * np.max(np.abs(kz))
+
+
+def _narrow_beam_xyt(orc, qparaxial, N=64, xmax=0.05, ymax=0.08, ax=0.004, ay=0.006, Nt=256, k=30, A=0.5):
+    """A monochromatic elliptical Gaussian beam exp(-x^2/(2ax^2) - y^2/(2ay^2))
+    at t-bin k on an XYT grid (mm, fs) with different x and y windows (so kx
+    and ky differ) in a constant medium (n = 1), Kerr only, no guards: kt/beta
+    reaches ~0.1 over its spectrum."""
+    c = base_cfg(geometry=2, Nr=0, Nx=N, Ny=N, xmin=-xmax, xmax=xmax, ymin=-ymax, ymax=ymax, xu=1e-3, yu=1e-3,
+                 xguard=0.0, yguard=0.0, kxguard=90.0, kyguard=90.0, guard3d_bug=0, Nt=Nt, tmin=-100.0,
+                 tmax=100.0, nonlinearity=1, cubic=2, medium=1, n_const=1.0, rguard=0.0, tguard=0.0,
+                 qparaxial=qparaxial)
+    o = orc.Oracle(c, np.zeros((N * N, Nt)))
+    gx = o.grid_xyt()
+    X, Y = np.meshgrid(gx["x"], gx["y"], indexing="ij")
+    Es = np.zeros((N * N, Nt), dtype=complex)
+    Es[:, k] = H.xyt_rows((A * np.exp(-X ** 2 / (2 * ax * ax) - Y ** 2 / (2 * ay * ay)))[:, :, None])[:, 0]
+    return o, gx, Es, k
+
+
+def test_xyt_nonparaxial_rhs_is_qz_ratio_in_kx_ky_space(orc):
+    """func! with QPARAXIAL = false for XYT (nonlinear_propagators.jl:58-63,
+    QZfunc XYT :184-204, Qfunc_nonparaxial :232-242): after the 2-D FFT over
+    x and y the right-hand side is the paraxial one times beta / kz(kx, ky),
+    kz = sqrt(beta^2 - kx^2 - ky^2) (the guards are 1 to double precision
+    here).  beta/kz - 1 reaches ~0.1 over the beam's spectrum; a transform
+    taken the wrong way round, kx and ky exchanged (the beam is elliptical)
+    or the paraxial factor kept fails by far more than the 1e-9 gate."""
+    op, gx, Es, k = _narrow_beam_xyt(orc, 1)
+    onp, _, _, _ = _narrow_beam_xyt(orc, 0)
+    N = 64
+    rp, rn = op.nl_rhs(Es)[:, k], onp.nl_rhs(Es)[:, k]
+    # rows are ix + N iy: [iy][ix] in C order
+    Fp = np.fft.fft2(rp.reshape(N, N))
+    Fn = np.fft.fft2(rn.reshape(N, N))
+    w = op.grid()["w"][k] * 1e15
+    beta = w / C0
+    KX, KY = np.meshgrid(gx["kx"] * 1e3, gx["ky"] * 1e3, indexing="xy")   # [iy][ix]
+    kz = np.sqrt(beta ** 2 - KX ** 2 - KY ** 2 + 0j)
+    expect = Fp * beta / kz
+    sup = np.abs(Fp) > 1e-3 * np.abs(Fp).max()
+    print(f"max beta/kz - 1 over the spectrum: {np.max(np.abs(beta / kz[sup] - 1)):.2e}")
+    assert np.max(np.abs(beta / kz[sup] - 1)) > 1e-3
+    assert np.max(np.abs(Fn - expect)) < 1e-9 * np.max(np.abs(Fp))
+    assert np.max(np.abs(onp.nl_rhs(Es)[:, np.arange(256) != k])) < 1e-12 * np.max(np.abs(rn))
