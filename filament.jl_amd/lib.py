@@ -7,8 +7,6 @@ not been built -- there is no CPU fallback.
 import ctypes
 import os
 
-import numpy as np
-
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libfilament_hip.so")
 
