@@ -11,11 +11,13 @@ namespace fil {
 // ---------------------------------------------------------------------------
 // Rate of one plasma component: TabulatedFunctions.TFunction evaluated as
 // log-log linear interpolation, clamped to the end nodes
-// (photoionization_avalanche.jl:158,184,196).  `power` gives the local
-// log-log slope (tfpower, :196).
+// (photoionization_avalanche.jl:158,184,196).  `power` receives the local
+// log-log slope (tfpower, :196).  It is a reference that is always written: a
+// pointer chosen at run time between a local and null would keep that local
+// in scratch memory.
 // ---------------------------------------------------------------------------
 template <typename R>
-__device__ __forceinline__ R tf_eval(const PlasmaParams<R>& pp, int q, R I, R* power) {
+__device__ __forceinline__ R tf_eval(const PlasmaParams<R>& pp, int q, R I, R& power) {
   const int n = pp.ntab[q];
   int i;
   R lc;
@@ -27,9 +29,14 @@ __device__ __forceinline__ R tf_eval(const PlasmaParams<R>& pp, int q, R I, R* p
     else { i = (int)u; if (i > n - 2) i = n - 2; lc = l; }
   }
   const R s = (R)pp.slope[q][i];
-  if (power) *power = s;
+  power = s;
   const R y = (R)pp.ly[q][i] + s * (lc - ((R)pp.lx0[q] + (R)i * (R)pp.dlx[q]));
   return (sizeof(R) == 8) ? (R)exp2((double)y) : (R)exp2f((float)y);
+}
+template <typename R>
+__device__ __forceinline__ R tf_eval(const PlasmaParams<R>& pp, int q, R I) {
+  R unused;
+  return tf_eval<R>(pp, q, I, unused);
 }
 
 // bilinear interpolation of one pooled plane at pooled coordinates (y, x), clamped
@@ -158,7 +165,7 @@ __device__ void plasma_row(cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>& pp, R
         R E2st[3] = {E0.x * E0.x, Em.x * Em.x, E1.x * E1.x};
         R W[3][FIL_MAXCOMP];
         for (int st = 0; st < 3; st++)
-          for (int q = 0; q < nc; q++) W[st][q] = tf_eval<R>(pp, q, Ist[st], nullptr);
+          for (int q = 0; q < nc; q++) W[st][q] = tf_eval<R>(pp, q, Ist[st]);
         // avalanche integrator
         R k[3][FIL_MAXCOMP], tmp[FIL_MAXCOMP];
         for (int st = 0; st < 3; st++) {
@@ -192,7 +199,7 @@ __device__ void plasma_row(cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>& pp, R
         for (int q = 0; q < nc; q++) {
           ut[q] = ut[q] + dt * (k[0][q] / (R)6 + (R)2 * k[1][q] / (R)3 + k[2][q] / (R)6);
           R pw;
-          R w1 = tf_eval<R>(pp, q, Ist[0], &pw);
+          R w1 = tf_eval<R>(pp, q, Ist[0], pw);
           R K = pp.kdep ? pw : (R)pp.K[q];
           kdv += K * (w1 * ((R)pp.frhont[q] - ut[q]));
         }
@@ -235,14 +242,14 @@ __device__ void plasma_row(cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>& pp, R
     E1c = E1;
     cx<R> Em = (R)0.5 * (E0 + E1);
     R I0 = fiarg(E0, pp.ereal), Im = fiarg(Em, pp.ereal), I1 = fiarg(E1, pp.ereal);
-    R hq[2] = {0, 0};
+    R hq0 = 0, hq1 = 0;   // scalars: the q loop is not unrolled (nc is a run-time count)
 #pragma unroll
     for (int q = 0; q < FIL_MAXCOMP; q++) {
       if (q >= nc) break;
       R pw1 = have_prev ? Pprev[q] : (R)0, pw3 = 0;
-      R W1 = have_prev ? Wprev[q] : tf_eval<R>(pp, q, I0, slope ? &pw1 : nullptr);
-      R W2 = tf_eval<R>(pp, q, Im, nullptr);
-      R W3 = tf_eval<R>(pp, q, I1, slope ? &pw3 : nullptr);
+      R W1 = have_prev ? Wprev[q] : tf_eval<R>(pp, q, I0, pw1);
+      R W2 = tf_eval<R>(pp, q, Im);
+      R W3 = tf_eval<R>(pp, q, I1, pw3);
       Wprev[q] = W3;
       Pprev[q] = pw3;
       A w1 = W1, w2 = W2, w3 = W3;
@@ -254,14 +261,15 @@ __device__ void plasma_row(cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>& pp, R
         if (q == 0) rho[rk_index<R>(j + 1)] = cache ? hr : kw1;
         if (q == 0 && cache) kd[rk_index<R>(j)] = kw1;
         if (q == 1) kd[rk_index<R>(j)] = kw1;
-        if (q < 2) hq[q] = hr;
+        if (q == 0) hq0 = hr;
+        if (q == 1) hq1 = hr;
       }
       Aff<A> st; st.a = (A)1 - h; st.b = h * (A)pp.frhont[q];
       m[q] = compose(st, m[q]);
     }
     if (cache2) {
-      if (c == 0) hz = mk(hq[0], hq[1]);
-      else Z[lds_index(j)] = mk(hq[0], hq[1]);
+      if (c == 0) hz = mk(hq0, hq1);
+      else Z[lds_index(j)] = mk(hq0, hq1);
     }
     have_prev = true;
   }
@@ -326,9 +334,9 @@ __device__ void plasma_row(cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>& pp, R
     for (int q = 0; q < FIL_MAXCOMP; q++) {
       if (q >= nc) break;
       R pw = 0;
-      R W1 = tf_eval<R>(pp, q, I0, pp.kdep ? &pw : nullptr);
-      R W2 = tf_eval<R>(pp, q, Im, nullptr);
-      R W3 = tf_eval<R>(pp, q, I1, nullptr);
+      R W1 = tf_eval<R>(pp, q, I0, pw);
+      R W2 = tf_eval<R>(pp, q, Im);
+      R W3 = tf_eval<R>(pp, q, I1);
       A w1 = W1, w2 = W2, w3 = W3;
       const A h = rk3_gain<A>(dtA, w1, w2, w3);
       r[q] = ((A)1 - h) * r[q] + h * (A)pp.frhont[q];
