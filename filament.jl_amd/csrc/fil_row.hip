@@ -168,7 +168,14 @@ __global__ __launch_bounds__(NTH) void to_spectrum_kernel(const cx<R>* __restric
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   const int i = blockIdx.x, tid = threadIdx.x;
-  for (int t = tid; t < N; t += NTH) Z[lds_index(t)] = Et[(size_t)i * N + t];
+  {
+    // all of the thread's loads in flight before its first LDS write
+    cx<R> v[N / NTH];
+#pragma unroll
+    for (int u = 0; u < N / NTH; u++) v[u] = Et[(size_t)i * N + tid + u * NTH];
+#pragma unroll
+    for (int u = 0; u < N / NTH; u++) Z[lds_index(tid + u * NTH)] = v[u];
+  }
   __syncthreads();
   fft_lds<R, N, NTH, +1>(Z, tw);
   const R inv = (R)1 / (R)N;
@@ -182,9 +189,15 @@ __global__ __launch_bounds__(NTH) void to_time_kernel(const cx<R>* __restrict__ 
   extern __shared__ __align__(16) unsigned char smem[];
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   const int i = blockIdx.x, tid = threadIdx.x;
-  for (int w = tid; w < N / 2; w += NTH) {
-    Z[lds_index(w)] = S[((size_t)i * rs + ro) * (N / 2) + w];
-    Z[lds_index(N / 2 + w)] = mk((R)0, (R)0);
+  {
+    cx<R> v[N / 2 / NTH];
+#pragma unroll
+    for (int m = 0; m < N / 2 / NTH; m++) v[m] = S[((size_t)i * rs + ro) * (N / 2) + tid + m * NTH];
+#pragma unroll
+    for (int m = 0; m < N / 2 / NTH; m++) {
+      Z[lds_index(tid + m * NTH)] = v[m];
+      Z[lds_index(N / 2 + tid + m * NTH)] = mk((R)0, (R)0);
+    }
   }
   __syncthreads();
   fft_lds<R, N, NTH, -1>(Z, tw);
@@ -212,7 +225,17 @@ __global__ __launch_bounds__(NTH) void analyze_rows_kernel(cx<R>* S, const cx<R>
   const int i = blockIdx.x, tid = threadIdx.x;
   constexpr int NH = N / 2;
   cx<R>* Srow = S + ((size_t)i * rs + ro) * NH;
-  for (int w = tid; w < NH; w += NTH) { Z[lds_index(w)] = Srow[w]; Z[lds_index(NH + w)] = mk((R)0, (R)0); }
+  {
+    // all of the thread's loads in flight before its first LDS write
+    cx<R> v[NH / NTH];
+#pragma unroll
+    for (int m = 0; m < NH / NTH; m++) v[m] = Srow[tid + m * NTH];
+#pragma unroll
+    for (int m = 0; m < NH / NTH; m++) {
+      Z[lds_index(tid + m * NTH)] = v[m];
+      Z[lds_index(NH + tid + m * NTH)] = mk((R)0, (R)0);
+    }
+  }
   __syncthreads();
   fft_lds<R, N, NTH, -1>(Z, tw);
   const R g = apply_filter ? Rg[i / gdiv] : (R)1;
@@ -312,7 +335,16 @@ __global__ void colsum_part_kernel(const T* __restrict__ in, int Nr, int ncol, c
   if (j >= ncol) return;
   const int i0 = blockIdx.y * rows, i1 = min(Nr, i0 + rows);
   double s = 0;
-  for (int i = i0; i < i1; i++) s += (double)in[(size_t)i * ncol + j] * wt[i];
+  // eight loads in flight, added in the same row order as one at a time
+  int i = i0;
+  for (; i + 8 <= i1; i += 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = in[(size_t)(i + u) * ncol + j];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += (double)v[u] * wt[i + u];
+  }
+  for (; i < i1; i++) s += (double)in[(size_t)i * ncol + j] * wt[i];
   part[(size_t)blockIdx.y * ncol + j] = s;
 }
 
@@ -320,7 +352,16 @@ __global__ void colsum_final_kernel(const double* __restrict__ part, int nchunk,
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= ncol) return;
   double s = 0;
-  for (int c = 0; c < nchunk; c++) s += part[(size_t)c * ncol + j];
+  // eight loads in flight, added in the same chunk order as one at a time
+  int c = 0;
+  for (; c + 8 <= nchunk; c += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = part[(size_t)(c + u) * ncol + j];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += v[u];
+  }
+  for (; c < nchunk; c++) s += part[(size_t)c * ncol + j];
   out[j] = s;
 }
 

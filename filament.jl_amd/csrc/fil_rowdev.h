@@ -233,7 +233,6 @@ __device__ void plasma_row(cx<R>* Z, R* rho, R* kd, const PlasmaParams<R>& pp, R
   for (int q = 0; q < FIL_MAXCOMP; q++) { m[q].a = (A)1; m[q].b = (A)0; }
   R Wprev[FIL_MAXCOMP], Pprev[FIL_MAXCOMP];
   bool have_prev = false;
-  const bool slope = (cache || cache2) && pp.kdep;
   cx<R> E1c = mk((R)0, (R)0), hz = mk((R)0, (R)0);
   for (int c = 0; c < C; c++) {
     const int j = j0 + c;

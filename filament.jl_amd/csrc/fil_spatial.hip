@@ -182,10 +182,35 @@ __global__ __launch_bounds__(sp_line_threads<N>()) void sp_xline_kernel(cx<R>* _
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   const int tid = threadIdx.x;
   const size_t base = (size_t)blockIdx.x * N;
-  for (int t = tid; t < N; t += NTH) {
-    cx<R> v = E[base + t];
-    if constexpr (FWD && ALG > 0) v = sp_rk_point<R, ALG>(v, Xg[t] * Yg[blockIdx.x], rnl, QZ[base + t], h);
-    Z[lds_index(t)] = v;
+  {
+    // all of the thread's loads in flight before its first use
+    constexpr int PER = (N + NTH - 1) / NTH;
+    constexpr bool FULL = N % NTH == 0;      // else lines shorter than the workgroup
+    const cx<R> zero = mk((R)0, (R)0);
+    cx<R> v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int t = tid + u * NTH;
+      v[u] = (FULL || t < N) ? E[base + t] : zero;
+    }
+    if constexpr (FWD && ALG > 0) {
+      cx<R> qz[PER];
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        const int t = tid + u * NTH;
+        qz[u] = (FULL || t < N) ? QZ[base + t] : zero;
+      }
+#pragma unroll
+      for (int u = 0; u < PER; u++) {
+        const int t = tid + u * NTH;
+        if (FULL || t < N) v[u] = sp_rk_point<R, ALG>(v[u], Xg[t] * Yg[blockIdx.x], rnl, qz[u], h);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int t = tid + u * NTH;
+      if (FULL || t < N) Z[lds_index(t)] = v[u];
+    }
   }
   __syncthreads();
   fft_lds<R, N, NTH, FWD ? -1 : +1>(Z, tw, tid);
@@ -219,22 +244,44 @@ __global__ __launch_bounds__((sp_col_width<R, N>() * sp_col_threads<R, N>())) vo
   cx<R>* Z = reinterpret_cast<cx<R>*>(smem);
   const int tid = threadIdx.x, col = tid / NTHC, ctid = tid % NTHC;
   const size_t ix0 = (size_t)blockIdx.x * WB;
-  for (int q = tid; q < N * WB; q += NT) {
-    const int e = q / WB, c = q % WB;
-    Z[c * CS + lds_index(e)] = E[(size_t)e * Nx + ix0 + c];
+  // Each thread moves PER elements of one column (NT is a multiple of WB);
+  // each batch of loads is in flight together before its first use.
+  constexpr int PER = N * WB / NT;
+  static_assert(PER * NT == N * WB, "whole elements per thread");
+  constexpr int EST = NT / WB;
+  const int c = tid % WB, e0 = tid / WB;
+  {
+    cx<R> v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) v[k] = E[(size_t)(e0 + k * EST) * Nx + ix0 + c];
+#pragma unroll
+    for (int k = 0; k < PER; k++) Z[c * CS + lds_index(e0 + k * EST)] = v[k];
   }
   __syncthreads();
   cx<R>* Zc = Z + col * CS;
   fft_lds<R, N, NTHC, -1>(Zc, tw, ctid);
-  for (int q = tid; q < N * WB; q += NT) {
-    const int e = q / WB, c = q % WB;
-    Z[c * CS + lds_index(e)] = Z[c * CS + lds_index(e)] * P[(size_t)e * Nx + ix0 + c];
+  {
+    cx<R> p[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) p[k] = P[(size_t)(e0 + k * EST) * Nx + ix0 + c];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int e = e0 + k * EST;
+      Z[c * CS + lds_index(e)] = Z[c * CS + lds_index(e)] * p[k];
+    }
   }
   __syncthreads();
   fft_lds<R, N, NTHC, +1>(Zc, tw, ctid);
-  for (int q = tid; q < N * WB; q += NT) {
-    const int e = q / WB, c = q % WB;
-    E[(size_t)e * Nx + ix0 + c] = Z[c * CS + lds_index(e)];
+  {
+    // the indices pass through an empty asm so that the store addresses are
+    // formed here rather than kept live across the transforms
+    int cb = c, eb = e0;
+    asm volatile("" : "+v"(cb), "+v"(eb));
+    cx<R> v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) v[k] = Z[cb * CS + lds_index(eb + k * EST)];
+#pragma unroll
+    for (int k = 0; k < PER; k++) E[(size_t)(eb + k * EST) * Nx + ix0 + cb] = v[k];
   }
 }
 

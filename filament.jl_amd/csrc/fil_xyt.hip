@@ -157,9 +157,20 @@ __global__ __launch_bounds__((xyt_cols<R, N>() * xyt_col_threads<N>())) void xyt
   const long line = (MODE == 2 && pp.xsym) ? xyt_paired_line(blockIdx.x / nwb, pp.Nx) : (long)(blockIdx.x / nwb);
   const int w0 = (blockIdx.x % nwb) * WB;
   const size_t base = (size_t)(line * line_stride + w0);
-  for (int q = tid; q < N * WB; q += NT) {
-    const int e = q / WB, c = q % WB;
-    Z[c * CS + lds_index(e)] = S[base + (size_t)e * elem_stride + c];
+  // Each thread moves PER elements of one column (NT is a multiple of WB).
+  // All its loads are issued before its first LDS write: a load followed by
+  // its write in the same loop iteration would wait out each load's latency
+  // in turn.
+  constexpr int PER = N * WB / NT;
+  static_assert(PER * NT == N * WB, "whole elements per thread");
+  const int c = tid % WB, e0 = tid / WB;
+  constexpr int EST = NT / WB;                    // element step between a thread's elements
+  {
+    cx<R> v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) v[k] = S[base + (size_t)(e0 + k * EST) * elem_stride + c];
+#pragma unroll
+    for (int k = 0; k < PER; k++) Z[c * CS + lds_index(e0 + k * EST)] = v[k];
   }
   __syncthreads();
   cx<R>* Zc = Z + col * CS;
@@ -169,29 +180,52 @@ __global__ __launch_bounds__((xyt_cols<R, N>() * xyt_col_threads<N>())) void xyt
     fft_lds<R, N, NTHC, -1>(Zc, tw, ctid);
   }
   if constexpr (MODE == 2) {
-    // the line is ix, the elements iy: P row (min(ix, Nx - ix) or ix) + Px iy
+    // This thread's propagator values, all loads issued before the first
+    // multiply (held across the forward transform they would cost the second
+    // workgroup per CU its registers).  The line is ix, the elements iy: P row
+    // min(ix, Nx - ix) (xsym) or ix, plus Px iy; with ysym the value of
+    // iy <= Ny/2 also serves Ny - iy.
+    constexpr int PLIM = (N / 2 + 1) * WB;
+    constexpr int PPER = (PLIM + NT - 1) / NT;
     const long ixp = (pp.xsym && line > pp.Nx / 2) ? pp.Nx - line : line;
-    const cx<R>* Pl = P + (size_t)ixp * Nh + w0;
+    const cx<R>* Pl = P + (size_t)ixp * Nh + w0 + c;
     const size_t pstride = (size_t)pp.Px * Nh;
+    cx<R>* Zt = Z + c * CS;
     if (pp.ysym) {
-      for (int q = tid; q < (N / 2 + 1) * WB; q += NT) {
-        const int e = q / WB, c = q % WB;
-        const cx<R> p = Pl[(size_t)e * pstride + c];
-        Z[c * CS + lds_index(e)] = Z[c * CS + lds_index(e)] * p;
-        if (e != 0 && e != N / 2) Z[c * CS + lds_index(N - e)] = Z[c * CS + lds_index(N - e)] * p;
+      cx<R> pv[PPER];
+#pragma unroll
+      for (int k = 0; k < PPER; k++) {
+        const int e = e0 + k * EST;
+        if (e <= N / 2) pv[k] = Pl[(size_t)e * pstride];
+      }
+#pragma unroll
+      for (int k = 0; k < PPER; k++) {
+        const int e = e0 + k * EST;
+        if (e <= N / 2) {
+          Zt[lds_index(e)] = Zt[lds_index(e)] * pv[k];
+          if (e != 0 && e != N / 2) Zt[lds_index(N - e)] = Zt[lds_index(N - e)] * pv[k];
+        }
       }
     } else {
-      for (int q = tid; q < N * WB; q += NT) {
-        const int e = q / WB, c = q % WB;
-        Z[c * CS + lds_index(e)] = Z[c * CS + lds_index(e)] * Pl[(size_t)e * pstride + c];
+      // tables that are not even in ky (not the FFT grid's): one element at a time
+      for (int k = 0; k < PER; k++) {
+        const int e = e0 + k * EST;
+        Zt[lds_index(e)] = Zt[lds_index(e)] * Pl[(size_t)e * pstride];
       }
     }
     __syncthreads();
     fft_lds<R, N, NTHC, +1>(Zc, tw, ctid);
   }
-  for (int q = tid; q < N * WB; q += NT) {
-    const int e = q / WB, c = q % WB;
-    S[base + (size_t)e * elem_stride + c] = Z[c * CS + lds_index(e)];
+  {
+    // the element indices pass through an empty asm so that the store
+    // addresses are formed here rather than kept live across the transforms
+    int cb = c, eb = e0;
+    asm volatile("" : "+v"(cb), "+v"(eb));
+    cx<R> v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) v[k] = Z[cb * CS + lds_index(eb + k * EST)];
+#pragma unroll
+    for (int k = 0; k < PER; k++) S[base + (size_t)(eb + k * EST) * elem_stride + cb] = v[k];
   }
 }
 
