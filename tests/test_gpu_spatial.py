@@ -161,3 +161,17 @@ def test_spatial_abi_refuses_bad_input(pkg):
     assert rc == 0
     assert lib.fil_spatial_zstep(h, 0.0, 1e-3) == 3    # tables not set: FIL_ESTATE
     assert lib.fil_spatial_destroy(h) == 0
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_xy_lines_shorter_than_a_workgroup(pkg, orc, precision):
+    """Nx = 32, Ny = 16: the x-line kernel runs its 64 threads over 32 points
+    (the loads and writes of the threads past the line are skipped), the y
+    columns are 16 points; against the oracle."""
+    p = case(pkg, "XY", "wide")
+    p["Nx"], p["Ny"] = 32, 16
+    m, o = run_both(pkg, orc, p, precision, 20)
+    err = rel(m.field(), o.field())
+    m.close()
+    print(f"XY 32 x 16 precision={precision}: rel_l2 {err:.3e}")
+    assert err < TOL[precision]
