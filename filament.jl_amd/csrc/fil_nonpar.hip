@@ -85,9 +85,21 @@ __global__ __launch_bounds__(NTH) void np_stage_kernel(RowArgs<R> a, const cx<R>
 #pragma unroll
     for (int m = 0; m < SP; m++) Us[m] = U[(size_t)i * NH + tid + m * NTH];
     stage_field<R, N, NTH>(Us, e, Z, a.tw);
-    for (int t = tid; t < N; t += NTH) {
-      rho[rk_index<R>(t)] = a.plasma ? rho_g[(size_t)i * N + t] : (R)0;
-      kd[rk_index<R>(t)] = a.plasma ? kd_g[(size_t)i * N + t] : (R)0;
+    {
+      // the row's density and kd from the first stage, all loads in flight
+      // before the first LDS write
+      R rv[TP], kv[TP];
+#pragma unroll
+      for (int u = 0; u < TP; u++) {
+        const size_t q = (size_t)i * N + tid + u * NTH;
+        rv[u] = a.plasma ? rho_g[q] : (R)0;
+        kv[u] = a.plasma ? kd_g[q] : (R)0;
+      }
+#pragma unroll
+      for (int u = 0; u < TP; u++) {
+        rho[rk_index<R>(tid + u * NTH)] = rv[u];
+        kd[rk_index<R>(tid + u * NTH)] = kv[u];
+      }
     }
     __syncthreads();
   }
